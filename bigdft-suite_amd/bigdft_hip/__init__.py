@@ -1,0 +1,294 @@
+"""Host side of the MI355X local-Hamiltonian library (ctypes over the C ABI).
+
+Mirrors the accelerator interface the reference drives from
+LocalHamiltonianApplication (bigdft/src/hpsiortho.f90:892-903):
+  init_acceleration_OCL  -> Context(device)
+  allocate_data_OCL      -> Context.region(Region)   (keys validated+uploaded)
+  local_hamiltonian_OCL  -> local_hamiltonian(...)    (orbital batch on device)
+  finish_hamiltonian_OCL -> the weighted energy sums returned by the above
+Device memory is handled with torch tensors (plumbing only); every compute
+call goes through libbigdft_hip.so, and importing this package fails loudly
+when that library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import descriptors  # noqa: F401
+from .descriptors import Region, dense_region, sphere_region  # noqa: F401
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+ERRORS = {1: "BDFT_ERR_ARG", 2: "BDFT_ERR_HIP", 3: "BDFT_ERR_DESC",
+          4: "BDFT_ERR_NOMEM", 5: "BDFT_ERR_UNSUPPORTED"}
+
+# entry points declared in include/bigdft_hip.h
+EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
+           "bdft_last_error", "bdft_lr_create", "bdft_lr_destroy",
+           "bdft_lr_info", "bdft_locham_d", "bdft_locham_batch_d",
+           "bdft_plan_create", "bdft_plan_execute", "bdft_plan_destroy",
+           "bdft_local_hamiltonian", "bdft_uncompress_d",
+           "bdft_compress_acc_d", "bdft_profile_enable", "bdft_profile_read",
+           "bdft_stage_name", "bdft_sync")
+
+
+class BigDFTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    """Load libbigdft_hip.so and declare the C signatures.  Raises if the
+    library has not been built (there is no fallback path)."""
+    if not os.path.exists(path):
+        raise ImportError(f"{path} not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    I3 = ctypes.c_int * 3
+    D3 = ctypes.c_double * 3
+    sig = {
+        "bdft_ctx_create": [I, ctypes.POINTER(P)],
+        "bdft_ctx_destroy": [P],
+        "bdft_ctx_set_stream": [P, P],
+        "bdft_lr_create": [P, ctypes.c_char, I3, I3, I3, D3, I, I, P, P, I, I,
+                           P, P, ctypes.POINTER(P)],
+        "bdft_lr_destroy": [P],
+        "bdft_lr_info": [P, I3, ctypes.POINTER(ctypes.c_int64)],
+        "bdft_locham_d": [P, P, I, P, P, P, P, P, I],
+        "bdft_locham_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
+        "bdft_plan_create": [P, I, P, P, P, P, P, P, P, P, P, P, I,
+                             ctypes.POINTER(P)],
+        "bdft_plan_execute": [P],
+        "bdft_plan_destroy": [P],
+        "bdft_local_hamiltonian": [P, P, I, P, P, P, P, P, I],
+        "bdft_uncompress_d": [P, P, I, P, P],
+        "bdft_compress_acc_d": [P, P, I, P, P],
+        "bdft_sync": [P],
+        "bdft_profile_enable": [P, I],
+        "bdft_profile_read": [P, I, P, P, I],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = I
+    lib.bdft_last_error.argtypes = [P]
+    lib.bdft_last_error.restype = ctypes.c_char_p
+    lib.bdft_stage_name.argtypes = [I]
+    lib.bdft_stage_name.restype = ctypes.c_char_p
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _ptr(t):
+    """Device (or host) address of a torch tensor / numpy array / None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(int(t))
+
+
+class LocReg:
+    """A region registered on a context (bdft_lr)."""
+
+    def __init__(self, ctx, region):
+        self.ctx, self.region = ctx, region
+        h = ctypes.c_void_p()
+        I3, D3 = ctypes.c_int * 3, ctypes.c_double * 3
+        kg_c = np.ascontiguousarray(region.keyg_c, dtype=np.int32)
+        kv_c = np.ascontiguousarray(region.keyv_c, dtype=np.int32)
+        kg_f = np.ascontiguousarray(region.keyg_f, dtype=np.int32)
+        kv_f = np.ascontiguousarray(region.keyv_f, dtype=np.int32)
+        ctx._check(lib().bdft_lr_create(
+            ctx.handle, region.geocode.encode(), I3(*region.n),
+            I3(*region.nfl), I3(*region.nfu), D3(*region.hgrid),
+            region.nseg_c, region.nvctr_c, _ptr(kg_c), _ptr(kv_c),
+            region.nseg_f, region.nvctr_f, _ptr(kg_f), _ptr(kv_f),
+            ctypes.byref(h)))
+        self.handle = h
+
+    def info(self):
+        nf = (ctypes.c_int * 3)()
+        nc = ctypes.c_int64()
+        lib().bdft_lr_info(self.handle, nf, ctypes.byref(nc))
+        return tuple(nf), nc.value
+
+    def close(self):
+        if self.handle:
+            lib().bdft_lr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A captured batch (bdft_plan); execute() replays its hipGraph."""
+
+    def __init__(self, ctx, handle, keep):
+        self.ctx, self.handle, self._keep = ctx, handle, keep
+
+    def execute(self):
+        self.ctx._check(lib().bdft_plan_execute(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().bdft_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """bdft_ctx on one device (init_acceleration_OCL)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().bdft_ctx_create(device, ctypes.byref(h))
+        if rc:
+            raise BigDFTError(rc, f"cannot create a context on device {device}")
+        self.handle = h
+
+    def _check(self, rc):
+        if rc:
+            msg = lib().bdft_last_error(self.handle)
+            raise BigDFTError(rc, msg.decode() if msg else "")
+
+    def region(self, region):
+        return LocReg(self, region)
+
+    def set_stream(self, stream):
+        """Run on a torch/HIP stream (object with .cuda_stream) or None."""
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        self._check(lib().bdft_ctx_set_stream(self.handle, s))
+
+    def sync(self):
+        self._check(lib().bdft_sync(self.handle))
+
+    def profile(self, on=True):
+        self._check(lib().bdft_profile_enable(self.handle, int(on)))
+
+    def profile_read(self, reset=True):
+        """{stage name: accumulated ms}, number of profiled launches"""
+        n = 16
+        ms = np.zeros(n)
+        cnt = ctypes.c_int64()
+        self._check(lib().bdft_profile_read(self.handle, n, _ptr(ms),
+                                            ctypes.byref(cnt), int(reset)))
+        out = {}
+        for i in range(n):
+            name = lib().bdft_stage_name(i).decode()
+            if name:
+                out[name] = float(ms[i])
+        return out, cnt.value
+
+    # ---- device-pointer entry points (torch tensors) ----
+    def locham(self, lr, pot, psi, hpsi, ekin, epot, precision=PREC_FP64):
+        norb = ekin.numel()
+        self._check(lib().bdft_locham_d(self.handle, lr.handle, norb,
+                                        _ptr(pot), _ptr(psi), _ptr(hpsi),
+                                        _ptr(ekin), _ptr(epot), precision))
+
+    @staticmethod
+    def _batch_args(lrs, pot_off, potdim, ishift, psi_off):
+        arr = (ctypes.c_void_p * len(lrs))(*[l.handle.value for l in lrs])
+
+        def a(x, dt):
+            return None if x is None else np.ascontiguousarray(x, dtype=dt)
+        return (arr, a(pot_off, np.int64), a(potdim, np.int32),
+                a(ishift, np.int32), a(psi_off, np.int64))
+
+    def locham_batch(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None,
+                     potdim=None, ishift=None, psi_off=None,
+                     precision=PREC_FP64):
+        arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
+                                               psi_off)
+        self._check(lib().bdft_locham_batch_d(
+            self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
+            _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
+            _ptr(epot), precision))
+
+    def plan(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None, potdim=None,
+             ishift=None, psi_off=None, precision=PREC_FP64):
+        arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
+                                               psi_off)
+        h = ctypes.c_void_p()
+        self._check(lib().bdft_plan_create(
+            self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
+            _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
+            _ptr(epot), precision, ctypes.byref(h)))
+        return Plan(self, h, (lrs, pot, psi, hpsi, ekin, epot))
+
+    def uncompress(self, lr, norb, psi, psig):
+        self._check(lib().bdft_uncompress_d(self.handle, lr.handle, norb,
+                                            _ptr(psi), _ptr(psig)))
+
+    def compress_acc(self, lr, norb, psig, hpsi):
+        self._check(lib().bdft_compress_acc_d(self.handle, lr.handle, norb,
+                                              _ptr(psig), _ptr(hpsi)))
+
+    # ---- host-array entry point (numpy) ----
+    def locham_host(self, lr, pot, psi, hpsi, precision=PREC_FP64):
+        """numpy in/out; hpsi is accumulated in place; returns (ekin, epot)."""
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        norb = psi.shape[0] if psi.ndim > 1 else 1
+        assert hpsi.flags["C_CONTIGUOUS"] and hpsi.dtype == np.float64
+        ek, ep = np.zeros(norb), np.zeros(norb)
+        pot = np.ascontiguousarray(pot, dtype=np.float64)
+        self._check(lib().bdft_local_hamiltonian(
+            self.handle, lr.handle, norb, _ptr(pot), _ptr(psi), _ptr(hpsi),
+            _ptr(ek), _ptr(ep), precision))
+        return ek, ep
+
+    def close(self):
+        if self.handle:
+            lib().bdft_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def local_hamiltonian(ctx, lr, occup, pot, psi, hpsi, kwgt=1.0,
+                      precision=PREC_FP64):
+    """local_hamiltonian_OCL + finish_hamiltonian_OCL semantics
+    (bigdft/src/interface_ocl.f90:391-607) on device tensors: hpsi += H_loc psi
+    for every orbital, and the returned energies are
+    ekin_sum = sum_i kwgt*occup_i*ekin_i, epot_sum = sum_i kwgt*occup_i*epot_i
+    (the per-orbital weighting of hpsiortho.f90:946-954)."""
+    import torch
+    norb = len(occup)
+    ekin = torch.zeros(norb, dtype=torch.float64, device=psi.device)
+    epot = torch.zeros(norb, dtype=torch.float64, device=psi.device)
+    ctx.locham(lr, pot, psi, hpsi, ekin, epot, precision)
+    ctx.sync()
+    w = np.asarray(occup, dtype=np.float64) * kwgt
+    ek, ep = ekin.cpu().numpy(), epot.cpu().numpy()
+    return float((w * ek).sum()), float((w * ep).sum()), ek, ep

@@ -1,0 +1,753 @@
+// C-ABI layer: contexts, localisation regions (descriptor validation and
+// upload), batch plans and their hipGraph capture.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bigdft_hip.h"
+#include "bdft_internal.h"
+
+using namespace bdft;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct bdft_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // grow-only workspace used by one-shot calls
+  DevBuf ws;
+  // cache of the last one-shot plan
+  bdft_plan* cached = nullptr;
+  std::vector<uintptr_t> cached_key;
+  // stage profiling (non-graph launches)
+  bool profile = false;
+  std::vector<hipEvent_t> ev_free;                 // reusable events
+  std::vector<std::vector<hipEvent_t>> ev_pending; // one set per launch
+  double prof_ms[S_COUNT] = {0};
+  long long prof_n = 0;
+};
+
+struct bdft_lr {
+  bdft_ctx* ctx = nullptr;
+  char geocode = 'P';
+  int n[3] = {0, 0, 0}, nfl[3] = {0, 0, 0}, nfu[3] = {0, 0, 0};
+  int N[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+  double hgrid[3] = {0, 0, 0};
+  int nvctr_c = 0, nvctr_f = 0;
+  int* d_cmap = nullptr;
+  int* d_fmap = nullptr;
+  bool dense = false;
+};
+
+struct bdft_plan {
+  bdft_ctx* ctx = nullptr;
+  int nbox = 0;
+  int precision = 0;
+  int maxN = 0;
+  long long wtotal = 0;
+  bool need_zero = true;
+  bool owns_ws = false;
+  DevBuf ws;           // own workspace (persistent plans)
+  DevBuf meta;         // box descriptors + tile starts + energy partials
+  Batch B{};
+  PassTables T{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int set_err(bdft_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                     \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return set_err(ctx, BDFT_ERR_HIP, "%s failed: %s", #call,               \
+                     hipGetErrorString(e_));                                  \
+  } while (0)
+
+static int dev_alloc(bdft_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return BDFT_OK;
+  if (b.p) {
+    hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return BDFT_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return set_err(ctx, BDFT_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes,
+                   hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return BDFT_OK;
+}
+
+static void dev_free(DevBuf& b) {
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int bdft_ctx_create(int device, bdft_ctx** out) {
+  if (!out) return BDFT_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return BDFT_ERR_ARG;
+  bdft_ctx* ctx = new bdft_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return BDFT_ERR_HIP;
+  }
+  ctx->stream = ctx->own;
+  *out = ctx;
+  return BDFT_OK;
+}
+
+extern "C" int bdft_plan_destroy(bdft_plan* plan);
+
+extern "C" int bdft_ctx_destroy(bdft_ctx* ctx) {
+  if (!ctx) return BDFT_ERR_ARG;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& set : ctx->ev_pending)
+    for (auto x : set) hipEventDestroy(x);
+  for (auto x : ctx->ev_free) hipEventDestroy(x);
+  if (ctx->cached) bdft_plan_destroy(ctx->cached);
+  dev_free(ctx->ws);
+  if (ctx->own) hipStreamDestroy(ctx->own);
+  delete ctx;
+  return BDFT_OK;
+}
+
+extern "C" int bdft_ctx_set_stream(bdft_ctx* ctx, void* s) {
+  if (!ctx) return BDFT_ERR_ARG;
+  ctx->stream = s ? (hipStream_t)s : ctx->own;
+  return BDFT_OK;
+}
+
+extern "C" const char* bdft_last_error(const bdft_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int bdft_sync(bdft_ctx* ctx) {
+  if (!ctx) return BDFT_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return BDFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// localisation regions: key validation and decode into dense index maps
+// (segment decode of uncompress_per, bigdft/src/wavelib/un-compress.f90:903-913)
+// ---------------------------------------------------------------------------
+static int build_map(bdft_ctx* ctx, const char* what, const int N[3], int nseg,
+                     int nvctr, const int32_t* keyg, const int32_t* keyv,
+                     const int* lo, const int* hi, std::vector<int>& map,
+                     std::vector<unsigned char>& occ) {
+  map.assign((size_t)nvctr, -1);
+  if (nvctr == 0) return BDFT_OK;
+  if (nseg <= 0 || !keyg || !keyv)
+    return set_err(ctx, BDFT_ERR_DESC, "%s: nvctr=%d but nseg=%d", what, nvctr, nseg);
+  const long long M = (long long)N[0] * N[1] * N[2];
+  for (int s = 0; s < nseg; ++s) {
+    const long long j0 = keyg[2 * s], j1 = keyg[2 * s + 1], jj = keyv[s];
+    if (j0 < 1 || j1 < j0 || j1 > M)
+      return set_err(ctx, BDFT_ERR_DESC, "%s segment %d: keyg (%lld,%lld) outside 1..%lld",
+                     what, s + 1, j0, j1, M);
+    long long ii = j0 - 1;
+    const int i3 = (int)(ii / ((long long)N[0] * N[1]));
+    ii -= (long long)i3 * N[0] * N[1];
+    const int i2 = (int)(ii / N[0]);
+    const int i0 = (int)(ii - (long long)i2 * N[0]);
+    const int i1 = i0 + (int)(j1 - j0);
+    if (i1 >= N[0])
+      return set_err(ctx, BDFT_ERR_DESC, "%s segment %d crosses a row", what, s + 1);
+    if (jj < 1 || jj - 1 + (j1 - j0) >= nvctr)
+      return set_err(ctx, BDFT_ERR_DESC, "%s segment %d: keyv %lld out of range", what,
+                     s + 1, jj);
+    if (lo && (i2 < lo[1] || i2 > hi[1] || i3 < lo[2] || i3 > hi[2] || i0 < lo[0] ||
+               i1 > hi[0]))
+      return set_err(ctx, BDFT_ERR_DESC, "%s segment %d outside the fine box", what, s + 1);
+    for (int i = i0; i <= i1; ++i) {
+      const long long v = jj - 1 + (i - i0);
+      const long long lin = i + (long long)N[0] * (i2 + (long long)N[1] * i3);
+      if (map[v] != -1)
+        return set_err(ctx, BDFT_ERR_DESC, "%s: coefficient %lld addressed twice", what, v + 1);
+      if (occ[lin])
+        return set_err(ctx, BDFT_ERR_DESC, "%s: grid point %lld addressed twice", what, lin + 1);
+      occ[lin] = 1;
+      map[v] = (int)lin;
+    }
+  }
+  for (int v = 0; v < nvctr; ++v)
+    if (map[v] < 0)
+      return set_err(ctx, BDFT_ERR_DESC, "%s: coefficient %d not covered by keys", what, v + 1);
+  return BDFT_OK;
+}
+
+extern "C" int bdft_lr_create(bdft_ctx* ctx, char geocode, const int n[3],
+                              const int nfl[3], const int nfu[3],
+                              const double hgrid[3], int nseg_c, int nvctr_c,
+                              const int32_t* keyg_c, const int32_t* keyv_c,
+                              int nseg_f, int nvctr_f, const int32_t* keyg_f,
+                              const int32_t* keyv_f, bdft_lr** out) {
+  if (!ctx || !out || !n || !hgrid) return set_err(ctx, BDFT_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (geocode != 'P' && geocode != 'F')
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED, "geocode '%c' not supported (P or F)", geocode);
+  if (nvctr_c < 0 || nvctr_f < 0 || nseg_c < 0 || nseg_f < 0)
+    return set_err(ctx, BDFT_ERR_ARG, "negative descriptor size");
+  bdft_lr lr;
+  lr.ctx = ctx;
+  lr.geocode = geocode;
+  for (int d = 0; d < 3; ++d) {
+    lr.n[d] = n[d];
+    lr.N[d] = n[d] + 1;
+    if (lr.N[d] < 1 || lr.N[d] > 4096)
+      return set_err(ctx, BDFT_ERR_UNSUPPORTED, "box extent n%d=%d outside 0..4095", d + 1,
+                     n[d]);
+    lr.L[d] = geocode == 'P' ? 2 * lr.N[d] : 2 * lr.N[d] + 29;
+    if (!(hgrid[d] > 0.0)) return set_err(ctx, BDFT_ERR_ARG, "hgrid must be positive");
+    lr.hgrid[d] = hgrid[d];
+    lr.nfl[d] = (geocode == 'F' && nfl) ? nfl[d] : 0;
+    lr.nfu[d] = (geocode == 'F' && nfu) ? nfu[d] : n[d];
+  }
+  if (geocode == 'F' && nvctr_f > 0) {
+    if (!nfl || !nfu) return set_err(ctx, BDFT_ERR_ARG, "free BC needs nfl/nfu");
+    for (int d = 0; d < 3; ++d)
+      if (lr.nfl[d] < 0 || lr.nfu[d] > n[d] || lr.nfl[d] > lr.nfu[d])
+        return set_err(ctx, BDFT_ERR_DESC, "fine box [%d,%d] outside 0..%d", lr.nfl[d],
+                       lr.nfu[d], n[d]);
+  }
+  const long long M = (long long)lr.N[0] * lr.N[1] * lr.N[2];
+  if (nvctr_c > M || nvctr_f > M)
+    return set_err(ctx, BDFT_ERR_DESC, "more coefficients than grid points");
+  std::vector<int> cmap, fmap;
+  std::vector<unsigned char> occ((size_t)M, 0);
+  int rc = build_map(ctx, "coarse", lr.N, nseg_c, nvctr_c, keyg_c, keyv_c, nullptr,
+                     nullptr, cmap, occ);
+  if (rc) return rc;
+  std::fill(occ.begin(), occ.end(), 0);
+  const bool fbox = geocode == 'F';
+  rc = build_map(ctx, "fine", lr.N, nseg_f, nvctr_f, keyg_f, keyv_f,
+                 fbox ? lr.nfl : nullptr, fbox ? lr.nfu : nullptr, fmap, occ);
+  if (rc) return rc;
+  lr.nvctr_c = nvctr_c;
+  lr.nvctr_f = nvctr_f;
+  lr.dense = (nvctr_c == M && nvctr_f == M);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (nvctr_c > 0) {
+    HIPCHK(ctx, hipMalloc(&lr.d_cmap, sizeof(int) * nvctr_c));
+    HIPCHK(ctx, hipMemcpy(lr.d_cmap, cmap.data(), sizeof(int) * nvctr_c,
+                          hipMemcpyHostToDevice));
+  }
+  if (nvctr_f > 0) {
+    HIPCHK(ctx, hipMalloc(&lr.d_fmap, sizeof(int) * nvctr_f));
+    HIPCHK(ctx, hipMemcpy(lr.d_fmap, fmap.data(), sizeof(int) * nvctr_f,
+                          hipMemcpyHostToDevice));
+  }
+  *out = new bdft_lr(lr);
+  return BDFT_OK;
+}
+
+extern "C" int bdft_lr_destroy(bdft_lr* lr) {
+  if (!lr) return BDFT_ERR_ARG;
+  if (lr->ctx) {
+    hipSetDevice(lr->ctx->device);
+    hipStreamSynchronize(lr->ctx->stream);
+    if (lr->ctx->cached) {  // a cached plan may reference this region's maps
+      bdft_plan_destroy(lr->ctx->cached);
+      lr->ctx->cached = nullptr;
+      lr->ctx->cached_key.clear();
+    }
+  }
+  if (lr->d_cmap) hipFree(lr->d_cmap);
+  if (lr->d_fmap) hipFree(lr->d_fmap);
+  delete lr;
+  return BDFT_OK;
+}
+
+extern "C" int bdft_lr_info(const bdft_lr* lr, int nfine[3], int64_t* ncoef) {
+  if (!lr) return BDFT_ERR_ARG;
+  if (nfine)
+    for (int d = 0; d < 3; ++d) nfine[d] = lr->L[d];
+  if (ncoef) *ncoef = (int64_t)lr->nvctr_c + 7 * (int64_t)lr->nvctr_f;
+  return BDFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+static int box_tiles(const BoxDesc& b, int pass) {
+  const int N1 = b.N[0], N2 = b.N[1], N3 = b.N[2], L1 = b.L[0], L2 = b.L[1];
+  switch (pass) {
+    case P_UNC: return cdiv(b.nvctr_c, 256) + cdiv(b.nvctr_f, 256);
+    case P_XF:
+    case P_XB: return cdiv(4LL * N2 * N3, TW);
+    case P_YF:
+    case P_YB: return cdiv(L1, TW) * 2 * N3;
+    case P_KY: return cdiv(2 * N1, TW) * 2 * N3;
+    case P_KZ: return cdiv(2 * N1, TW) * 2 * N2;
+    case P_ZP: return cdiv(L1, TW) * L2;
+  }
+  return 0;
+}
+
+static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                     const double* pot, const int64_t* pot_off, const int32_t* potdim,
+                     const int32_t* ishift, const double* psi, const int64_t* psi_off,
+                     double* hpsi, double* ekin, double* epot, int precision,
+                     bool own_ws, bdft_plan** out) {
+  if (!ctx || !out) return BDFT_ERR_ARG;
+  *out = nullptr;
+  if (norb <= 0 || !lrs || !pot || !psi || !hpsi || !ekin || !epot)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  if (precision != BDFT_PREC_FP64)
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
+  std::vector<BoxDesc> box((size_t)norb);
+  long long wtot = 0, t1tot = 0, t2tot = 0;
+  int maxN = 0;
+  bool need_zero = false;
+  for (int i = 0; i < norb; ++i) {
+    const bdft_lr* lr = lrs[i];
+    if (!lr || lr->ctx != ctx)
+      return set_err(ctx, BDFT_ERR_ARG, "orbital %d: region missing or from another context", i);
+    BoxDesc& b = box[i];
+    memset(&b, 0, sizeof b);
+    for (int d = 0; d < 3; ++d) {
+      b.N[d] = lr->N[d];
+      b.L[d] = lr->L[d];
+      b.kscale[d] = -0.5 / (lr->hgrid[d] * lr->hgrid[d]);
+      b.potdim[d] = potdim ? potdim[3 * i + d] : lr->L[d];
+      b.ishift[d] = ishift ? ishift[3 * i + d] : 0;
+      if (b.potdim[d] <= 0)
+        return set_err(ctx, BDFT_ERR_ARG, "orbital %d: potential extent <= 0", i);
+      maxN = std::max(maxN, lr->N[d]);
+      if (lr->N[d] > MAXN)
+        return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                       "orbital %d: %d coarse points along axis %d, at most %d supported", i,
+                       lr->N[d], d + 1, MAXN);
+    }
+    b.per = lr->geocode == 'P';
+    b.nvctr_c = lr->nvctr_c;
+    b.nvctr_f = lr->nvctr_f;
+    b.cmap = lr->d_cmap;
+    b.fmap = lr->d_fmap;
+    b.w_off = wtot;
+    b.t1_off = t1tot;
+    b.t2_off = t2tot;
+    b.pot_off = pot_off ? pot_off[i] : 0;
+    const long long ncoef = (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
+    b.psi_off = psi_off ? psi_off[i] : ncoef * i;
+    wtot += 8LL * b.N[0] * b.N[1] * b.N[2];
+    t1tot += (long long)b.L[0] * 4 * b.N[1] * b.N[2];
+    t2tot += (long long)b.L[0] * b.L[1] * 2 * b.N[2];
+    if (!lr->dense) need_zero = true;
+  }
+  // tile tables
+  std::vector<int> starts;
+  PassTables T{};
+  int start_off[P_COUNT];
+  for (int p = 0; p < P_COUNT; ++p) {
+    const int t0 = box_tiles(box[0], p);
+    bool uni = true;
+    long long tot = 0;
+    for (int i = 0; i < norb; ++i) {
+      const int t = box_tiles(box[i], p);
+      uni = uni && t == t0;
+      tot += t;
+    }
+    if (tot > 0x7fffffffLL) return set_err(ctx, BDFT_ERR_UNSUPPORTED, "batch too large");
+    T.ntiles[p] = (int)tot;
+    T.t[p].nbox = norb;
+    T.t[p].uniform = uni ? t0 : 0;
+    start_off[p] = -1;
+    if (!uni) {
+      start_off[p] = (int)starts.size();
+      int acc = 0;
+      for (int i = 0; i < norb; ++i) {
+        starts.push_back(acc);
+        acc += box_tiles(box[i], p);
+      }
+      starts.push_back(acc);
+    }
+  }
+  bdft_plan* plan = new bdft_plan();
+  plan->ctx = ctx;
+  plan->nbox = norb;
+  plan->precision = precision;
+  plan->maxN = maxN;
+  plan->wtotal = wtot;
+  plan->need_zero = need_zero;
+  plan->owns_ws = own_ws;
+  // metadata: boxes | starts | partials
+  const size_t nb_box = sizeof(BoxDesc) * norb;
+  const size_t nb_st = sizeof(int) * starts.size();
+  const int epass[4] = {P_XF, P_KY, P_KZ, P_ZP};
+  size_t nb_part = 0;
+  for (int e = 0; e < 4; ++e) nb_part += sizeof(double) * T.ntiles[epass[e]];
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  int rc = dev_alloc(ctx, plan->meta, al(nb_box) + al(nb_st) + al(nb_part));
+  if (rc) {
+    delete plan;
+    return rc;
+  }
+  char* m = (char*)plan->meta.p;
+  BoxDesc* dbox = (BoxDesc*)m;
+  int* dst = (int*)(m + al(nb_box));
+  double* dpart = (double*)(m + al(nb_box) + al(nb_st));
+  hipError_t e1 = hipMemcpy(dbox, box.data(), nb_box, hipMemcpyHostToDevice);
+  hipError_t e2 = starts.empty() ? hipSuccess
+                                 : hipMemcpy(dst, starts.data(), nb_st, hipMemcpyHostToDevice);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    dev_free(plan->meta);
+    delete plan;
+    return set_err(ctx, BDFT_ERR_HIP, "descriptor upload failed");
+  }
+  for (int p = 0; p < P_COUNT; ++p) T.t[p].start = start_off[p] >= 0 ? dst + start_off[p] : nullptr;
+  {
+    double* q = dpart;
+    for (int e = 0; e < 4; ++e) {
+      T.part[e] = q;
+      q += T.ntiles[epass[e]];
+    }
+  }
+  // workspace: W | K | T1 | T2
+  const size_t ws_bytes = sizeof(double) * (2 * al(wtot) + al(t1tot) + al(t2tot));
+  DevBuf& ws = own_ws ? plan->ws : ctx->ws;
+  rc = dev_alloc(ctx, ws, ws_bytes);
+  if (rc) {
+    dev_free(plan->meta);
+    delete plan;
+    return rc;
+  }
+  double* w = (double*)ws.p;
+  Batch& B = plan->B;
+  B.box = dbox;
+  B.nbox = norb;
+  B.psi = psi;
+  B.hpsi = hpsi;
+  B.pot = pot;
+  B.W = w;
+  B.K = w + al(wtot);
+  B.T1 = w + 2 * al(wtot);
+  B.T2 = w + 2 * al(wtot) + al(t1tot);
+  B.ekin = ekin;
+  B.epot = epot;
+  plan->T = T;
+  *out = plan;
+  return BDFT_OK;
+}
+
+static int run_plan(bdft_plan* plan) {
+  bdft_ctx* ctx = plan->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (plan->exec) {
+    HIPCHK(ctx, hipGraphLaunch(plan->exec, ctx->stream));
+    return BDFT_OK;
+  }
+  hipEvent_t* ev = nullptr;
+  if (ctx->profile) {
+    std::vector<hipEvent_t> set(2 * S_COUNT);
+    for (auto& x : set) {
+      if (!ctx->ev_free.empty()) {
+        x = ctx->ev_free.back();
+        ctx->ev_free.pop_back();
+      } else {
+        HIPCHK(ctx, hipEventCreate(&x));
+      }
+    }
+    ctx->ev_pending.push_back(std::move(set));
+    ev = ctx->ev_pending.back().data();
+  }
+  hipError_t e = launch_fp64(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero,
+                             ctx->stream, ev);
+  if (e != hipSuccess)
+    return set_err(ctx, BDFT_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  return BDFT_OK;
+}
+
+extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                                const double* pot, const int64_t* pot_off,
+                                const int32_t* potdim, const int32_t* ishift,
+                                const double* psi, const int64_t* psi_off, double* hpsi,
+                                double* ekin, double* epot, int precision,
+                                bdft_plan** out) {
+  int rc = make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
+                     epot, precision, true, out);
+  if (rc) return rc;
+  bdft_plan* plan = *out;
+  // capture the launch sequence once; replays reuse the instantiated graph
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipError_t el = launch_fp64(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero, cs);
+    e = hipStreamEndCapture(cs, &plan->graph);
+    if (el != hipSuccess) e = el;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+  if (cs) hipStreamDestroy(cs);
+  if (e != hipSuccess) {
+    bdft_plan_destroy(plan);
+    *out = nullptr;
+    return set_err(ctx, BDFT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  }
+  return BDFT_OK;
+}
+
+extern "C" int bdft_plan_execute(bdft_plan* plan) {
+  if (!plan) return BDFT_ERR_ARG;
+  return run_plan(plan);
+}
+
+extern "C" int bdft_plan_destroy(bdft_plan* plan) {
+  if (!plan) return BDFT_ERR_ARG;
+  if (plan->ctx) {
+    hipSetDevice(plan->ctx->device);
+    hipStreamSynchronize(plan->ctx->stream);
+  }
+  if (plan->exec) hipGraphExecDestroy(plan->exec);
+  if (plan->graph) hipGraphDestroy(plan->graph);
+  dev_free(plan->meta);
+  if (plan->owns_ws) dev_free(plan->ws);
+  delete plan;
+  return BDFT_OK;
+}
+
+// one-shot batch: reuses the context's last plan when called again with the
+// same arguments (the common case of an SCF loop)
+extern "C" int bdft_locham_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                                   const double* pot, const int64_t* pot_off,
+                                   const int32_t* potdim, const int32_t* ishift,
+                                   const double* psi, const int64_t* psi_off, double* hpsi,
+                                   double* ekin, double* epot, int precision) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (norb <= 0 || !lrs) return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  std::vector<uintptr_t> key;
+  key.reserve(16 + (size_t)norb * 9);
+  key.push_back((uintptr_t)norb);
+  key.push_back((uintptr_t)precision);
+  key.push_back((uintptr_t)pot);
+  key.push_back((uintptr_t)psi);
+  key.push_back((uintptr_t)hpsi);
+  key.push_back((uintptr_t)ekin);
+  key.push_back((uintptr_t)epot);
+  key.push_back((uintptr_t)(pot_off != nullptr) | ((uintptr_t)(potdim != nullptr) << 1) |
+                ((uintptr_t)(ishift != nullptr) << 2) | ((uintptr_t)(psi_off != nullptr) << 3));
+  for (int i = 0; i < norb; ++i) {
+    key.push_back((uintptr_t)lrs[i]);
+    if (pot_off) key.push_back((uintptr_t)pot_off[i]);
+    if (psi_off) key.push_back((uintptr_t)psi_off[i]);
+    for (int d = 0; d < 3; ++d) {
+      if (potdim) key.push_back((uintptr_t)(uint32_t)potdim[3 * i + d]);
+      if (ishift) key.push_back((uintptr_t)(uint32_t)ishift[3 * i + d]);
+    }
+  }
+  if (!ctx->cached || key != ctx->cached_key) {
+    if (ctx->cached) {
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      bdft_plan_destroy(ctx->cached);
+      ctx->cached = nullptr;
+      ctx->cached_key.clear();
+    }
+    bdft_plan* plan = nullptr;
+    int rc = make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
+                       epot, precision, false, &plan);
+    if (rc) return rc;
+    ctx->cached = plan;
+    ctx->cached_key.swap(key);
+  }
+  return run_plan(ctx->cached);
+}
+
+extern "C" int bdft_locham_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* pot,
+                             const double* psi, double* hpsi, double* ekin, double* epot,
+                             int precision) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!lr || norb <= 0) return set_err(ctx, BDFT_ERR_ARG, "null region or norb <= 0");
+  std::vector<const bdft_lr*> lrs((size_t)norb, lr);
+  return bdft_locham_batch_d(ctx, norb, lrs.data(), pot, nullptr, nullptr, nullptr, psi,
+                             nullptr, hpsi, ekin, epot, precision);
+}
+
+extern "C" int bdft_local_hamiltonian(bdft_ctx* ctx, const bdft_lr* lr, int norb,
+                                      const double* pot, const double* psi, double* hpsi,
+                                      double* ekin, double* epot, int precision) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!lr || norb <= 0 || !pot || !psi || !hpsi || !ekin || !epot)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t ncoef = (size_t)lr->nvctr_c + 7 * (size_t)lr->nvctr_f;
+  const size_t npot = (size_t)lr->L[0] * lr->L[1] * lr->L[2];
+  double *dpot = nullptr, *dpsi = nullptr, *dh = nullptr, *den = nullptr;
+  int rc = BDFT_OK;
+  hipError_t e = hipMalloc(&dpot, npot * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dpsi, ncoef * norb * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dh, ncoef * norb * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&den, 2 * (size_t)norb * sizeof(double));
+  if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_NOMEM, "device buffers: %s", hipGetErrorString(e));
+  hipStream_t s = ctx->stream;
+  if (!rc) {
+    e = hipMemcpyAsync(dpot, pot, npot * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dpsi, psi, ncoef * norb * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dh, hpsi, ncoef * norb * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_HIP, "H2D copy: %s", hipGetErrorString(e));
+  }
+  if (!rc) {
+    std::vector<const bdft_lr*> lrs((size_t)norb, lr);
+    bdft_plan* plan = nullptr;
+    rc = make_plan(ctx, norb, lrs.data(), dpot, nullptr, nullptr, nullptr, dpsi, nullptr, dh,
+                   den, den + norb, precision, true, &plan);
+    if (!rc) {
+      rc = run_plan(plan);
+      if (!rc) {
+        e = hipMemcpyAsync(hpsi, dh, ncoef * norb * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(ekin, den, norb * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(epot, den + norb, norb * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_HIP, "D2H copy: %s", hipGetErrorString(e));
+      }
+      bdft_plan_destroy(plan);
+    }
+  }
+  hipStreamSynchronize(s);
+  if (dpot) hipFree(dpot);
+  if (dpsi) hipFree(dpsi);
+  if (dh) hipFree(dh);
+  if (den) hipFree(den);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// compressed <-> mixed layout transfers
+// ---------------------------------------------------------------------------
+static int transfer(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* psi,
+                    double* hpsi, double* psig, bool compress) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!lr || norb <= 0 || !psig || (compress ? !hpsi : !psi))
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::vector<BoxDesc> box((size_t)norb);
+  const long long vol = 8LL * lr->N[0] * lr->N[1] * lr->N[2];
+  const long long ncoef = (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
+  for (int i = 0; i < norb; ++i) {
+    BoxDesc& b = box[i];
+    memset(&b, 0, sizeof b);
+    for (int d = 0; d < 3; ++d) {
+      b.N[d] = lr->N[d];
+      b.L[d] = lr->L[d];
+    }
+    b.per = lr->geocode == 'P';
+    b.nvctr_c = lr->nvctr_c;
+    b.nvctr_f = lr->nvctr_f;
+    b.cmap = lr->d_cmap;
+    b.fmap = lr->d_fmap;
+    b.w_off = vol * i;
+    b.psi_off = ncoef * i;
+  }
+  DevBuf meta;
+  int rc = dev_alloc(ctx, meta, sizeof(BoxDesc) * norb);
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(meta.p, box.data(), sizeof(BoxDesc) * norb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    Batch B{};
+    B.box = (const BoxDesc*)meta.p;
+    B.nbox = norb;
+    B.psi = psi;
+    B.hpsi = hpsi;
+    B.W = psig;
+    TileTable t{nullptr, norb, box_tiles(box[0], P_UNC)};
+    const long long nt = (long long)t.uniform * norb;
+    if (t.uniform == 0 || nt > 0x7fffffffLL) {
+      dev_free(meta);
+      return t.uniform == 0 ? BDFT_OK : set_err(ctx, BDFT_ERR_UNSUPPORTED, "too many points");
+    }
+    e = launch_transfer(B, t, (int)nt, vol * norb, compress, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
+  dev_free(meta);
+  if (e != hipSuccess) return set_err(ctx, BDFT_ERR_HIP, "transfer: %s", hipGetErrorString(e));
+  return BDFT_OK;
+}
+
+extern "C" int bdft_uncompress_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* psi,
+                                 double* psig) {
+  return transfer(ctx, lr, norb, psi, nullptr, psig, false);
+}
+
+extern "C" int bdft_compress_acc_d(bdft_ctx* ctx, const bdft_lr* lr, int norb,
+                                   const double* psig, double* hpsi) {
+  return transfer(ctx, lr, norb, nullptr, hpsi, const_cast<double*>(psig), true);
+}
+
+// ---------------------------------------------------------------------------
+// stage profiling with HIP events
+// ---------------------------------------------------------------------------
+static const char* kStageNames[S_COUNT] = {
+    "zero",   "uncompress", "x_grow_kin", "y_grow", "y_kin",   "z_kin",
+    "z_pot",  "y_shrink",   "x_shrink",   "compress", "energy"};
+
+extern "C" int bdft_profile_enable(bdft_ctx* ctx, int on) {
+  if (!ctx) return BDFT_ERR_ARG;
+  ctx->profile = on != 0;
+  return BDFT_OK;
+}
+
+extern "C" int bdft_profile_read(bdft_ctx* ctx, int nstage, double* ms, int64_t* nlaunch,
+                                 int reset) {
+  if (!ctx || nstage < 0 || (nstage > 0 && !ms)) return BDFT_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& set : ctx->ev_pending) {
+    for (int s = 0; s < S_COUNT; ++s) {
+      float t = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&t, set[2 * s], set[2 * s + 1]));
+      ctx->prof_ms[s] += t;
+    }
+    ++ctx->prof_n;
+    for (auto x : set) ctx->ev_free.push_back(x);
+  }
+  ctx->ev_pending.clear();
+  for (int s = 0; s < nstage && s < S_COUNT; ++s) ms[s] = ctx->prof_ms[s];
+  if (nlaunch) *nlaunch = ctx->prof_n;
+  if (reset) {
+    for (int s = 0; s < S_COUNT; ++s) ctx->prof_ms[s] = 0.0;
+    ctx->prof_n = 0;
+  }
+  return BDFT_OK;
+}
+
+extern "C" const char* bdft_stage_name(int stage) {
+  return (stage >= 0 && stage < S_COUNT) ? kStageNames[stage] : "";
+}

@@ -1,0 +1,669 @@
+// Plain FP64 kernels of the local-Hamiltonian apply.
+//
+// hpsi += G^T V G psi + K psi, evaluated as separable line passes over
+// bundles of 32 lines staged in LDS:
+//   G   = (synthesis o magic filter) per axis, the combined filter fil2 of
+//         bigdft/src/orbitals/growshrink.f90 (comb_grow_all), periodic wrap
+//         or zero padding;
+//   G^T = (magic^T o analysis) per axis (comb_shrink,
+//         growshrink.f90:310-340);
+//   K   = kinetic operator in the wavelet basis, 2x2 block of 29-tap filters
+//         per axis (ConvolkineticT, convolut_ib_optim.f90:921-1740).
+// For periodic boxes the reference applies the fine-scaling kinetic between
+// synthesis and analysis (convolut_kinetic_per_T, convolut_optim_per.f90:
+// 1169-1260); since analysis o synthesis = identity per periodic axis, that
+// equals the wavelet-basis operator used here.
+//
+// Pass order for one box (x fastest in every volume):
+//   uncompress   psi -> W                      (wavelet volume)
+//   xfwd         W  x-lines: T1 = G_x W, K = K_x W        (+ekin)
+//   yfwd         T1 y-lines: T2 = G_y T1
+//   kin(y), kin(z) W lines: K += K_d W                    (+ekin)
+//   zpot         T2 z-lines: T2 = G_z^T V G_z T2 (in place) (+epot)
+//   ybwd         T2 y-lines: T1 = G_y^T T2
+//   xbwd         T1 x-lines: W = G_x^T T1 + K
+//   compress     hpsi += compress(W)
+#include "bdft_internal.h"
+
+namespace bdft {
+
+__device__ __forceinline__ int pmod(int i, int n) {
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// ---------------------------------------------------------------------------
+// LDS staging.  Layout: lds[row * LDW + line].
+// ---------------------------------------------------------------------------
+
+// Wavelet line bundle: two sections (c, d) of NS = N + HL + HR rows; row r of
+// a section holds coarse index k = r - HL (wrapped or zero outside [0,N)).
+// CONTIG: lines are contiguous in memory (x pass), else positions are.
+template <bool CONTIG>
+__device__ __forceinline__ void fill_wav(double* lds, const double* __restrict__ src,
+                                         long long lstride, long long pstride,
+                                         int nl, int N, int per) {
+  const int NS = N + HL + HR;
+  const int rows = 2 * NS;
+  for (int idx = threadIdx.x; idx < rows * TW; idx += NT) {
+    int row, l;
+    if (CONTIG) {
+      l = idx / rows;
+      row = idx - l * rows;
+    } else {
+      row = idx / TW;
+      l = idx - row * TW;
+    }
+    const int sec = row >= NS;
+    int k = row - sec * NS - HL;
+    bool ok = l < nl;
+    if (per) k = pmod(k, N);
+    else ok = ok && k >= 0 && k < N;
+    double v = 0.0;
+    if (ok) v = src[(long long)l * lstride + (long long)(sec * N + k) * pstride];
+    lds[row * LDW + l] = v;
+  }
+}
+
+// Real-space line bundle for G^T: rows q = I + 14 with fine index
+// I in [-14, 2N+22); periodic wraps mod 2N, free reads position p = q < L.
+template <bool CONTIG>
+__device__ __forceinline__ void fill_real(double* lds, const double* __restrict__ src,
+                                          long long lstride, long long pstride,
+                                          int nl, int N, int L, int per) {
+  const int Q = 2 * N + 36;
+  for (int idx = threadIdx.x; idx < Q * TW; idx += NT) {
+    int q, l;
+    if (CONTIG) {
+      l = idx / Q;
+      q = idx - l * Q;
+    } else {
+      q = idx / TW;
+      l = idx - q * TW;
+    }
+    int pos;
+    bool ok = l < nl;
+    if (per) pos = pmod(q - 14, 2 * N);
+    else {
+      pos = q;
+      ok = ok && q < L;
+    }
+    double v = 0.0;
+    if (ok) v = src[(long long)l * lstride + (long long)pos * pstride];
+    lds[q * LDW + l] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-blocked 1D operators on one LDS column.
+// ---------------------------------------------------------------------------
+
+// G: outputs at fine positions I0..I0+7 (I0 even) from coarse k = I0/2-8+kk,
+// kk = 0..18: out(I) += fil2(I-2k,1) c(k) + fil2(I-2k,2) d(k).
+// cc points at the c-section row of k = I0/2-8, dd at the d-section row.
+__device__ __forceinline__ void grow8(const double* __restrict__ cc,
+                                      const double* __restrict__ dd, double o[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) o[r] = 0.0;
+#pragma unroll
+  for (int kk = 0; kk < 19; ++kk) {
+    const double c = cc[kk * LDW];
+    const double d = dd[kk * LDW];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int j = r + 16 - 2 * kk;  // fil2 index, valid in [-14, 16]
+      if (j >= -14 && j <= 16) {
+        o[r] = fma(Fil::F2S[j + 14], c, o[r]);
+        o[r] = fma(Fil::F2W[j + 14], d, o[r]);
+      }
+    }
+  }
+}
+
+// G^T: coarse outputs i0..i0+3 (c and d) from fine rows q = 2*i0 + qq,
+// qq = 0..36 (row q holds fine index q - 14): c(i) = sum_j fil2(j,1) x(2i+j).
+__device__ __forceinline__ void shrink4(const double* __restrict__ xq,
+                                        double oc[4], double od[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    oc[r] = 0.0;
+    od[r] = 0.0;
+  }
+#pragma unroll
+  for (int qq = 0; qq < 37; ++qq) {
+    const double x = xq[qq * LDW];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = qq - 2 * r - 14;
+      if (j >= -14 && j <= 16) {
+        oc[r] = fma(Fil::F2S[j + 14], x, oc[r]);
+        od[r] = fma(Fil::F2W[j + 14], x, od[r]);
+      }
+    }
+  }
+}
+
+// Wavelet-basis kinetic (unscaled): coarse outputs i0..i0+3 from rows of
+// k = i0-14+kk, kk = 0..31 (cc, dd point at k = i0-14 in each section).
+//   c_out(i) += a(l) c(i+l) + c(-l) d(i+l);  d_out(i) += c(l) c(i+l) + e(l) d(i+l)
+__device__ __forceinline__ void kin4(const double* __restrict__ cc,
+                                     const double* __restrict__ dd,
+                                     double oc[4], double od[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    oc[r] = 0.0;
+    od[r] = 0.0;
+  }
+#pragma unroll
+  for (int kk = 0; kk < 32; ++kk) {
+    const double c = cc[kk * LDW];
+    const double d = dd[kk * LDW];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int l = kk - 14 - r;
+      if (l >= -14 && l <= 14) {
+        const int al = l < 0 ? -l : l;
+        oc[r] = fma(Fil::KA[al], c, oc[r]);
+        oc[r] = fma(Fil::KC[-l + 14], d, oc[r]);
+        od[r] = fma(Fil::KC[l + 14], c, od[r]);
+        od[r] = fma(Fil::KE[al], d, od[r]);
+      }
+    }
+  }
+}
+
+// Block reduction of one double per thread; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
+  return s;
+}
+
+// grow-pass row of the first coarse k used by output chunk starting at p0
+__device__ __forceinline__ int grow_row0(int p0, int per) {
+  // periodic: I0 = p0, k0 = p0/2 - 8, row = k0 + HL;
+  // free:     I0 = p0 - 14, k0 = p0/2 - 15, row = k0 + HL
+  return per ? (p0 >> 1) + 7 : (p0 >> 1);
+}
+
+// ---------------------------------------------------------------------------
+// uncompress / compress (keys pre-decoded into dense coarse linear indices)
+// ---------------------------------------------------------------------------
+constexpr int CPT = 256;  // compressed points per tile
+
+__device__ __forceinline__ long long psig_index(const BoxDesc& bd, int lin, int s1,
+                                                int s2, int s3) {
+  const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2];
+  const int i1 = lin % N1;
+  const int t = lin / N1;
+  const int i2 = t % N2;
+  const int i3 = t / N2;
+  return (long long)(i1 + N1 * s1) +
+         2LL * N1 * ((long long)(i2 + N2 * s2) + 2LL * N2 * (i3 + N3 * s3));
+}
+
+// fine component t = 1..7 -> (s1,s2,s3) bits, un-compress.f90:930-936
+__device__ __forceinline__ void fsel(int t, int& s1, int& s2, int& s3) {
+  s1 = t & 1;
+  s2 = (t >> 1) & 1;
+  s3 = (t >> 2) & 1;
+}
+
+__global__ __launch_bounds__(NT) void k_zero(double* __restrict__ p, long long n) {
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n;
+       i += (long long)gridDim.x * NT)
+    p[i] = 0.0;
+}
+
+template <bool COMPRESS>
+__global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  double* W = B.W + bd.w_off;
+  if (lt < ctiles) {
+    const int k = lt * CPT + threadIdx.x;
+    if (k < bd.nvctr_c) {
+      const long long w = psig_index(bd, bd.cmap[k], 0, 0, 0);
+      if (COMPRESS) B.hpsi[bd.psi_off + k] += W[w];
+      else W[w] = B.psi[bd.psi_off + k];
+    }
+  } else {
+    const int k = (lt - ctiles) * CPT + threadIdx.x;
+    if (k < bd.nvctr_f) {
+      const int lin = bd.fmap[k];
+      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
+#pragma unroll
+      for (int t = 1; t <= 7; ++t) {
+        int s1, s2, s3;
+        fsel(t, s1, s2, s3);
+        const long long w = psig_index(bd, lin, s1, s2, s3);
+        if (COMPRESS) B.hpsi[base + t - 1] += W[w];
+        else W[w] = B.psi[base + t - 1];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass, forward: T1 = G_x W, K = K_x W, ekin partial
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part) {
+  extern __shared__ double lds[];
+  __shared__ double red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int N1 = bd.N[0], L1 = bd.L[0], per = bd.per;
+  const int W1 = 2 * N1;
+  const int nlines = 4 * bd.N[1] * bd.N[2];
+  const int l0 = lt * TW;
+  const int nl = min(TW, nlines - l0);
+  const double* src = B.W + bd.w_off + (long long)l0 * W1;
+  fill_wav<true>(lds, src, W1, 1, nl, N1, per);
+  __syncthreads();
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int NS = N1 + HL + HR;
+  const double* col = lds + tx;
+  double g[3][8];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int p0 = 8 * (ty + m * TY);
+    if (p0 < L1) {
+      const int r0 = grow_row0(p0, per);
+      grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
+    }
+  }
+  double kc[2][4], kd[2][4];
+  double ek = 0.0;
+  const double ks = bd.kscale[0];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int i0 = 4 * (ty + m * TY);
+    if (i0 < N1) {
+      kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, kc[m], kd[m]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        kc[m][r] *= ks;
+        kd[m][r] *= ks;
+        if (i0 + r < N1 && tx < nl)
+          ek += kc[m][r] * col[(i0 + r + HL) * LDW] +
+                kd[m][r] * col[(NS + i0 + r + HL) * LDW];
+      }
+    }
+  }
+  const double eks = block_sum(ek, red);
+  if (threadIdx.x == 0) part[tile] = eks;
+  __syncthreads();
+  // stage G outputs as [p][line] and store contiguous lines of T1
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int p0 = 8 * (ty + m * TY);
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (p0 + r < L1) lds[(p0 + r) * LDW + tx] = g[m][r];
+  }
+  __syncthreads();
+  double* dst = B.T1 + bd.t1_off + (long long)l0 * L1;
+  for (int idx = threadIdx.x; idx < nl * L1; idx += NT) {
+    const int l = idx / L1, p = idx - l * L1;
+    dst[idx] = lds[p * LDW + l];
+  }
+  __syncthreads();
+  // stage K outputs and store contiguous lines of K
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int i0 = 4 * (ty + m * TY);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N1) {
+        lds[(i0 + r) * LDW + tx] = kc[m][r];
+        lds[(N1 + i0 + r) * LDW + tx] = kd[m][r];
+      }
+  }
+  __syncthreads();
+  double* kdst = B.K + bd.w_off + (long long)l0 * W1;
+  for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
+    const int l = idx / W1, p = idx - l * W1;
+    kdst[idx] = lds[p * LDW + l];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y pass, forward: T2 = G_y T1
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_yfwd(Batch B, TileTable tt) {
+  extern __shared__ double lds[];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int L1 = bd.L[0], N2 = bd.N[1], L2 = bd.L[1], per = bd.per;
+  const int W2 = 2 * N2;
+  const int nxt = (L1 + TW - 1) / TW;
+  const int j3 = lt / nxt;
+  const int j1 = (lt - j3 * nxt) * TW;
+  const int nl = min(TW, L1 - j1);
+  const double* src = B.T1 + bd.t1_off + j1 + (long long)L1 * W2 * j3;
+  fill_wav<false>(lds, src, 1, L1, nl, N2, per);
+  __syncthreads();
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int NS = N2 + HL + HR;
+  const double* col = lds + tx;
+  double* dst = B.T2 + bd.t2_off + j1 + tx + (long long)L1 * L2 * j3;
+  for (int p0 = 8 * ty; p0 < L2; p0 += 8 * TY) {
+    double g[8];
+    const int r0 = grow_row0(p0, per);
+    grow8(col + r0 * LDW, col + (NS + r0) * LDW, g);
+    if (tx < nl) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (p0 + r < L2) dst[(long long)L1 * (p0 + r)] = g[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kinetic along y (AX=1) or z (AX=2) on the wavelet volume: K += K_d W
+// ---------------------------------------------------------------------------
+template <int AX>
+__global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part) {
+  extern __shared__ double lds[];
+  __shared__ double red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1];
+  const int Nd = bd.N[AX];
+  const int nxt = (W1 + TW - 1) / TW;
+  const int jo = lt / nxt;  // the other (slow) index: j3 for AX=1, j2 for AX=2
+  const int j1 = (lt - jo * nxt) * TW;
+  const int nl = min(TW, W1 - j1);
+  const long long pstride = AX == 1 ? (long long)W1 : (long long)W1 * W2;
+  const long long base = bd.w_off + j1 +
+                         (AX == 1 ? (long long)W1 * W2 * jo : (long long)W1 * jo);
+  fill_wav<false>(lds, B.W + base, 1, pstride, nl, Nd, bd.per);
+  __syncthreads();
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int NS = Nd + HL + HR;
+  const double* col = lds + tx;
+  const double ks = bd.kscale[AX];
+  double ek = 0.0;
+  double* kp = B.K + base + tx;
+  for (int i0 = 4 * ty; i0 < Nd; i0 += 4 * TY) {
+    double oc[4], od[4];
+    kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, oc, od);
+    if (tx < nl) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (i0 + r < Nd) {
+          const double vc = oc[r] * ks, vd = od[r] * ks;
+          ek += vc * col[(i0 + r + HL) * LDW] + vd * col[(NS + i0 + r + HL) * LDW];
+          kp[pstride * (i0 + r)] += vc;
+          kp[pstride * (Nd + i0 + r)] += vd;
+        }
+    }
+  }
+  const double eks = block_sum(ek, red);
+  if (threadIdx.x == 0) part[tile] = eks;
+}
+
+// ---------------------------------------------------------------------------
+// z pass with the potential: T2 <- G_z^T ( V . G_z T2 ), epot partial
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part) {
+  extern __shared__ double lds[];
+  __shared__ double red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int L1 = bd.L[0], L2 = bd.L[1], N3 = bd.N[2], L3 = bd.L[2], per = bd.per;
+  const int nxt = (L1 + TW - 1) / TW;
+  const int j2 = lt / nxt;
+  const int j1 = (lt - j2 * nxt) * TW;
+  const int nl = min(TW, L1 - j1);
+  const long long pstride = (long long)L1 * L2;
+  double* line = B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
+  fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
+  __syncthreads();
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int NS = N3 + HL + HR;
+  const double* col = lds + tx;
+  // potential index of this line; outside the potential box V = 0, which
+  // reproduces the zeroing of psir there (potential.f90:874-947)
+  const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
+  const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
+                    q2 < bd.potdim[1];
+  const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
+  const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  double g[3][8];
+  double ep = 0.0;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int p0 = 8 * (ty + m * TY);
+    if (p0 < L3) {
+      const int r0 = grow_row0(p0, per);
+      grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int q3 = p0 + r - bd.ishift[2];
+        double v = 0.0;
+        if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) v = pot[pz * q3];
+        const double t = v * g[m][r];
+        ep = fma(t, g[m][r], ep);
+        g[m][r] = t;
+      }
+    }
+  }
+  const double eps = block_sum(ep, red);
+  if (threadIdx.x == 0) part[tile] = eps;
+  __syncthreads();
+  // V psir as a real line bundle: row q holds fine index q-14
+  const int Q = 2 * N3 + 36;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int p0 = 8 * (ty + m * TY);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int p = p0 + r;
+      if (p0 < L3 && p < L3) {
+        if (per) {
+          for (int q = p + 14 - 2 * N3 * ((p + 14) / (2 * N3)); q < Q; q += 2 * N3)
+            lds[q * LDW + tx] = g[m][r];
+        } else {
+          lds[p * LDW + tx] = g[m][r];
+        }
+      }
+    }
+  }
+  if (!per)
+    for (int q = L3 + ty; q < Q; q += TY) lds[q * LDW + tx] = 0.0;
+  __syncthreads();
+  for (int i0 = 4 * ty; i0 < N3; i0 += 4 * TY) {
+    double oc[4], od[4];
+    shrink4(col + 2 * i0 * LDW, oc, od);
+    if (tx < nl) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (i0 + r < N3) {
+          line[tx + pstride * (i0 + r)] = oc[r];
+          line[tx + pstride * (N3 + i0 + r)] = od[r];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y pass, backward: T1 = G_y^T T2
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_ybwd(Batch B, TileTable tt) {
+  extern __shared__ double lds[];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int L1 = bd.L[0], N2 = bd.N[1], L2 = bd.L[1], per = bd.per;
+  const int W2 = 2 * N2;
+  const int nxt = (L1 + TW - 1) / TW;
+  const int j3 = lt / nxt;
+  const int j1 = (lt - j3 * nxt) * TW;
+  const int nl = min(TW, L1 - j1);
+  const double* src = B.T2 + bd.t2_off + j1 + (long long)L1 * L2 * j3;
+  fill_real<false>(lds, src, 1, L1, nl, N2, L2, per);
+  __syncthreads();
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const double* col = lds + tx;
+  double* dst = B.T1 + bd.t1_off + j1 + tx + (long long)L1 * W2 * j3;
+  for (int i0 = 4 * ty; i0 < N2; i0 += 4 * TY) {
+    double oc[4], od[4];
+    shrink4(col + 2 * i0 * LDW, oc, od);
+    if (tx < nl) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (i0 + r < N2) {
+          dst[(long long)L1 * (i0 + r)] = oc[r];
+          dst[(long long)L1 * (N2 + i0 + r)] = od[r];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass, backward: W = G_x^T T1 + K
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
+  extern __shared__ double lds[];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int N1 = bd.N[0], L1 = bd.L[0], per = bd.per;
+  const int W1 = 2 * N1;
+  const int nlines = 4 * bd.N[1] * bd.N[2];
+  const int l0 = lt * TW;
+  const int nl = min(TW, nlines - l0);
+  const double* src = B.T1 + bd.t1_off + (long long)l0 * L1;
+  fill_real<true>(lds, src, L1, 1, nl, N1, L1, per);
+  __syncthreads();
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const double* col = lds + tx;
+  double oc[2][4], od[2][4];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int i0 = 4 * (ty + m * TY);
+    if (i0 < N1) shrink4(col + 2 * i0 * LDW, oc[m], od[m]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int i0 = 4 * (ty + m * TY);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N1) {
+        lds[(i0 + r) * LDW + tx] = oc[m][r];
+        lds[(N1 + i0 + r) * LDW + tx] = od[m][r];
+      }
+  }
+  __syncthreads();
+  const long long o = bd.w_off + (long long)l0 * W1;
+  for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
+    const int l = idx / W1, p = idx - l * W1;
+    B.W[o + idx] = lds[p * LDW + l] + B.K[o + idx];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// energies: fixed-order sums of the per-tile partials of each box
+// ---------------------------------------------------------------------------
+struct EnergyTables {
+  TileTable t[4];       // xfwd, kin_y, kin_z (ekin) ; zpot (epot)
+  const double* p[4];   // partial arrays
+};
+
+__global__ void k_energy(EnergyTables et, int nbox, double* ekin, double* epot) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbox) return;
+  double e[4];
+  for (int s = 0; s < 4; ++s) {
+    const TileTable& tt = et.t[s];
+    const int t0 = box_first_tile(tt, b);
+    const int t1 = tt.uniform > 0 ? t0 + tt.uniform : tt.start[b + 1];
+    double acc = 0.0;
+    for (int t = t0; t < t1; ++t) acc += et.p[s][t];
+    e[s] = acc;
+  }
+  ekin[b] = e[0] + e[1] + e[2];
+  epot[b] = e[3];
+}
+
+// ---------------------------------------------------------------------------
+// host-side launcher of the whole FP64 sequence
+// ---------------------------------------------------------------------------
+size_t lds_bytes_fp64(int maxN) {
+  return (size_t)2 * (maxN + HL + HR) * LDW * sizeof(double);
+}
+
+hipError_t launch_fp64(const Batch& B, const PassTables& T, int maxN,
+                       long long wtotal, bool need_zero, hipStream_t s,
+                       hipEvent_t* ev) {
+  const size_t lds = lds_bytes_fp64(maxN);
+  auto mark = [&](int stage, int end) {
+    if (ev) (void)hipEventRecord(ev[2 * stage + end], s);
+  };
+  mark(S_ZERO, 0);
+  if (need_zero) {
+    const long long nb = (wtotal + NT - 1) / NT;
+    k_zero<<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>(B.W, wtotal);
+  }
+  mark(S_ZERO, 1);
+#define BDFT_STAGE(S, P, SH, KERNEL, ...)                    \
+  mark(S, 0);                                                \
+  if (T.ntiles[P] > 0) KERNEL<<<T.ntiles[P], NT, SH, s>>>(__VA_ARGS__); \
+  mark(S, 1);
+  BDFT_STAGE(S_UNC, P_UNC, 0, k_uncompress<false>, B, T.t[P_UNC])
+  BDFT_STAGE(S_XF, P_XF, lds, k_xfwd, B, T.t[P_XF], T.part[0])
+  BDFT_STAGE(S_YF, P_YF, lds, k_yfwd, B, T.t[P_YF])
+  BDFT_STAGE(S_KY, P_KY, lds, k_kin<1>, B, T.t[P_KY], T.part[1])
+  BDFT_STAGE(S_KZ, P_KZ, lds, k_kin<2>, B, T.t[P_KZ], T.part[2])
+  BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot, B, T.t[P_ZP], T.part[3])
+  BDFT_STAGE(S_YB, P_YB, lds, k_ybwd, B, T.t[P_YB])
+  BDFT_STAGE(S_XB, P_XB, lds, k_xbwd, B, T.t[P_XB])
+  BDFT_STAGE(S_COMP, P_UNC, 0, k_uncompress<true>, B, T.t[P_UNC])
+#undef BDFT_STAGE
+  EnergyTables et;
+  et.t[0] = T.t[P_XF];
+  et.t[1] = T.t[P_KY];
+  et.t[2] = T.t[P_KZ];
+  et.t[3] = T.t[P_ZP];
+  for (int i = 0; i < 4; ++i) et.p[i] = T.part[i];
+  mark(S_ENERGY, 0);
+  k_energy<<<(B.nbox + 127) / 128, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);
+  mark(S_ENERGY, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
+                           long long wtotal, bool compress, hipStream_t s) {
+  if (!compress) {
+    const long long nb = (wtotal + NT - 1) / NT;
+    k_zero<<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>(B.W, wtotal);
+    k_uncompress<false><<<ntiles, NT, 0, s>>>(B, t);
+  } else {
+    k_uncompress<true><<<ntiles, NT, 0, s>>>(B, t);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace bdft

@@ -1,0 +1,203 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+//
+// Data model (per orbital "box", i.e. one localisation region instance):
+//   N[d]   coarse points along axis d (reference: n_d + 1)
+//   L[d]   real-space (ISF) points along d: 2N (periodic) or 2N+29 (free),
+//          the reference's n1i rule (bigdft/src/locregs/locregs.f90:871-884)
+//   wavelet volume  "psig" layout of the reference (0:n1,2,0:n2,2,0:n3,2),
+//          i.e. per axis [c(0..N-1) ; d(0..N-1)], x fastest
+//          (bigdft/src/wavelib/un-compress.f90:877-948)
+//   real volume     (L1, L2, L3), x fastest; free-BC index p <-> fine -14+p
+//                   (bigdft/src/orbitals/growshrink.f90:12-37)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bdft {
+
+// Line-bundle tiling shared by every pass kernel.
+constexpr int TW = 32;          // lines per tile
+constexpr int TY = 8;           // chunk-threads per line
+constexpr int NT = TW * TY;     // threads per block
+constexpr int LDW = TW + 1;     // padded LDS row (doubles)
+constexpr int HL = 15;          // halo rows left of a coarse section
+constexpr int HR = 18;          // halo rows right of a coarse section
+constexpr int MAXN = 64;        // largest coarse extent handled per axis
+
+// Per-box descriptor (device resident).
+struct BoxDesc {
+  int N[3];
+  int L[3];
+  int per;              // 1 periodic, 0 free
+  int nvctr_c, nvctr_f;
+  int potdim[3];        // potential box extents (n1ip, n2ip, n3ip)
+  int ishift[3];        // potential box offset in wavefunction box coordinates
+  int pad_;
+  double kscale[3];     // -1/2/h_d^2, coarse grid spacing
+  long long w_off;      // offset of the wavelet volume (8 N1 N2 N3)
+  long long t1_off;     // offset of (L1, 2N2, 2N3)
+  long long t2_off;     // offset of (L1, L2, 2N3)
+  long long pot_off;    // offset into the potential array
+  long long psi_off;    // offset of this orbital's compressed coefficients
+  const int* cmap;      // coarse points: dense linear index i1+N1*(i2+N2*i3)
+  const int* fmap;      // fine points: same encoding
+};
+
+// Tile bookkeeping for one pass: tiles of box b are [start[b], start[b+1]).
+struct TileTable {
+  const int* start;     // nbox + 1 cumulative tile counts (device)
+  int nbox;
+  int uniform;          // >0: every box has exactly this many tiles
+};
+
+__device__ __forceinline__ int find_box(const TileTable& tt, int tile) {
+  if (tt.uniform > 0) return tile / tt.uniform;
+  int lo = 0, hi = tt.nbox;  // invariant start[lo] <= tile < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tt.start[mid] <= tile) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int box_first_tile(const TileTable& tt, int b) {
+  return tt.uniform > 0 ? b * tt.uniform : tt.start[b];
+}
+
+// ---------------------------------------------------------------------------
+// Filter tables (values of the reference; host copies live in bdft_filters.cpp)
+// fil2(j,s), j=-14..16: bigdft/src/orbitals/v.inc:13-76
+// kinetic a,c,e:        bigdft/src/orbitals/convolut_ib_optim.f90:962-1038
+// ---------------------------------------------------------------------------
+struct Fil {
+  static constexpr double F2S[31] = {
+      -9.2250149921474082015e-9, 1.739330656197903949e-7,
+      -1.5578172714074496288e-6, 9.7488426741565769756e-6,
+      -0.000028480442793096733611, -0.000034162902320983755896,
+      0.00054731040109973893777, -0.0034543155764822098308,
+      -0.0043299995331208240697, 0.031835414345970668137,
+      0.023424614735397134395, -0.13852477567405100085,
+      -0.11023008167833446826, 0.41606693676621257536,
+      0.78046048025074402849, 0.4259879826969471933,
+      -0.014743648317973476936, -0.031312897214485659659,
+      0.0437933264161550095, 0.0080208847559691655459,
+      -0.01336723942112679464, -0.0017135145673762282241,
+      0.0015547744749237938993, 0.00023987828928177507517,
+      0.000025057266882581997398, -0.00001414752521104005874,
+      2.0755675626515876328e-6, -4.0033729064957944938e-7,
+      1.5691554225771085691e-7, -2.4646355857433108359e-8,
+      1.5938753881171638248e-9};
+  static constexpr double F2W[31] = {
+      -5.1545464563622292493e-9, 9.7186407588266778366e-8,
+      -8.7796306212502899553e-7, 5.5890429568834202313e-6,
+      -0.000017169835895815067013, -0.000011402461428743860613,
+      0.00028479503466830096257, -0.0019699454319373247538,
+      -0.0019602139595384833547, 0.015281781287179308495,
+      0.0072888369266039604623, -0.039920047232423474172,
+      -0.052935814515665507352, 0.049247868599583189247,
+      0.40346722112554718347, -0.78110468414482695849,
+      0.43810509543421050422, 0.077858113707232306275,
+      -0.11950673101135450945, -0.022857677317437822508,
+      0.028206423027424364667, 0.0037436631993987390831,
+      -0.0027213277354099953613, -0.0004645045737819340081,
+      -0.000040668222535397420851, 0.000024486891695262527278,
+      -3.3186107540444071037e-6, 6.5211830898659553296e-7,
+      -2.7666729880873517609e-7, 4.4109215856612868616e-8,
+      -2.8525350339693075057e-9};
+  static constexpr double KA[15] = {
+      -3.5536922899131901941296809374, 2.2191465938911163898794546405,
+      -0.6156141465570069496314853949, 0.2371780582153805636239247476,
+      -0.0822663999742123340987663521, 0.02207029188482255523789911295638968409,
+      -0.409765689342633823899327051188315485e-2,
+      0.45167920287502235349480037639758496e-3,
+      -0.2398228524507599670405555359023135e-4,
+      2.0904234952920365957922889447361e-6,
+      -3.7230763047369275848791496973044e-7,
+      -1.05857055496741470373494132287e-8,
+      -5.813879830282540547959250667e-11,
+      2.70800493626319438269856689037647576e-13,
+      -6.924474940639200152025730585882e-18};
+  static constexpr double KC[29] = {
+      -3.869102413147656535541850057188e-18,
+      1.5130616560866154733900029272077362e-13,
+      -3.2264702314010525539061647271983988409e-11,
+      -5.96264938781402337319841002642e-9,
+      -2.1656830629214041470164889350342e-7,
+      8.7969704055286288323596890609625e-7,
+      -0.00001133456724516819987751818232711775,
+      0.00021710795484646138591610188464622454,
+      -0.0021356291838797986414312219042358542,
+      0.00713761218453631422925717625758502986,
+      -0.0284696165863973422636410524436931061,
+      0.14327329352510759457155821037742893841,
+      -0.42498050943780130143385739554118569733,
+      0.65703074007121357894896358254040272157,
+      -0.42081655293724308770919536332797729898,
+      -0.21716117505137104371463587747283267899,
+      0.63457035267892488185929915286969303251,
+      -0.53298223962800395684936080758073568406,
+      0.23370490631751294307619384973520033236,
+      -0.05657736973328755112051544344507997075,
+      0.0080872029411844780634067667008050127,
+      -0.00093423623304808664741804536808932984,
+      0.00005075807947289728306309081261461095,
+      -4.62561497463184262755416490048242e-6,
+      6.3919128513793415587294752371778e-7,
+      1.87909235155149902916133888931e-8,
+      1.04757345962781829480207861447155543883e-10,
+      -4.84665690596158959648731537084025836e-13,
+      1.2392629629188986192855777620877e-17};
+  static constexpr double KE[15] = {
+      -24.875846029392331358907766562, -7.1440597663471719869313377994,
+      -0.04251705323669172315864542163525830944,
+      -0.26995931336279126953587091167128839196,
+      0.08207454169225172612513390763444496516,
+      -0.02207327034586634477996701627614752761,
+      0.00409765642831595181639002667514310145,
+      -0.00045167920287507774929432548999880117,
+      0.00002398228524507599670405555359023135,
+      -2.0904234952920365957922889447361e-6,
+      3.7230763047369275848791496973044e-7,
+      1.05857055496741470373494132287e-8,
+      5.8138798302825405479592506674648873655e-11,
+      -2.70800493626319438269856689037647576e-13,
+      6.924474940639200152025730585882e-18};
+};
+
+// Launch arguments of one batched local-H application.
+struct Batch {
+  const BoxDesc* box;   // device, nbox entries
+  int nbox;
+  const double* psi;    // compressed coefficients (all orbitals)
+  double* hpsi;         // accumulated output, same layout as psi
+  const double* pot;    // potential(s)
+  double* W;            // wavelet volumes (uncompressed psi, later output)
+  double* K;            // kinetic accumulator volumes (wavelet layout)
+  double* T1;           // (L1,2N2,2N3) volumes
+  double* T2;           // (L1,L2,2N3) volumes
+  double* part;         // energy partial sums, one slot per (pass, tile)
+  double* ekin;         // per-box kinetic energy (device)
+  double* epot;         // per-box potential energy (device)
+};
+
+// Passes of one application; each has its own tile table.
+enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_COUNT };
+
+struct PassTables {
+  TileTable t[P_COUNT];
+  int ntiles[P_COUNT];   // total tiles (grid size) of each pass
+  double* part[4];       // energy partials: xfwd, kin_y, kin_z, zpot
+};
+
+// Launch stages in order; profiling records one event pair per stage.
+enum Stage { S_ZERO = 0, S_UNC, S_XF, S_YF, S_KY, S_KZ, S_ZP, S_YB, S_XB,
+             S_COMP, S_ENERGY, S_COUNT };
+
+size_t lds_bytes_fp64(int maxN);
+hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
+                           long long wtotal, bool compress, hipStream_t s);
+// ev: null, or 2*S_COUNT events recorded before/after each stage
+hipError_t launch_fp64(const Batch& B, const PassTables& T, int maxN,
+                       long long wtotal, bool need_zero, hipStream_t s,
+                       hipEvent_t* ev = nullptr);
+
+}  // namespace bdft

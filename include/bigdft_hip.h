@@ -1,0 +1,161 @@
+/*
+ * bigdft_hip.h -- C ABI of the MI355X local-Hamiltonian library.
+ *
+ * Drop-in replacement for the accelerator layer of BigDFT's local
+ * Hamiltonian application (the GPU%OCLconv branch of
+ * LocalHamiltonianApplication, bigdft/src/hpsiortho.f90:892-903, completed in
+ * SynchronizeHamiltonianApplication, hpsiortho.f90:1313-1323).
+ *
+ * Conventions (kept from the reference):
+ *   - keys are int32, 1-based: keyg(2,nseg) = first/last linear coarse index
+ *     i3*(n1+1)*(n2+1) + i2*(n1+1) + i1 + 1, keyv(nseg) = 1-based offset into
+ *     the coefficient vector (bigdft/src/locregs/initlocregs.f90:1633-1686);
+ *     coarse and fine descriptors are passed separately, as allocate_data_OCL
+ *     uploads them (bigdft/src/interface_ocl.f90:129-148);
+ *   - one orbital = [psi_c(nvctr_c) ; psi_f(7, nvctr_f)] doubles
+ *     (bigdft/src/wavelib/un-compress.f90:2187-2189);
+ *   - potential: n1i*n2i*n3i doubles, x fastest, n1i = 2n+2 (periodic) or
+ *     2n+31 (free) (bigdft/src/locregs/locregs.f90:871-884);
+ *   - hpsi is ACCUMULATED (hpsi += H_loc psi), as compress_and_accumulate_*
+ *     does on the CPU (un-compress.f90:2105-2238) and as the OpenCL path's
+ *     result is added after finish (hpsiortho.f90:1316-1321);
+ *   - ekin, epot are per orbital and unweighted: ekin = <psi|T|psi>,
+ *     epot = sum V psir^2; the caller applies kwgt*occup as
+ *     finish_hamiltonian_OCL does (interface_ocl.f90:589-607).
+ * Every entry point returns 0 on success or a BDFT_ERR_* code; the message
+ * of the last failure of a context is available from bdft_last_error.
+ * Calls on one context are asynchronous with respect to the host until
+ * bdft_sync (like the OpenCL queue, interface_ocl.f90:37).
+ */
+#ifndef BIGDFT_HIP_H
+#define BIGDFT_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bdft_ctx bdft_ctx;
+typedef struct bdft_lr bdft_lr;
+typedef struct bdft_plan bdft_plan;
+
+enum {
+  BDFT_OK = 0,
+  BDFT_ERR_ARG = 1,         /* null handle, bad size or geocode */
+  BDFT_ERR_HIP = 2,         /* HIP runtime failure */
+  BDFT_ERR_DESC = 3,        /* inconsistent keyg/keyv descriptors */
+  BDFT_ERR_NOMEM = 4,       /* device allocation failed */
+  BDFT_ERR_UNSUPPORTED = 5  /* box or mode outside the supported range */
+};
+
+/* arithmetic of the convolution passes */
+enum {
+  BDFT_PREC_FP64 = 0,    /* plain FP64 everywhere */
+  BDFT_PREC_FP32 = 1,    /* FP32 MFMA passes, FP64 energies and accumulate */
+  BDFT_PREC_BF16X3 = 2   /* split-BF16 (3 products) MFMA, FP32 accumulation */
+};
+
+/* Context on one device.  Replaces init_acceleration_OCL
+ * (bigdft/src/interface_ocl.f90:20-41: ocl_create_context,
+ * ocl_build_programs, ocl_create_command_queue_id). */
+int bdft_ctx_create(int device, bdft_ctx **ctx);
+/* Replaces free_gpu_OCL (interface_ocl.f90:196) for the context part. */
+int bdft_ctx_destroy(bdft_ctx *ctx);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int bdft_ctx_set_stream(bdft_ctx *ctx, void *hip_stream);
+/* Message of the last failure on this context ("" if none).  Replaces the
+ * print-and-exit of oclErrorCheck (bigdft/src/OpenCL/OpenCL_wrappers.h:89). */
+const char *bdft_last_error(const bdft_ctx *ctx);
+
+/* Localisation region: box, grid spacing and compression keys; keys are
+ * validated and uploaded once.  Replaces allocate_data_OCL
+ * (interface_ocl.f90:43-193).  geocode 'P' (periodic) or 'F' (free); n[d] is
+ * the reference's n_d (coarse points n_d+1; the Hamiltonian
+ * entry points take at most 64 per axis, the transfers at most 4096); nfl/nfu
+ * bound the fine region (ignored for 'P'); hgrid is the coarse spacing. */
+int bdft_lr_create(bdft_ctx *ctx, char geocode, const int n[3],
+                   const int nfl[3], const int nfu[3], const double hgrid[3],
+                   int nseg_c, int nvctr_c, const int32_t *keyg_c,
+                   const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                   const int32_t *keyg_f, const int32_t *keyv_f,
+                   bdft_lr **lr);
+/* Replaces free_gpu_OCL (interface_ocl.f90:196-269) for the descriptor part */
+int bdft_lr_destroy(bdft_lr *lr);
+/* n1i,n2i,n3i and nvctr_c + 7*nvctr_f of a region */
+int bdft_lr_info(const bdft_lr *lr, int nfine[3], int64_t *ncoef);
+
+/* Local Hamiltonian of norb orbitals sharing one region, device pointers:
+ * psi, hpsi: norb consecutive orbitals; pot: n1i*n2i*n3i shared;
+ * ekin, epot: norb doubles each (device).  Replaces ocl_fulllocham_generic_k
+ * (bigdft/src/OpenCL/Hamiltonian.c:105-172) called in the orbital loop of
+ * local_hamiltonian_OCL (interface_ocl.f90:459-535), all orbitals in one
+ * batch. */
+int bdft_locham_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
+                  const double *pot, const double *psi, double *hpsi,
+                  double *ekin, double *epot, int precision);
+
+/* Batched application over orbitals living in different regions (the ket
+ * loop over localisation regions, hpsiortho.f90:934-958 with
+ * modules/orbital_basis.f90:760-858).  Orbital i uses region lrs[i], its
+ * coefficients at psi + psi_off[i] (hpsi likewise), and the potential box
+ * pot + pot_off[i] of extents potdim[3i..3i+2] placed at ishift[3i..3i+2] in
+ * the region's fine grid (apply_potential_lr_nobounds,
+ * bigdft/src/orbitals/potential.f90:850-879).  Device pointers. */
+int bdft_locham_batch_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
+                        const double *pot, const int64_t *pot_off,
+                        const int32_t *potdim, const int32_t *ishift,
+                        const double *psi, const int64_t *psi_off,
+                        double *hpsi, double *ekin, double *epot,
+                        int precision);
+
+/* Plans: the same batch with fixed pointers, captured once into a hipGraph
+ * and replayed per SCF iteration.  No counterpart in the reference (the
+ * OpenCL path re-enqueues every kernel per orbital). */
+int bdft_plan_create(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
+                     const double *pot, const int64_t *pot_off,
+                     const int32_t *potdim, const int32_t *ishift,
+                     const double *psi, const int64_t *psi_off, double *hpsi,
+                     double *ekin, double *epot, int precision,
+                     bdft_plan **plan);
+int bdft_plan_execute(bdft_plan *plan);
+int bdft_plan_destroy(bdft_plan *plan);
+
+/* Host-array convenience with the argument meaning of local_hamiltonian_OCL
+ * (interface_ocl.f90:391-407): copies pot and psi in, hpsi (accumulated) and
+ * the per-orbital energies out, then synchronises. */
+int bdft_local_hamiltonian(bdft_ctx *ctx, const bdft_lr *lr, int norb,
+                           const double *pot, const double *psi, double *hpsi,
+                           double *ekin, double *epot, int precision);
+
+/* Compressed <-> mixed-layout transfers of norb orbitals of one region,
+ * device pointers; psig holds norb consecutive (0:n1,2,0:n2,2,0:n3,2)
+ * volumes.  bdft_uncompress_d zeroes psig and scatters psi (uncompress_per
+ * without the synthesis, un-compress.f90:877-936; OpenCL uncompress_d,
+ * bigdft/src/OpenCL/Uncompress.c:23-86); bdft_compress_acc_d does
+ * hpsi += gather(psig) (compress_and_accumulate_mixed,
+ * un-compress.f90:2174-2238).  Both are pure index work and bit-exact. */
+int bdft_uncompress_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
+                      const double *psi, double *psig);
+int bdft_compress_acc_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
+                        const double *psig, double *hpsi);
+
+/* Stage timing with HIP events around every kernel of non-graph launches
+ * (replaces the OpenCL event list, init_event_list/print_event_list,
+ * bigdft/src/OpenCL/OpenCL_wrappers.h:309-311).  bdft_profile_read
+ * synchronises, then returns the accumulated milliseconds of the first
+ * nstage stages (names from bdft_stage_name, launch order) and the number of
+ * profiled launches; reset != 0 clears the counters. */
+int bdft_profile_enable(bdft_ctx *ctx, int on);
+int bdft_profile_read(bdft_ctx *ctx, int nstage, double *ms, int64_t *nlaunch,
+                      int reset);
+const char *bdft_stage_name(int stage);
+
+/* Wait for all work of the context.  Replaces ocl_finish in
+ * finish_hamiltonian_OCL (interface_ocl.f90:599). */
+int bdft_sync(bdft_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+/*
+ * CPU oracle for BigDFT's local-Hamiltonian path on Daubechies-16 wavelets.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain-C restatement of the
+ * reference's FP64 Fortran routines.  Only the test-suite, the smoke entry
+ * and the CPU-baseline leg of the benchmark may load it, as the checker.
+ * The product (the HIP library) never calls it.
+ *
+ * Array conventions follow the reference: Fortran column-major storage,
+ * 1-based keys (keyg, keyv) exactly as produced by segkeys
+ * (bigdft/src/locregs/initlocregs.f90:1633-1686).
+ */
+#ifndef BDFT_ORACLE_H
+#define BDFT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- filter tables (copied values; see bdft_oracle.c for the citation) ---- */
+void oref_filters(double ch[16], double cg[16], double magic[16],
+                  double kin_a[15], double kin_c[29], double kin_e[15],
+                  double fil2[62]);
+
+/* ---- periodic 1D rotating kernels (convolut_optim_per.f90) ---- */
+void oref_ana_rot_per(int n, int ndat, const double *x, double *y);
+void oref_syn_rot_per(int n, int ndat, const double *x, double *y);
+void oref_convrot_n_per(int n1, int ndat, const double *x, double *y);
+void oref_convrot_t_per(int n1, int ndat, const double *x, double *y);
+void oref_convolut_kinetic_per_T(int n1, int n2, int n3, const double hgrid[3],
+                                 const double *x, double *y, double kstrten[3]);
+
+/* ---- periodic 3D drivers (convolut_common_per.f90) ---- */
+void oref_synthese_per_self(int n1, int n2, int n3, double *x, double *y);
+void oref_analyse_per_self(int n1, int n2, int n3, double *y, double *x);
+void oref_convolut_magic_n_per(int n1, int n2, int n3, const double *x,
+                               double *y, double *ww);
+void oref_convolut_magic_t_per_self(int n1, int n2, int n3, double *x,
+                                    double *y);
+
+/* ---- compression (wavelib/un-compress.f90) ---- */
+void oref_uncompress_per(int n1, int n2, int n3, int nseg_c, int nvctr_c,
+                         const int32_t *keyg_c, const int32_t *keyv_c,
+                         int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                         const int32_t *keyv_f, const double *psi_c,
+                         const double *psi_f, double *psifscf, double *psig);
+void oref_uncompress_psig(int n1, int n2, int n3, int nseg_c,
+                          const int32_t *keyg_c, const int32_t *keyv_c,
+                          int nseg_f, const int32_t *keyg_f,
+                          const int32_t *keyv_f, const double *psi_c,
+                          const double *psi_f, double *psig);
+void oref_compress_and_accumulate_mixed(int n1, int n2, int n3, int nseg_c,
+                                        const int32_t *keyg_c,
+                                        const int32_t *keyv_c, int nseg_f,
+                                        const int32_t *keyg_f,
+                                        const int32_t *keyv_f,
+                                        const double *psig, double *psi_c,
+                                        double *psi_f);
+void oref_uncompress_forstandard(int n1, int n2, int n3, const int nfl[3],
+                                 const int nfu[3], int nseg_c,
+                                 const int32_t *keyg_c, const int32_t *keyv_c,
+                                 int nseg_f, const int32_t *keyg_f,
+                                 const int32_t *keyv_f, const double *psi_c,
+                                 const double *psi_f, double *x_c,
+                                 double *x_f);
+void oref_compress_and_accumulate_standard(
+    int n1, int n2, int n3, const int nfl[3], const int nfu[3], int nseg_c,
+    const int32_t *keyg_c, const int32_t *keyv_c, int nseg_f,
+    const int32_t *keyg_f, const int32_t *keyv_f, const double *y_c,
+    const double *y_f, double *psi_c, double *psi_f);
+
+/* ---- potential (orbitals/potential.f90) ---- */
+void oref_apply_potential_lr(int n1i, int n2i, int n3i, int n1ip, int n2ip,
+                             int n3ip, const int ishift[3], double *psir,
+                             const double *pot, double *epot);
+
+/* ---- free boundary conditions (growshrink.f90, convolut_ib_optim.f90) ---- */
+void oref_comb_grow_all(int n1, int n2, int n3, const int nfl[3],
+                        const int nfu[3], const double *x_c, const double *x_f,
+                        double *psir);
+void oref_comb_shrink(int n1, int n2, int n3, const int nfl[3],
+                      const int nfu[3], const double *psir, double *y_c,
+                      double *y_f);
+void oref_convolkinetic_T(int n1, int n2, int n3, const int nfl[3],
+                          const int nfu[3], const double hgrid[3],
+                          const double *x_c, const double *x_f, double *y_c,
+                          double *y_f, double *ekin);
+
+/* ---- per-orbital drivers (modules/orbital_basis.f90:760-858) ---- */
+/* periodic: psi/hpsi compressed; pot has n1i*n2i*n3i = (2n+2)^3 entries;
+ * hpsi is accumulated (+=) */
+void oref_locham_per(const int n[3], const double hgrid[3], int nseg_c,
+                     int nvctr_c, const int32_t *keyg_c, const int32_t *keyv_c,
+                     int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                     const int32_t *keyv_f, const double *pot,
+                     const double *psi, double *hpsi, double *ekin,
+                     double *epot);
+/* free: pot has (2n+31)^3 entries laid out like psir */
+void oref_locham_free(const int n[3], const int nfl[3], const int nfu[3],
+                      const double hgrid[3], int nseg_c, int nvctr_c,
+                      const int32_t *keyg_c, const int32_t *keyv_c,
+                      int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                      const int32_t *keyv_f, const double *pot,
+                      const double *psi, double *hpsi, double *ekin,
+                      double *epot);
+/* many orbitals sharing one locreg (the ket loop, hpsiortho.f90:945-958);
+ * orbital loop serial, OpenMP inside the kernels */
+void oref_locham_many(char geocode, const int n[3], const int nfl[3],
+                      const int nfu[3], const double hgrid[3], int nseg_c,
+                      int nvctr_c, const int32_t *keyg_c,
+                      const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                      const int32_t *keyg_f, const int32_t *keyv_f,
+                      const double *pot, int norb, const double *psi,
+                      double *hpsi, double *ekin, double *epot);
+
+/* ---- descriptors (locregs/initlocregs.f90:1573-1686) ---- */
+int oref_num_segkeys(int n1, int n2, int n3, int nl1, int nu1, int nl2,
+                     int nu2, int nl3, int nu3, const uint8_t *logrid,
+                     int *mseg, int *mvctr);
+int oref_segkeys(int n1, int n2, int n3, int nl1, int nu1, int nl2, int nu2,
+                 int nl3, int nu3, const uint8_t *logrid, int mseg,
+                 int32_t *keyg, int32_t *keyv);
+
+int oref_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,262 @@
+"""HIP path vs the oracle, through the C ABI, on seeded inputs.
+
+Bars: compression transfers bit-exact; FP64 local Hamiltonian within 1e-12
+per element (relative error guarded near zeros: |a-b| / max(|b|,
+1e-3 max|b|)) and 1e-12 relative in ekin and epot.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (dense_region, mixed_sphere_region, okeys, random_pot,
+                     random_psi, rel_err, region_from_grids, sphere_region)
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import bigdft_hip
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    c = bigdft_hip.Context(0)
+    yield c
+    c.close()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def run_gpu(ctx, region, pot, psi, hpsi0=None, precision=0):
+    lr = ctx.region(region)
+    norb = psi.shape[0]
+    d_psi = dev(psi)
+    d_h = dev(np.zeros_like(psi) if hpsi0 is None else hpsi0)
+    d_pot = dev(pot)
+    ek = torch.zeros(norb, dtype=torch.float64, device="cuda")
+    ep = torch.zeros(norb, dtype=torch.float64, device="cuda")
+    ctx.locham(lr, d_pot, d_psi, d_h, ek, ep, precision)
+    ctx.sync()
+    out = d_h.cpu().numpy(), ek.cpu().numpy(), ep.cpu().numpy()
+    lr.close()
+    return out
+
+
+def check(region, norb, seed, ctx):
+    print(f"case {region.geocode} n={region.n} nvctr=({region.nvctr_c},"
+          f"{region.nvctr_f}) norb={norb} seed={seed}", flush=True)
+    psi = random_psi(region, norb, seed)
+    pot = random_pot(region, seed + 1)
+    h0 = random_psi(region, norb, seed + 2)
+    hg, eg, pg = run_gpu(ctx, region, pot, psi, h0)
+    ho, eo, po = oracle.locham(region.geocode, region.n, region.nfl,
+                               region.nfu, region.hgrid, okeys(region), pot,
+                               psi, h0)
+    assert rel_err(hg - h0, ho - h0) < TOL
+    assert rel_err(eg, eo) < TOL
+    assert rel_err(pg, po) < TOL
+
+
+@pytest.mark.parametrize("n", [(0, 0, 0), (3, 4, 2), (7, 6, 8), (9, 13, 5)])
+def test_periodic_small(ctx, n):
+    check(dense_region("P", n, (0.45, 0.4, 0.5)), 2, 10 + sum(n), ctx)
+
+
+def test_periodic_config1(ctx):
+    """24^3 coarse, hgrid 0.45, V ~ U[-2,0] (the single-orbital case)"""
+    check(dense_region("P", (23, 23, 23), (0.45,) * 3), 1, 1, ctx)
+
+
+@pytest.mark.parametrize("n", [(4, 5, 6), (11, 9, 10), (15, 15, 15)])
+def test_free_dense(ctx, n):
+    check(dense_region("F", n, (0.4, 0.45, 0.5)), 2, 20 + sum(n), ctx)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_free_sphere(ctx, seed):
+    check(mixed_sphere_region((14, 12, 16), 0.4, seed), 2, 30 + seed, ctx)
+
+
+def test_free_no_fine_points(ctx):
+    lg = np.ones((9, 8, 7), dtype=bool)
+    reg = region_from_grids("F", (8, 7, 6), (0.4,) * 3, lg, np.zeros_like(lg))
+    check(reg, 1, 40, ctx)
+
+
+@pytest.mark.parametrize("geo", ["P", "F"])
+def test_uncompress_compress_bit_exact(ctx, geo):
+    if geo == "P":
+        reg = dense_region("P", (9, 7, 8), (0.4,) * 3)
+        lg = np.zeros((10, 8, 9), dtype=bool)
+        lg[2:7] = True   # partial rows
+        reg = region_from_grids("P", (9, 7, 8), (0.4,) * 3, lg, lg)
+    else:
+        reg = mixed_sphere_region((14, 12, 16), 0.4, 4)
+    norb = 3
+    psi = random_psi(reg, norb, 50)
+    lr = ctx.region(reg)
+    vol = 8 * np.prod([x + 1 for x in reg.n])
+    psig = torch.full((norb * vol,), 7.0, dtype=torch.float64, device="cuda")
+    ctx.uncompress(lr, norb, dev(psi), psig)
+    h0 = random_psi(reg, norb, 51)
+    d_h = dev(h0)
+    ctx.compress_acc(lr, norb, psig, d_h)
+    ctx.sync()
+    g = psig.cpu().numpy().reshape(norb, vol)
+    for i in range(norb):
+        o = oracle.uncompress_psig(reg.n, okeys(reg), psi[i])
+        assert np.array_equal(g[i], o)
+        oc = oracle.compress_acc_mixed(reg.n, okeys(reg), o, h0[i])
+        assert np.array_equal(d_h.cpu().numpy()[i], oc)
+    # compress(uncompress(psi)) onto zeros gives psi back exactly
+    z = torch.zeros((norb, reg.ncoef), dtype=torch.float64, device="cuda")
+    ctx.compress_acc(lr, norb, psig, z)
+    ctx.sync()
+    assert np.array_equal(z.cpu().numpy(), psi)
+    lr.close()
+
+
+def test_conv_check_pattern_bit_exact(ctx):
+    """keys and values of bigdft/src/OpenCL/conv_check.f90:1622-1660"""
+    n1 = 76
+    lg = np.zeros((n1 + 1,) * 3, dtype=bool)
+    lg[5:n1 - 4] = True
+    reg = region_from_grids("P", (n1,) * 3, (0.4,) * 3, lg, lg)
+    nv = reg.nvctr_c
+    psi = np.concatenate([np.arange(1, nv + 1, dtype=np.float64),
+                          (np.arange(1, nv + 1)[:, None] +
+                           0.1 * np.arange(1, 8)[None, :]).ravel()])[None, :]
+    lr = ctx.region(reg)
+    psig = torch.empty(8 * (n1 + 1) ** 3, dtype=torch.float64, device="cuda")
+    ctx.uncompress(lr, 1, dev(psi), psig)
+    z = torch.zeros_like(dev(psi))
+    ctx.compress_acc(lr, 1, psig, z)
+    ctx.sync()
+    assert np.array_equal(z.cpu().numpy(), psi)
+    assert np.array_equal(psig.cpu().numpy(),
+                          oracle.uncompress_psig(reg.n, okeys(reg), psi[0]))
+    lr.close()
+
+
+def test_uneven_batch_with_potential_offsets(ctx):
+    """orbitals in different free-BC regions, each with its own potential box
+    inside one concatenated array (linear-scaling layout)"""
+    import bigdft_hip
+    rng = np.random.default_rng(60)
+    regs = [mixed_sphere_region(tuple(int(x) for x in rng.integers(6, 15, 3)),
+                                0.4, 61 + i) for i in range(5)]
+    regs.append(dense_region("F", (5, 9, 7), (0.4,) * 3))
+    lrs = [ctx.region(r) for r in regs]
+    psis = [random_psi(r, 1, 70 + i)[0] for i, r in enumerate(regs)]
+    pots = [random_pot(r, 80 + i) for i, r in enumerate(regs)]
+    psi_off = np.cumsum([0] + [p.size for p in psis])[:-1]
+    pot_off = np.cumsum([0] + [p.size for p in pots])[:-1]
+    d_psi = dev(np.concatenate(psis))
+    d_h = torch.zeros_like(d_psi)
+    d_pot = dev(np.concatenate(pots))
+    potdim = np.array([r.nfine for r in regs], dtype=np.int32).ravel()
+    ek = torch.zeros(len(regs), dtype=torch.float64, device="cuda")
+    ep = torch.zeros_like(ek)
+    ctx.locham_batch(lrs, d_pot, d_psi, d_h, ek, ep, pot_off=pot_off,
+                     potdim=potdim, psi_off=psi_off)
+    ctx.sync()
+    h = d_h.cpu().numpy()
+    for i, r in enumerate(regs):
+        ho, eo, po = oracle.locham("F", r.n, r.nfl, r.nfu, r.hgrid, okeys(r),
+                                   pots[i], psis[i])
+        assert rel_err(h[psi_off[i]:psi_off[i] + psis[i].size], ho[0]) < TOL
+        assert rel_err(ek.cpu().numpy()[i], eo[0]) < TOL
+        assert rel_err(ep.cpu().numpy()[i], po[0]) < TOL
+    for l in lrs:
+        l.close()
+
+
+def test_potential_box_shift(ctx):
+    """global potential, region placed by ishift (apply_potential_lr_nobounds
+    with ishift), compared with the oracle on the extracted box"""
+    reg, nl = sphere_region((30, 30, 30), (0.4,) * 3, (6.1, 5.7, 6.6), 3.0, 1.5)
+    gn = tuple(2 * 30 + 31 for _ in range(3))
+    rng = np.random.default_rng(90)
+    gpot = rng.uniform(-2, 0, size=np.prod(gn))
+    ishift = [-2 * nl[d] for d in range(3)]
+    nf = reg.nfine
+    G = gpot.reshape(gn, order="F")
+    box = G[2 * nl[0]:2 * nl[0] + nf[0], 2 * nl[1]:2 * nl[1] + nf[1],
+            2 * nl[2]:2 * nl[2] + nf[2]].ravel(order="F")
+    psi = random_psi(reg, 1, 91)
+    lr = ctx.region(reg)
+    d_psi = dev(psi)
+    d_h = torch.zeros_like(d_psi)
+    ek = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ep = torch.zeros_like(ek)
+    ctx.locham_batch([lr], dev(gpot), d_psi, d_h, ek, ep, pot_off=[0],
+                     potdim=np.array(gn, dtype=np.int32), ishift=np.array(ishift))
+    ctx.sync()
+    ho, eo, po = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), box, psi)
+    assert rel_err(d_h.cpu().numpy(), ho) < TOL
+    assert rel_err(ep.cpu().numpy(), po) < TOL
+    lr.close()
+
+
+def test_graph_plan_replays(ctx):
+    """captured plan replayed several times with eager work in between; every
+    replay adds the same H psi"""
+    reg = dense_region("P", (11, 11, 11), (0.4,) * 3)
+    psi = random_psi(reg, 4, 100)
+    pot = random_pot(reg, 101)
+    lr = ctx.region(reg)
+    d_psi, d_pot = dev(psi), dev(pot)
+    d_h = torch.zeros_like(d_psi)
+    ek = torch.zeros(4, dtype=torch.float64, device="cuda")
+    ep = torch.zeros_like(ek)
+    plan = ctx.plan([lr] * 4, d_pot, d_psi, d_h, ek, ep)
+    ho, eo, po = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    prev = np.zeros_like(psi)
+    for rep in range(4):
+        plan.execute()
+        ctx.sync()
+        cur = d_h.cpu().numpy()
+        assert rel_err(cur - prev, ho) < 1e-11
+        assert rel_err(ek.cpu().numpy(), eo) < TOL
+        prev = cur
+        _ = (torch.randn(1000, device="cuda") * 2).sum().item()  # eager work
+    plan.close()
+    lr.close()
+
+
+def test_host_entry_point(ctx):
+    reg = dense_region("F", (6, 7, 5), (0.45,) * 3)
+    psi = random_psi(reg, 2, 110)
+    pot = random_pot(reg, 111)
+    lr = ctx.region(reg)
+    h = np.zeros_like(psi)
+    ek, ep = ctx.locham_host(lr, pot, psi, h)
+    ho, eo, po = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    assert rel_err(h, ho) < TOL and rel_err(ek, eo) < TOL and rel_err(ep, po) < TOL
+    lr.close()
+
+
+def test_descriptor_errors_are_reported(ctx):
+    import bigdft_hip
+    reg = dense_region("P", (3, 3, 3), (0.4,) * 3)
+    bad = reg.keyv_c.copy()
+    bad[2] += 1
+    reg.keyv_c = bad
+    with pytest.raises(bigdft_hip.BigDFTError) as e:
+        ctx.region(reg)
+    assert e.value.code == 3
+    big = dense_region("P", (70, 3, 3), (0.4,) * 3)
+    lr = ctx.region(big)   # transfers accept it, the Hamiltonian does not
+    with pytest.raises(bigdft_hip.BigDFTError) as e:
+        run = torch.zeros(big.ncoef, dtype=torch.float64, device="cuda")
+        ek = torch.zeros(1, dtype=torch.float64, device="cuda")
+        ctx.locham(lr, torch.zeros(big.n_isf, dtype=torch.float64, device="cuda"),
+                   run, run.clone(), ek, ek.clone())
+    assert e.value.code == 5
+    lr.close()
