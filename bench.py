@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched local-Hamiltonian apply on one MI355X.
+
+Workload (default): 256 orbitals on a periodic 48^3 coarse grid (96^3 fine),
+hgrid 0.35 bohr, coefficients N(0,1) normalised per orbital, shared
+potential = sum of 8 Gaussians (centres uniform in the cell, depth U[-5,-1]
+Ha, width U[0.5,2] bohr, minimum-image distance).  One step = one full
+application hpsi += H_loc psi (+ ekin, epot per orbital) over the batch,
+replayed from a captured hipGraph.  Inputs are resident in HBM before the
+timed region.
+
+Prints one JSON line.  value = grid-point*orbital applications per second
+(grid point = fine ISF point, n1i*n2i*n3i per orbital, the reference's unit,
+bigdft/src/memguess.f90:2269).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bigdft-suite_amd"))
+
+METRIC = ("grid-pt·orbital/s of local-H apply: mixed vs FP64 at "
+          "≤1e-6 rel err")
+PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32
+PEAK_HBM_GBS = 8000.0
+# Algorithmic FP64 flops per fine point of each stage: the reference's
+# 524 = 192 (synthesis+magic) + 46*3 (kinetic) + 192 (magic^T+analysis) + 2
+# (bigdft/src/memguess.f90:2269), split per axis pass.
+STAGE_FLOPS = {"x_grow_kin": 64 + 46, "y_grow": 64, "y_kin": 46, "z_kin": 46,
+               "z_pot": 64 + 2 + 64, "y_shrink": 64, "x_shrink": 64}
+
+# Compulsory volume traffic of each stage per fine point: element reads and
+# writes of the working type (esz bytes) plus the FP64 psi/hpsi accesses.
+STAGE_VOLS = {"uncompress": (1, 8), "x_grow_kin": (3, 0), "y_grow": (2, 0),
+              "y_kin": (3, 0), "z_kin": (3, 0), "z_pot": (2, 0),
+              "y_shrink": (2, 0), "x_shrink": (3, 0), "compress": (1, 16)}
+
+
+def stage_bytes(stage, esz):
+    v, extra = STAGE_VOLS[stage]
+    return v * esz + extra
+
+
+CONFIGS = {
+    "c2": dict(workload="periodic 48^3 coarse (96^3 fine), 256 orbitals, "
+               "hgrid 0.35, 8-Gaussian potential", geocode="P", n=47,
+               hgrid=0.35, norb=256),
+    "c3": dict(workload="periodic 64^3 coarse (128^3 fine), 128 orbitals, "
+               "hgrid 0.35, 8-Gaussian potential", geocode="P", n=63,
+               hgrid=0.35, norb=128),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp32",
+                    choices=["fp64", "fp32"],
+                    help="arithmetic of the timed headline run (fp32 = mixed)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="time budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the "
+              "batch shards by orbital across GPUs but no multi-GPU run is "
+              "part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import bigdft_hip
+    from bigdft_hip import inputs
+    from bigdft_hip.descriptors import dense_region
+
+    cfg = CONFIGS[args.config]
+    prec = bigdft_hip.PRECISIONS[args.precision]
+    dev = "cuda"
+    n, h, norb = cfg["n"], cfg["hgrid"], cfg["norb"]
+    reg = dense_region(cfg["geocode"], (n, n, n), (h, h, h))
+    units = reg.n_isf * norb
+
+    # ---- inputs, generated on the GPU ----
+    psi = inputs.coefficients(norb, reg.ncoef, args.seed, device=dev,
+                              normalise=True)
+    cell = [(n + 1) * h] * 3
+    gp = inputs.gaussian_params(args.seed + 1, cell)
+    pot = inputs.gaussian_field(gp, inputs.fine_coords("P", (n, n, n),
+                                                       (h, h, h), device=dev),
+                                cell=cell).contiguous()
+
+    ctx = bigdft_hip.Context(0)
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
+    lr = ctx.region(reg)
+
+    def timed(prec):
+        """graph-replay timing, then per-stage HIP-event timing, then one
+        clean application for the error analysis"""
+        h = torch.zeros_like(psi)
+        ek = torch.zeros(norb, dtype=torch.float64, device=dev)
+        ep = torch.zeros_like(ek)
+        plan = ctx.plan([lr] * norb, pot, psi, h, ek, ep, precision=prec)
+        for _ in range(args.warmup):
+            plan.execute()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        t0.record(stream)
+        for _ in range(args.steps):
+            plan.execute()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        ms = t0.elapsed_time(t1) / args.steps
+        plan.close()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        for _ in range(max(3, min(args.steps, 10))):
+            ctx.locham(lr, pot, psi, h, ek, ep, prec)
+        stages, nl = ctx.profile_read(reset=True)
+        ctx.profile(False)
+        h.zero_()
+        ctx.locham(lr, pot, psi, h, ek, ep, prec)
+        ctx.sync()
+        return ms, wall, {k: v / nl for k, v in stages.items()}, h, ek, ep
+
+    ms64, wall64, st64, h64, ek64, ep64 = timed(bigdft_hip.PREC_FP64)
+    if prec != bigdft_hip.PREC_FP64:
+        ms_step, wall, avg, hm, ekm, epm = timed(prec)
+        hmax = h64.abs().max()
+        d = (hm - h64).abs()
+        guard = torch.maximum(h64.abs(), 1e-3 * hmax)
+        mixed_err = {
+            "hpsi_max_abs_over_max": float(d.max() / hmax),
+            "hpsi_rms_over_rms": float(torch.sqrt((d * d).mean() / (h64 * h64).mean())),
+            "hpsi_max_rel_guarded": float((d / guard).max()),
+            "ekin_max_rel": float(((ekm - ek64).abs() / ek64.abs()).max()),
+            "epot_max_rel": float(((epm - ep64).abs() / ep64.abs()).max()),
+            "reference": "FP64 GPU run on the same inputs"}
+    else:
+        ms_step, wall, avg, hm, ekm, epm = ms64, wall64, st64, h64, ek64, ep64
+        mixed_err = None
+    esz = 8 if prec == bigdft_hip.PREC_FP64 else 4
+    peak_c = PEAK_FP64_TFLOPS if esz == 8 else PEAK_FP32_TFLOPS
+    dom = max((k for k in avg if k in STAGE_FLOPS), key=lambda k: avg[k])
+    t_dom = avg[dom] * 1e-3
+    ach_c = STAGE_FLOPS[dom] * units / t_dom / 1e12
+    ach_b = stage_bytes(dom, esz) * units / t_dom / 1e9
+    if ach_c / peak_c >= ach_b / PEAK_HBM_GBS:
+        roof = {"bound": "mfma", "achieved": ach_c, "peak": peak_c,
+                "unit": "TFLOP/s", "frac": ach_c / peak_c}
+        other = {"hbm_GBps": ach_b, "hbm_frac": ach_b / PEAK_HBM_GBS}
+    else:
+        roof = {"bound": "hbm", "achieved": ach_b, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": ach_b / PEAK_HBM_GBS}
+        other = {"flop_TFLOPs": ach_c, "flop_frac": ach_c / peak_c}
+    roof.update({"traffic": None, "kernel": dom, "kernel_ms": avg[dom]})
+    roof.update(other)
+
+    # ---- parity on a sample + CPU baseline (oracle, host cores) ----
+    parity, cpu = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        keys = oracle.Keys(reg.nseg_c, reg.nvctr_c, reg.keyg_c, reg.keyv_c,
+                           reg.nseg_f, reg.nvctr_f, reg.keyg_f, reg.keyv_f)
+        potn = pot.cpu().numpy()
+        done, ts, errs, eks, eps = 0, 0.0, [], [], []
+        ek_g, ep_g = ek64.cpu().numpy(), ep64.cpu().numpy()
+        while done < norb and (done < 4 or ts < args.cpu_seconds):
+            p = psi[done].cpu().numpy()[None, :]
+            tt = time.perf_counter()
+            ho, eo, po = oracle.locham(cfg["geocode"], reg.n, reg.nfl,
+                                       reg.nfu, reg.hgrid, keys, potn, p)
+            ts += time.perf_counter() - tt
+            hg = h64[done].cpu().numpy()
+            sc = np.maximum(np.abs(ho[0]), 1e-3 * np.abs(ho[0]).max())
+            errs.append(float((np.abs(hg - ho[0]) / sc).max()))
+            eks.append(abs(ek_g[done] - eo[0]) / abs(eo[0]))
+            eps.append(abs(ep_g[done] - po[0]) / abs(po[0]))
+            done += 1
+        parity = {"orbitals": done, "fp64_hpsi_max_rel": max(errs),
+                  "fp64_ekin_max_rel": float(max(eks)),
+                  "fp64_epot_max_rel": float(max(eps)), "reference": "oracle"}
+        cpu = {"value": done * reg.n_isf / ts, "unit": "grid-pt*orbital/s",
+               "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"{done} of the {norb} orbitals, oracle (C+OpenMP "
+                         f"restatement of the reference Fortran), {ts:.1f} s"}
+
+    out = {
+        "metric": METRIC, "value": units / (ms_step * 1e-3),
+        "unit": "grid-pt*orbital/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[args.precision],
+        "data": "synthetic (seeded N(0,1) coefficients, 8-Gaussian potential)",
+        "config": {"workload": cfg["workload"], "orbitals": norb,
+                   "grid_pts_per_orbital": reg.n_isf, "precision": args.precision,
+                   "graph": True, "wall_s": round(wall, 4)},
+        "roofline": roof,
+        "fp64": {"value": units / (ms64 * 1e-3), "ms_per_step": ms64,
+                 "stages_ms": {k: round(v, 4) for k, v in st64.items()}},
+        "mixed_speedup_vs_fp64": ms64 / ms_step,
+        "mixed_error_vs_fp64": mixed_err,
+        "stages_ms": {k: round(v, 4) for k, v in avg.items()},
+        "apply_equiv_tflops": 524.0 * units / (ms_step * 1e-3) / 1e12,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(out))
+    lr.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

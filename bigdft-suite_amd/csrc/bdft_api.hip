@@ -328,8 +328,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   *out = nullptr;
   if (norb <= 0 || !lrs || !pot || !psi || !hpsi || !ekin || !epot)
     return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
-  if (precision != BDFT_PREC_FP64)
+  if (precision != BDFT_PREC_FP64 && precision != BDFT_PREC_FP32)
     return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
+  const size_t esz = precision == BDFT_PREC_FP64 ? sizeof(double) : sizeof(float);
   std::vector<BoxDesc> box((size_t)norb);
   long long wtot = 0, t1tot = 0, t2tot = 0;
   int maxN = 0;
@@ -439,7 +440,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     }
   }
   // workspace: W | K | T1 | T2
-  const size_t ws_bytes = sizeof(double) * (2 * al(wtot) + al(t1tot) + al(t2tot));
+  const size_t ws_bytes = esz * (2 * al(wtot) + al(t1tot) + al(t2tot));
   DevBuf& ws = own_ws ? plan->ws : ctx->ws;
   rc = dev_alloc(ctx, ws, ws_bytes);
   if (rc) {
@@ -447,7 +448,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     delete plan;
     return rc;
   }
-  double* w = (double*)ws.p;
+  char* w = (char*)ws.p;
   Batch& B = plan->B;
   B.box = dbox;
   B.nbox = norb;
@@ -455,9 +456,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   B.hpsi = hpsi;
   B.pot = pot;
   B.W = w;
-  B.K = w + al(wtot);
-  B.T1 = w + 2 * al(wtot);
-  B.T2 = w + 2 * al(wtot) + al(t1tot);
+  B.K = w + esz * al(wtot);
+  B.T1 = w + esz * 2 * al(wtot);
+  B.T2 = w + esz * (2 * al(wtot) + al(t1tot));
   B.ekin = ekin;
   B.epot = epot;
   plan->T = T;
@@ -486,8 +487,8 @@ static int run_plan(bdft_plan* plan) {
     ctx->ev_pending.push_back(std::move(set));
     ev = ctx->ev_pending.back().data();
   }
-  hipError_t e = launch_fp64(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero,
-                             ctx->stream, ev);
+  hipError_t e = launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero,
+                               plan->precision == BDFT_PREC_FP32, ctx->stream, ev);
   if (e != hipSuccess)
     return set_err(ctx, BDFT_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return BDFT_OK;
@@ -508,7 +509,8 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t el = launch_fp64(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero, cs);
+    hipError_t el = launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
+                                  plan->need_zero, plan->precision == BDFT_PREC_FP32, cs);
     e = hipStreamEndCapture(cs, &plan->graph);
     if (el != hipSuccess) e = el;
   }

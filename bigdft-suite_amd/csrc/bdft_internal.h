@@ -170,10 +170,11 @@ struct Batch {
   const double* psi;    // compressed coefficients (all orbitals)
   double* hpsi;         // accumulated output, same layout as psi
   const double* pot;    // potential(s)
-  double* W;            // wavelet volumes (uncompressed psi, later output)
-  double* K;            // kinetic accumulator volumes (wavelet layout)
-  double* T1;           // (L1,2N2,2N3) volumes
-  double* T2;           // (L1,L2,2N3) volumes
+  void* W;              // wavelet volumes (uncompressed psi, later output)
+  void* K;              // kinetic accumulator volumes (wavelet layout)
+  void* T1;             // (L1,2N2,2N3) volumes
+  void* T2;             // (L1,L2,2N3) volumes
+                        // (element type: double, or float for mixed precision)
   double* part;         // energy partial sums, one slot per (pass, tile)
   double* ekin;         // per-box kinetic energy (device)
   double* epot;         // per-box potential energy (device)
@@ -192,12 +193,13 @@ struct PassTables {
 enum Stage { S_ZERO = 0, S_UNC, S_XF, S_YF, S_KY, S_KZ, S_ZP, S_YB, S_XB,
              S_COMP, S_ENERGY, S_COUNT };
 
-size_t lds_bytes_fp64(int maxN);
+size_t lds_bytes(int maxN, int elem);
 hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
                            long long wtotal, bool compress, hipStream_t s);
-// ev: null, or 2*S_COUNT events recorded before/after each stage
-hipError_t launch_fp64(const Batch& B, const PassTables& T, int maxN,
-                       long long wtotal, bool need_zero, hipStream_t s,
-                       hipEvent_t* ev = nullptr);
+// ev: null, or 2*S_COUNT events recorded before/after each stage;
+// fp32 selects the mixed-precision (float working type) kernels
+hipError_t launch_passes(const Batch& B, const PassTables& T, int maxN,
+                         long long wtotal, bool need_zero, int fp32,
+                         hipStream_t s, hipEvent_t* ev = nullptr);
 
 }  // namespace bdft

@@ -1,4 +1,6 @@
-// Plain FP64 kernels of the local-Hamiltonian apply.
+// Pass kernels of the local-Hamiltonian apply, templated on the working type
+// T of the convolutions (double: plain FP64; float: mixed precision with FP64
+// input/potential/energies/accumulation).
 //
 // hpsi += G^T V G psi + K psi, evaluated as separable line passes over
 // bundles of 32 lines staged in LDS:
@@ -27,9 +29,19 @@
 
 namespace bdft {
 
-__device__ __forceinline__ int pmod(int i, int n) {
-  const int r = i % n;
-  return r < 0 ? r + n : r;
+// i mod n for i in a small window around [0,n): conditional steps, no division
+__device__ __forceinline__ int wrap(int i, int n) {
+  while (i < 0) i += n;
+  while (i >= n) i -= n;
+  return i;
+}
+
+// a / b for 0 <= a < 2^22 with a float reciprocal, corrected to exact
+__device__ __forceinline__ int qdiv(int a, int b, float inv) {
+  int q = (int)((float)a * inv);
+  if ((q + 1) * b <= a) ++q;
+  if (q * b > a) --q;
+  return q;
 }
 
 // ---------------------------------------------------------------------------
@@ -39,58 +51,104 @@ __device__ __forceinline__ int pmod(int i, int n) {
 // Wavelet line bundle: two sections (c, d) of NS = N + HL + HR rows; row r of
 // a section holds coarse index k = r - HL (wrapped or zero outside [0,N)).
 // CONTIG: lines are contiguous in memory (x pass), else positions are.
-template <bool CONTIG>
-__device__ __forceinline__ void fill_wav(double* lds, const double* __restrict__ src,
+template <bool CONTIG, typename T, typename S>
+__device__ __forceinline__ void fill_wav(T* lds, const S* __restrict__ src,
                                          long long lstride, long long pstride,
                                          int nl, int N, int per) {
   const int NS = N + HL + HR;
   const int rows = 2 * NS;
-  for (int idx = threadIdx.x; idx < rows * TW; idx += NT) {
-    int row, l;
-    if (CONTIG) {
-      l = idx / rows;
-      row = idx - l * rows;
-    } else {
-      row = idx / TW;
-      l = idx - row * TW;
+  if (CONTIG) {
+    // lines contiguous (pstride 1, lstride = 2N): read the tile as one flat
+    // run of nl*2N elements, then fill the halo rows from LDS
+    const int W = 2 * N;
+    const float inv = 1.0f / (float)W;
+    for (int idx = threadIdx.x; idx < nl * W; idx += NT) {
+      const int l = qdiv(idx, W, inv);
+      const int pos = idx - l * W;
+      const int sec = pos >= N;
+      const int row = sec * NS + (pos - sec * N) + HL;
+      lds[row * LDW + l] = (T)src[(long long)l * lstride + pos];
     }
-    const int sec = row >= NS;
-    int k = row - sec * NS - HL;
-    bool ok = l < nl;
-    if (per) k = pmod(k, N);
-    else ok = ok && k >= 0 && k < N;
-    double v = 0.0;
-    if (ok) v = src[(long long)l * lstride + (long long)(sec * N + k) * pstride];
-    lds[row * LDW + l] = v;
+    for (int idx = threadIdx.x; idx < (TW - nl) * W; idx += NT) {
+      const int l = nl + qdiv(idx, W, inv);
+      const int pos = idx - (l - nl) * W;
+      const int sec = pos >= N;
+      lds[(sec * NS + (pos - sec * N) + HL) * LDW + l] = (T)0;
+    }
+    __syncthreads();
+    const int H = HL + HR;
+    for (int idx = threadIdx.x; idx < 2 * H * TW; idx += NT) {
+      const int hr = idx >> 5, l = idx & (TW - 1);
+      const int sec = hr >= H;
+      const int r = hr - sec * H;
+      const int k = r < HL ? r - HL : N + (r - HL);  // halo coarse index
+      T v = (T)0;
+      if (per) v = lds[(sec * NS + wrap(k, N) + HL) * LDW + l];
+      lds[(sec * NS + k + HL) * LDW + l] = v;
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < rows * TW; idx += NT) {
+      const int row = idx >> 5, l = idx & (TW - 1);
+      const int sec = row >= NS;
+      int k = row - sec * NS - HL;
+      bool ok = l < nl;
+      if (per) k = wrap(k, N);
+      else ok = ok && k >= 0 && k < N;
+      T v = (T)0;
+      if (ok) v = (T)src[(long long)l * lstride + (long long)(sec * N + k) * pstride];
+      lds[row * LDW + l] = v;
+    }
   }
 }
 
 // Real-space line bundle for G^T: rows q = I + 14 with fine index
 // I in [-14, 2N+22); periodic wraps mod 2N, free reads position p = q < L.
-template <bool CONTIG>
-__device__ __forceinline__ void fill_real(double* lds, const double* __restrict__ src,
+template <bool CONTIG, typename T>
+__device__ __forceinline__ void fill_real(T* lds, const T* __restrict__ src,
                                           long long lstride, long long pstride,
                                           int nl, int N, int L, int per) {
   const int Q = 2 * N + 36;
-  for (int idx = threadIdx.x; idx < Q * TW; idx += NT) {
-    int q, l;
-    if (CONTIG) {
-      l = idx / Q;
-      q = idx - l * Q;
-    } else {
-      q = idx / TW;
-      l = idx - q * TW;
+  if (CONTIG) {
+    const float inv = 1.0f / (float)L;
+    for (int idx = threadIdx.x; idx < nl * L; idx += NT) {
+      const int l = qdiv(idx, L, inv);
+      const int p = idx - l * L;
+      lds[(per ? p + 14 : p) * LDW + l] = src[(long long)l * lstride + p];
     }
-    int pos;
-    bool ok = l < nl;
-    if (per) pos = pmod(q - 14, 2 * N);
-    else {
-      pos = q;
-      ok = ok && q < L;
+    for (int idx = threadIdx.x; idx < (TW - nl) * L; idx += NT) {
+      const int l = nl + qdiv(idx, L, inv);
+      const int p = idx - (l - nl) * L;
+      lds[(per ? p + 14 : p) * LDW + l] = (T)0;
     }
-    double v = 0.0;
-    if (ok) v = src[(long long)l * lstride + (long long)pos * pstride];
-    lds[q * LDW + l] = v;
+    __syncthreads();
+    // rows not covered above: periodic halos [0,14) and [2N+14,Q), free tail
+    const int nh = per ? Q - 2 * N : Q - L;
+    for (int idx = threadIdx.x; idx < nh * TW; idx += NT) {
+      const int hr = idx >> 5, l = idx & (TW - 1);
+      int q;
+      T v = (T)0;
+      if (per) {
+        q = hr < 14 ? hr : 2 * N + hr;
+        v = lds[(wrap(q - 14, 2 * N) + 14) * LDW + l];
+      } else {
+        q = L + hr;
+      }
+      lds[q * LDW + l] = v;
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < Q * TW; idx += NT) {
+      const int q = idx >> 5, l = idx & (TW - 1);
+      int pos;
+      bool ok = l < nl;
+      if (per) pos = wrap(q - 14, 2 * N);
+      else {
+        pos = q;
+        ok = ok && q < L;
+      }
+      T v = (T)0;
+      if (ok) v = src[(long long)l * lstride + (long long)pos * pstride];
+      lds[q * LDW + l] = v;
+    }
   }
 }
 
@@ -101,20 +159,21 @@ __device__ __forceinline__ void fill_real(double* lds, const double* __restrict_
 // G: outputs at fine positions I0..I0+7 (I0 even) from coarse k = I0/2-8+kk,
 // kk = 0..18: out(I) += fil2(I-2k,1) c(k) + fil2(I-2k,2) d(k).
 // cc points at the c-section row of k = I0/2-8, dd at the d-section row.
-__device__ __forceinline__ void grow8(const double* __restrict__ cc,
-                                      const double* __restrict__ dd, double o[8]) {
+template <typename T>
+__device__ __forceinline__ void grow8(const T* __restrict__ cc,
+                                      const T* __restrict__ dd, T o[8]) {
 #pragma unroll
-  for (int r = 0; r < 8; ++r) o[r] = 0.0;
+  for (int r = 0; r < 8; ++r) o[r] = (T)0;
 #pragma unroll
   for (int kk = 0; kk < 19; ++kk) {
-    const double c = cc[kk * LDW];
-    const double d = dd[kk * LDW];
+    const T c = cc[kk * LDW];
+    const T d = dd[kk * LDW];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int j = r + 16 - 2 * kk;  // fil2 index, valid in [-14, 16]
       if (j >= -14 && j <= 16) {
-        o[r] = fma(Fil::F2S[j + 14], c, o[r]);
-        o[r] = fma(Fil::F2W[j + 14], d, o[r]);
+        o[r] = fma((T)Fil::F2S[j + 14], c, o[r]);
+        o[r] = fma((T)Fil::F2W[j + 14], d, o[r]);
       }
     }
   }
@@ -122,22 +181,22 @@ __device__ __forceinline__ void grow8(const double* __restrict__ cc,
 
 // G^T: coarse outputs i0..i0+3 (c and d) from fine rows q = 2*i0 + qq,
 // qq = 0..36 (row q holds fine index q - 14): c(i) = sum_j fil2(j,1) x(2i+j).
-__device__ __forceinline__ void shrink4(const double* __restrict__ xq,
-                                        double oc[4], double od[4]) {
+template <typename T>
+__device__ __forceinline__ void shrink4(const T* __restrict__ xq, T oc[4], T od[4]) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    oc[r] = 0.0;
-    od[r] = 0.0;
+    oc[r] = (T)0;
+    od[r] = (T)0;
   }
 #pragma unroll
   for (int qq = 0; qq < 37; ++qq) {
-    const double x = xq[qq * LDW];
+    const T x = xq[qq * LDW];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int j = qq - 2 * r - 14;
       if (j >= -14 && j <= 16) {
-        oc[r] = fma(Fil::F2S[j + 14], x, oc[r]);
-        od[r] = fma(Fil::F2W[j + 14], x, od[r]);
+        oc[r] = fma((T)Fil::F2S[j + 14], x, oc[r]);
+        od[r] = fma((T)Fil::F2W[j + 14], x, od[r]);
       }
     }
   }
@@ -146,27 +205,27 @@ __device__ __forceinline__ void shrink4(const double* __restrict__ xq,
 // Wavelet-basis kinetic (unscaled): coarse outputs i0..i0+3 from rows of
 // k = i0-14+kk, kk = 0..31 (cc, dd point at k = i0-14 in each section).
 //   c_out(i) += a(l) c(i+l) + c(-l) d(i+l);  d_out(i) += c(l) c(i+l) + e(l) d(i+l)
-__device__ __forceinline__ void kin4(const double* __restrict__ cc,
-                                     const double* __restrict__ dd,
-                                     double oc[4], double od[4]) {
+template <typename T>
+__device__ __forceinline__ void kin4(const T* __restrict__ cc, const T* __restrict__ dd,
+                                     T oc[4], T od[4]) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    oc[r] = 0.0;
-    od[r] = 0.0;
+    oc[r] = (T)0;
+    od[r] = (T)0;
   }
 #pragma unroll
   for (int kk = 0; kk < 32; ++kk) {
-    const double c = cc[kk * LDW];
-    const double d = dd[kk * LDW];
+    const T c = cc[kk * LDW];
+    const T d = dd[kk * LDW];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int l = kk - 14 - r;
       if (l >= -14 && l <= 14) {
         const int al = l < 0 ? -l : l;
-        oc[r] = fma(Fil::KA[al], c, oc[r]);
-        oc[r] = fma(Fil::KC[-l + 14], d, oc[r]);
-        od[r] = fma(Fil::KC[l + 14], c, od[r]);
-        od[r] = fma(Fil::KE[al], d, od[r]);
+        oc[r] = fma((T)Fil::KA[al], c, oc[r]);
+        oc[r] = fma((T)Fil::KC[-l + 14], d, oc[r]);
+        od[r] = fma((T)Fil::KC[l + 14], c, od[r]);
+        od[r] = fma((T)Fil::KE[al], d, od[r]);
       }
     }
   }
@@ -198,44 +257,40 @@ __device__ __forceinline__ int grow_row0(int p0, int per) {
 // ---------------------------------------------------------------------------
 constexpr int CPT = 256;  // compressed points per tile
 
+// psig index of coarse point lin = i1 + N1*(i2 + N2*i3), component bits s
 __device__ __forceinline__ long long psig_index(const BoxDesc& bd, int lin, int s1,
                                                 int s2, int s3) {
   const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2];
-  const int i1 = lin % N1;
-  const int t = lin / N1;
-  const int i2 = t % N2;
-  const int i3 = t / N2;
+  const int t = qdiv(lin, N1, 1.0f / (float)N1);
+  const int i1 = lin - t * N1;
+  const int i3 = qdiv(t, N2, 1.0f / (float)N2);
+  const int i2 = t - i3 * N2;
   return (long long)(i1 + N1 * s1) +
          2LL * N1 * ((long long)(i2 + N2 * s2) + 2LL * N2 * (i3 + N3 * s3));
 }
 
-// fine component t = 1..7 -> (s1,s2,s3) bits, un-compress.f90:930-936
-__device__ __forceinline__ void fsel(int t, int& s1, int& s2, int& s3) {
-  s1 = t & 1;
-  s2 = (t >> 1) & 1;
-  s3 = (t >> 2) & 1;
-}
-
-__global__ __launch_bounds__(NT) void k_zero(double* __restrict__ p, long long n) {
+template <typename T>
+__global__ __launch_bounds__(NT) void k_zero(T* __restrict__ p, long long n) {
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n;
        i += (long long)gridDim.x * NT)
-    p[i] = 0.0;
+    p[i] = (T)0;
 }
 
-template <bool COMPRESS>
+// uncompress (psi -> W, converted to T) or compress-accumulate (hpsi += W)
+template <typename T, bool COMPRESS>
 __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
   const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
-  double* W = B.W + bd.w_off;
+  T* W = (T*)B.W + bd.w_off;
   if (lt < ctiles) {
     const int k = lt * CPT + threadIdx.x;
     if (k < bd.nvctr_c) {
       const long long w = psig_index(bd, bd.cmap[k], 0, 0, 0);
-      if (COMPRESS) B.hpsi[bd.psi_off + k] += W[w];
-      else W[w] = B.psi[bd.psi_off + k];
+      if (COMPRESS) B.hpsi[bd.psi_off + k] += (double)W[w];
+      else W[w] = (T)B.psi[bd.psi_off + k];
     }
   } else {
     const int k = (lt - ctiles) * CPT + threadIdx.x;
@@ -244,11 +299,10 @@ __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
       const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
 #pragma unroll
       for (int t = 1; t <= 7; ++t) {
-        int s1, s2, s3;
-        fsel(t, s1, s2, s3);
-        const long long w = psig_index(bd, lin, s1, s2, s3);
-        if (COMPRESS) B.hpsi[base + t - 1] += W[w];
-        else W[w] = B.psi[base + t - 1];
+        // component t = 1..7 <-> (s1,s2,s3) = bits of t, un-compress.f90:930-936
+        const long long w = psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1);
+        if (COMPRESS) B.hpsi[base + t - 1] += (double)W[w];
+        else W[w] = (T)B.psi[base + t - 1];
       }
     }
   }
@@ -257,8 +311,10 @@ __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // x pass, forward: T1 = G_x W, K = K_x W, ekin partial
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
   __shared__ double red[NT / 64];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
@@ -269,13 +325,13 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
   const int nlines = 4 * bd.N[1] * bd.N[2];
   const int l0 = lt * TW;
   const int nl = min(TW, nlines - l0);
-  const double* src = B.W + bd.w_off + (long long)l0 * W1;
+  const T* src = (const T*)B.W + bd.w_off + (long long)l0 * W1;
   fill_wav<true>(lds, src, W1, 1, nl, N1, per);
   __syncthreads();
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N1 + HL + HR;
-  const double* col = lds + tx;
-  double g[3][8];
+  const T* col = lds + tx;
+  T g[3][8];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const int p0 = 8 * (ty + m * TY);
@@ -284,9 +340,9 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
       grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
     }
   }
-  double kc[2][4], kd[2][4];
+  T kc[2][4], kd[2][4];
   double ek = 0.0;
-  const double ks = bd.kscale[0];
+  const T ks = (T)bd.kscale[0];
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
     const int i0 = 4 * (ty + m * TY);
@@ -297,8 +353,8 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
         kc[m][r] *= ks;
         kd[m][r] *= ks;
         if (i0 + r < N1 && tx < nl)
-          ek += kc[m][r] * col[(i0 + r + HL) * LDW] +
-                kd[m][r] * col[(NS + i0 + r + HL) * LDW];
+          ek += (double)kc[m][r] * (double)col[(i0 + r + HL) * LDW] +
+                (double)kd[m][r] * (double)col[(NS + i0 + r + HL) * LDW];
       }
     }
   }
@@ -314,10 +370,13 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
       if (p0 + r < L1) lds[(p0 + r) * LDW + tx] = g[m][r];
   }
   __syncthreads();
-  double* dst = B.T1 + bd.t1_off + (long long)l0 * L1;
-  for (int idx = threadIdx.x; idx < nl * L1; idx += NT) {
-    const int l = idx / L1, p = idx - l * L1;
-    dst[idx] = lds[p * LDW + l];
+  T* dst = (T*)B.T1 + bd.t1_off + (long long)l0 * L1;
+  {
+    const float inv = 1.0f / (float)L1;
+    for (int idx = threadIdx.x; idx < nl * L1; idx += NT) {
+      const int l = qdiv(idx, L1, inv), p = idx - l * L1;
+      dst[idx] = lds[p * LDW + l];
+    }
   }
   __syncthreads();
   // stage K outputs and store contiguous lines of K
@@ -332,18 +391,23 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
       }
   }
   __syncthreads();
-  double* kdst = B.K + bd.w_off + (long long)l0 * W1;
-  for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
-    const int l = idx / W1, p = idx - l * W1;
-    kdst[idx] = lds[p * LDW + l];
+  T* kdst = (T*)B.K + bd.w_off + (long long)l0 * W1;
+  {
+    const float inv = 1.0f / (float)W1;
+    for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
+      const int l = qdiv(idx, W1, inv), p = idx - l * W1;
+      kdst[idx] = lds[p * LDW + l];
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // y pass, forward: T2 = G_y T1
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(NT) void k_yfwd(Batch B, TileTable tt) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
@@ -354,15 +418,15 @@ __global__ __launch_bounds__(NT) void k_yfwd(Batch B, TileTable tt) {
   const int j3 = lt / nxt;
   const int j1 = (lt - j3 * nxt) * TW;
   const int nl = min(TW, L1 - j1);
-  const double* src = B.T1 + bd.t1_off + j1 + (long long)L1 * W2 * j3;
+  const T* src = (const T*)B.T1 + bd.t1_off + j1 + (long long)L1 * W2 * j3;
   fill_wav<false>(lds, src, 1, L1, nl, N2, per);
   __syncthreads();
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N2 + HL + HR;
-  const double* col = lds + tx;
-  double* dst = B.T2 + bd.t2_off + j1 + tx + (long long)L1 * L2 * j3;
+  const T* col = lds + tx;
+  T* dst = (T*)B.T2 + bd.t2_off + j1 + tx + (long long)L1 * L2 * j3;
   for (int p0 = 8 * ty; p0 < L2; p0 += 8 * TY) {
-    double g[8];
+    T g[8];
     const int r0 = grow_row0(p0, per);
     grow8(col + r0 * LDW, col + (NS + r0) * LDW, g);
     if (tx < nl) {
@@ -376,9 +440,10 @@ __global__ __launch_bounds__(NT) void k_yfwd(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // kinetic along y (AX=1) or z (AX=2) on the wavelet volume: K += K_d W
 // ---------------------------------------------------------------------------
-template <int AX>
+template <typename T, int AX>
 __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
   __shared__ double red[NT / 64];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
@@ -393,23 +458,24 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
   const long long pstride = AX == 1 ? (long long)W1 : (long long)W1 * W2;
   const long long base = bd.w_off + j1 +
                          (AX == 1 ? (long long)W1 * W2 * jo : (long long)W1 * jo);
-  fill_wav<false>(lds, B.W + base, 1, pstride, nl, Nd, bd.per);
+  fill_wav<false>(lds, (const T*)B.W + base, 1, pstride, nl, Nd, bd.per);
   __syncthreads();
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = Nd + HL + HR;
-  const double* col = lds + tx;
-  const double ks = bd.kscale[AX];
+  const T* col = lds + tx;
+  const T ks = (T)bd.kscale[AX];
   double ek = 0.0;
-  double* kp = B.K + base + tx;
+  T* kp = (T*)B.K + base + tx;
   for (int i0 = 4 * ty; i0 < Nd; i0 += 4 * TY) {
-    double oc[4], od[4];
+    T oc[4], od[4];
     kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, oc, od);
     if (tx < nl) {
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         if (i0 + r < Nd) {
-          const double vc = oc[r] * ks, vd = od[r] * ks;
-          ek += vc * col[(i0 + r + HL) * LDW] + vd * col[(NS + i0 + r + HL) * LDW];
+          const T vc = oc[r] * ks, vd = od[r] * ks;
+          ek += (double)vc * (double)col[(i0 + r + HL) * LDW] +
+                (double)vd * (double)col[(NS + i0 + r + HL) * LDW];
           kp[pstride * (i0 + r)] += vc;
           kp[pstride * (Nd + i0 + r)] += vd;
         }
@@ -422,8 +488,10 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
 // ---------------------------------------------------------------------------
 // z pass with the potential: T2 <- G_z^T ( V . G_z T2 ), epot partial
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
   __shared__ double red[NT / 64];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
@@ -435,12 +503,12 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   const int j1 = (lt - j2 * nxt) * TW;
   const int nl = min(TW, L1 - j1);
   const long long pstride = (long long)L1 * L2;
-  double* line = B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
+  T* line = (T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
   fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
   __syncthreads();
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
-  const double* col = lds + tx;
+  const T* col = lds + tx;
   // potential index of this line; outside the potential box V = 0, which
   // reproduces the zeroing of psir there (potential.f90:874-947)
   const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
@@ -448,7 +516,7 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
                     q2 < bd.potdim[1];
   const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
   const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
-  double g[3][8];
+  T g[3][8];
   double ep = 0.0;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -461,9 +529,9 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
         const int q3 = p0 + r - bd.ishift[2];
         double v = 0.0;
         if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) v = pot[pz * q3];
-        const double t = v * g[m][r];
-        ep = fma(t, g[m][r], ep);
-        g[m][r] = t;
+        const double pr = (double)g[m][r];
+        ep = fma(v * pr, pr, ep);
+        g[m][r] = (T)(v * pr);
       }
     }
   }
@@ -480,7 +548,7 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
       const int p = p0 + r;
       if (p0 < L3 && p < L3) {
         if (per) {
-          for (int q = p + 14 - 2 * N3 * ((p + 14) / (2 * N3)); q < Q; q += 2 * N3)
+          for (int q = wrap(p + 14, 2 * N3); q < Q; q += 2 * N3)
             lds[q * LDW + tx] = g[m][r];
         } else {
           lds[p * LDW + tx] = g[m][r];
@@ -489,10 +557,10 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
     }
   }
   if (!per)
-    for (int q = L3 + ty; q < Q; q += TY) lds[q * LDW + tx] = 0.0;
+    for (int q = L3 + ty; q < Q; q += TY) lds[q * LDW + tx] = (T)0;
   __syncthreads();
   for (int i0 = 4 * ty; i0 < N3; i0 += 4 * TY) {
-    double oc[4], od[4];
+    T oc[4], od[4];
     shrink4(col + 2 * i0 * LDW, oc, od);
     if (tx < nl) {
 #pragma unroll
@@ -508,8 +576,10 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
 // ---------------------------------------------------------------------------
 // y pass, backward: T1 = G_y^T T2
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(NT) void k_ybwd(Batch B, TileTable tt) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
@@ -520,14 +590,14 @@ __global__ __launch_bounds__(NT) void k_ybwd(Batch B, TileTable tt) {
   const int j3 = lt / nxt;
   const int j1 = (lt - j3 * nxt) * TW;
   const int nl = min(TW, L1 - j1);
-  const double* src = B.T2 + bd.t2_off + j1 + (long long)L1 * L2 * j3;
+  const T* src = (const T*)B.T2 + bd.t2_off + j1 + (long long)L1 * L2 * j3;
   fill_real<false>(lds, src, 1, L1, nl, N2, L2, per);
   __syncthreads();
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
-  const double* col = lds + tx;
-  double* dst = B.T1 + bd.t1_off + j1 + tx + (long long)L1 * W2 * j3;
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
+  const T* col = lds + tx;
+  T* dst = (T*)B.T1 + bd.t1_off + j1 + tx + (long long)L1 * W2 * j3;
   for (int i0 = 4 * ty; i0 < N2; i0 += 4 * TY) {
-    double oc[4], od[4];
+    T oc[4], od[4];
     shrink4(col + 2 * i0 * LDW, oc, od);
     if (tx < nl) {
 #pragma unroll
@@ -543,8 +613,10 @@ __global__ __launch_bounds__(NT) void k_ybwd(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // x pass, backward: W = G_x^T T1 + K
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
@@ -554,12 +626,12 @@ __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
   const int nlines = 4 * bd.N[1] * bd.N[2];
   const int l0 = lt * TW;
   const int nl = min(TW, nlines - l0);
-  const double* src = B.T1 + bd.t1_off + (long long)l0 * L1;
+  const T* src = (const T*)B.T1 + bd.t1_off + (long long)l0 * L1;
   fill_real<true>(lds, src, L1, 1, nl, N1, L1, per);
   __syncthreads();
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
-  const double* col = lds + tx;
-  double oc[2][4], od[2][4];
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
+  const T* col = lds + tx;
+  T oc[2][4], od[2][4];
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
     const int i0 = 4 * (ty + m * TY);
@@ -578,9 +650,12 @@ __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
   }
   __syncthreads();
   const long long o = bd.w_off + (long long)l0 * W1;
+  T* W = (T*)B.W;
+  const T* K = (const T*)B.K;
+  const float inv = 1.0f / (float)W1;
   for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
-    const int l = idx / W1, p = idx - l * W1;
-    B.W[o + idx] = lds[p * LDW + l] + B.K[o + idx];
+    const int l = qdiv(idx, W1, inv), p = idx - l * W1;
+    W[o + idx] = lds[p * LDW + l] + K[o + idx];
   }
 }
 
@@ -611,57 +686,64 @@ __global__ void k_energy(EnergyTables et, int nbox, double* ekin, double* epot) 
 // ---------------------------------------------------------------------------
 // host-side launcher of the whole FP64 sequence
 // ---------------------------------------------------------------------------
-size_t lds_bytes_fp64(int maxN) {
-  return (size_t)2 * (maxN + HL + HR) * LDW * sizeof(double);
+size_t lds_bytes(int maxN, int elem) {
+  return (size_t)2 * (maxN + HL + HR) * LDW * elem;
 }
 
-hipError_t launch_fp64(const Batch& B, const PassTables& T, int maxN,
-                       long long wtotal, bool need_zero, hipStream_t s,
-                       hipEvent_t* ev) {
-  const size_t lds = lds_bytes_fp64(maxN);
+template <typename T>
+static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
+                             long long wtotal, bool need_zero, hipStream_t s,
+                             hipEvent_t* ev) {
+  const size_t lds = lds_bytes(maxN, sizeof(T));
   auto mark = [&](int stage, int end) {
     if (ev) (void)hipEventRecord(ev[2 * stage + end], s);
   };
   mark(S_ZERO, 0);
   if (need_zero) {
     const long long nb = (wtotal + NT - 1) / NT;
-    k_zero<<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>(B.W, wtotal);
+    k_zero<T><<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>((T*)B.W, wtotal);
   }
   mark(S_ZERO, 1);
-#define BDFT_STAGE(S, P, SH, KERNEL, ...)                    \
-  mark(S, 0);                                                \
-  if (T.ntiles[P] > 0) KERNEL<<<T.ntiles[P], NT, SH, s>>>(__VA_ARGS__); \
+#define BDFT_STAGE(S, P, SH, KERNEL, ...)                              \
+  mark(S, 0);                                                          \
+  if (PT.ntiles[P] > 0) KERNEL<<<PT.ntiles[P], NT, SH, s>>>(__VA_ARGS__); \
   mark(S, 1);
-  BDFT_STAGE(S_UNC, P_UNC, 0, k_uncompress<false>, B, T.t[P_UNC])
-  BDFT_STAGE(S_XF, P_XF, lds, k_xfwd, B, T.t[P_XF], T.part[0])
-  BDFT_STAGE(S_YF, P_YF, lds, k_yfwd, B, T.t[P_YF])
-  BDFT_STAGE(S_KY, P_KY, lds, k_kin<1>, B, T.t[P_KY], T.part[1])
-  BDFT_STAGE(S_KZ, P_KZ, lds, k_kin<2>, B, T.t[P_KZ], T.part[2])
-  BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot, B, T.t[P_ZP], T.part[3])
-  BDFT_STAGE(S_YB, P_YB, lds, k_ybwd, B, T.t[P_YB])
-  BDFT_STAGE(S_XB, P_XB, lds, k_xbwd, B, T.t[P_XB])
-  BDFT_STAGE(S_COMP, P_UNC, 0, k_uncompress<true>, B, T.t[P_UNC])
+  BDFT_STAGE(S_UNC, P_UNC, 0, (k_uncompress<T, false>), B, PT.t[P_UNC])
+  BDFT_STAGE(S_XF, P_XF, lds, k_xfwd<T>, B, PT.t[P_XF], PT.part[0])
+  BDFT_STAGE(S_YF, P_YF, lds, k_yfwd<T>, B, PT.t[P_YF])
+  BDFT_STAGE(S_KY, P_KY, lds, (k_kin<T, 1>), B, PT.t[P_KY], PT.part[1])
+  BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
+  BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot<T>, B, PT.t[P_ZP], PT.part[3])
+  BDFT_STAGE(S_YB, P_YB, lds, k_ybwd<T>, B, PT.t[P_YB])
+  BDFT_STAGE(S_XB, P_XB, lds, k_xbwd<T>, B, PT.t[P_XB])
+  BDFT_STAGE(S_COMP, P_UNC, 0, (k_uncompress<T, true>), B, PT.t[P_UNC])
 #undef BDFT_STAGE
   EnergyTables et;
-  et.t[0] = T.t[P_XF];
-  et.t[1] = T.t[P_KY];
-  et.t[2] = T.t[P_KZ];
-  et.t[3] = T.t[P_ZP];
-  for (int i = 0; i < 4; ++i) et.p[i] = T.part[i];
+  et.t[0] = PT.t[P_XF];
+  et.t[1] = PT.t[P_KY];
+  et.t[2] = PT.t[P_KZ];
+  et.t[3] = PT.t[P_ZP];
+  for (int i = 0; i < 4; ++i) et.p[i] = PT.part[i];
   mark(S_ENERGY, 0);
   k_energy<<<(B.nbox + 127) / 128, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);
   mark(S_ENERGY, 1);
   return hipGetLastError();
 }
 
+hipError_t launch_passes(const Batch& B, const PassTables& T, int maxN, long long wtotal,
+                         bool need_zero, int fp32, hipStream_t s, hipEvent_t* ev) {
+  return fp32 ? launch_seq<float>(B, T, maxN, wtotal, need_zero, s, ev)
+              : launch_seq<double>(B, T, maxN, wtotal, need_zero, s, ev);
+}
+
 hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
                            long long wtotal, bool compress, hipStream_t s) {
   if (!compress) {
     const long long nb = (wtotal + NT - 1) / NT;
-    k_zero<<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>(B.W, wtotal);
-    k_uncompress<false><<<ntiles, NT, 0, s>>>(B, t);
+    k_zero<double><<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>((double*)B.W, wtotal);
+    k_uncompress<double, false><<<ntiles, NT, 0, s>>>(B, t);
   } else {
-    k_uncompress<true><<<ntiles, NT, 0, s>>>(B, t);
+    k_uncompress<double, true><<<ntiles, NT, 0, s>>>(B, t);
   }
   return hipGetLastError();
 }

@@ -260,3 +260,42 @@ def test_descriptor_errors_are_reported(ctx):
                    run, run.clone(), ek, ek.clone())
     assert e.value.code == 5
     lr.close()
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (float working type; FP64 input, potential, energies and
+# hpsi accumulation).  Bars: ekin, epot within 1e-6 relative of the FP64
+# oracle; hpsi within 2e-6 of max|hpsi| per element.
+# ---------------------------------------------------------------------------
+MIXED_E = 1e-6
+MIXED_H = 2e-6
+
+
+@pytest.mark.parametrize("geo,n,seed", [("P", (7, 6, 8), 1), ("P", (23, 23, 23), 2),
+                                        ("F", (11, 9, 10), 3), ("F", (15, 15, 15), 4)])
+def test_mixed_fp32(ctx, geo, n, seed):
+    reg = dense_region(geo, n, (0.45, 0.4, 0.5))
+    psi = random_psi(reg, 2, 200 + seed)
+    pot = random_pot(reg, 300 + seed)
+    hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=1)
+    ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    print(f"mixed {geo} {n}: hpsi {np.abs(hg - ho).max() / np.abs(ho).max():.2e} "
+          f"ekin {np.abs(eg - eo).max() / np.abs(eo).max():.2e} "
+          f"epot {np.abs(pg - po).max() / np.abs(po).max():.2e}")
+    assert np.abs(hg - ho).max() < MIXED_H * np.abs(ho).max()
+    assert np.all(np.abs(eg - eo) < MIXED_E * np.abs(eo))
+    assert np.all(np.abs(pg - po) < MIXED_E * np.abs(po))
+
+
+def test_mixed_sphere_batch(ctx):
+    regs = [mixed_sphere_region((14, 12, 16), 0.4, s) for s in (5, 6)]
+    for i, reg in enumerate(regs):
+        psi = random_psi(reg, 1, 400 + i)
+        pot = random_pot(reg, 500 + i)
+        hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=1)
+        ho, eo, po = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                                   okeys(reg), pot, psi)
+        assert np.abs(hg - ho).max() < MIXED_H * np.abs(ho).max()
+        assert abs(eg[0] - eo[0]) < MIXED_E * abs(eo[0])
+        assert abs(pg[0] - po[0]) < MIXED_E * abs(po[0])
