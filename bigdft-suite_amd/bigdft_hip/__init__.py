@@ -34,7 +34,7 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_plan_create", "bdft_plan_execute", "bdft_plan_destroy",
            "bdft_local_hamiltonian", "bdft_uncompress_d",
            "bdft_compress_acc_d", "bdft_profile_enable", "bdft_profile_read",
-           "bdft_stage_name", "bdft_sync")
+           "bdft_stage_name", "bdft_ctx_info", "bdft_sync")
 
 
 class BigDFTError(RuntimeError):
@@ -72,6 +72,7 @@ def load_library(path=LIB_PATH):
         "bdft_sync": [P],
         "bdft_profile_enable": [P, I],
         "bdft_profile_read": [P, I, P, P, I],
+        "bdft_ctx_info": [P, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -188,6 +189,13 @@ class Context:
 
     def sync(self):
         self._check(lib().bdft_sync(self.handle))
+
+    def info(self):
+        """(LDS bytes per work-group, plane sequence enabled)"""
+        lds, pl = ctypes.c_int64(), ctypes.c_int()
+        self._check(lib().bdft_ctx_info(self.handle, ctypes.byref(lds),
+                                        ctypes.byref(pl)))
+        return lds.value, bool(pl.value)
 
     def profile(self, on=True):
         self._check(lib().bdft_profile_enable(self.handle, int(on)))

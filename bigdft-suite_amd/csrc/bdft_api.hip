@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,6 +34,8 @@ struct bdft_ctx {
   std::vector<std::vector<hipEvent_t>> ev_pending; // one set per launch
   double prof_ms[S_COUNT] = {0};
   long long prof_n = 0;
+  bool force_lines = false;  // disable the plane-fused sequence
+  size_t plane_lds = 0;      // LDS bytes the plane kernels may use
 };
 
 struct bdft_lr {
@@ -44,6 +47,8 @@ struct bdft_lr {
   int nvctr_c = 0, nvctr_f = 0;
   int* d_cmap = nullptr;
   int* d_fmap = nullptr;
+  int* d_cinv = nullptr;  // null when the coarse map is the identity
+  int* d_finv = nullptr;  // null when the fine map is the identity
   bool dense = false;
 };
 
@@ -124,6 +129,9 @@ extern "C" int bdft_ctx_create(int device, bdft_ctx** out) {
     return BDFT_ERR_HIP;
   }
   ctx->stream = ctx->own;
+  ctx->plane_lds = plane_attributes();
+  const char* env = getenv("BDFT_LINE_PASSES");
+  ctx->force_lines = env && env[0] == '1';
   *out = ctx;
   return BDFT_OK;
 }
@@ -270,6 +278,26 @@ extern "C" int bdft_lr_create(bdft_ctx* ctx, char geocode, const int n[3],
     HIPCHK(ctx, hipMemcpy(lr.d_fmap, fmap.data(), sizeof(int) * nvctr_f,
                           hipMemcpyHostToDevice));
   }
+  // inverse maps (grid point -> coefficient index, -1 if absent), skipped
+  // when every point is present in grid order
+  auto inverse = [&](const std::vector<int>& map, int*& dptr) -> int {
+    bool ident = (long long)map.size() == M;
+    for (size_t v = 0; ident && v < map.size(); ++v) ident = map[v] == (int)v;
+    if (ident) return BDFT_OK;
+    std::vector<int> inv((size_t)M, -1);
+    for (size_t v = 0; v < map.size(); ++v) inv[map[v]] = (int)v;
+    HIPCHK(ctx, hipMalloc(&dptr, sizeof(int) * M));
+    HIPCHK(ctx, hipMemcpy(dptr, inv.data(), sizeof(int) * M, hipMemcpyHostToDevice));
+    return BDFT_OK;
+  };
+  rc = inverse(cmap, lr.d_cinv);
+  if (!rc) rc = inverse(fmap, lr.d_finv);
+  if (rc) {
+    if (lr.d_cmap) hipFree(lr.d_cmap);
+    if (lr.d_fmap) hipFree(lr.d_fmap);
+    if (lr.d_cinv) hipFree(lr.d_cinv);
+    return rc;
+  }
   *out = new bdft_lr(lr);
   return BDFT_OK;
 }
@@ -287,6 +315,8 @@ extern "C" int bdft_lr_destroy(bdft_lr* lr) {
   }
   if (lr->d_cmap) hipFree(lr->d_cmap);
   if (lr->d_fmap) hipFree(lr->d_fmap);
+  if (lr->d_cinv) hipFree(lr->d_cinv);
+  if (lr->d_finv) hipFree(lr->d_finv);
   delete lr;
   return BDFT_OK;
 }
@@ -315,6 +345,8 @@ static int box_tiles(const BoxDesc& b, int pass) {
     case P_KY: return cdiv(2 * N1, TW) * 2 * N3;
     case P_KZ: return cdiv(2 * N1, TW) * 2 * N2;
     case P_ZP: return cdiv(L1, TW) * L2;
+    case P_PF:
+    case P_PB: return 2 * N3;
   }
   return 0;
 }
@@ -360,6 +392,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     b.nvctr_f = lr->nvctr_f;
     b.cmap = lr->d_cmap;
     b.fmap = lr->d_fmap;
+    b.cinv = lr->d_cinv;
+    b.finv = lr->d_finv;
     b.w_off = wtot;
     b.t1_off = t1tot;
     b.t2_off = t2tot;
@@ -399,6 +433,23 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       starts.push_back(acc);
     }
   }
+  // plane-fused sequence when every box qualifies: periodic axes need at
+  // least 18 coarse points (single wrap of the in-plane stencils) and both
+  // plane kernels must fit in LDS
+  {
+    bool plane = !ctx->force_lines;
+    size_t lpf = 0, lpb = 0;
+    for (int i = 0; i < norb && plane; ++i) {
+      const BoxDesc& b = box[i];
+      if (b.per && (b.N[0] < 18 || b.N[1] < 18)) plane = false;
+      lpf = std::max(lpf, plane_lds_fwd(b.N[0], b.N[1], b.L[0], (int)esz));
+      lpb = std::max(lpb, plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)esz));
+    }
+    if (std::max(lpf, lpb) > ctx->plane_lds) plane = false;
+    T.plane = plane ? 1 : 0;
+    T.lds_pf = lpf;
+    T.lds_pb = lpb;
+  }
   bdft_plan* plan = new bdft_plan();
   plan->ctx = ctx;
   plan->nbox = norb;
@@ -410,7 +461,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // metadata: boxes | starts | partials
   const size_t nb_box = sizeof(BoxDesc) * norb;
   const size_t nb_st = sizeof(int) * starts.size();
-  const int epass[4] = {P_XF, P_KY, P_KZ, P_ZP};
+  int epass[4] = {P_XF, P_KY, P_KZ, P_ZP};
+  if (T.plane) epass[0] = P_PF;
   size_t nb_part = 0;
   for (int e = 0; e < 4; ++e) nb_part += sizeof(double) * T.ntiles[epass[e]];
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -440,6 +492,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     }
   }
   // workspace: W | K | T1 | T2
+  if (T.plane) t1tot = 0;
   const size_t ws_bytes = esz * (2 * al(wtot) + al(t1tot) + al(t2tot));
   DevBuf& ws = own_ws ? plan->ws : ctx->ws;
   rc = dev_alloc(ctx, ws, ws_bytes);
@@ -717,8 +770,9 @@ extern "C" int bdft_compress_acc_d(bdft_ctx* ctx, const bdft_lr* lr, int norb,
 // stage profiling with HIP events
 // ---------------------------------------------------------------------------
 static const char* kStageNames[S_COUNT] = {
-    "zero",   "uncompress", "x_grow_kin", "y_grow", "y_kin",   "z_kin",
-    "z_pot",  "y_shrink",   "x_shrink",   "compress", "energy"};
+    "zero",   "uncompress", "x_grow_kin", "y_grow", "y_kin",    "z_kin",
+    "z_pot",  "y_shrink",   "x_shrink",   "compress", "energy", "xy_fwd",
+    "xy_bwd_compress"};
 
 extern "C" int bdft_profile_enable(bdft_ctx* ctx, int on) {
   if (!ctx) return BDFT_ERR_ARG;
@@ -747,6 +801,13 @@ extern "C" int bdft_profile_read(bdft_ctx* ctx, int nstage, double* ms, int64_t*
     for (int s = 0; s < S_COUNT; ++s) ctx->prof_ms[s] = 0.0;
     ctx->prof_n = 0;
   }
+  return BDFT_OK;
+}
+
+extern "C" int bdft_ctx_info(const bdft_ctx* ctx, int64_t* lds_per_block, int* plane_enabled) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (lds_per_block) *lds_per_block = (int64_t)ctx->plane_lds;
+  if (plane_enabled) *plane_enabled = ctx->force_lines ? 0 : 1;
   return BDFT_OK;
 }
 

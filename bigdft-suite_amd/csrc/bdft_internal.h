@@ -41,6 +41,9 @@ struct BoxDesc {
   long long psi_off;    // offset of this orbital's compressed coefficients
   const int* cmap;      // coarse points: dense linear index i1+N1*(i2+N2*i3)
   const int* fmap;      // fine points: same encoding
+  const int* cinv;      // grid point -> coarse coefficient index or -1
+  const int* finv;      // grid point -> fine point index or -1
+                        // (null: identity, every point present in key order)
 };
 
 // Tile bookkeeping for one pass: tiles of box b are [start[b], start[b+1]).
@@ -181,19 +184,31 @@ struct Batch {
 };
 
 // Passes of one application; each has its own tile table.
-enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_COUNT };
+enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_PF, P_PB,
+            P_COUNT };
 
 struct PassTables {
   TileTable t[P_COUNT];
   int ntiles[P_COUNT];   // total tiles (grid size) of each pass
-  double* part[4];       // energy partials: xfwd, kin_y, kin_z, zpot
+  double* part[4];       // energy partials: xfwd|plane_fwd, kin_y, kin_z, zpot
+  int plane;             // 1: plane-fused sequence, 0: line passes
+  size_t lds_pf, lds_pb; // LDS bytes of the plane kernels
 };
 
 // Launch stages in order; profiling records one event pair per stage.
 enum Stage { S_ZERO = 0, S_UNC, S_XF, S_YF, S_KY, S_KZ, S_ZP, S_YB, S_XB,
-             S_COMP, S_ENERGY, S_COUNT };
+             S_COMP, S_ENERGY, S_PF, S_PB, S_COUNT };
 
 size_t lds_bytes(int maxN, int elem);
+size_t plane_lds_fwd(int N1, int N2, int L1, int elem);
+size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem);
+size_t plane_attributes();
+template <typename T>
+hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                            double* part, hipStream_t s);
+template <typename T>
+hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                            hipStream_t s);
 hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
                            long long wtotal, bool compress, hipStream_t s);
 // ev: null, or 2*S_COUNT events recorded before/after each stage;

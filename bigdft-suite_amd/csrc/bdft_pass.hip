@@ -51,6 +51,10 @@ __device__ __forceinline__ int qdiv(int a, int b, float inv) {
 // Wavelet line bundle: two sections (c, d) of NS = N + HL + HR rows; row r of
 // a section holds coarse index k = r - HL (wrapped or zero outside [0,N)).
 // CONTIG: lines are contiguous in memory (x pass), else positions are.
+// Loads are issued in batches of UNR per thread before their LDS stores, so
+// that many global loads are in flight at once.
+constexpr int UNR = 8;
+
 template <bool CONTIG, typename T, typename S>
 __device__ __forceinline__ void fill_wav(T* lds, const S* __restrict__ src,
                                          long long lstride, long long pstride,
@@ -59,22 +63,30 @@ __device__ __forceinline__ void fill_wav(T* lds, const S* __restrict__ src,
   const int rows = 2 * NS;
   if (CONTIG) {
     // lines contiguous (pstride 1, lstride = 2N): read the tile as one flat
-    // run of nl*2N elements, then fill the halo rows from LDS
+    // run of TW*2N elements (lines >= nl read as zero), then fill the halo
+    // rows from LDS
     const int W = 2 * N;
+    const int tot = TW * W;
     const float inv = 1.0f / (float)W;
-    for (int idx = threadIdx.x; idx < nl * W; idx += NT) {
-      const int l = qdiv(idx, W, inv);
-      const int pos = idx - l * W;
-      const int sec = pos >= N;
-      const int row = sec * NS + (pos - sec * N) + HL;
-      lds[row * LDW + l] = (T)src[(long long)l * lstride + pos];
+    for (int base = threadIdx.x; base < tot; base += NT * UNR) {
+      T v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        v[u] = (idx < nl * W) ? (T)src[idx] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        if (idx < tot) {
+          const int l = qdiv(idx, W, inv);
+          const int pos = idx - l * W;
+          const int sec = pos >= N;
+          lds[(sec * NS + (pos - sec * N) + HL) * LDW + l] = v[u];
+        }
+      }
     }
-    for (int idx = threadIdx.x; idx < (TW - nl) * W; idx += NT) {
-      const int l = nl + qdiv(idx, W, inv);
-      const int pos = idx - (l - nl) * W;
-      const int sec = pos >= N;
-      lds[(sec * NS + (pos - sec * N) + HL) * LDW + l] = (T)0;
-    }
+    (void)lstride;
     __syncthreads();
     const int H = HL + HR;
     for (int idx = threadIdx.x; idx < 2 * H * TW; idx += NT) {
@@ -87,16 +99,26 @@ __device__ __forceinline__ void fill_wav(T* lds, const S* __restrict__ src,
       lds[(sec * NS + k + HL) * LDW + l] = v;
     }
   } else {
-    for (int idx = threadIdx.x; idx < rows * TW; idx += NT) {
-      const int row = idx >> 5, l = idx & (TW - 1);
-      const int sec = row >= NS;
-      int k = row - sec * NS - HL;
-      bool ok = l < nl;
-      if (per) k = wrap(k, N);
-      else ok = ok && k >= 0 && k < N;
-      T v = (T)0;
-      if (ok) v = (T)src[(long long)l * lstride + (long long)(sec * N + k) * pstride];
-      lds[row * LDW + l] = v;
+    const int tot = rows * TW;
+    for (int base = threadIdx.x; base < tot; base += NT * UNR) {
+      T v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        const int row = idx >> 5, l = idx & (TW - 1);
+        const int sec = row >= NS;
+        int k = row - sec * NS - HL;
+        bool ok = l < nl && idx < tot;
+        if (per) k = wrap(k, N);
+        else ok = ok && k >= 0 && k < N;
+        v[u] = ok ? (T)src[(long long)l * lstride + (long long)(sec * N + k) * pstride]
+                  : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        if (idx < tot) lds[(idx >> 5) * LDW + (idx & (TW - 1))] = v[u];
+      }
     }
   }
 }
@@ -109,17 +131,26 @@ __device__ __forceinline__ void fill_real(T* lds, const T* __restrict__ src,
                                           int nl, int N, int L, int per) {
   const int Q = 2 * N + 36;
   if (CONTIG) {
+    const int tot = TW * L;
     const float inv = 1.0f / (float)L;
-    for (int idx = threadIdx.x; idx < nl * L; idx += NT) {
-      const int l = qdiv(idx, L, inv);
-      const int p = idx - l * L;
-      lds[(per ? p + 14 : p) * LDW + l] = src[(long long)l * lstride + p];
+    for (int base = threadIdx.x; base < tot; base += NT * UNR) {
+      T v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        v[u] = (idx < nl * L) ? src[idx] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        if (idx < tot) {
+          const int l = qdiv(idx, L, inv);
+          const int p = idx - l * L;
+          lds[(per ? p + 14 : p) * LDW + l] = v[u];
+        }
+      }
     }
-    for (int idx = threadIdx.x; idx < (TW - nl) * L; idx += NT) {
-      const int l = nl + qdiv(idx, L, inv);
-      const int p = idx - (l - nl) * L;
-      lds[(per ? p + 14 : p) * LDW + l] = (T)0;
-    }
+    (void)lstride;
     __syncthreads();
     // rows not covered above: periodic halos [0,14) and [2N+14,Q), free tail
     const int nh = per ? Q - 2 * N : Q - L;
@@ -136,18 +167,27 @@ __device__ __forceinline__ void fill_real(T* lds, const T* __restrict__ src,
       lds[q * LDW + l] = v;
     }
   } else {
-    for (int idx = threadIdx.x; idx < Q * TW; idx += NT) {
-      const int q = idx >> 5, l = idx & (TW - 1);
-      int pos;
-      bool ok = l < nl;
-      if (per) pos = wrap(q - 14, 2 * N);
-      else {
-        pos = q;
-        ok = ok && q < L;
+    const int tot = Q * TW;
+    for (int base = threadIdx.x; base < tot; base += NT * UNR) {
+      T v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        const int q = idx >> 5, l = idx & (TW - 1);
+        int pos;
+        bool ok = l < nl && idx < tot;
+        if (per) pos = wrap(q - 14, 2 * N);
+        else {
+          pos = q;
+          ok = ok && q < L;
+        }
+        v[u] = ok ? src[(long long)l * lstride + (long long)pos * pstride] : (T)0;
       }
-      T v = (T)0;
-      if (ok) v = src[(long long)l * lstride + (long long)pos * pstride];
-      lds[q * LDW + l] = v;
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int idx = base + u * NT;
+        if (idx < tot) lds[(idx >> 5) * LDW + (idx & (TW - 1))] = v[u];
+      }
     }
   }
 }
@@ -653,9 +693,22 @@ __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
   T* W = (T*)B.W;
   const T* K = (const T*)B.K;
   const float inv = 1.0f / (float)W1;
-  for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
-    const int l = qdiv(idx, W1, inv), p = idx - l * W1;
-    W[o + idx] = lds[p * LDW + l] + K[o + idx];
+  const int tot = nl * W1;
+  for (int base = threadIdx.x; base < tot; base += NT * UNR) {
+    T kv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int idx = base + u * NT;
+      kv[u] = idx < tot ? K[o + idx] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int idx = base + u * NT;
+      if (idx < tot) {
+        const int l = qdiv(idx, W1, inv), p = idx - l * W1;
+        W[o + idx] = lds[p * LDW + l] + kv[u];
+      }
+    }
   }
 }
 
@@ -663,24 +716,37 @@ __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
 // energies: fixed-order sums of the per-tile partials of each box
 // ---------------------------------------------------------------------------
 struct EnergyTables {
-  TileTable t[4];       // xfwd, kin_y, kin_z (ekin) ; zpot (epot)
-  const double* p[4];   // partial arrays
+  TileTable t[4];       // partial sums of ekin (first nk entries) then epot
+  const double* p[4];
+  int nk, n;
 };
 
-__global__ void k_energy(EnergyTables et, int nbox, double* ekin, double* epot) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbox) return;
-  double e[4];
-  for (int s = 0; s < 4; ++s) {
+// one work-group per box; fixed summation order (strided per thread, then a
+// fixed tree) so the energies are reproducible run to run
+__global__ __launch_bounds__(128) void k_energy(EnergyTables et, int nbox, double* ekin,
+                                                double* epot) {
+  __shared__ double sh[128];
+  const int b = blockIdx.x;
+  double e[2] = {0.0, 0.0};
+  for (int s = 0; s < et.n; ++s) {
     const TileTable& tt = et.t[s];
     const int t0 = box_first_tile(tt, b);
     const int t1 = tt.uniform > 0 ? t0 + tt.uniform : tt.start[b + 1];
     double acc = 0.0;
-    for (int t = t0; t < t1; ++t) acc += et.p[s][t];
-    e[s] = acc;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += 128) acc += et.p[s][t];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 64; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) e[s < et.nk ? 0 : 1] += sh[0];
+    __syncthreads();
   }
-  ekin[b] = e[0] + e[1] + e[2];
-  epot[b] = e[3];
+  if (threadIdx.x == 0) {
+    ekin[b] = e[0];
+    epot[b] = e[1];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -709,24 +775,55 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
   if (PT.ntiles[P] > 0) KERNEL<<<PT.ntiles[P], NT, SH, s>>>(__VA_ARGS__); \
   mark(S, 1);
   BDFT_STAGE(S_UNC, P_UNC, 0, (k_uncompress<T, false>), B, PT.t[P_UNC])
-  BDFT_STAGE(S_XF, P_XF, lds, k_xfwd<T>, B, PT.t[P_XF], PT.part[0])
-  BDFT_STAGE(S_YF, P_YF, lds, k_yfwd<T>, B, PT.t[P_YF])
-  BDFT_STAGE(S_KY, P_KY, lds, (k_kin<T, 1>), B, PT.t[P_KY], PT.part[1])
-  BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
-  BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot<T>, B, PT.t[P_ZP], PT.part[3])
-  BDFT_STAGE(S_YB, P_YB, lds, k_ybwd<T>, B, PT.t[P_YB])
-  BDFT_STAGE(S_XB, P_XB, lds, k_xbwd<T>, B, PT.t[P_XB])
-  BDFT_STAGE(S_COMP, P_UNC, 0, (k_uncompress<T, true>), B, PT.t[P_UNC])
-#undef BDFT_STAGE
   EnergyTables et;
-  et.t[0] = PT.t[P_XF];
-  et.t[1] = PT.t[P_KY];
-  et.t[2] = PT.t[P_KZ];
-  et.t[3] = PT.t[P_ZP];
-  for (int i = 0; i < 4; ++i) et.p[i] = PT.part[i];
+  if (PT.plane) {
+    mark(S_PF, 0);
+    if (PT.ntiles[P_PF] > 0)
+      launch_plane_fwd<T>(B, PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pf, PT.part[0], s);
+    mark(S_PF, 1);
+    BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
+    BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot<T>, B, PT.t[P_ZP], PT.part[3])
+    mark(S_PB, 0);
+    if (PT.ntiles[P_PB] > 0)
+      launch_plane_bwd<T>(B, PT.t[P_PB], PT.ntiles[P_PB], PT.lds_pb, s);
+    mark(S_PB, 1);
+    et.t[0] = PT.t[P_PF];
+    et.p[0] = PT.part[0];
+    et.t[1] = PT.t[P_KZ];
+    et.p[1] = PT.part[2];
+    et.t[2] = PT.t[P_ZP];
+    et.p[2] = PT.part[3];
+    et.nk = 2;
+    et.n = 3;
+  } else {
+    BDFT_STAGE(S_XF, P_XF, lds, k_xfwd<T>, B, PT.t[P_XF], PT.part[0])
+    BDFT_STAGE(S_YF, P_YF, lds, k_yfwd<T>, B, PT.t[P_YF])
+    BDFT_STAGE(S_KY, P_KY, lds, (k_kin<T, 1>), B, PT.t[P_KY], PT.part[1])
+    BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
+    BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot<T>, B, PT.t[P_ZP], PT.part[3])
+    BDFT_STAGE(S_YB, P_YB, lds, k_ybwd<T>, B, PT.t[P_YB])
+    BDFT_STAGE(S_XB, P_XB, lds, k_xbwd<T>, B, PT.t[P_XB])
+    BDFT_STAGE(S_COMP, P_UNC, 0, (k_uncompress<T, true>), B, PT.t[P_UNC])
+    et.t[0] = PT.t[P_XF];
+    et.t[1] = PT.t[P_KY];
+    et.t[2] = PT.t[P_KZ];
+    et.t[3] = PT.t[P_ZP];
+    for (int i = 0; i < 4; ++i) et.p[i] = PT.part[i];
+    et.nk = 3;
+    et.n = 4;
+  }
+#undef BDFT_STAGE
   mark(S_ENERGY, 0);
-  k_energy<<<(B.nbox + 127) / 128, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);
+  k_energy<<<B.nbox, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);
   mark(S_ENERGY, 1);
+  if (ev) {  // stages of the other sequence: empty event pairs
+    const int unused_plane[] = {S_XF, S_YF, S_KY, S_YB, S_XB, S_COMP};
+    const int unused_lines[] = {S_PF, S_PB};
+    if (PT.plane)
+      for (int st : unused_plane) { mark(st, 0); mark(st, 1); }
+    else
+      for (int st : unused_lines) { mark(st, 0); mark(st, 1); }
+  }
   return hipGetLastError();
 }
 

@@ -1,0 +1,466 @@
+// Plane-fused kernels: one work-group holds a whole x-y plane of one box in
+// LDS and applies both in-plane axes there.
+//
+//   k_plane_fwd  plane j3 of W (wavelet x,y at wavelet z index j3):
+//                K  = K_x W + K_y W              (kinetic, + ekin partial)
+//                T2 = G_y G_x W                  (real x,y at wavelet z)
+//   k_plane_bwd  plane j3 of T2 after the z pass (G_z^T V G_z):
+//                out = G_x^T G_y^T T2 + K        -> hpsi += compress(out)
+//
+// Together with the z-line kernels (k_zpot, k_kin<2>) this replaces the
+// x/y line passes and the compress pass: the (L1,2N2,2N3) intermediate and
+// the separate wavelet output volume never reach memory.  Operators and
+// filters are those of the line kernels (growshrink.f90 comb_grow/comb_shrink,
+// convolut_ib_optim.f90 ConvolkineticT); periodic wrap and free zero padding
+// are applied by index arithmetic, with a branch-free path for chunks whose
+// stencil lies inside the line.
+#include "bdft_internal.h"
+
+namespace bdft {
+
+constexpr int NTP = 512;  // threads per plane work-group
+
+template <typename T>
+struct PLine {
+  const T* p;  // element 0 of the line
+  int es;      // element stride
+};
+
+// G on a wavelet line [c(0..N-1); d(0..N-1)]: 8 fine outputs from
+// k = k0 .. k0+18 (k0 = first coarse index of the chunk).
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ void grow8p(PLine<T> ln, int N, int k0, int per, T o[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) o[r] = (T)0;
+#pragma unroll
+  for (int kk = 0; kk < 19; ++kk) {
+    const int k = k0 + kk;
+    T c, d;
+    if (INTERIOR) {
+      c = ln.p[k * ln.es];
+      d = ln.p[(N + k) * ln.es];
+    } else if (per) {
+      const int kw = k < 0 ? k + N : (k >= N ? k - N : k);
+      c = ln.p[kw * ln.es];
+      d = ln.p[(N + kw) * ln.es];
+    } else {
+      const bool v = (unsigned)k < (unsigned)N;
+      const int kc = v ? k : 0;
+      c = ln.p[kc * ln.es];
+      d = ln.p[(N + kc) * ln.es];
+      c = v ? c : (T)0;
+      d = v ? d : (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int j = r + 16 - 2 * kk;
+      if (j >= -14 && j <= 16) {
+        o[r] = fma((T)Fil::F2S[j + 14], c, o[r]);
+        o[r] = fma((T)Fil::F2W[j + 14], d, o[r]);
+      }
+    }
+  }
+}
+
+// wavelet-basis kinetic (unscaled) on a wavelet line: 4 coarse outputs
+// i0..i0+3 (c and d) from k = i0-14 .. i0+17.
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ void kin4p(PLine<T> ln, int N, int i0, int per, T oc[4],
+                                      T od[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    oc[r] = (T)0;
+    od[r] = (T)0;
+  }
+#pragma unroll
+  for (int kk = 0; kk < 32; ++kk) {
+    const int k = i0 - 14 + kk;
+    T c, d;
+    if (INTERIOR) {
+      c = ln.p[k * ln.es];
+      d = ln.p[(N + k) * ln.es];
+    } else if (per) {
+      int kw = k;
+      while (kw < 0) kw += N;
+      while (kw >= N) kw -= N;
+      c = ln.p[kw * ln.es];
+      d = ln.p[(N + kw) * ln.es];
+    } else {
+      const bool v = (unsigned)k < (unsigned)N;
+      const int kc = v ? k : 0;
+      c = ln.p[kc * ln.es];
+      d = ln.p[(N + kc) * ln.es];
+      c = v ? c : (T)0;
+      d = v ? d : (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int l = kk - 14 - r;
+      if (l >= -14 && l <= 14) {
+        const int al = l < 0 ? -l : l;
+        oc[r] = fma((T)Fil::KA[al], c, oc[r]);
+        oc[r] = fma((T)Fil::KC[-l + 14], d, oc[r]);
+        od[r] = fma((T)Fil::KC[l + 14], c, od[r]);
+        od[r] = fma((T)Fil::KE[al], d, od[r]);
+      }
+    }
+  }
+}
+
+// G^T on a real line of L points (periodic: fine index p, L = 2N; free: fine
+// index p-14): 4 coarse outputs i0..i0+3 (c and d) from fine I = 2i0-14+qq.
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ void shrink4p(PLine<T> ln, int N, int L, int i0, int per,
+                                         T oc[4], T od[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    oc[r] = (T)0;
+    od[r] = (T)0;
+  }
+#pragma unroll
+  for (int qq = 0; qq < 37; ++qq) {
+    T x;
+    if (INTERIOR) {
+      x = ln.p[(2 * i0 + qq - (per ? 14 : 0)) * ln.es];
+    } else if (per) {
+      int p = 2 * i0 - 14 + qq;
+      p = p < 0 ? p + L : (p >= L ? p - L : p);
+      x = ln.p[p * ln.es];
+    } else {
+      const int p = 2 * i0 + qq;
+      const bool v = p < L;
+      x = ln.p[(v ? p : 0) * ln.es];
+      x = v ? x : (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = qq - 2 * r - 14;
+      if (j >= -14 && j <= 16) {
+        oc[r] = fma((T)Fil::F2S[j + 14], x, oc[r]);
+        od[r] = fma((T)Fil::F2W[j + 14], x, od[r]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double block_sum_p(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NTP / 64; ++i) s += red[i];
+  return s;
+}
+
+// first coarse index of the grow chunk starting at fine output p0
+__device__ __forceinline__ int grow_k0(int p0, int per) {
+  return per ? (p0 >> 1) - 8 : (p0 >> 1) - 15;
+}
+
+template <typename T>
+__device__ __forceinline__ void grow8_any(PLine<T> ln, int N, int k0, int per, T o[8]) {
+  if (k0 >= 0 && k0 + 18 < N) grow8p<T, true>(ln, N, k0, per, o);
+  else grow8p<T, false>(ln, N, k0, per, o);
+}
+template <typename T>
+__device__ __forceinline__ void kin4_any(PLine<T> ln, int N, int i0, int per, T oc[4],
+                                         T od[4]) {
+  if (i0 - 14 >= 0 && i0 + 17 < N) kin4p<T, true>(ln, N, i0, per, oc, od);
+  else kin4p<T, false>(ln, N, i0, per, oc, od);
+}
+template <typename T>
+__device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, int per,
+                                            T oc[4], T od[4]) {
+  const bool in = per ? (2 * i0 - 14 >= 0 && 2 * i0 + 22 < L) : (2 * i0 + 36 < L);
+  if (in) shrink4p<T, true>(ln, N, L, i0, per, oc, od);
+  else shrink4p<T, false>(ln, N, L, i0, per, oc, od);
+}
+
+// ---------------------------------------------------------------------------
+// forward plane kernel
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTP) void k_plane_fwd(Batch B, TileTable tt, double* part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NTP / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int j3 = tile - box_first_tile(tt, b);
+  const int N1 = bd.N[0], N2 = bd.N[1], L1 = bd.L[0], L2 = bd.L[1], per = bd.per;
+  const int W1 = 2 * N1, W2 = 2 * N2;
+  const int RA = W1 + 1;            // row stride of the wavelet plane
+  const int RB = L1 + 1;            // row stride of the G_x output
+  T* A = (T*)smem;                  // W plane: A[j2*RA + j1]
+  T* C = A + W2 * RA;               // kinetic accumulator, later G_x output
+  const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
+  const int tid = threadIdx.x;
+  // load the plane (contiguous), UNR loads in flight per thread
+  {
+    const int tot = W1 * W2;
+    const float inv = 1.0f / (float)W1;
+    for (int base = tid; base < tot; base += NTP * 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + u * NTP;
+        v[u] = i < tot ? src[i] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + u * NTP;
+        if (i < tot) {
+          int r = (int)((float)i * inv);
+          if ((r + 1) * W1 <= i) ++r;
+          if (r * W1 > i) --r;
+          A[r * RA + (i - r * W1)] = v[u];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // K_x: rows j2, chunks of 4 coarse x positions -> C
+  const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1];
+  const int cx = (N1 + 3) >> 2;
+  for (int it = tid; it < W2 * cx; it += NTP) {
+    const int ch = it / W2, j2 = it - ch * W2;
+    const int i0 = 4 * ch;
+    T oc[4], od[4];
+    kin4_any(PLine<T>{A + j2 * RA, 1}, N1, i0, per, oc, od);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N1) {
+        C[j2 * RA + i0 + r] = oc[r] * kx;
+        C[j2 * RA + N1 + i0 + r] = od[r] * kx;
+      }
+  }
+  __syncthreads();
+  // K_y: columns j1, chunks of 4 coarse y positions, accumulated into C
+  double ek = 0.0;
+  const int cy = (N2 + 3) >> 2;
+  for (int it = tid; it < W1 * cy; it += NTP) {
+    const int ch = it / W1, j1 = it - ch * W1;
+    const int i0 = 4 * ch;
+    T oc[4], od[4];
+    kin4_any(PLine<T>{A + j1, RA}, N2, i0, per, oc, od);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N2) {
+        const int a = (i0 + r) * RA + j1, d = (N2 + i0 + r) * RA + j1;
+        const T vc = C[a] + oc[r] * ky, vd = C[d] + od[r] * ky;
+        C[a] = vc;
+        C[d] = vd;
+        ek += (double)vc * (double)A[a] + (double)vd * (double)A[d];
+      }
+  }
+  const double eks = block_sum_p(ek, red);
+  if (tid == 0) part[tile] = eks;
+  __syncthreads();
+  // K plane -> memory (contiguous)
+  {
+    T* kd = (T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
+    const int tot = W1 * W2;
+    const float inv = 1.0f / (float)W1;
+    for (int i = tid; i < tot; i += NTP) {
+      int r = (int)((float)i * inv);
+      if ((r + 1) * W1 <= i) ++r;
+      if (r * W1 > i) --r;
+      kd[i] = C[r * RA + (i - r * W1)];
+    }
+  }
+  __syncthreads();
+  // G_x: rows j2 -> C as (W2 rows) x (L1 cols), row stride RB
+  const int gx = (L1 + 7) >> 3;
+  for (int it = tid; it < W2 * gx; it += NTP) {
+    const int ch = it / W2, j2 = it - ch * W2;
+    const int p0 = 8 * ch;
+    T o[8];
+    grow8_any(PLine<T>{A + j2 * RA, 1}, N1, grow_k0(p0, per), per, o);
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (p0 + r < L1) C[j2 * RB + p0 + r] = o[r];
+  }
+  __syncthreads();
+  // G_y: columns p1 of C -> T2 plane (real x,y at wavelet z j3)
+  T* dst = (T*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
+  const int gy = (L2 + 7) >> 3;
+  for (int it = tid; it < L1 * gy; it += NTP) {
+    const int ch = it / L1, p1 = it - ch * L1;
+    const int p0 = 8 * ch;
+    T o[8];
+    grow8_any(PLine<T>{C + p1, RB}, N2, grow_k0(p0, per), per, o);
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (p0 + r < L2) dst[p1 + (long long)L1 * (p0 + r)] = o[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward plane kernel with compress-accumulate
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTP) void k_plane_bwd(Batch B, TileTable tt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int j3 = tile - box_first_tile(tt, b);
+  const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], L1 = bd.L[0], L2 = bd.L[1],
+            per = bd.per;
+  const int W1 = 2 * N1, W2 = 2 * N2;
+  const int RU = L1 + 1;
+  const int RO = W1 + 1;
+  T* U = (T*)smem;                // T2 plane: U[p2*RU + p1], later output O
+  T* Bq = U + max(L2 * RU, W2 * RO);  // G_y^T output: Bq[j2*RU + p1]
+  const int tid = threadIdx.x;
+  const T* src = (const T*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
+  {
+    const int tot = L1 * L2;
+    const float inv = 1.0f / (float)L1;
+    for (int base = tid; base < tot; base += NTP * 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + u * NTP;
+        v[u] = i < tot ? src[i] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + u * NTP;
+        if (i < tot) {
+          int r = (int)((float)i * inv);
+          if ((r + 1) * L1 <= i) ++r;
+          if (r * L1 > i) --r;
+          U[r * RU + (i - r * L1)] = v[u];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // G_y^T: columns p1 -> Bq rows (c: i2, d: N2+i2)
+  const int cy = (N2 + 3) >> 2;
+  for (int it = tid; it < L1 * cy; it += NTP) {
+    const int ch = it / L1, p1 = it - ch * L1;
+    const int i0 = 4 * ch;
+    T oc[4], od[4];
+    shrink4_any(PLine<T>{U + p1, RU}, N2, L2, i0, per, oc, od);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N2) {
+        Bq[(i0 + r) * RU + p1] = oc[r];
+        Bq[(N2 + i0 + r) * RU + p1] = od[r];
+      }
+  }
+  __syncthreads();
+  // G_x^T: rows j2 of Bq -> output plane O (reuses U), row stride RO
+  T* O = U;
+  const int cx = (N1 + 3) >> 2;
+  for (int it = tid; it < W2 * cx; it += NTP) {
+    const int ch = it / W2, j2 = it - ch * W2;
+    const int i0 = 4 * ch;
+    T oc[4], od[4];
+    shrink4_any(PLine<T>{Bq + j2 * RU, 1}, N1, L1, i0, per, oc, od);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N1) {
+        O[j2 * RO + i0 + r] = oc[r];
+        O[j2 * RO + N1 + i0 + r] = od[r];
+      }
+  }
+  __syncthreads();
+  // out = O + K, accumulated into hpsi at the compressed positions of the
+  // plane's four components (s3 fixed by j3), compress_and_accumulate_mixed
+  // (un-compress.f90:2174-2238)
+  const int s3 = j3 >= N3;
+  const int i3 = j3 - s3 * N3;
+  const T* Kp = (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
+  const int npl = N1 * N2;
+  double* hc = B.hpsi + bd.psi_off;
+  double* hf = B.hpsi + bd.psi_off + bd.nvctr_c;
+  const float inv1 = 1.0f / (float)N1;
+  for (int i = tid; i < npl; i += NTP) {
+    int i2 = (int)((float)i * inv1);
+    if ((i2 + 1) * N1 <= i) ++i2;
+    if (i2 * N1 > i) --i2;
+    const int i1 = i - i2 * N1;
+    const int lin = i + npl * i3;
+    const int ci = bd.cinv ? bd.cinv[lin] : lin;
+    const int fi = bd.finv ? bd.finv[lin] : lin;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int s1 = s & 1, s2 = s >> 1;
+      const int j1 = i1 + N1 * s1, j2 = i2 + N2 * s2;
+      const int t = s1 + 2 * s2 + 4 * s3;
+      const double v = (double)(O[j2 * RO + j1] + Kp[j1 + W1 * j2]);
+      if (t == 0) {
+        if (ci >= 0) hc[ci] += v;
+      } else if (fi >= 0) {
+        hf[7LL * fi + t - 1] += v;
+      }
+    }
+  }
+}
+
+// LDS bytes of the two plane kernels for the largest box extents
+size_t plane_lds_fwd(int N1, int N2, int L1, int elem) {
+  const size_t RA = 2 * N1 + 1, RB = L1 + 1;
+  const size_t a = 2 * N2 * RA, c = 2 * N2 * (RA > RB ? RA : RB);
+  return (a + c) * elem;
+}
+size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem) {
+  const size_t RU = L1 + 1, RO = 2 * N1 + 1;
+  const size_t u = (size_t)L2 * RU > 2 * N2 * RO ? (size_t)L2 * RU : 2 * N2 * RO;
+  return (u + 2 * N2 * RU) * elem;
+}
+
+template <typename T>
+hipError_t plane_setup(int bytes) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_plane_fwd<T>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_plane_bwd<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  return e;
+}
+
+// LDS bytes one work-group may use on the current device; the plane kernels'
+// dynamic-LDS limit is raised to it where the runtime asks for that.
+size_t plane_attributes() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (plane_setup<double>(v) != hipSuccess || plane_setup<float>(v) != hipSuccess)
+    (void)hipGetLastError();  // not required on this runtime; clear the error
+  return (size_t)v;
+}
+
+template <typename T>
+hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                            double* part, hipStream_t s) {
+  k_plane_fwd<T><<<ntiles, NTP, lds, s>>>(B, t, part);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                            hipStream_t s) {
+  k_plane_bwd<T><<<ntiles, NTP, lds, s>>>(B, t);
+  return hipGetLastError();
+}
+template hipError_t launch_plane_fwd<double>(const Batch&, const TileTable&, int, size_t,
+                                             double*, hipStream_t);
+template hipError_t launch_plane_fwd<float>(const Batch&, const TileTable&, int, size_t,
+                                            double*, hipStream_t);
+template hipError_t launch_plane_bwd<double>(const Batch&, const TileTable&, int, size_t,
+                                             hipStream_t);
+template hipError_t launch_plane_bwd<float>(const Batch&, const TileTable&, int, size_t,
+                                            hipStream_t);
+
+}  // namespace bdft

@@ -150,6 +150,11 @@ int bdft_profile_enable(bdft_ctx *ctx, int on);
 int bdft_profile_read(bdft_ctx *ctx, int nstage, double *ms, int64_t *nlaunch,
                       int reset);
 const char *bdft_stage_name(int stage);
+/* LDS bytes a work-group may use on the context's device, and whether the
+ * plane-fused kernel sequence is enabled (environment BDFT_LINE_PASSES=1 at
+ * context creation selects the line-pass sequence for every batch). */
+int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
+                  int *plane_enabled);
 
 /* Wait for all work of the context.  Replaces ocl_finish in
  * finish_hamiltonian_OCL (interface_ocl.f90:599). */

@@ -22,6 +22,8 @@ def ctx():
     import bigdft_hip
     assert torch.cuda.is_available(), "GPU tests need a HIP device"
     c = bigdft_hip.Context(0)
+    print("device LDS per work-group, plane sequence:", c.info(), flush=True)
+    torch.cuda.synchronize()
     yield c
     c.close()
 
@@ -299,3 +301,47 @@ def test_mixed_sphere_batch(ctx):
         assert np.abs(hg - ho).max() < MIXED_H * np.abs(ho).max()
         assert abs(eg[0] - eo[0]) < MIXED_E * abs(eo[0])
         assert abs(pg[0] - po[0]) < MIXED_E * abs(po[0])
+
+
+# ---------------------------------------------------------------------------
+# both kernel sequences: plane-fused (default where it applies: periodic axes
+# of at least 18 points, planes fitting in LDS) and the line passes
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def ctx_lines():
+    import os
+
+    import bigdft_hip
+    os.environ["BDFT_LINE_PASSES"] = "1"
+    try:
+        c = bigdft_hip.Context(0)
+    finally:
+        del os.environ["BDFT_LINE_PASSES"]
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("geo,n", [("P", (23, 19, 18)), ("F", (15, 13, 17)),
+                                   ("P", (31, 31, 31))])
+def test_line_and_plane_sequences_agree_with_oracle(ctx, ctx_lines, geo, n):
+    reg = dense_region(geo, n, (0.41, 0.43, 0.39))
+    for c in (ctx, ctx_lines):
+        check(reg, 2, 700 + n[0], c)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_plane_sparse_keys(ctx, seed):
+    """sphere regions (inverse maps) on boxes large enough for the plane path"""
+    check(mixed_sphere_region((24, 22, 26), 0.4, seed), 2, 800 + seed, ctx)
+
+
+def test_mixed_line_sequence(ctx_lines):
+    reg = dense_region("P", (23, 23, 23), (0.45,) * 3)
+    psi = random_psi(reg, 2, 900)
+    pot = random_pot(reg, 901)
+    hg, eg, pg = run_gpu(ctx_lines, reg, pot, psi, precision=1)
+    ho, eo, po = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    assert np.abs(hg - ho).max() < MIXED_H * np.abs(ho).max()
+    assert np.all(np.abs(eg - eo) < MIXED_E * np.abs(eo))
+    assert np.all(np.abs(pg - po) < MIXED_E * np.abs(po))
