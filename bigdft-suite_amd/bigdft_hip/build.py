@@ -32,5 +32,32 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN = os.path.join(PKG, "fortran")
+FDRIVER = os.path.join(FORTRAN, "locham_hip_driver")
+
+
+def build_fortran():
+    """Compile the iso_c_binding module and its driver with amdflang (when
+    the compiler is present); returns the driver path or None."""
+    fc = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        mod = os.path.join(tmp, "module_locham_hip.o")
+        r = subprocess.run([fc, "-O2", "-module-dir", tmp, "-c",
+                            os.path.join(FORTRAN, "module_locham_hip.f90"),
+                            "-o", mod], capture_output=True, text=True)
+        if r.returncode == 0:
+            r = subprocess.run([fc, "-O2", "-I", tmp,
+                                os.path.join(FORTRAN, "locham_hip_driver.f90"),
+                                mod, "-L" + LIBDIR, "-lbigdft_hip",
+                                "-Wl,-rpath,$ORIGIN/../lib", "-o", FDRIVER],
+                               capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stderr[-4000:])
+    return FDRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -1,0 +1,71 @@
+!> Driver of module_locham_hip: a dense periodic region (keys built the way
+!! segkeys does, bigdft/src/locregs/initlocregs.f90:1633-1686), deterministic
+!! orbitals and potential, one call of local_hamiltonian_hip, and the
+!! weighted energies printed for the test-suite to compare with the oracle.
+!! Usage: locham_hip_driver n norb precision
+program locham_hip_driver
+  use iso_c_binding
+  use module_locham_hip
+  implicit none
+  type(c_ptr) :: ctx, lr
+  integer :: n, norb, prec, ierr, nseg, nvctr, iseg, i2, i3, i, iorb, ncoef, npot
+  integer, dimension(:, :), allocatable :: keyg
+  integer, dimension(:), allocatable :: keyv
+  real(kind=8), dimension(:, :), allocatable :: psi, hpsi
+  real(kind=8), dimension(:), allocatable :: pot, occup
+  real(kind=8) :: ekin, epot, hsum
+  character(len=32) :: arg
+
+  call get_command_argument(1, arg); read(arg, *) n
+  call get_command_argument(2, arg); read(arg, *) norb
+  call get_command_argument(3, arg); read(arg, *) prec
+
+  ! every (i2,i3) row is one segment, coarse and fine alike
+  nseg = (n + 1) ** 2
+  nvctr = (n + 1) ** 3
+  allocate(keyg(2, 2 * nseg), keyv(2 * nseg))
+  do i3 = 0, n
+     do i2 = 0, n
+        iseg = i2 + 1 + (n + 1) * i3
+        keyg(1, iseg) = i3 * (n + 1) ** 2 + i2 * (n + 1) + 1
+        keyg(2, iseg) = keyg(1, iseg) + n
+        keyv(iseg) = (iseg - 1) * (n + 1) + 1
+     end do
+  end do
+  keyg(:, nseg + 1:) = keyg(:, 1:nseg)
+  keyv(nseg + 1:) = keyv(1:nseg)
+
+  ncoef = nvctr + 7 * nvctr
+  npot = (2 * n + 2) ** 3
+  allocate(psi(ncoef, norb), hpsi(ncoef, norb), pot(npot), occup(norb))
+  do iorb = 1, norb
+     do i = 1, ncoef
+        psi(i, iorb) = sin(0.37d0 * i + 1.3d0 * iorb) * cos(0.011d0 * i)
+     end do
+     occup(iorb) = 2.0d0 / iorb
+  end do
+  do i = 1, npot
+     pot(i) = -1.0d0 - cos(0.05d0 * i)
+  end do
+  hpsi = 0.0d0
+
+  call init_acceleration_hip(0, ctx, ierr)
+  if (ierr /= 0) stop 'context creation failed'
+  call allocate_data_hip(ctx, 'P', n, n, n, [0, 0, 0], [n, n, n], &
+       [0.45d0, 0.45d0, 0.45d0], nseg, nvctr, nseg, nvctr, keyg, keyv, lr, ierr)
+  if (ierr /= 0) then
+     print '(a)', trim(bdft_error_message(ctx))
+     stop 'region creation failed'
+  end if
+  call local_hamiltonian_hip(ctx, lr, ncoef, norb, 1.0d0, occup, pot, psi, hpsi, &
+       ekin, epot, prec, ierr)
+  if (ierr /= 0) then
+     print '(a)', trim(bdft_error_message(ctx))
+     stop 'local hamiltonian failed'
+  end if
+  hsum = sum(hpsi * psi)
+  print '(a,es25.17)', 'ekin_sum ', ekin
+  print '(a,es25.17)', 'epot_sum ', epot
+  print '(a,es25.17)', 'psi_hpsi ', hsum
+  call free_gpu_hip(ctx, lr)
+end program locham_hip_driver

@@ -1,0 +1,153 @@
+!> Fortran binding of the MI355X local-Hamiltonian library (C ABI in
+!! include/bigdft_hip.h).  It plays the role of the OpenCL glue of the
+!! reference (bigdft/src/interface_ocl.f90): init_acceleration_OCL,
+!! allocate_data_OCL, local_hamiltonian_OCL, finish_hamiltonian_OCL and
+!! free_gpu_OCL have counterparts here with the same argument meaning,
+!! reduced to plain arrays so the module compiles on its own.
+module module_locham_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter, public :: BDFT_PREC_FP64 = 0, BDFT_PREC_FP32 = 1, &
+       BDFT_PREC_BF16X3 = 2
+
+  public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
+       free_gpu_hip, bdft_error_message
+
+  interface
+     integer(c_int) function bdft_ctx_create(device, ctx) bind(C, name='bdft_ctx_create')
+       import :: c_int, c_ptr
+       integer(c_int), value :: device
+       type(c_ptr) :: ctx
+     end function bdft_ctx_create
+     integer(c_int) function bdft_ctx_destroy(ctx) bind(C, name='bdft_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function bdft_ctx_destroy
+     type(c_ptr) function bdft_last_error(ctx) bind(C, name='bdft_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function bdft_last_error
+     integer(c_int) function bdft_lr_create(ctx, geocode, n, nfl, nfu, hgrid, &
+          nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f, nvctr_f, keyg_f, keyv_f, lr) &
+          bind(C, name='bdft_lr_create')
+       import :: c_int, c_ptr, c_char, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       character(kind=c_char), value :: geocode
+       integer(c_int), dimension(3), intent(in) :: n, nfl, nfu
+       real(c_double), dimension(3), intent(in) :: hgrid
+       integer(c_int), value :: nseg_c, nvctr_c, nseg_f, nvctr_f
+       integer(c_int32_t), dimension(*), intent(in) :: keyg_c, keyv_c, keyg_f, keyv_f
+       type(c_ptr) :: lr
+     end function bdft_lr_create
+     integer(c_int) function bdft_lr_destroy(lr) bind(C, name='bdft_lr_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: lr
+     end function bdft_lr_destroy
+     integer(c_int) function bdft_local_hamiltonian(ctx, lr, norb, pot, psi, hpsi, &
+          ekin, epot, precision) bind(C, name='bdft_local_hamiltonian')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, lr
+       integer(c_int), value :: norb, precision
+       real(c_double), dimension(*), intent(in) :: pot, psi
+       real(c_double), dimension(*), intent(inout) :: hpsi
+       real(c_double), dimension(*), intent(out) :: ekin, epot
+     end function bdft_local_hamiltonian
+  end interface
+
+contains
+
+  !> message of the last failure on the context
+  function bdft_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=512) :: msg
+    character(kind=c_char), dimension(:), pointer :: s
+    integer :: i
+    msg = ''
+    call c_f_pointer(bdft_last_error(ctx), s, [512])
+    do i = 1, 512
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    end do
+  end function bdft_error_message
+
+  !> counterpart of init_acceleration_OCL (interface_ocl.f90:20-41)
+  subroutine init_acceleration_hip(device, ctx, ierr)
+    integer, intent(in) :: device
+    type(c_ptr), intent(out) :: ctx
+    integer, intent(out) :: ierr
+    ctx = c_null_ptr
+    ierr = bdft_ctx_create(int(device, c_int), ctx)
+  end subroutine init_acceleration_hip
+
+  !> counterpart of allocate_data_OCL (interface_ocl.f90:43-193): the region's
+  !! keys (1-based keyg(2,nseg_c+nseg_f), keyv(nseg_c+nseg_f), coarse first,
+  !! as in wavefunctions_descriptors) are validated and uploaded once
+  subroutine allocate_data_hip(ctx, geocode, n1, n2, n3, nfl, nfu, hgrid, &
+       nseg_c, nvctr_c, nseg_f, nvctr_f, keyg, keyv, lr, ierr)
+    type(c_ptr), intent(in) :: ctx
+    character(len=1), intent(in) :: geocode
+    integer, intent(in) :: n1, n2, n3, nseg_c, nvctr_c, nseg_f, nvctr_f
+    integer, dimension(3), intent(in) :: nfl, nfu
+    real(kind=8), dimension(3), intent(in) :: hgrid
+    integer, dimension(2, nseg_c + nseg_f), intent(in) :: keyg
+    integer, dimension(nseg_c + nseg_f), intent(in) :: keyv
+    type(c_ptr), intent(out) :: lr
+    integer, intent(out) :: ierr
+    integer(c_int32_t), dimension(2) :: dummy_g
+    integer(c_int32_t), dimension(1) :: dummy_v
+    lr = c_null_ptr
+    dummy_g = 0
+    dummy_v = 0
+    if (nseg_f > 0) then
+       ierr = bdft_lr_create(ctx, geocode, [n1, n2, n3], nfl, nfu, hgrid, &
+            nseg_c, nvctr_c, keyg(:, 1:nseg_c), keyv(1:nseg_c), &
+            nseg_f, nvctr_f, keyg(:, nseg_c + 1:), keyv(nseg_c + 1:), lr)
+    else
+       ierr = bdft_lr_create(ctx, geocode, [n1, n2, n3], nfl, nfu, hgrid, &
+            nseg_c, nvctr_c, keyg(:, 1:nseg_c), keyv(1:nseg_c), &
+            0, 0, dummy_g, dummy_v, lr)
+    end if
+  end subroutine allocate_data_hip
+
+  !> counterpart of local_hamiltonian_OCL followed by finish_hamiltonian_OCL
+  !! (interface_ocl.f90:391-607): hpsi(:,iorb) += H_loc psi(:,iorb) for the
+  !! norbp orbitals, and the weighted sums
+  !!   ekin_sum = sum kwgt*occup(iorb)*ekin(iorb),
+  !!   epot_sum = sum kwgt*occup(iorb)*epot(iorb)   (hpsiortho.f90:946-954)
+  subroutine local_hamiltonian_hip(ctx, lr, ncoef, norbp, kwgt, occup, pot, psi, &
+       hpsi, ekin_sum, epot_sum, precision, ierr)
+    type(c_ptr), intent(in) :: ctx, lr
+    integer, intent(in) :: ncoef, norbp, precision
+    real(kind=8), intent(in) :: kwgt
+    real(kind=8), dimension(norbp), intent(in) :: occup
+    real(kind=8), dimension(*), intent(in) :: pot
+    real(kind=8), dimension(ncoef, norbp), intent(in) :: psi
+    real(kind=8), dimension(ncoef, norbp), intent(inout) :: hpsi
+    real(kind=8), intent(out) :: ekin_sum, epot_sum
+    integer, intent(out) :: ierr
+    real(kind=8), dimension(norbp) :: ekin, epot
+    integer :: iorb
+    ierr = bdft_local_hamiltonian(ctx, lr, int(norbp, c_int), pot, psi, hpsi, &
+         ekin, epot, int(precision, c_int))
+    ekin_sum = 0.0d0
+    epot_sum = 0.0d0
+    if (ierr /= 0) return
+    do iorb = 1, norbp
+       ekin_sum = ekin_sum + kwgt * occup(iorb) * ekin(iorb)
+       epot_sum = epot_sum + kwgt * occup(iorb) * epot(iorb)
+    end do
+  end subroutine local_hamiltonian_hip
+
+  !> counterpart of free_gpu_OCL (interface_ocl.f90:196-269)
+  subroutine free_gpu_hip(ctx, lr)
+    type(c_ptr), intent(inout) :: ctx, lr
+    integer(c_int) :: ierr
+    if (c_associated(lr)) ierr = bdft_lr_destroy(lr)
+    if (c_associated(ctx)) ierr = bdft_ctx_destroy(ctx)
+    lr = c_null_ptr
+    ctx = c_null_ptr
+  end subroutine free_gpu_hip
+
+end module module_locham_hip

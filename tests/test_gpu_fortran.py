@@ -1,0 +1,34 @@
+"""The Fortran iso_c_binding module (bigdft-suite_amd/fortran) drives the
+library like local_hamiltonian_OCL + finish_hamiltonian_OCL would; its
+weighted energies must match the oracle on the same inputs."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import dense_region, okeys
+
+pytestmark = pytest.mark.gpu
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "bigdft-suite_amd", "fortran", "locham_hip_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no amdflang)")
+@pytest.mark.parametrize("n,norb", [(11, 3), (23, 2)])
+def test_fortran_driver_matches_oracle(n, norb):
+    r = subprocess.run([DRIVER, str(n), str(norb), "0"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = {l.split()[0]: float(l.split()[1]) for l in r.stdout.splitlines() if l.strip()}
+    reg = dense_region("P", (n, n, n), (0.45,) * 3)
+    i = np.arange(1, reg.ncoef + 1, dtype=np.float64)
+    psi = np.stack([np.sin(0.37 * i + 1.3 * o) * np.cos(0.011 * i)
+                    for o in range(1, norb + 1)])
+    occup = 2.0 / np.arange(1, norb + 1)
+    pot = -1.0 - np.cos(0.05 * np.arange(1, reg.n_isf + 1, dtype=np.float64))
+    h, ek, ep = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
+    assert abs(vals["ekin_sum"] - (occup * ek).sum()) < 1e-12 * abs((occup * ek).sum())
+    assert abs(vals["epot_sum"] - (occup * ep).sum()) < 1e-12 * abs((occup * ep).sum())
+    assert abs(vals["psi_hpsi"] - (ek + ep).sum()) < 1e-11 * abs((ek + ep).sum())
