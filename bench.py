@@ -26,23 +26,56 @@ METRIC = ("grid-pt·orbital/s of local-H apply: mixed vs FP64 at "
           "≤1e-6 rel err")
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 matrix
 PEAK_HBM_GBS = 8000.0
-# Algorithmic FP64 flops per fine point of each stage: the reference's
-# 524 = 192 (synthesis+magic) + 46*3 (kinetic) + 192 (magic^T+analysis) + 2
-# (bigdft/src/memguess.f90:2269), split per axis pass.
-STAGE_FLOPS = {"x_grow_kin": 64 + 46, "y_grow": 64, "y_kin": 46, "z_kin": 46,
-               "z_pot": 64 + 2 + 64, "y_shrink": 64, "x_shrink": 64}
 
-# Compulsory volume traffic of each stage per fine point: element reads and
-# writes of the working type (esz bytes) plus the FP64 psi/hpsi accesses.
-STAGE_VOLS = {"uncompress": (1, 8), "x_grow_kin": (3, 0), "y_grow": (2, 0),
-              "y_kin": (3, 0), "z_kin": (3, 0), "z_pot": (2, 0),
-              "y_shrink": (2, 0), "x_shrink": (3, 0), "compress": (1, 16)}
+# Per stage of each code path: (algorithmic flops per fine point, elements of
+# the working type read+written per fine point, extra FP64 bytes per point).
+# Flops follow the reference's count 524 = 192 (synthesis+magic) + 46*3
+# (kinetic) + 192 (magic^T+analysis) + 2 (bigdft/src/memguess.f90:2269):
+# one 1-D grow or shrink pass is 64, one kinetic pass 46, V*psi 2.
+# uncompress reads psi (8 B) and writes the working grid; compress reads and
+# writes hpsi (16 B).
+PATH_STAGES = {
+    "lines": {"uncompress": (0, 1, 8), "x_grow_kin": (110, 3, 0),
+              "y_grow": (64, 2, 0), "y_kin": (46, 3, 0), "z_kin": (46, 3, 0),
+              "z_pot": (130, 2, 0), "y_shrink": (64, 2, 0),
+              "x_shrink": (64, 3, 0), "compress": (0, 1, 16)},
+    "plane": {"uncompress": (0, 1, 8), "xy_fwd": (220, 3, 0),
+              "z_kin": (46, 3, 0), "z_pot": (130, 2, 0),
+              "xy_bwd_compress": (128, 2, 16)},
+    "mfma": {"uncompress": (0, 1, 8), "x_grow_kin": (46, 2, 0),
+             "y_kin": (46, 3, 0), "z_kin": (46, 3, 0), "xy_fwd": (128, 2, 0),
+             "z_pot": (130, 2, 0), "xy_bwd_compress": (128, 2, 16)},
+}
 
 
-def stage_bytes(stage, esz):
-    v, extra = STAGE_VOLS[stage]
-    return v * esz + extra
+def path_of(prec, plane_enabled):
+    if prec in ("bf16x3", "bf16x4"):
+        return "mfma"
+    return "plane" if plane_enabled else "lines"
+
+
+def roofline(path, avg, units, esz, peak_c):
+    """roofline entry of the dominant (longest) kernel stage"""
+    tab = PATH_STAGES[path]
+    dom = max((k for k in avg if k in tab and avg[k] > 0), key=lambda k: avg[k])
+    fl, el, extra = tab[dom]
+    t_dom = avg[dom] * 1e-3
+    ach_c = fl * units / t_dom / 1e12
+    ach_b = (el * esz + extra) * units / t_dom / 1e9
+    if ach_c / peak_c >= ach_b / PEAK_HBM_GBS:
+        roof = {"bound": "mfma", "achieved": ach_c, "peak": peak_c,
+                "unit": "TFLOP/s", "frac": ach_c / peak_c,
+                "hbm_GBps": ach_b, "hbm_frac": ach_b / PEAK_HBM_GBS}
+    else:
+        roof = {"bound": "hbm", "achieved": ach_b, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": ach_b / PEAK_HBM_GBS,
+                "flop_TFLOPs": ach_c, "flop_frac": ach_c / peak_c}
+    roof.update({"traffic": None, "kernel": dom, "kernel_ms": avg[dom],
+                 "path": path, "flops_per_pt": fl,
+                 "bytes_per_pt": el * esz + extra})
+    return roof
 
 
 CONFIGS = {
@@ -62,8 +95,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="fp32",
-                    choices=["fp64", "fp32"],
-                    help="arithmetic of the timed headline run (fp32 = mixed)")
+                    choices=["fp64", "fp32", "bf16x3", "bf16x4"],
+                    help="arithmetic of the headline run (fp32: FP32 working "
+                         "grids; bf16x4: split-BF16 matrix-core path)")
+    ap.add_argument("--variants", default="fp64,fp32,bf16x4",
+                    help="precisions timed beside the headline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -138,38 +174,47 @@ def main():
         ctx.sync()
         return ms, wall, {k: v / nl for k, v in stages.items()}, h, ek, ep
 
+    _, plane_enabled = ctx.info()
     ms64, wall64, st64, h64, ek64, ep64 = timed(bigdft_hip.PREC_FP64)
-    if prec != bigdft_hip.PREC_FP64:
-        ms_step, wall, avg, hm, ekm, epm = timed(prec)
-        hmax = h64.abs().max()
+    hmax = h64.abs().max()
+
+    def errors(hm, ekm, epm):
         d = (hm - h64).abs()
         guard = torch.maximum(h64.abs(), 1e-3 * hmax)
-        mixed_err = {
-            "hpsi_max_abs_over_max": float(d.max() / hmax),
-            "hpsi_rms_over_rms": float(torch.sqrt((d * d).mean() / (h64 * h64).mean())),
-            "hpsi_max_rel_guarded": float((d / guard).max()),
-            "ekin_max_rel": float(((ekm - ek64).abs() / ek64.abs()).max()),
-            "epot_max_rel": float(((epm - ep64).abs() / ep64.abs()).max()),
-            "reference": "FP64 GPU run on the same inputs"}
-    else:
-        ms_step, wall, avg, hm, ekm, epm = ms64, wall64, st64, h64, ek64, ep64
-        mixed_err = None
+        return {"hpsi_max_abs_over_max": float(d.max() / hmax),
+                "hpsi_rms_over_rms": float(torch.sqrt((d * d).mean() /
+                                                      (h64 * h64).mean())),
+                "hpsi_max_rel_guarded": float((d / guard).max()),
+                "ekin_max_rel": float(((ekm - ek64).abs() / ek64.abs()).max()),
+                "epot_max_rel": float(((epm - ep64).abs() / ep64.abs()).max()),
+                "reference": "FP64 GPU run on the same inputs"}
+
+    variants = {}
+    names = [v for v in args.variants.split(",") if v]
+    if args.precision not in names:
+        names.append(args.precision)
+    for name in names:
+        if name == "fp64":
+            ms, wall, avg, hm, ekm, epm = ms64, wall64, st64, h64, ek64, ep64
+            err = None
+        else:
+            ms, wall, avg, hm, ekm, epm = timed(bigdft_hip.PRECISIONS[name])
+            err = errors(hm, ekm, epm)
+            del hm
+        variants[name] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
+                          "wall_s": round(wall, 4), "path":
+                          path_of(name, plane_enabled),
+                          "stages_ms": {k: round(v, 4) for k, v in avg.items()},
+                          "error_vs_fp64": err}
+    head = variants[args.precision]
+    ms_step, wall = head["ms_per_step"], head["wall_s"]
+    avg = head["stages_ms"]
     esz = 8 if prec == bigdft_hip.PREC_FP64 else 4
-    peak_c = PEAK_FP64_TFLOPS if esz == 8 else PEAK_FP32_TFLOPS
-    dom = max((k for k in avg if k in STAGE_FLOPS), key=lambda k: avg[k])
-    t_dom = avg[dom] * 1e-3
-    ach_c = STAGE_FLOPS[dom] * units / t_dom / 1e12
-    ach_b = stage_bytes(dom, esz) * units / t_dom / 1e9
-    if ach_c / peak_c >= ach_b / PEAK_HBM_GBS:
-        roof = {"bound": "mfma", "achieved": ach_c, "peak": peak_c,
-                "unit": "TFLOP/s", "frac": ach_c / peak_c}
-        other = {"hbm_GBps": ach_b, "hbm_frac": ach_b / PEAK_HBM_GBS}
+    if args.precision in ("bf16x3", "bf16x4"):
+        peak_c = PEAK_BF16_TFLOPS
     else:
-        roof = {"bound": "hbm", "achieved": ach_b, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": ach_b / PEAK_HBM_GBS}
-        other = {"flop_TFLOPs": ach_c, "flop_frac": ach_c / peak_c}
-    roof.update({"traffic": None, "kernel": dom, "kernel_ms": avg[dom]})
-    roof.update(other)
+        peak_c = PEAK_FP64_TFLOPS if esz == 8 else PEAK_FP32_TFLOPS
+    roof = roofline(head["path"], avg, units, esz, peak_c)
 
     # ---- parity on a sample + CPU baseline (oracle, host cores) ----
     parity, cpu = None, None
@@ -206,17 +251,16 @@ def main():
         "unit": "grid-pt*orbital/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16",
+                  "bf16x4": "bf16"}[args.precision],
         "data": "synthetic (seeded N(0,1) coefficients, 8-Gaussian potential)",
         "config": {"workload": cfg["workload"], "orbitals": norb,
                    "grid_pts_per_orbital": reg.n_isf, "precision": args.precision,
-                   "graph": True, "wall_s": round(wall, 4)},
+                   "graph": True, "wall_s": wall},
         "roofline": roof,
-        "fp64": {"value": units / (ms64 * 1e-3), "ms_per_step": ms64,
-                 "stages_ms": {k: round(v, 4) for k, v in st64.items()}},
-        "mixed_speedup_vs_fp64": ms64 / ms_step,
-        "mixed_error_vs_fp64": mixed_err,
-        "stages_ms": {k: round(v, 4) for k, v in avg.items()},
+        "speedup_vs_fp64": ms64 / ms_step,
+        "error_vs_fp64": head["error_vs_fp64"],
+        "variants": variants,
         "apply_equiv_tflops": 524.0 * units / (ms_step * 1e-3) / 1e12,
         "cpu_baseline": cpu,
         "parity": parity,

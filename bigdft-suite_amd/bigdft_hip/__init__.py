@@ -21,8 +21,9 @@ from .descriptors import Region, dense_region, sphere_region  # noqa: F401
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
 
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4 = 0, 1, 2, 3
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
+              "bf16x4": PREC_BF16X4}
 
 ERRORS = {1: "BDFT_ERR_ARG", 2: "BDFT_ERR_HIP", 3: "BDFT_ERR_DESC",
           4: "BDFT_ERR_NOMEM", 5: "BDFT_ERR_UNSUPPORTED"}

@@ -1,6 +1,8 @@
 // C-ABI layer: contexts, localisation regions (descriptor validation and
 // upload), batch plans and their hipGraph capture.
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -64,6 +66,8 @@ struct bdft_plan {
   DevBuf meta;         // box descriptors + tile starts + energy partials
   Batch B{};
   PassTables T{};
+  MfmaLds mlds{};
+  DevBuf tabbuf;       // matrix-core operator tables
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -130,6 +134,7 @@ extern "C" int bdft_ctx_create(int device, bdft_ctx** out) {
   }
   ctx->stream = ctx->own;
   ctx->plane_lds = plane_attributes();
+  if (ctx->plane_lds) mfma_attributes(ctx->plane_lds);
   const char* env = getenv("BDFT_LINE_PASSES");
   ctx->force_lines = env && env[0] == '1';
   *out = ctx;
@@ -330,6 +335,114 @@ extern "C" int bdft_lr_info(const bdft_lr* lr, int nfine[3], int64_t* ncoef) {
 }
 
 // ---------------------------------------------------------------------------
+// operator tables of the matrix-core path
+// ---------------------------------------------------------------------------
+static uint16_t f2bf(float x) {  // round to nearest even
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static double bf2d(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+// Dense operator of one axis: op 0 = G (synthesis+magic, fil2 of
+// bigdft/src/orbitals/v.inc), 1 = G^T, 2 = wavelet-basis kinetic (unscaled;
+// ConvolkineticT filters, convolut_ib_optim.f90:962-1038).  Periodic boxes
+// fold every image of the stencil into the matrix; free boxes cut it.
+static void build_op(int op, int N, int per, std::vector<double>& M, int& rows, int& cols) {
+  const int W = 2 * N, L = per ? 2 * N : 2 * N + 29;
+  if (op == 2) {
+    rows = cols = W;
+    M.assign((size_t)rows * cols, 0.0);
+    for (int so = 0; so < 2; ++so)
+      for (int i = 0; i < N; ++i)
+        for (int si = 0; si < 2; ++si)
+          for (int l = -14; l <= 14; ++l) {
+            int k = i + l;
+            if (per) k = ((k % N) + N) % N;
+            else if (k < 0 || k >= N) continue;
+            const int al = l < 0 ? -l : l;
+            const double f = so == 0 ? (si == 0 ? Fil::KA[al] : Fil::KC[-l + 14])
+                                     : (si == 0 ? Fil::KC[l + 14] : Fil::KE[al]);
+            M[(size_t)(so * N + i) * cols + si * N + k] += f;
+          }
+    return;
+  }
+  // G: rows = fine outputs, cols = [c ; d]
+  std::vector<double> G((size_t)L * W, 0.0);
+  for (int k = 0; k < N; ++k)
+    for (int s = 0; s < 2; ++s)
+      for (int j = -14; j <= 16; ++j) {
+        int p = 2 * k + j;
+        if (per) p = ((p % W) + W) % W;
+        else p += 14;
+        G[(size_t)p * W + s * N + k] += s == 0 ? Fil::F2S[j + 14] : Fil::F2W[j + 14];
+      }
+  if (op == 0) {
+    rows = L;
+    cols = W;
+    M.swap(G);
+  } else {
+    rows = W;
+    cols = L;
+    M.assign((size_t)W * L, 0.0);
+    for (int p = 0; p < L; ++p)
+      for (int q = 0; q < W; ++q) M[(size_t)q * L + p] = G[(size_t)p * W + q];
+  }
+}
+
+struct TabSet {
+  std::vector<OpTab> tabs;
+  std::vector<uint16_t> hi, lo, lo2;
+  std::vector<uint32_t> masks;
+  std::map<std::tuple<int, int, int>, int> index;
+  int get(int op, int N, int per) {
+    const auto key = std::make_tuple(op, N, per);
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    std::vector<double> M;
+    int rows = 0, cols = 0;
+    build_op(op, N, per, M, rows, cols);
+    OpTab t{};
+    t.rows = rows;
+    t.cols = cols;
+    t.Dp = (rows + 31) / 32 * 32;
+    t.Kp = (cols + 15) / 16 * 16;
+    t.off = (long long)hi.size();
+    t.moff = (int)masks.size();
+    hi.resize(hi.size() + (size_t)t.Dp * t.Kp, 0);
+    lo.resize(lo.size() + (size_t)t.Dp * t.Kp, 0);
+    lo2.resize(lo2.size() + (size_t)t.Dp * t.Kp, 0);
+    for (int r = 0; r < rows; ++r)
+      for (int c = 0; c < cols; ++c) {
+        const double v = M[(size_t)r * cols + c];
+        const uint16_t h = f2bf((float)v);
+        const size_t o = (size_t)t.off + (size_t)r * t.Kp + c;
+        hi[o] = h;
+        const uint16_t l1 = f2bf((float)(v - bf2d(h)));
+        lo[o] = l1;
+        lo2[o] = f2bf((float)(v - bf2d(h) - bf2d(l1)));
+      }
+    for (int q = 0; q < t.Dp / 32; ++q) {
+      uint32_t m = 0;
+      for (int r = 32 * q; r < std::min(rows, 32 * q + 32); ++r)
+        for (int c = 0; c < cols; ++c)
+          if (M[(size_t)r * cols + c] != 0.0) m |= 1u << (c / 16);
+      masks.push_back(m);
+    }
+    const int id = (int)tabs.size();
+    tabs.push_back(t);
+    index[key] = id;
+    return id;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // plans
 // ---------------------------------------------------------------------------
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -360,9 +473,11 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   *out = nullptr;
   if (norb <= 0 || !lrs || !pot || !psi || !hpsi || !ekin || !epot)
     return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
-  if (precision != BDFT_PREC_FP64 && precision != BDFT_PREC_FP32)
+  if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_BF16X4)
     return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
   const size_t esz = precision == BDFT_PREC_FP64 ? sizeof(double) : sizeof(float);
+  const bool mfma = precision == BDFT_PREC_BF16X3 || precision == BDFT_PREC_BF16X4;
+  TabSet tabs;
   std::vector<BoxDesc> box((size_t)norb);
   long long wtot = 0, t1tot = 0, t2tot = 0;
   int maxN = 0;
@@ -400,6 +515,12 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     b.pot_off = pot_off ? pot_off[i] : 0;
     const long long ncoef = (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
     b.psi_off = psi_off ? psi_off[i] : ncoef * i;
+    if (mfma)
+      for (int d = 0; d < 3; ++d) {
+        b.opG[d] = tabs.get(0, b.N[d], b.per);
+        b.opS[d] = tabs.get(1, b.N[d], b.per);
+        b.opK[d] = tabs.get(2, b.N[d], b.per);
+      }
     wtot += 8LL * b.N[0] * b.N[1] * b.N[2];
     t1tot += (long long)b.L[0] * 4 * b.N[1] * b.N[2];
     t2tot += (long long)b.L[0] * b.L[1] * 2 * b.N[2];
@@ -436,8 +557,29 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // plane-fused sequence when every box qualifies: periodic axes need at
   // least 18 coarse points (single wrap of the in-plane stencils) and both
   // plane kernels must fit in LDS
+  MfmaLds ml{};
+  if (mfma) {
+    for (int i = 0; i < norb; ++i) {
+      const BoxDesc& b = box[i];
+      const OpTab* tb = tabs.tabs.data();
+      for (int d = 0; d < 3; ++d)
+        ml.kin[d] = std::max(ml.kin[d], mfma_lds_kin(2 * b.N[d], tb[b.opK[d]].Kp));
+      ml.grow = std::max(ml.grow, mfma_lds_grow(2 * b.N[1], b.L[0], tb[b.opG[0]].Kp,
+                                                tb[b.opG[1]].Kp));
+      ml.zpot = std::max(ml.zpot, mfma_lds_zpot(tb[b.opG[2]].Kp, tb[b.opS[2]].Kp));
+      ml.shrink = std::max(ml.shrink, mfma_lds_shrink(b.L[0], 2 * b.N[0], 2 * b.N[1],
+                                                      tb[b.opS[1]].Kp, tb[b.opS[0]].Kp));
+    }
+    const size_t mx = std::max(std::max(std::max(ml.kin[0], ml.kin[1]), ml.kin[2]),
+                               std::max(std::max(ml.grow, ml.zpot), ml.shrink));
+    if (mx > ctx->plane_lds)
+      return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                     "split-bf16 kernels need %zu bytes of LDS per work-group, the device "
+                     "offers %zu",
+                     mx, ctx->plane_lds);
+  }
   {
-    bool plane = !ctx->force_lines;
+    bool plane = !ctx->force_lines && !mfma;
     size_t lpf = 0, lpb = 0;
     for (int i = 0; i < norb && plane; ++i) {
       const BoxDesc& b = box[i];
@@ -463,6 +605,38 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   const size_t nb_st = sizeof(int) * starts.size();
   int epass[4] = {P_XF, P_KY, P_KZ, P_ZP};
   if (T.plane) epass[0] = P_PF;
+  plan->mlds = ml;
+  if (mfma) {
+    const size_t nt = sizeof(OpTab) * tabs.tabs.size();
+    const size_t nh = sizeof(uint16_t) * tabs.hi.size();
+    const size_t nm = sizeof(uint32_t) * tabs.masks.size();
+    auto al2 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    int rc2 = dev_alloc(ctx, plan->tabbuf, al2(nt) + 3 * al2(nh) + al2(nm));
+    if (rc2) {
+      delete plan;
+      return rc2;
+    }
+    char* q = (char*)plan->tabbuf.p;
+    hipError_t e = hipMemcpy(q, tabs.tabs.data(), nt, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(q + al2(nt), tabs.hi.data(), nh, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(q + al2(nt) + al2(nh), tabs.lo.data(), nh, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(q + al2(nt) + 2 * al2(nh), tabs.lo2.data(), nh, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(q + al2(nt) + 3 * al2(nh), tabs.masks.data(), nm, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      dev_free(plan->tabbuf);
+      delete plan;
+      return set_err(ctx, BDFT_ERR_HIP, "operator table upload failed");
+    }
+    plan->B.tabs = (const OpTab*)q;
+    plan->B.thi = q + al2(nt);
+    plan->B.tlo = q + al2(nt) + al2(nh);
+    plan->B.tlo2 = q + al2(nt) + 2 * al2(nh);
+    plan->B.tmask = (const unsigned*)(q + al2(nt) + 3 * al2(nh));
+  }
   size_t nb_part = 0;
   for (int e = 0; e < 4; ++e) nb_part += sizeof(double) * T.ntiles[epass[e]];
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -492,7 +666,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     }
   }
   // workspace: W | K | T1 | T2
-  if (T.plane) t1tot = 0;
+  if (T.plane || mfma) t1tot = 0;
   const size_t ws_bytes = esz * (2 * al(wtot) + al(t1tot) + al(t2tot));
   DevBuf& ws = own_ws ? plan->ws : ctx->ws;
   rc = dev_alloc(ctx, ws, ws_bytes);
@@ -540,8 +714,12 @@ static int run_plan(bdft_plan* plan) {
     ctx->ev_pending.push_back(std::move(set));
     ev = ctx->ev_pending.back().data();
   }
-  hipError_t e = launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero,
-                               plan->precision == BDFT_PREC_FP32, ctx->stream, ev);
+  const bool mf = plan->precision >= BDFT_PREC_BF16X3;
+  hipError_t e = mf ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal, plan->need_zero,
+                                  plan->precision == BDFT_PREC_BF16X4 ? 4 : 3, ctx->stream, ev)
+                     : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
+                                     plan->need_zero, plan->precision == BDFT_PREC_FP32,
+                                     ctx->stream, ev);
   if (e != hipSuccess)
     return set_err(ctx, BDFT_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return BDFT_OK;
@@ -562,8 +740,13 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t el = launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
-                                  plan->need_zero, plan->precision == BDFT_PREC_FP32, cs);
+    hipError_t el = plan->precision >= BDFT_PREC_BF16X3
+                         ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal,
+                                       plan->need_zero,
+                                       plan->precision == BDFT_PREC_BF16X4 ? 4 : 3, cs, nullptr)
+                         : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
+                                         plan->need_zero, plan->precision == BDFT_PREC_FP32,
+                                         cs);
     e = hipStreamEndCapture(cs, &plan->graph);
     if (el != hipSuccess) e = el;
   }
@@ -591,6 +774,7 @@ extern "C" int bdft_plan_destroy(bdft_plan* plan) {
   if (plan->exec) hipGraphExecDestroy(plan->exec);
   if (plan->graph) hipGraphDestroy(plan->graph);
   dev_free(plan->meta);
+  dev_free(plan->tabbuf);
   if (plan->owns_ws) dev_free(plan->ws);
   delete plan;
   return BDFT_OK;

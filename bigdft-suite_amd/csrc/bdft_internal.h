@@ -44,6 +44,18 @@ struct BoxDesc {
   const int* cinv;      // grid point -> coarse coefficient index or -1
   const int* finv;      // grid point -> fine point index or -1
                         // (null: identity, every point present in key order)
+  int opG[3], opS[3], opK[3];  // operator tables per axis (matrix-core path)
+};
+
+// A 1D operator as a padded matrix for the matrix-core path: rows = outputs,
+// cols = inputs, split in bf16 hi/lo; mask bit s of 32-row block q set when
+// columns [16s,16s+16) of those rows hold non-zeros.
+struct OpTab {
+  int rows, cols;   // Dout, Din
+  int Dp, Kp;       // padded to multiples of 32 and 16
+  long long off;    // element offset in the hi/lo arrays
+  int moff;         // offset of the block masks
+  int pad_;
 };
 
 // Tile bookkeeping for one pass: tiles of box b are [start[b], start[b+1]).
@@ -181,6 +193,11 @@ struct Batch {
   double* part;         // energy partial sums, one slot per (pass, tile)
   double* ekin;         // per-box kinetic energy (device)
   double* epot;         // per-box potential energy (device)
+  const OpTab* tabs;    // matrix-core operator tables (device)
+  const void* thi;      // bf16 hi parts
+  const void* tlo;      // bf16 lo parts
+  const void* tlo2;     // bf16 third parts (residual of hi + lo)
+  const unsigned* tmask;
 };
 
 // Passes of one application; each has its own tile table.
@@ -199,7 +216,28 @@ struct PassTables {
 enum Stage { S_ZERO = 0, S_UNC, S_XF, S_YF, S_KY, S_KZ, S_ZP, S_YB, S_XB,
              S_COMP, S_ENERGY, S_PF, S_PB, S_COUNT };
 
+struct EnergyTables {
+  TileTable t[4];       // partial sums of ekin (first nk entries) then epot
+  const double* p[4];
+  int nk, n;
+};
+
+struct MfmaLds {
+  size_t kin[3], grow, zpot, shrink;
+};
+
 size_t lds_bytes(int maxN, int elem);
+void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s);
+void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, hipStream_t s);
+void launch_energy(const EnergyTables& et, const Batch& B, hipStream_t s);
+size_t mfma_lds_kin(int Wd, int Kp);
+size_t mfma_lds_grow(int W2, int L1, int Kp1, int Kp2);
+size_t mfma_lds_zpot(int KpG, int KpS);
+size_t mfma_lds_shrink(int L1, int W1, int W2, int KpU, int KpB);
+size_t mfma_attributes(size_t lds);
+hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
+                       long long wtotal, bool need_zero, int nterm, hipStream_t s,
+                       hipEvent_t* ev);
 size_t plane_lds_fwd(int N1, int N2, int L1, int elem);
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem);
 size_t plane_attributes();

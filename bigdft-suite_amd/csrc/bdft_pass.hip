@@ -715,12 +715,6 @@ __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // energies: fixed-order sums of the per-tile partials of each box
 // ---------------------------------------------------------------------------
-struct EnergyTables {
-  TileTable t[4];       // partial sums of ekin (first nk entries) then epot
-  const double* p[4];
-  int nk, n;
-};
-
 // one work-group per box; fixed summation order (strided per thread, then a
 // fixed tree) so the energies are reproducible run to run
 __global__ __launch_bounds__(128) void k_energy(EnergyTables et, int nbox, double* ekin,
@@ -843,6 +837,16 @@ hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
     k_uncompress<double, true><<<ntiles, NT, 0, s>>>(B, t);
   }
   return hipGetLastError();
+}
+
+void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s) {
+  k_zero<float><<<nblocks, NT, 0, s>>>(p, n);
+}
+void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, hipStream_t s) {
+  k_uncompress<float, false><<<ntiles, NT, 0, s>>>(B, t);
+}
+void launch_energy(const EnergyTables& et, const Batch& B, hipStream_t s) {
+  k_energy<<<B.nbox, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);
 }
 
 }  // namespace bdft

@@ -52,7 +52,10 @@ enum {
 enum {
   BDFT_PREC_FP64 = 0,    /* plain FP64 everywhere */
   BDFT_PREC_FP32 = 1,    /* FP32 MFMA passes, FP64 energies and accumulate */
-  BDFT_PREC_BF16X3 = 2   /* split-BF16 (3 products) MFMA, FP32 accumulation */
+  BDFT_PREC_BF16X3 = 2,  /* split-BF16 MFMA, FP32 accumulation: filter and data
+                            split hi+lo, products hi*hi + hi*lo + lo*hi */
+  BDFT_PREC_BF16X4 = 3   /* as BF16X3 plus the third filter piece times the data
+                            hi part (removes the filter's fixed 2^-16 error) */
 };
 
 /* Context on one device.  Replaces init_acceleration_OCL

@@ -345,3 +345,75 @@ def test_mixed_line_sequence(ctx_lines):
     assert np.abs(hg - ho).max() < MIXED_H * np.abs(ho).max()
     assert np.all(np.abs(eg - eo) < MIXED_E * np.abs(eo))
     assert np.all(np.abs(pg - po) < MIXED_E * np.abs(po))
+
+
+# ---------------------------------------------------------------------------
+# split-BF16 matrix-core path (FP32 accumulation, FP64 energies and
+# accumulation).  Bars for the four-product form: ekin, epot within 1e-6
+# relative of the FP64 oracle; hpsi within 1e-4 of max|hpsi| per element.
+# The three-product form is held to 1e-4 per element and 2e-5 in the
+# energies (its filter carries a fixed 2^-16 representation error).
+# ---------------------------------------------------------------------------
+BF_E = 1e-6
+BF_H = 1e-4
+
+
+@pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
+                                        ("F", (15, 13, 17), 3), ("P", (31, 19, 24), 4),
+                                        ("F", (39, 39, 39), 5)])
+def test_bf16x4(ctx, geo, n, seed):
+    reg = dense_region(geo, n, (0.45, 0.4, 0.5))
+    print(f"bf16x4 case {geo} {n}", flush=True)
+    psi = random_psi(reg, 2, 1000 + seed)
+    pot = random_pot(reg, 1100 + seed)
+    hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=3)
+    ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    eh = np.abs(hg - ho).max() / np.abs(ho).max()
+    ee = np.max(np.abs(eg - eo) / np.abs(eo))
+    ep_ = np.max(np.abs(pg - po) / np.abs(po))
+    print(f"bf16x4 {geo} {n}: hpsi {eh:.2e} ekin {ee:.2e} epot {ep_:.2e}", flush=True)
+    assert eh < BF_H
+    assert ee < BF_E and ep_ < BF_E
+
+
+def test_bf16x4_sphere_batch(ctx):
+    regs = [mixed_sphere_region((24, 22, 26), 0.4, s) for s in (21, 22, 23)]
+    lrs = [ctx.region(r) for r in regs]
+    psis = [random_psi(r, 1, 1200 + i)[0] for i, r in enumerate(regs)]
+    pots = [random_pot(r, 1300 + i) for i, r in enumerate(regs)]
+    psi_off = np.cumsum([0] + [p.size for p in psis])[:-1]
+    pot_off = np.cumsum([0] + [p.size for p in pots])[:-1]
+    d_psi = dev(np.concatenate(psis))
+    d_h = torch.zeros_like(d_psi)
+    potdim = np.array([r.nfine for r in regs], dtype=np.int32).ravel()
+    ek = torch.zeros(len(regs), dtype=torch.float64, device="cuda")
+    ep = torch.zeros_like(ek)
+    ctx.locham_batch(lrs, dev(np.concatenate(pots)), d_psi, d_h, ek, ep,
+                     pot_off=pot_off, potdim=potdim, psi_off=psi_off, precision=3)
+    ctx.sync()
+    h = d_h.cpu().numpy()
+    for i, r in enumerate(regs):
+        ho, eo, po = oracle.locham("F", r.n, r.nfl, r.nfu, r.hgrid, okeys(r),
+                                   pots[i], psis[i])
+        hs = h[psi_off[i]:psi_off[i] + psis[i].size]
+        assert np.abs(hs - ho[0]).max() < BF_H * np.abs(ho).max()
+        assert abs(ek.cpu().numpy()[i] - eo[0]) < BF_E * abs(eo[0])
+        assert abs(ep.cpu().numpy()[i] - po[0]) < BF_E * abs(po[0])
+    for l in lrs:
+        l.close()
+
+
+def test_bf16x3_three_products(ctx):
+    reg = dense_region("P", (23, 23, 23), (0.45,) * 3)
+    psi = random_psi(reg, 2, 1400)
+    pot = random_pot(reg, 1401)
+    hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=2)
+    ho, eo, po = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    print(f"bf16x3: hpsi {np.abs(hg - ho).max() / np.abs(ho).max():.2e} "
+          f"ekin {np.max(np.abs(eg - eo) / np.abs(eo)):.2e} "
+          f"epot {np.max(np.abs(pg - po) / np.abs(po)):.2e}")
+    assert np.abs(hg - ho).max() < BF_H * np.abs(ho).max()
+    assert np.all(np.abs(eg - eo) < 2e-5 * np.abs(eo))
+    assert np.all(np.abs(pg - po) < 2e-5 * np.abs(po))
