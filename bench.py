@@ -104,6 +104,9 @@ def parse():
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (the "
+                         "known-size kernel that scales profiler byte counters)")
     return ap.parse_args()
 
 
@@ -136,6 +139,14 @@ def main():
     pot = inputs.gaussian_field(gp, inputs.fine_coords("P", (n, n, n),
                                                        (h, h, h), device=dev),
                                 cell=cell).contiguous()
+
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ctx = bigdft_hip.Context(0)
     stream = torch.cuda.Stream()

@@ -396,10 +396,19 @@ static void build_op(int op, int N, int per, std::vector<double>& M, int& rows, 
   }
 }
 
+// Operator tables for the matrix-core kernels.  Every entry of the dense
+// operator (rows = outputs, cols = inputs, periodic wrap or zero padding
+// folded in) is split into three bf16 pieces (hi, lo = bf16(v - hi),
+// lo2 = bf16(v - hi - lo)).  Storage is in matrix-core operand order: for
+// 32-row block q and each K step s (16 columns) holding a non-zero, one
+// fragment of 64 lanes x 8 bf16 per piece (lane l: row 32q + l%32, columns
+// 16s + 8(l/32) .. +8), pieces of one step adjacent, so a wave reads each
+// piece as one contiguous 1 KB block.  Per row block: {bit mask of the K
+// steps stored, index of its first fragment}.
 struct TabSet {
   std::vector<OpTab> tabs;
-  std::vector<uint16_t> hi, lo, lo2;
-  std::vector<uint32_t> masks;
+  std::vector<uint16_t> frag;
+  std::vector<uint32_t> mq;
   std::map<std::tuple<int, int, int>, int> index;
   int get(int op, int N, int per) {
     const auto key = std::make_tuple(op, N, per);
@@ -413,27 +422,33 @@ struct TabSet {
     t.cols = cols;
     t.Dp = (rows + 31) / 32 * 32;
     t.Kp = (cols + 15) / 16 * 16;
-    t.off = (long long)hi.size();
-    t.moff = (int)masks.size();
-    hi.resize(hi.size() + (size_t)t.Dp * t.Kp, 0);
-    lo.resize(lo.size() + (size_t)t.Dp * t.Kp, 0);
-    lo2.resize(lo2.size() + (size_t)t.Dp * t.Kp, 0);
-    for (int r = 0; r < rows; ++r)
-      for (int c = 0; c < cols; ++c) {
-        const double v = M[(size_t)r * cols + c];
-        const uint16_t h = f2bf((float)v);
-        const size_t o = (size_t)t.off + (size_t)r * t.Kp + c;
-        hi[o] = h;
-        const uint16_t l1 = f2bf((float)(v - bf2d(h)));
-        lo[o] = l1;
-        lo2[o] = f2bf((float)(v - bf2d(h) - bf2d(l1)));
-      }
+    t.off = 0;
+    t.moff = (int)(mq.size() / 2);
     for (int q = 0; q < t.Dp / 32; ++q) {
       uint32_t m = 0;
       for (int r = 32 * q; r < std::min(rows, 32 * q + 32); ++r)
         for (int c = 0; c < cols; ++c)
           if (M[(size_t)r * cols + c] != 0.0) m |= 1u << (c / 16);
-      masks.push_back(m);
+      const uint32_t first = (uint32_t)(frag.size() / (3 * 512));
+      mq.push_back(m);
+      mq.push_back(first);
+      for (int s = 0; s < t.Kp / 16; ++s) {
+        if (!(m & (1u << s))) continue;
+        const size_t base = frag.size();
+        frag.resize(base + 3 * 512, 0);
+        for (int l = 0; l < 64; ++l)
+          for (int u = 0; u < 8; ++u) {
+            const int r = 32 * q + (l & 31), c = 16 * s + 8 * (l >> 5) + u;
+            if (r >= rows || c >= cols) continue;
+            const double v = M[(size_t)r * cols + c];
+            const uint16_t h = f2bf((float)v);
+            const uint16_t l1 = f2bf((float)(v - bf2d(h)));
+            const uint16_t l2 = f2bf((float)(v - bf2d(h) - bf2d(l1)));
+            frag[base + 0 * 512 + 8 * l + u] = h;
+            frag[base + 1 * 512 + 8 * l + u] = l1;
+            frag[base + 2 * 512 + 8 * l + u] = l2;
+          }
+      }
     }
     const int id = (int)tabs.size();
     tabs.push_back(t);
@@ -460,6 +475,8 @@ static int box_tiles(const BoxDesc& b, int pass) {
     case P_ZP: return cdiv(L1, TW) * L2;
     case P_PF:
     case P_PB: return 2 * N3;
+    case P_KZG: return cdiv((long long)cdiv(2 * N1, TW) * 2 * N2, ZGROUP);
+    case P_ZPG: return cdiv((long long)cdiv(L1, TW) * L2, ZGROUP);
   }
   return 0;
 }
@@ -570,8 +587,23 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       ml.shrink = std::max(ml.shrink, mfma_lds_shrink(b.L[0], 2 * b.N[0], 2 * b.N[1],
                                                       tb[b.opS[1]].Kp, tb[b.opS[0]].Kp));
     }
-    const size_t mx = std::max(std::max(std::max(ml.kin[0], ml.kin[1]), ml.kin[2]),
-                               std::max(std::max(ml.grow, ml.zpot), ml.shrink));
+    for (int i = 0; i < norb; ++i) {
+      const BoxDesc& b = box[i];
+      const OpTab* tb = tabs.tabs.data();
+      ml.fwd = std::max(ml.fwd, mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]));
+      ml.bwd = std::max(ml.bwd, mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]));
+      ml.kz = std::max(ml.kz, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp));
+      ml.zp = std::max(ml.zp, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp));
+    }
+    // plane-fused sequence when both plane kernels fit; the per-pass one
+    // otherwise (or when line passes are forced)
+    ml.fused = !ctx->force_lines && std::max(ml.fwd, ml.bwd) <= ctx->plane_lds;
+    size_t mx = std::max(ml.kin[2], ml.zpot);
+    if (ml.fused)
+      mx = std::max(std::max(ml.kz, ml.zp), std::max(ml.fwd, ml.bwd));
+    else
+      mx = std::max(std::max(mx, std::max(ml.kin[0], ml.kin[1])),
+                    std::max(ml.grow, ml.shrink));
     if (mx > ctx->plane_lds)
       return set_err(ctx, BDFT_ERR_UNSUPPORTED,
                      "split-bf16 kernels need %zu bytes of LDS per work-group, the device "
@@ -608,34 +640,28 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   plan->mlds = ml;
   if (mfma) {
     const size_t nt = sizeof(OpTab) * tabs.tabs.size();
-    const size_t nh = sizeof(uint16_t) * tabs.hi.size();
-    const size_t nm = sizeof(uint32_t) * tabs.masks.size();
+    const size_t nf = sizeof(uint16_t) * tabs.frag.size();
+    const size_t nm = sizeof(uint32_t) * tabs.mq.size();
     auto al2 = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    int rc2 = dev_alloc(ctx, plan->tabbuf, al2(nt) + 3 * al2(nh) + al2(nm));
+    int rc2 = dev_alloc(ctx, plan->tabbuf, al2(nt) + al2(nf) + al2(nm));
     if (rc2) {
       delete plan;
       return rc2;
     }
     char* q = (char*)plan->tabbuf.p;
     hipError_t e = hipMemcpy(q, tabs.tabs.data(), nt, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nf)
+      e = hipMemcpy(q + al2(nt), tabs.frag.data(), nf, hipMemcpyHostToDevice);
     if (e == hipSuccess)
-      e = hipMemcpy(q + al2(nt), tabs.hi.data(), nh, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(q + al2(nt) + al2(nh), tabs.lo.data(), nh, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(q + al2(nt) + 2 * al2(nh), tabs.lo2.data(), nh, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(q + al2(nt) + 3 * al2(nh), tabs.masks.data(), nm, hipMemcpyHostToDevice);
+      e = hipMemcpy(q + al2(nt) + al2(nf), tabs.mq.data(), nm, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       dev_free(plan->tabbuf);
       delete plan;
       return set_err(ctx, BDFT_ERR_HIP, "operator table upload failed");
     }
     plan->B.tabs = (const OpTab*)q;
-    plan->B.thi = q + al2(nt);
-    plan->B.tlo = q + al2(nt) + al2(nh);
-    plan->B.tlo2 = q + al2(nt) + 2 * al2(nh);
-    plan->B.tmask = (const unsigned*)(q + al2(nt) + 3 * al2(nh));
+    plan->B.tfrag = q + al2(nt);
+    plan->B.tmq = (const uint2*)(q + al2(nt) + al2(nf));
   }
   size_t nb_part = 0;
   for (int e = 0; e < 4; ++e) nb_part += sizeof(double) * T.ntiles[epass[e]];

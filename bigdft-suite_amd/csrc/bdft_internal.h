@@ -53,8 +53,8 @@ struct BoxDesc {
 struct OpTab {
   int rows, cols;   // Dout, Din
   int Dp, Kp;       // padded to multiples of 32 and 16
-  long long off;    // element offset in the hi/lo arrays
-  int moff;         // offset of the block masks
+  long long off;    // unused (kept for layout)
+  int moff;         // offset of the row blocks' {mask, fragment} entries
   int pad_;
 };
 
@@ -194,15 +194,16 @@ struct Batch {
   double* ekin;         // per-box kinetic energy (device)
   double* epot;         // per-box potential energy (device)
   const OpTab* tabs;    // matrix-core operator tables (device)
-  const void* thi;      // bf16 hi parts
-  const void* tlo;      // bf16 lo parts
-  const void* tlo2;     // bf16 third parts (residual of hi + lo)
-  const unsigned* tmask;
+  const void* tfrag;    // operator fragments, matrix-core operand order
+  const uint2* tmq;     // per 32-row block: {K-step mask, first fragment}
 };
 
 // Passes of one application; each has its own tile table.
-enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_PF, P_PB,
+// P_KZG, P_ZPG: z-line passes of the matrix-core sequence, ZGROUP bundles of
+// 32 lines per tile
+enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_PF, P_PB, P_KZG, P_ZPG,
             P_COUNT };
+constexpr int ZGROUP = 3;
 
 struct PassTables {
   TileTable t[P_COUNT];
@@ -223,7 +224,9 @@ struct EnergyTables {
 };
 
 struct MfmaLds {
-  size_t kin[3], grow, zpot, shrink;
+  size_t kin[3], grow, zpot, shrink;  // per-pass sequence
+  size_t fwd, bwd, kz, zp;            // plane-fused sequence
+  int fused;                          // 1: plane-fused sequence
 };
 
 size_t lds_bytes(int maxN, int elem);
@@ -234,6 +237,10 @@ size_t mfma_lds_kin(int Wd, int Kp);
 size_t mfma_lds_grow(int W2, int L1, int Kp1, int Kp2);
 size_t mfma_lds_zpot(int KpG, int KpS);
 size_t mfma_lds_shrink(int L1, int W1, int W2, int KpU, int KpB);
+size_t mfma_lds_fwd(int W1, int W2, int L1);
+size_t mfma_lds_kz(int W3, int Kp);
+size_t mfma_lds_zp(int KpG, int DpG);
+size_t mfma_lds_bwd(int W1, int L1, int L2);
 size_t mfma_attributes(size_t lds);
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
                        long long wtotal, bool need_zero, int nterm, hipStream_t s,

@@ -39,39 +39,178 @@ __device__ __forceinline__ int drow(int r, int lane) {
   return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
 
-// acc += T[m0:m0+32, :] X[:, line bundle], X in LDS as [line*XS + k].
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// One 32x32 output tile of an operator T (table rows o0..o0+31, all K steps
+// flagged in the row block's mask) applied to a bundle of 32 lines held in
+// LDS as bf16 pieces [line * XS + k].
+//   TABB false: D = T X      -> D[m = output o][n = line]
+//   TABB true : D = X^T T^T  -> D[m = line][n = output o]
+// Both read the same fragments (lane l: row l%32, k 8(l/32)..+8 of the
+// step); only the operand slots differ, which picks the lane <-> index map
+// of the result (lane l, register r: m = drow(r, l), n = l % 32).
 // T = T0 + T1 + T2 and X = X0 + X1 (bf16 pieces); the products kept are
 // T0 X0 + T0 X1 + T1 X0 (NTERM 3) and also T2 X0 (NTERM 4), which removes
 // the fixed representation error of the filter (2^-16 relative).
+// one K step: acc[j] += (fragments f) x (bundle j's columns 16s..16s+15)
+template <int NTERM, bool TABB, int NBMAX>
+__device__ __forceinline__ void mma_step(const bf16x8 (&f)[3], const __bf16* bh,
+                                         const __bf16* bl, int LS, int s, int nb,
+                                         f32x16* acc) {
+#pragma unroll
+  for (int j = 0; j < NBMAX; ++j) {
+    if (j < nb) {
+      const bf16x8 xh8 = *(const bf16x8*)(bh + j * LS + 16 * s);
+      const bf16x8 xl8 = *(const bf16x8*)(bl + j * LS + 16 * s);
+      if (TABB) {
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, f[0], acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl8, f[0], acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, f[1], acc[j], 0, 0, 0);
+        if (NTERM == 4)
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, f[2], acc[j], 0, 0, 0);
+      } else {
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], xh8, acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], xl8, acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], xh8, acc[j], 0, 0, 0);
+        if (NTERM == 4)
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], xh8, acc[j], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int NTERM>
+__device__ __forceinline__ void load_frag(const bf16x8* fp, bf16x8 (&f)[3]) {
+  f[0] = fp[0];
+  f[1] = fp[64];
+  if (NTERM == 4) f[2] = fp[128];
+}
+
+// Sweep of the K steps of one 32-row block for nb <= NBMAX bundles; the
+// fragments of the next step are loaded while the current step computes
+// (two register sets used in turn, so no copy waits on a load).
+template <int NTERM, bool TABB, int NBMAX>
+__device__ __forceinline__ void mma_sweep(const Batch& B, const OpTab& t, int o0,
+                                          const __bf16* __restrict__ xh,
+                                          const __bf16* __restrict__ xl, int XS, int LS,
+                                          int nb, f32x16* acc) {
+  const int lane = threadIdx.x & 63;
+  const int kh = 8 * (lane >> 5);
+  const uint2 mq = B.tmq[t.moff + (o0 >> 5)];
+  unsigned mask = mq.x;
+  if (!mask) return;
+  const bf16x8* fp = (const bf16x8*)B.tfrag + (size_t)mq.y * 192 + lane;
+  const __bf16* bh = xh + (lane & 31) * XS + kh;
+  const __bf16* bl = xl + (lane & 31) * XS + kh;
+  bf16x8 fa[3], fb[3];
+  load_frag<NTERM>(fp, fa);
+  while (true) {
+    int s = __builtin_ctz(mask);
+    mask &= mask - 1;
+    if (mask) load_frag<NTERM>(fp += 192, fb);
+    mma_step<NTERM, TABB, NBMAX>(fa, bh, bl, LS, s, nb, acc);
+    if (!mask) break;
+    s = __builtin_ctz(mask);
+    mask &= mask - 1;
+    if (mask) load_frag<NTERM>(fp += 192, fa);
+    mma_step<NTERM, TABB, NBMAX>(fb, bh, bl, LS, s, nb, acc);
+    if (!mask) break;
+  }
+}
+
+template <int NTERM, bool TABB>
+__device__ __forceinline__ void mma(const Batch& B, const OpTab& t, int o0,
+                                    const __bf16* __restrict__ xh,
+                                    const __bf16* __restrict__ xl, int XS, f32x16& acc) {
+  mma_sweep<NTERM, TABB, 1>(B, t, o0, xh, xl, XS, 0, 1, &acc);
+}
+
 template <int NTERM>
 __device__ __forceinline__ void mfma_block(const Batch& B, const OpTab& t, int m0,
                                            const __bf16* __restrict__ xh,
                                            const __bf16* __restrict__ xl, int XS,
                                            f32x16& acc) {
+  mma<NTERM, false>(B, t, m0, xh, xl, XS, acc);
+}
+
+// Transpose of a 32 x 32 block of a line bundle through the matrix cores:
+// D = X I with X = the lines' columns n0..n0+31, so D[m = line][n = column]
+// holds hi + lo exactly (FP32), and lane l gets column n0 + l%32 of the 32
+// lines, four consecutive lines per register group.
+__device__ __forceinline__ void mma_ident(const __bf16* __restrict__ xh,
+                                          const __bf16* __restrict__ xl, int XS, int n0,
+                                          f32x16& acc) {
   const int lane = threadIdx.x & 63;
   const int kh = 8 * (lane >> 5);
-  const long long ro = t.off + (long long)(m0 + (lane & 31)) * t.Kp + kh;
-  const __bf16* th = (const __bf16*)B.thi + ro;
-  const __bf16* tl = (const __bf16*)B.tlo + ro;
-  const __bf16* t2 = (const __bf16*)B.tlo2 + ro;
-  const __bf16* bh = xh + (lane & 31) * XS + kh;
-  const __bf16* bl = xl + (lane & 31) * XS + kh;
-  unsigned mask = B.tmask[t.moff + (m0 >> 5)];
-  while (mask) {
-    const int s = __builtin_ctz(mask);
-    mask &= mask - 1;
-    const bf16x8 ah = *(const bf16x8*)(th + 16 * s);
-    const bf16x8 al = *(const bf16x8*)(tl + 16 * s);
-    const bf16x8 xh8 = *(const bf16x8*)(bh + 16 * s);
-    const bf16x8 xl8 = *(const bf16x8*)(bl + 16 * s);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, xh8, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, xl8, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, xh8, acc, 0, 0, 0);
-    if (NTERM == 4) {
-      const bf16x8 a2 = *(const bf16x8*)(t2 + 16 * s);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, xh8, acc, 0, 0, 0);
-    }
+  const int n = n0 + (lane & 31);
+  const __bf16* ah = xh + (lane & 31) * XS + kh;
+  const __bf16* al = xl + (lane & 31) * XS + kh;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int k0 = n0 + 16 * q;
+    bf16x8 id;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) id[u] = (__bf16)(k0 + kh + u == n ? 1.0f : 0.0f);
+    const bf16x8 xh8 = *(const bf16x8*)(ah + k0);
+    const bf16x8 xl8 = *(const bf16x8*)(al + k0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, id, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl8, id, acc, 0, 0, 0);
   }
+}
+
+// The same product for up to MB line bundles at once (bundle j at xh + j*LS),
+// so that each table fragment is fetched once per K step for all of them:
+// acc[j] += T[o0..o0+31, :] X_j (orientation as in mma).
+constexpr int MB = 2;   // bundles per sweep in the plane kernels
+template <int NTERM, bool TABB, int NBMAX>
+__device__ __forceinline__ void mma_multi(const Batch& B, const OpTab& t, int o0,
+                                          const __bf16* __restrict__ xh,
+                                          const __bf16* __restrict__ xl, int XS, int LS,
+                                          int nb, f32x16 (&acc)[NBMAX]) {
+  mma_sweep<NTERM, TABB, NBMAX>(B, t, o0, xh, xl, XS, LS, nb, acc);
+}
+
+// Store the 16 results of a lane (one column n, rows m in four groups of four
+// consecutive m) into a bf16 hi/lo buffer laid out [n * XS + m], 8 bytes per
+// group and piece.
+__device__ __forceinline__ void store_cols(const f32x16& acc, __bf16* __restrict__ yh,
+                                           __bf16* __restrict__ yl, int XS, int n, int m0) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    bf16x4 h, l;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      __bf16 a, b;
+      split2(acc[4 * g + u], a, b);
+      h[u] = a;
+      l[u] = b;
+    }
+    const int o = n * XS + m0 + 8 * g + 4 * (lane >> 5);
+    *(bf16x4*)(yh + o) = h;
+    *(bf16x4*)(yl + o) = l;
+  }
+}
+
+// Work split of a phase: nunits (operator, output block) units over nlb line
+// bundles, cut into chunks so that about nw waves have work; task q covers
+// bundles [lb_begin, lb_end) of unit u.
+struct Task {
+  int u, lb_begin, lb_end;
+};
+__device__ __forceinline__ int task_chunks(int nunits, int nlb, int nw) {
+  const int c = nw / max(nunits, 1);
+  return max(1, min(nlb, c));
+}
+__device__ __forceinline__ Task task_of(int q, int chunks, int nlb) {
+  const int cs = (nlb + chunks - 1) / chunks;
+  Task t;
+  t.u = q / chunks;
+  const int c = q - t.u * chunks;
+  t.lb_begin = c * cs;
+  t.lb_end = min(nlb, t.lb_begin + cs);
+  return t;
 }
 
 __device__ __forceinline__ void zero_lds(unsigned* p, int nwords) {
@@ -94,9 +233,14 @@ __device__ __forceinline__ double wave_block_sum(double v, double* red, int nwav
 __device__ __forceinline__ int rup(int a, int m) { return (a + m - 1) / m * m; }
 
 // ---------------------------------------------------------------------------
-// kinetic along AX on the wavelet volume: K (=, AX 0) or (+=) Ktilde_AX W
+// kinetic along AX on the wavelet volume: K (=, AX 0) or (+=) Ktilde_AX W.
+// The per-tile partial of ekin is sum W * (Ktilde_AX W) or, with TOTAL, the
+// sum of W times the complete K after this pass (the kinetic along the other
+// axes already accumulated in K), i.e. the whole <psi|T psi> of the tile.
+// x lines take the table as the B operand so that lanes run along x, the
+// contiguous direction of K; y and z lines run along x across lanes anyway.
 // ---------------------------------------------------------------------------
-template <int AX, int NT4>
+template <int AX, int NT4, bool TOTAL>
 __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
@@ -131,51 +275,61 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
     lstride = 1;
     pstride = AX == 1 ? (long long)W1 : (long long)W1 * W2;
   }
-  zero_lds((unsigned*)smem, 32 * XS);  // hi and lo planes (2 bytes each)
-  __syncthreads();
+  // the K steps read columns k < t.Kp: columns Wd..Kp-1 and lines nl..31
+  // are zero
   const float* W = (const float*)B.W;
-  const int tot = 32 * Wd;
+  const int KpW = t.Kp;
+  const int tot = 32 * KpW;
   for (int i0 = threadIdx.x; i0 < tot; i0 += NTM * 8) {
     float v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u * NTM;
       int l, k;
-      if (AX == 0) { l = i / Wd; k = i - l * Wd; } else { k = i >> 5; l = i & 31; }
-      v[u] = (i < tot && l < nl) ? W[base + l * lstride + k * pstride] : 0.0f;
+      if (AX == 0) { l = i / KpW; k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
+      v[u] = (i < tot && l < nl && k < Wd) ? W[base + l * lstride + k * pstride] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u * NTM;
       if (i < tot) {
         int l, k;
-        if (AX == 0) { l = i / Wd; k = i - l * Wd; } else { k = i >> 5; l = i & 31; }
+        if (AX == 0) { l = i / KpW; k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
         __bf16 h, lo;
         split2(v[u], h, lo);
         xh[l * XS + k] = h;
         xl[l * XS + k] = lo;
-        xf[l * XF + k] = v[u];
+        if (k < Wd) xf[l * XF + k] = v[u];
       }
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const float ks = (float)bd.kscale[AX];
   float* K = (float*)B.K;
   double ek = 0.0;
-  const int line = lane & 31;
   for (int ob = wv; ob < t.Dp / 32; ob += NTM / 64) {
     f32x16 acc = {};
-    mfma_block<NT4>(B, t, 32 * ob, xh, xl, XS, acc);
+    constexpr bool TB = AX == 0;
+    mma<NT4, TB>(B, t, 32 * ob, xh, xl, XS, acc);
+    float kold[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int o = 32 * ob + drow(r, lane);
+      const int line = TB ? drow(r, lane) : (lane & 31);
+      const int o = 32 * ob + (TB ? (lane & 31) : drow(r, lane));
+      kold[r] = 0.0f;
+      if (AX != 0 && o < Wd && line < nl) kold[r] = K[base + line * lstride + o * pstride];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int line = TB ? drow(r, lane) : (lane & 31);
+      const int o = 32 * ob + (TB ? (lane & 31) : drow(r, lane));
       if (o < Wd && line < nl) {
         const float v = acc[r] * ks;
-        ek += (double)v * (double)xf[line * XF + o];
-        const long long a = base + line * lstride + o * pstride;
-        if (AX == 0) K[a] = v;
-        else K[a] += v;
+        const float kn = v + kold[r];
+        ek += (double)(TOTAL ? kn : v) * (double)xf[line * XF + o];
+        K[base + line * lstride + o * pstride] = kn;
       }
     }
   }
@@ -225,7 +379,8 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // G_x: lines j2, outputs p1
   {
     const int nob = tx.Dp / 32, nlb = n1 / 32;
@@ -313,7 +468,8 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int line = lane & 31;
   const int q1 = j1 + line - bd.ishift[0], q2 = j2 - bd.ishift[1];
   const bool in12 = line < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
@@ -400,7 +556,8 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   {  // G_y^T: lines p1, outputs j2
     const int nob = sy.Dp / 32, nlb = nu / 32;
     for (int it = wv; it < nob * nlb; it += NTMP / 64) {
@@ -465,6 +622,553 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
 }
 
 // ---------------------------------------------------------------------------
+// Plane-fused forward pass, one work-group per x-y plane j3 of a box:
+//   K  = (Ktilde_x + Ktilde_y) W      (kinetic of the plane)
+//   T2 = G_y G_x W                    (real space in x, y; wavelet z)
+// LDS: R = the W plane as lines j2 [j2][j1] (bf16 hi, lo); C = G_x W as lines
+// p1 [p1][j2]; after G_y has consumed C the same space holds W^T [j1][j2]
+// (made by the matrix cores, mma_ident) for the y kinetic.
+// Work units: one wave per (operator, 32-row output block), sweeping all line
+// bundles of the plane, so each table fragment is read once per plane.
+// Orientation: x kinetic with the table as B (rows j2, lanes j1 -> K stored
+// along x); y kinetic with the table as A (rows j2, lanes j1 -> added to K
+// along x); G_x as B (lanes p1 -> columns of C); G_y as A (lanes p1 -> T2).
+// ---------------------------------------------------------------------------
+template <int NT4>
+__global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int j3 = tile - box_first_tile(tt, b);
+  const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], L1 = bd.L[0], L2 = bd.L[1];
+  const OpTab tx = B.tabs[bd.opG[0]], ty = B.tabs[bd.opG[1]];
+  const OpTab kx = B.tabs[bd.opK[0]], ky = B.tabs[bd.opK[1]];
+  const int KX = rup(W1, 32), nR = rup(W2, 32);
+  const int XR = KX + 8, XC = nR + 8;
+  const int nC = max(rup(L1, 32), KX);
+  __bf16* rh = (__bf16*)smem;  // [j2 * XR + j1], j2 < nR, j1 < KX (zero padded)
+  __bf16* rl = rh + nR * XR;
+  __bf16* ch = rl + nR * XR;   // [p1 * XC + j2] then [j1 * XC + j2]
+  __bf16* cl = ch + nC * XC;
+  // load the plane, two x-neighbours per thread (W1 is even)
+  {
+    const float2* src =
+        (const float2*)((const float*)B.W + bd.w_off + (long long)W1 * W2 * j3);
+    const int hp = KX / 2;
+    const int tot = nR * hp;
+    for (int i0 = threadIdx.x; i0 < tot; i0 += NTMP * 8) {
+      float2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * NTMP;
+        const int j2 = i / hp, j1 = 2 * (i - j2 * hp);
+        v[u] = (i < tot && j2 < W2 && j1 < W1) ? src[(j2 * W1 + j1) >> 1]
+                                                : make_float2(0.0f, 0.0f);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * NTMP;
+        if (i < tot) {
+          const int j2 = i / hp, j1 = 2 * (i - j2 * hp);
+          __bf16 h0, l0, h1, l1;
+          split2(v[u].x, h0, l0);
+          split2(v[u].y, h1, l1);
+          *(bf16x2*)(rh + j2 * XR + j1) = bf16x2{h0, h1};
+          *(bf16x2*)(rl + j2 * XR + j1) = bf16x2{l0, l1};
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int NW = NTMP / 64;
+  const int nlb = nR / 32;          // line blocks j2
+  const int nobK = KX / 32;         // output blocks j1 of the kinetic
+  const int nobG = tx.Dp / 32;      // output blocks p1 of G_x
+  float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
+  const float ksx = (float)bd.kscale[0], ksy = (float)bd.kscale[1];
+  // x kinetic -> K, G_x -> C
+  const int ch2 = task_chunks(nobK + nobG, nlb, NW);
+  for (int q = wv; q < (nobK + nobG) * ch2; q += NW) {
+    const Task tk = task_of(q, ch2, nlb);
+    const bool kin = tk.u < nobK;
+    const int ob = kin ? tk.u : tk.u - nobK;
+    const OpTab& t = kin ? kx : tx;
+    for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
+      const int nb = min(MB, tk.lb_end - lb0);
+      f32x16 acc[MB] = {};
+      mma_multi<NT4, true, MB>(B, t, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
+                           32 * XR, nb, acc);
+      const int n = 32 * ob + (lane & 31);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j >= nb) break;
+        const int lb = lb0 + j;
+        if (kin) {
+          if (n < W1) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const int j2 = 32 * lb + drow(r, lane);
+              if (j2 < W2) K[n + W1 * j2] = acc[j][r] * ksx;
+            }
+          }
+        } else {
+          store_cols(acc[j], ch, cl, XC, n, 32 * lb);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // G_y: lines p1, outputs p2 -> T2 plane, lanes along p1
+  {
+    float* dst = (float*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
+    const int nl1 = rup(L1, 32) / 32, nob = ty.Dp / 32;
+    const int ch3 = task_chunks(nob, nl1, NW);
+    for (int q = wv; q < nob * ch3; q += NW) {
+      const Task tk = task_of(q, ch3, nl1);
+      const int ob = tk.u;
+      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
+        const int nb = min(MB, tk.lb_end - lb0);
+        f32x16 acc[MB] = {};
+        mma_multi<NT4, false, MB>(B, ty, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
+                              32 * XC, nb, acc);
+#pragma unroll
+        for (int j = 0; j < MB; ++j) {
+          if (j >= nb) break;
+          const int p1 = 32 * (lb0 + j) + (lane & 31);
+          if (p1 < L1) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const int p2 = 32 * ob + drow(r, lane);
+              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r];
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // W^T into the C space
+  for (int q = wv; q < nlb * nobK; q += NW) {
+    const int lb = q / nobK, nb = q - lb * nobK;
+    f32x16 acc = {};
+    mma_ident(rh + 32 * lb * XR, rl + 32 * lb * XR, XR, 32 * nb, acc);
+    store_cols(acc, ch, cl, XC, 32 * nb + (lane & 31), 32 * lb);
+  }
+  __syncthreads();
+  // y kinetic: table rows = output block j2 (lb), lines = j1 blocks; K +=
+  const int ch5 = task_chunks(nlb, nobK, NW);
+  for (int q = wv; q < nlb * ch5; q += NW) {
+    const Task tk = task_of(q, ch5, nobK);
+    const int lb = tk.u;
+    for (int ob0 = tk.lb_begin; ob0 < tk.lb_end; ob0 += MB) {
+      const int nb = min(MB, tk.lb_end - ob0);
+      f32x16 acc[MB] = {};
+      mma_multi<NT4, false, MB>(B, ky, 32 * lb, ch + 32 * ob0 * XC, cl + 32 * ob0 * XC, XC,
+                            32 * XC, nb, acc);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j >= nb) break;
+        const int j1 = 32 * (ob0 + j) + (lane & 31);
+        if (j1 >= W1) continue;
+        float kold[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int j2 = 32 * lb + drow(r, lane);
+          kold[r] = j2 < W2 ? K[j1 + W1 * j2] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int j2 = 32 * lb + drow(r, lane);
+          if (j2 < W2) K[j1 + W1 * j2] = kold[r] + acc[j][r] * ksy;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plane-fused backward pass, one work-group per plane j3:
+//   hpsi += compress(G_y^T G_x^T T2 + K)        (compress_and_accumulate_*)
+// LDS: R = the T2 plane as lines p2 [p2][p1]; C = G_x^T T2 as lines j1
+// [j1][p2] (table as B operand); G_y^T with the table as A operand puts
+// lanes along j1 for the K reads and the compress.  One wave per output
+// block, sweeping all line bundles.
+// ---------------------------------------------------------------------------
+template <int NT4>
+__global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int j3 = tile - box_first_tile(tt, b);
+  const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], L1 = bd.L[0], L2 = bd.L[1];
+  const int W1 = 2 * N1, W2 = 2 * N2;
+  const OpTab sx = B.tabs[bd.opS[0]], sy = B.tabs[bd.opS[1]];
+  const int KX = rup(L1, 32), nR = rup(L2, 32);
+  const int XR = KX + 8, XC = nR + 8;
+  const int nC = rup(W1, 32);
+  __bf16* rh = (__bf16*)smem;  // [p2 * XR + p1]
+  __bf16* rl = rh + nR * XR;
+  __bf16* ch = rl + nR * XR;   // [j1 * XC + p2]
+  __bf16* cl = ch + nC * XC;
+  {
+    const float* src = (const float*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
+    const int tot = nR * KX;
+    for (int i0 = threadIdx.x; i0 < tot; i0 += NTMP * 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * NTMP;
+        const int p2 = i / KX, p1 = i - p2 * KX;
+        v[u] = (i < tot && p2 < L2 && p1 < L1) ? src[p1 + L1 * p2] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * NTMP;
+        if (i < tot) {
+          const int p2 = i / KX, p1 = i - p2 * KX;
+          __bf16 h, l;
+          split2(v[u], h, l);
+          rh[p2 * XR + p1] = h;
+          rl[p2 * XR + p1] = l;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int NW = NTMP / 64;
+  {  // G_x^T: lines p2, outputs j1 -> C[j1][p2]
+    const int nlb = nR / 32, nob = sx.Dp / 32;
+    const int chx = task_chunks(nob, nlb, NW);
+    for (int q = wv; q < nob * chx; q += NW) {
+      const Task tk = task_of(q, chx, nlb);
+      const int ob = tk.u;
+      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
+        const int nb = min(MB, tk.lb_end - lb0);
+        f32x16 acc[MB] = {};
+        mma_multi<NT4, true, MB>(B, sx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
+                             32 * XR, nb, acc);
+#pragma unroll
+        for (int j = 0; j < MB; ++j) {
+          if (j >= nb) break;
+          store_cols(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // G_y^T: lines j1, outputs j2, into the plane O [j2][j1] (FP32, over the
+  // dead R space), lanes along j1
+  const int RO = W1 + 1;
+  float* O = (float*)smem;
+  const int nl1 = nC / 32, nob = sy.Dp / 32;
+  const int chy = task_chunks(nob, nl1, NW);
+  for (int q = wv; q < nob * chy; q += NW) {
+    const Task tk = task_of(q, chy, nl1);
+    const int ob = tk.u;
+    for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
+      const int nb = min(MB, tk.lb_end - lb0);
+      f32x16 acc[MB] = {};
+      mma_multi<NT4, false, MB>(B, sy, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
+                                32 * XC, nb, acc);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j >= nb) break;
+        const int j1 = 32 * (lb0 + j) + (lane & 31);
+        if (j1 >= W1) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int j2 = 32 * ob + drow(r, lane);
+          if (j2 < W2) O[j2 * RO + j1] = acc[j][r];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // + K, compress-accumulate (un-compress.f90:2174-2238 mixed, 2105-2170
+  // standard: the same scatter in this layout); one coarse point per thread,
+  // its four components of this plane (s3 fixed by j3) together
+  const int s3 = j3 >= N3;
+  const int i3 = j3 - s3 * N3;
+  const float* Kp = (const float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
+  double* hc = B.hpsi + bd.psi_off;
+  double* hf = B.hpsi + bd.psi_off + bd.nvctr_c;
+  const int npl = N1 * N2;
+  for (int i = threadIdx.x; i < npl; i += NTMP) {
+    const int i2 = i / N1, i1 = i - i2 * N1;
+    const int lin = i + npl * i3;
+    const int ci = bd.cinv ? bd.cinv[lin] : lin;
+    const int fi = bd.finv ? bd.finv[lin] : lin;
+    double v[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int jj1 = i1 + N1 * (s & 1), jj2 = i2 + N2 * (s >> 1);
+      v[s] = (double)O[jj2 * RO + jj1] + (double)Kp[jj1 + W1 * jj2];
+    }
+    if (s3 == 0) {
+      if (ci >= 0) hc[ci] += v[0];
+      if (fi >= 0) {
+        double* f = hf + 7LL * fi;
+        f[0] += v[1];
+        f[1] += v[2];
+        f[2] += v[3];
+      }
+    } else if (fi >= 0) {
+      double* f = hf + 7LL * fi + 3;
+      f[0] += v[0];
+      f[1] += v[1];
+      f[2] += v[2];
+      f[3] += v[3];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z lines, ZG bundles of 32 lines per work-group (each table fragment read
+// once for all of them).  Kinetic z: K += Ktilde_z W, and the tile's share of
+// ekin = sum W * K (K complete after this pass).
+// ---------------------------------------------------------------------------
+constexpr int ZG = ZGROUP;
+constexpr int ZU = 24;   // loads in flight per thread in the z-line loaders
+
+template <int NT4>
+__global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NTM / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], W3 = 2 * bd.N[2];
+  const OpTab t = B.tabs[bd.opK[2]];
+  const int XS = t.Kp + 8, XF = W3 + 1, KpW = t.Kp;
+  const int nxt = (W1 + 31) / 32;
+  const int nbund = nxt * W2;
+  const int nb = min(ZG, nbund - ZG * lt);
+  const long long pstride = (long long)W1 * W2;
+  __bf16* xh = (__bf16*)smem;          // [j][line][k]
+  __bf16* xl = xh + ZG * 32 * XS;
+  float* xf = (float*)(xl + ZG * 32 * XS);  // [j][line][k], k < W3
+  // z lines of 32 consecutive x: pairs of x-neighbours per load (W1 even,
+  // bundle starts at multiples of 32), all loads of a thread in flight at once
+  const float* W = (const float*)B.W;
+  const int per = 16 * KpW;         // pairs per bundle
+  const int tot = nb * per;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += NTM * ZU) {
+    float2 v[ZU];
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const int i = i0 + u * NTM;
+      const int j = i / per, rem = i - j * per;
+      const int k = rem >> 4, l = 2 * (rem & 15);
+      const int bi = ZG * lt + j;
+      const int jo = bi / nxt, j1 = (bi - jo * nxt) * 32 + l;
+      v[u] = (i < tot && j1 < W1 && k < W3)
+                 ? *(const float2*)(W + bd.w_off + j1 + (long long)W1 * jo + k * pstride)
+                 : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const int i = i0 + u * NTM;
+      if (i < tot) {
+        const int j = i / per, rem = i - j * per;
+        const int k = rem >> 4, l = 2 * (rem & 15);
+        __bf16 h, lo;
+        const int o = (j * 32 + l) * XS + k;
+        split2(v[u].x, h, lo);
+        xh[o] = h;
+        xl[o] = lo;
+        split2(v[u].y, h, lo);
+        xh[o + XS] = h;
+        xl[o + XS] = lo;
+        if (k < W3) {
+          xf[(j * 32 + l) * XF + k] = v[u].x;
+          xf[(j * 32 + l + 1) * XF + k] = v[u].y;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const float ks = (float)bd.kscale[2];
+  float* K = (float*)B.K;
+  double ek = 0.0;
+  for (int ob = wv; ob < t.Dp / 32; ob += NTM / 64) {
+    // the K values this wave updates, read while the matrix cores run
+    float kold[ZG][16];
+    long long base[ZG];
+    bool ok[ZG];
+#pragma unroll
+    for (int j = 0; j < ZG; ++j) {
+      const int bi = ZG * lt + j;
+      const int jo = bi / nxt, j1 = (bi - jo * nxt) * 32 + (lane & 31);
+      ok[j] = j < nb && j1 < W1;
+      base[j] = bd.w_off + j1 + (long long)W1 * jo;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = 32 * ob + drow(r, lane);
+        kold[j][r] = (ok[j] && o < W3) ? K[base[j] + o * pstride] : 0.0f;
+      }
+    }
+    f32x16 acc[ZG] = {};
+    mma_multi<NT4, false, ZG>(B, t, 32 * ob, xh, xl, XS, 32 * XS, nb, acc);
+#pragma unroll
+    for (int j = 0; j < ZG; ++j) {
+      if (!ok[j]) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = 32 * ob + drow(r, lane);
+        if (o < W3) {
+          const float kn = kold[j][r] + acc[j][r] * ks;
+          ek += (double)kn * (double)xf[(j * 32 + (lane & 31)) * XF + o];
+          K[base[j] + o * pstride] = kn;
+        }
+      }
+    }
+  }
+  const double eks = wave_block_sum(ek, red, NTM / 64);
+  if (threadIdx.x == 0) part[tile] = eks;
+}
+
+// z lines of T2, ZG bundles per work-group: G_z, V (epot), G_z^T in place.
+template <int NT4>
+__global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NTM / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int L1 = bd.L[0], L2 = bd.L[1], W3 = 2 * bd.N[2], L3 = bd.L[2];
+  const OpTab tg = B.tabs[bd.opG[2]], tsh = B.tabs[bd.opS[2]];
+  const int XS = tg.Kp + 8, KpG = tg.Kp;
+  const int RS = tg.Dp + 8;   // rows p3 < tg.Dp are written; tsh reads p3 < tsh.Kp
+  const int nxt = (L1 + 31) / 32;
+  const int nbund = nxt * L2;
+  const int nb = min(ZG, nbund - ZG * lt);
+  const long long pstride = (long long)L1 * L2;
+  float* T2 = (float*)B.T2 + bd.t2_off;
+  __bf16* xh = (__bf16*)smem;
+  __bf16* xl = xh + ZG * 32 * XS;
+  __bf16* rh = xl + ZG * 32 * XS;
+  __bf16* rl = rh + ZG * 32 * RS;
+  // z lines of 32 consecutive x; pairs of x-neighbours when L1 is even
+  const bool pairs = (L1 & 1) == 0;
+  const int ve = pairs ? 2 : 1;
+  const int per = (32 / ve) * KpG;
+  const int tot = nb * per;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += NTM * ZU) {
+    float2 v[ZU];
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const int i = i0 + u * NTM;
+      const int j = i / per, rem = i - j * per;
+      const int k = pairs ? rem >> 4 : rem >> 5;
+      const int l = pairs ? 2 * (rem & 15) : (rem & 31);
+      const int bi = ZG * lt + j;
+      const int p2 = bi / nxt, p1 = (bi - p2 * nxt) * 32 + l;
+      v[u] = make_float2(0.0f, 0.0f);
+      if (i < tot && p1 < L1 && k < W3) {
+        const float* a = T2 + p1 + (long long)L1 * p2 + k * pstride;
+        if (pairs) v[u] = *(const float2*)a;
+        else v[u].x = *a;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+      const int i = i0 + u * NTM;
+      if (i < tot) {
+        const int j = i / per, rem = i - j * per;
+        const int k = pairs ? rem >> 4 : rem >> 5;
+        const int l = pairs ? 2 * (rem & 15) : (rem & 31);
+        const int o = (j * 32 + l) * XS + k;
+        __bf16 h, lo;
+        split2(v[u].x, h, lo);
+        xh[o] = h;
+        xl[o] = lo;
+        if (pairs) {
+          split2(v[u].y, h, lo);
+          xh[o + XS] = h;
+          xl[o + XS] = lo;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  double ep = 0.0;
+  // G_z, V: rows p3 < tg.Dp (zero beyond L3) -> R[j][line][p3]
+  for (int ob = wv; ob < tg.Dp / 32; ob += NTM / 64) {
+    f32x16 acc[ZG] = {};
+    mma_multi<NT4, false, ZG>(B, tg, 32 * ob, xh, xl, XS, 32 * XS, nb, acc);
+#pragma unroll
+    for (int j = 0; j < ZG; ++j) {
+      if (j >= nb) break;
+      const int bi = ZG * lt + j;
+      const int p2 = bi / nxt, line = lane & 31;
+      const int p1 = (bi - p2 * nxt) * 32 + line;
+      const int q1 = p1 - bd.ishift[0], q2 = p2 - bd.ishift[1];
+      const bool in12 = p1 < L1 && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
+                        q2 < bd.potdim[1];
+      const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
+      double vv[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int p3 = 32 * ob + drow(r, lane);
+        const int q3 = p3 - bd.ishift[2];
+        vv[r] = (in12 && p3 < L3 && q3 >= 0 && q3 < bd.potdim[2]) ? pot[pz * q3] : 0.0;
+      }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        bf16x4 h, l;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int r = 4 * g + u;
+          const double pr = (double)acc[j][r];
+          ep = fma(vv[r] * pr, pr, ep);
+          __bf16 a, c;
+          split2((float)(vv[r] * pr), a, c);
+          h[u] = a;
+          l[u] = c;
+        }
+        const int o = (j * 32 + line) * RS + 32 * ob + 8 * g + 4 * (lane >> 5);
+        *(bf16x4*)(rh + o) = h;
+        *(bf16x4*)(rl + o) = l;
+      }
+    }
+  }
+  const double eps = wave_block_sum(ep, red, NTM / 64);
+  if (threadIdx.x == 0) part[tile] = eps;
+  __syncthreads();
+  // G_z^T back to the wavelet z lines, in place
+  for (int ob = wv; ob < tsh.Dp / 32; ob += NTM / 64) {
+    f32x16 acc[ZG] = {};
+    mma_multi<NT4, false, ZG>(B, tsh, 32 * ob, rh, rl, RS, 32 * RS, nb, acc);
+#pragma unroll
+    for (int j = 0; j < ZG; ++j) {
+      if (j >= nb) break;
+      const int bi = ZG * lt + j;
+      const int p2 = bi / nxt;
+      const int p1 = (bi - p2 * nxt) * 32 + (lane & 31);
+      if (p1 >= L1) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = 32 * ob + drow(r, lane);
+        if (o < W3) T2[p1 + (long long)L1 * p2 + o * pstride] = acc[j][r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // LDS sizes and launcher
 // ---------------------------------------------------------------------------
 static inline int rup_h(int a, int m) { return (a + m - 1) / m * m; }
@@ -484,11 +1188,30 @@ size_t mfma_lds_shrink(int L1, int W1, int W2, int KpU, int KpB) {
   return (u > o ? u : o) + (size_t)2 * 2 * rup_h(W2, 32) * (KpB + 8);
 }
 
+size_t mfma_lds_fwd(int W1, int W2, int L1) {
+  const int nR = rup_h(W2, 32), KX = rup_h(W1, 32);
+  const int nC = rup_h(L1, 32) > KX ? rup_h(L1, 32) : KX;
+  return (size_t)4 * nR * (KX + 8) + (size_t)4 * nC * (nR + 8);
+}
+size_t mfma_lds_kz(int W3, int Kp) {
+  return (size_t)ZGROUP * ((size_t)2 * 32 * (Kp + 8) * 2 + (size_t)32 * (W3 + 1) * 4);
+}
+size_t mfma_lds_zp(int KpG, int DpG) {
+  return (size_t)ZGROUP * ((size_t)2 * 32 * (KpG + 8) * 2 + (size_t)2 * 32 * (DpG + 8) * 2);
+}
+size_t mfma_lds_bwd(int W1, int L1, int L2) {
+  const int nR = rup_h(L2, 32), KX = rup_h(L1, 32), nC = rup_h(W1, 32);
+  return (size_t)4 * nR * (KX + 8) + (size_t)4 * nC * (nR + 8);
+}
+
 template <int NT4>
 static void set_attr(size_t lds) {
-  const void* k[] = {(const void*)k_mkin<0, NT4>, (const void*)k_mkin<1, NT4>,
-                     (const void*)k_mkin<2, NT4>, (const void*)k_mgrow_plane<NT4>,
-                     (const void*)k_mzpot<NT4>, (const void*)k_mshrink_plane<NT4>};
+  const void* k[] = {(const void*)k_mkin<0, NT4, false>, (const void*)k_mkin<1, NT4, false>,
+                     (const void*)k_mkin<2, NT4, false>, (const void*)k_mkin<2, NT4, true>,
+                     (const void*)k_mgrow_plane<NT4>, (const void*)k_mzpot<NT4>,
+                     (const void*)k_mshrink_plane<NT4>, (const void*)k_mfwd<NT4>,
+                     (const void*)k_mbwd<NT4>, (const void*)k_mkz<NT4>,
+                     (const void*)k_mzp<NT4>};
   for (const void* f : k)
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
@@ -517,41 +1240,69 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
   mark(S_UNC, 0);
   if (PT.ntiles[P_UNC] > 0) launch_uncompress_float(B, PT.t[P_UNC], PT.ntiles[P_UNC], s);
   mark(S_UNC, 1);
-  mark(S_XF, 0);
-  if (PT.ntiles[P_XF] > 0)
-    k_mkin<0, NT4><<<PT.ntiles[P_XF], NTM, L.kin[0], s>>>(B, PT.t[P_XF], PT.part[0]);
-  mark(S_XF, 1);
-  mark(S_KY, 0);
-  if (PT.ntiles[P_KY] > 0)
-    k_mkin<1, NT4><<<PT.ntiles[P_KY], NTM, L.kin[1], s>>>(B, PT.t[P_KY], PT.part[1]);
-  mark(S_KY, 1);
-  mark(S_KZ, 0);
-  if (PT.ntiles[P_KZ] > 0)
-    k_mkin<2, NT4><<<PT.ntiles[P_KZ], NTM, L.kin[2], s>>>(B, PT.t[P_KZ], PT.part[2]);
-  mark(S_KZ, 1);
-  mark(S_PF, 0);
-  if (PT.ntiles[P_PF] > 0)
-    k_mgrow_plane<NT4><<<PT.ntiles[P_PF], NTMP, L.grow, s>>>(B, PT.t[P_PF]);
-  mark(S_PF, 1);
-  mark(S_ZP, 0);
-  if (PT.ntiles[P_ZP] > 0)
-    k_mzpot<NT4><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);
-  mark(S_ZP, 1);
-  mark(S_PB, 0);
-  if (PT.ntiles[P_PB] > 0)
-    k_mshrink_plane<NT4><<<PT.ntiles[P_PB], NTMP, L.shrink, s>>>(B, PT.t[P_PB]);
-  mark(S_PB, 1);
   EnergyTables et;
-  et.t[0] = PT.t[P_XF];
-  et.p[0] = PT.part[0];
-  et.t[1] = PT.t[P_KY];
-  et.p[1] = PT.part[1];
-  et.t[2] = PT.t[P_KZ];
-  et.p[2] = PT.part[2];
-  et.t[3] = PT.t[P_ZP];
-  et.p[3] = PT.part[3];
-  et.nk = 3;
-  et.n = 4;
+  if (L.fused) {
+    mark(S_PF, 0);
+    if (PT.ntiles[P_PF] > 0)
+      k_mfwd<NT4><<<PT.ntiles[P_PF], NTMP, L.fwd, s>>>(B, PT.t[P_PF]);
+    mark(S_PF, 1);
+    mark(S_KZ, 0);
+    if (PT.ntiles[P_KZG] > 0)
+      k_mkz<NT4><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
+    mark(S_KZ, 1);
+    mark(S_ZP, 0);
+    if (PT.ntiles[P_ZPG] > 0)
+      k_mzp<NT4><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
+    mark(S_ZP, 1);
+    mark(S_PB, 0);
+    if (PT.ntiles[P_PB] > 0)
+      k_mbwd<NT4><<<PT.ntiles[P_PB], NTMP, L.bwd, s>>>(B, PT.t[P_PB]);
+    mark(S_PB, 1);
+    et.t[0] = PT.t[P_KZG];
+    et.p[0] = PT.part[2];
+    et.t[1] = PT.t[P_ZPG];
+    et.p[1] = PT.part[3];
+    et.nk = 1;
+    et.n = 2;
+  } else {
+    mark(S_XF, 0);
+    if (PT.ntiles[P_XF] > 0)
+      k_mkin<0, NT4, false><<<PT.ntiles[P_XF], NTM, L.kin[0], s>>>(B, PT.t[P_XF],
+                                                                   PT.part[0]);
+    mark(S_XF, 1);
+    mark(S_KY, 0);
+    if (PT.ntiles[P_KY] > 0)
+      k_mkin<1, NT4, false><<<PT.ntiles[P_KY], NTM, L.kin[1], s>>>(B, PT.t[P_KY],
+                                                                   PT.part[1]);
+    mark(S_KY, 1);
+    mark(S_KZ, 0);
+    if (PT.ntiles[P_KZ] > 0)
+      k_mkin<2, NT4, false><<<PT.ntiles[P_KZ], NTM, L.kin[2], s>>>(B, PT.t[P_KZ],
+                                                                   PT.part[2]);
+    mark(S_KZ, 1);
+    mark(S_PF, 0);
+    if (PT.ntiles[P_PF] > 0)
+      k_mgrow_plane<NT4><<<PT.ntiles[P_PF], NTMP, L.grow, s>>>(B, PT.t[P_PF]);
+    mark(S_PF, 1);
+    mark(S_ZP, 0);
+    if (PT.ntiles[P_ZP] > 0)
+      k_mzpot<NT4><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);
+    mark(S_ZP, 1);
+    mark(S_PB, 0);
+    if (PT.ntiles[P_PB] > 0)
+      k_mshrink_plane<NT4><<<PT.ntiles[P_PB], NTMP, L.shrink, s>>>(B, PT.t[P_PB]);
+    mark(S_PB, 1);
+    et.t[0] = PT.t[P_XF];
+    et.p[0] = PT.part[0];
+    et.t[1] = PT.t[P_KY];
+    et.p[1] = PT.part[1];
+    et.t[2] = PT.t[P_KZ];
+    et.p[2] = PT.part[2];
+    et.t[3] = PT.t[P_ZP];
+    et.p[3] = PT.part[3];
+    et.nk = 3;
+    et.n = 4;
+  }
   mark(S_ENERGY, 0);
   launch_energy(et, B, s);
   mark(S_ENERGY, 1);
@@ -560,6 +1311,12 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     for (int st : unused) {
       mark(st, 0);
       mark(st, 1);
+    }
+    if (L.fused) {
+      mark(S_XF, 0);
+      mark(S_XF, 1);
+      mark(S_KY, 0);
+      mark(S_KY, 1);
     }
   }
   return hipGetLastError();

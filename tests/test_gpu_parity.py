@@ -358,21 +358,25 @@ BF_E = 1e-6
 BF_H = 1e-4
 
 
+@pytest.mark.parametrize("seq", ["plane", "lines"])
 @pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
                                         ("F", (15, 13, 17), 3), ("P", (31, 19, 24), 4),
-                                        ("F", (39, 39, 39), 5)])
-def test_bf16x4(ctx, geo, n, seed):
+                                        ("F", (39, 39, 39), 5), ("P", (47, 20, 9), 6)])
+def test_bf16x4(ctx, ctx_lines, seq, geo, n, seed):
+    """plane: the plane-fused matrix-core sequence; lines: the per-pass one"""
     reg = dense_region(geo, n, (0.45, 0.4, 0.5))
-    print(f"bf16x4 case {geo} {n}", flush=True)
+    print(f"bf16x4 case {seq} {geo} {n}", flush=True)
     psi = random_psi(reg, 2, 1000 + seed)
     pot = random_pot(reg, 1100 + seed)
-    hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=3)
+    c = ctx if seq == "plane" else ctx_lines
+    hg, eg, pg = run_gpu(c, reg, pot, psi, precision=3)
     ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
                                okeys(reg), pot, psi)
     eh = np.abs(hg - ho).max() / np.abs(ho).max()
     ee = np.max(np.abs(eg - eo) / np.abs(eo))
     ep_ = np.max(np.abs(pg - po) / np.abs(po))
-    print(f"bf16x4 {geo} {n}: hpsi {eh:.2e} ekin {ee:.2e} epot {ep_:.2e}", flush=True)
+    print(f"bf16x4 {seq} {geo} {n}: hpsi {eh:.2e} ekin {ee:.2e} epot {ep_:.2e}",
+          flush=True)
     assert eh < BF_H
     assert ee < BF_E and ep_ < BF_E
 
