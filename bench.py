@@ -44,15 +44,21 @@ PATH_STAGES = {
     "plane": {"uncompress": (0, 1, 8), "xy_fwd": (220, 3, 0),
               "z_kin": (46, 3, 0), "z_pot": (130, 2, 0),
               "xy_bwd_compress": (128, 2, 16)},
-    "mfma": {"uncompress": (0, 1, 8), "x_grow_kin": (46, 2, 0),
-             "y_kin": (46, 3, 0), "z_kin": (46, 3, 0), "xy_fwd": (128, 2, 0),
+    # matrix-core plane-fused sequence: the forward plane kernel writes K
+    # (x+y kinetic) and T2 from one read of W
+    "mfma": {"uncompress": (0, 1, 8), "xy_fwd": (220, 3, 0), "z_kin": (46, 3, 0),
              "z_pot": (130, 2, 0), "xy_bwd_compress": (128, 2, 16)},
+    # matrix-core per-pass sequence (boxes whose planes do not fit in LDS)
+    "mfma_passes": {"uncompress": (0, 1, 8), "x_grow_kin": (46, 2, 0),
+                    "y_kin": (46, 3, 0), "z_kin": (46, 3, 0), "xy_fwd": (128, 2, 0),
+                    "z_pot": (130, 2, 0), "xy_bwd_compress": (128, 2, 16)},
 }
 
 
-def path_of(prec, plane_enabled):
+def path_of(prec, plane_enabled, stages):
+    """kernel sequence that ran, from the precision and the stages timed"""
     if prec in ("bf16x3", "bf16x4"):
-        return "mfma"
+        return "mfma_passes" if stages.get("x_grow_kin", 0.0) > 0.02 else "mfma"
     return "plane" if plane_enabled else "lines"
 
 
@@ -80,12 +86,45 @@ def roofline(path, avg, units, esz, peak_c):
 
 CONFIGS = {
     "c2": dict(workload="periodic 48^3 coarse (96^3 fine), 256 orbitals, "
-               "hgrid 0.35, 8-Gaussian potential", geocode="P", n=47,
+               "hgrid 0.35, 8-Gaussian potential", kind="dense", geocode="P", n=47,
                hgrid=0.35, norb=256),
     "c3": dict(workload="periodic 64^3 coarse (128^3 fine), 128 orbitals, "
-               "hgrid 0.35, 8-Gaussian potential", geocode="P", n=63,
+               "hgrid 0.35, 8-Gaussian potential", kind="dense", geocode="P", n=63,
                hgrid=0.35, norb=128),
+    "c4": dict(workload="2048 free boxes, N_d ~ U{16..40} coarse points per axis, "
+               "hgrid 0.4, dense keys, envelope exp(-r^2/32), potential per box "
+               "from one 8-Gaussian field", kind="uneven", norb=2048),
+    "c5": dict(workload="1024 compressed free orbitals in a 96^3 global box, "
+               "hgrid 0.4, coarse r<=7, fine r<=3.5 bohr, one global 8-Gaussian "
+               "potential read through ishift", kind="spheres", norb=1024),
 }
+
+
+def build_batch(cfg, seed, device):
+    """regions, device arrays and offsets of a configuration"""
+    import numpy as np
+
+    from bigdft_hip import inputs, workloads
+    from bigdft_hip.descriptors import dense_region
+    if cfg["kind"] == "uneven":
+        return workloads.uneven_free(cfg["norb"], seed, device=device)
+    if cfg["kind"] == "spheres":
+        return workloads.sphere_batch(cfg["norb"], seed, device=device)
+    n, h, norb = cfg["n"], cfg["hgrid"], cfg["norb"]
+    reg = dense_region(cfg["geocode"], (n, n, n), (h, h, h))
+    psi = inputs.coefficients(norb, reg.ncoef, seed, device=device,
+                              normalise=True).reshape(-1)
+    cell = [(n + 1) * h] * 3
+    gp = inputs.gaussian_params(seed + 1, cell)
+    pot = inputs.gaussian_field(gp, inputs.fine_coords("P", (n, n, n), (h, h, h),
+                                                       device=device),
+                                cell=cell).contiguous()
+    ncoef = np.full(norb, reg.ncoef, dtype=np.int64)
+    return dict(regions=[reg] * norb, psi=psi, ncoef=ncoef,
+                psi_off=np.arange(norb, dtype=np.int64) * reg.ncoef, pot=pot,
+                pot_off=np.zeros(norb, dtype=np.int64),
+                potdim=np.array(list(reg.nfine) * norb, dtype=np.int32), ishift=None,
+                units=reg.n_isf * norb)
 
 
 def parse():
@@ -121,24 +160,13 @@ def main():
     import torch
 
     import bigdft_hip
-    from bigdft_hip import inputs
-    from bigdft_hip.descriptors import dense_region
 
     cfg = CONFIGS[args.config]
     prec = bigdft_hip.PRECISIONS[args.precision]
     dev = "cuda"
-    n, h, norb = cfg["n"], cfg["hgrid"], cfg["norb"]
-    reg = dense_region(cfg["geocode"], (n, n, n), (h, h, h))
-    units = reg.n_isf * norb
-
-    # ---- inputs, generated on the GPU ----
-    psi = inputs.coefficients(norb, reg.ncoef, args.seed, device=dev,
-                              normalise=True)
-    cell = [(n + 1) * h] * 3
-    gp = inputs.gaussian_params(args.seed + 1, cell)
-    pot = inputs.gaussian_field(gp, inputs.fine_coords("P", (n, n, n),
-                                                       (h, h, h), device=dev),
-                                cell=cell).contiguous()
+    batch = build_batch(cfg, args.seed, dev)
+    regions, psi = batch["regions"], batch["psi"]
+    norb, units = len(regions), batch["units"]
 
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device=dev)
@@ -151,7 +179,13 @@ def main():
     ctx = bigdft_hip.Context(0)
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
-    lr = ctx.region(reg)
+    lr_of = {}
+    for r in regions:
+        if id(r) not in lr_of:
+            lr_of[id(r)] = ctx.region(r)
+    lrs = [lr_of[id(r)] for r in regions]
+    bargs = dict(pot_off=batch["pot_off"], potdim=batch["potdim"],
+                 ishift=batch["ishift"], psi_off=batch["psi_off"])
 
     def timed(prec):
         """graph-replay timing, then per-stage HIP-event timing, then one
@@ -159,7 +193,7 @@ def main():
         h = torch.zeros_like(psi)
         ek = torch.zeros(norb, dtype=torch.float64, device=dev)
         ep = torch.zeros_like(ek)
-        plan = ctx.plan([lr] * norb, pot, psi, h, ek, ep, precision=prec)
+        plan = ctx.plan(lrs, batch["pot"], psi, h, ek, ep, precision=prec, **bargs)
         for _ in range(args.warmup):
             plan.execute()
         torch.cuda.synchronize()
@@ -177,11 +211,11 @@ def main():
         ctx.profile(True)
         ctx.profile_read(reset=True)
         for _ in range(max(3, min(args.steps, 10))):
-            ctx.locham(lr, pot, psi, h, ek, ep, prec)
+            ctx.locham_batch(lrs, batch["pot"], psi, h, ek, ep, precision=prec, **bargs)
         stages, nl = ctx.profile_read(reset=True)
         ctx.profile(False)
         h.zero_()
-        ctx.locham(lr, pot, psi, h, ek, ep, prec)
+        ctx.locham_batch(lrs, batch["pot"], psi, h, ek, ep, precision=prec, **bargs)
         ctx.sync()
         return ms, wall, {k: v / nl for k, v in stages.items()}, h, ek, ep
 
@@ -214,7 +248,7 @@ def main():
             del hm
         variants[name] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
                           "wall_s": round(wall, 4), "path":
-                          path_of(name, plane_enabled),
+                          path_of(name, plane_enabled, avg),
                           "stages_ms": {k: round(v, 4) for k, v in avg.items()},
                           "error_vs_fp64": err}
     head = variants[args.precision]
@@ -232,30 +266,42 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        keys = oracle.Keys(reg.nseg_c, reg.nvctr_c, reg.keyg_c, reg.keyv_c,
-                           reg.nseg_f, reg.nvctr_f, reg.keyg_f, reg.keyv_f)
-        potn = pot.cpu().numpy()
-        done, ts, errs, eks, eps = 0, 0.0, [], [], []
+        from bigdft_hip import workloads
+        keys_of, pot_of = {}, {}
+        done, ts, pts, errs, eks, eps = 0, 0.0, 0, [], [], []
         ek_g, ep_g = ek64.cpu().numpy(), ep64.cpu().numpy()
+        order = np.random.default_rng(args.seed + 9).permutation(norb)
         while done < norb and (done < 4 or ts < args.cpu_seconds):
-            p = psi[done].cpu().numpy()[None, :]
+            b = int(order[done])
+            r = regions[b]
+            if id(r) not in keys_of:
+                keys_of[id(r)] = oracle.Keys(r.nseg_c, r.nvctr_c, r.keyg_c, r.keyv_c,
+                                             r.nseg_f, r.nvctr_f, r.keyg_f, r.keyv_f)
+            pkey = (int(batch["pot_off"][b]), None if batch["ishift"] is None else
+                    tuple(batch["ishift"][3 * b:3 * b + 3]), r.nfine)
+            if pkey not in pot_of:
+                pot_of = {pkey: workloads.local_potential(batch, b)}
+            o0, nc = int(batch["psi_off"][b]), int(batch["ncoef"][b])
+            p = psi[o0:o0 + nc].cpu().numpy()[None, :]
             tt = time.perf_counter()
-            ho, eo, po = oracle.locham(cfg["geocode"], reg.n, reg.nfl,
-                                       reg.nfu, reg.hgrid, keys, potn, p)
+            ho, eo, po = oracle.locham(r.geocode, r.n, r.nfl, r.nfu, r.hgrid,
+                                       keys_of[id(r)], pot_of[pkey], p)
             ts += time.perf_counter() - tt
-            hg = h64[done].cpu().numpy()
+            pts += r.n_isf
+            hg = h64[o0:o0 + nc].cpu().numpy()
             sc = np.maximum(np.abs(ho[0]), 1e-3 * np.abs(ho[0]).max())
             errs.append(float((np.abs(hg - ho[0]) / sc).max()))
-            eks.append(abs(ek_g[done] - eo[0]) / abs(eo[0]))
-            eps.append(abs(ep_g[done] - po[0]) / abs(po[0]))
+            eks.append(abs(ek_g[b] - eo[0]) / abs(eo[0]))
+            eps.append(abs(ep_g[b] - po[0]) / abs(po[0]))
             done += 1
         parity = {"orbitals": done, "fp64_hpsi_max_rel": max(errs),
                   "fp64_ekin_max_rel": float(max(eks)),
                   "fp64_epot_max_rel": float(max(eps)), "reference": "oracle"}
-        cpu = {"value": done * reg.n_isf / ts, "unit": "grid-pt*orbital/s",
+        cpu = {"value": pts / ts, "unit": "grid-pt*orbital/s",
                "cores": oracle.num_threads(), "kind": "port",
-               "sample": f"{done} of the {norb} orbitals, oracle (C+OpenMP "
-                         f"restatement of the reference Fortran), {ts:.1f} s"}
+               "sample": f"{done} of the {norb} orbitals (seeded order), oracle "
+                         f"(C+OpenMP restatement of the reference Fortran), "
+                         f"{ts:.1f} s"}
 
     out = {
         "metric": METRIC, "value": units / (ms_step * 1e-3),
@@ -266,7 +312,7 @@ def main():
                   "bf16x4": "bf16"}[args.precision],
         "data": "synthetic (seeded N(0,1) coefficients, 8-Gaussian potential)",
         "config": {"workload": cfg["workload"], "orbitals": norb,
-                   "grid_pts_per_orbital": reg.n_isf, "precision": args.precision,
+                   "grid_pts": units, "precision": args.precision,
                    "graph": True, "wall_s": wall},
         "roofline": roof,
         "speedup_vs_fp64": ms64 / ms_step,
@@ -277,7 +323,8 @@ def main():
         "parity": parity,
     }
     print(json.dumps(out))
-    lr.close()
+    for lr in lr_of.values():
+        lr.close()
     ctx.close()
 
 

@@ -35,7 +35,8 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_plan_create", "bdft_plan_execute", "bdft_plan_destroy",
            "bdft_local_hamiltonian", "bdft_uncompress_d",
            "bdft_compress_acc_d", "bdft_profile_enable", "bdft_profile_read",
-           "bdft_stage_name", "bdft_ctx_info", "bdft_sync")
+           "bdft_stage_name", "bdft_ctx_info", "bdft_sync", "bdft_density_d",
+           "bdft_density")
 
 
 class BigDFTError(RuntimeError):
@@ -63,6 +64,8 @@ def load_library(path=LIB_PATH):
         "bdft_lr_info": [P, I3, ctypes.POINTER(ctypes.c_int64)],
         "bdft_locham_d": [P, P, I, P, P, P, P, P, I],
         "bdft_locham_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
+        "bdft_density_d": [P, P, I, P, P, P, I, P, I],
+        "bdft_density": [P, P, I, P, P, P, I, P, I],
         "bdft_plan_create": [P, I, P, P, P, P, P, P, P, P, P, P, I,
                              ctypes.POINTER(P)],
         "bdft_plan_execute": [P],
@@ -241,6 +244,19 @@ class Context:
             _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
             _ptr(epot), precision))
 
+    def density(self, lr, psi, hfac, rho, spin=None, nspin=1, precision=PREC_FP64):
+        """rho[s] += hfac[i] psir_i^2 over the orbitals (rows of the device
+        array psi) of spin s; hfac, spin: host sequences; rho: device,
+        nspin x n1i*n2i*n3i, accumulated.  Synchronous."""
+        norb = len(hfac)
+        hf = np.ascontiguousarray(hfac, dtype=np.float64)
+        sp = None if spin is None else np.ascontiguousarray(spin, dtype=np.int32)
+        self._check(lib().bdft_density_d(
+            self.handle, lr.handle, norb, _ptr(psi),
+            hf.ctypes.data_as(ctypes.c_void_p),
+            None if sp is None else sp.ctypes.data_as(ctypes.c_void_p),
+            nspin, _ptr(rho), precision))
+
     def plan(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None, potdim=None,
              ishift=None, psi_off=None, precision=PREC_FP64):
         arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
@@ -301,3 +317,21 @@ def local_hamiltonian(ctx, lr, occup, pot, psi, hpsi, kwgt=1.0,
     w = np.asarray(occup, dtype=np.float64) * kwgt
     ek, ep = ekin.cpu().numpy(), epot.cpu().numpy()
     return float((w * ek).sum()), float((w * ep).sum()), ek, ep
+
+
+def local_partial_density(ctx, lr, occup, psi, rho, kwgt=1.0, spinsgn=None,
+                          nspin=1, precision=PREC_FP64):
+    """local_partial_density_OCL semantics (bigdft/src/interface_ocl.f90:
+    908-1040, CPU twin sumrho.f90:377-469) on device tensors:
+    rho[s] += kwgt*occup_i/volume_element * psir_i^2, volume_element the
+    fine-grid cell (hx/2)(hy/2)(hz/2); spin s = 0 where spinsgn_i > 0, else 1
+    (sumrho.f90:515-519).  rho: device tensor (nspin, n1i*n2i*n3i)."""
+    h = lr.region.hgrid
+    vol = (0.5 * h[0]) * (0.5 * h[1]) * (0.5 * h[2])
+    hfac = kwgt * (np.asarray(occup, dtype=np.float64) / vol)
+    spin = None
+    if spinsgn is not None:
+        spin = np.where(np.asarray(spinsgn) > 0, 0, 1).astype(np.int32)
+        if nspin == 1 and spin.any():
+            raise ValueError("spin-down orbitals need nspin = 2")
+    ctx.density(lr, psi, hfac, rho, spin=spin, nspin=nspin, precision=precision)

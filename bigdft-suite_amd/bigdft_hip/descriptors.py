@@ -114,12 +114,22 @@ def sphere_region(n_glob, hgrid, center, rcoarse, rfine):
     (coarse) and rfine (fine) of center; the local box is the bounding box of
     the coarse points.  Returns (region, nl) with nl the box origin in the
     global grid."""
-    glob_c = sphere_logrid(n_glob, hgrid, center, rcoarse)
-    idx = np.nonzero(glob_c)
-    nl = tuple(int(idx[d].min()) for d in range(3))
-    nu = tuple(int(idx[d].max()) for d in range(3))
+    # the criterion is evaluated on the index range that can hold points
+    # (one point of margin per side against rounding), which selects the same
+    # points as a scan of the whole global box
+    lo = [max(0, int(np.floor((center[d] - rcoarse) / hgrid[d])) - 1) for d in range(3)]
+    hi = [min(int(n_glob[d]), int(np.ceil((center[d] + rcoarse) / hgrid[d])) + 1)
+          for d in range(3)]
+    sub = sphere_logrid(tuple(hi[d] - lo[d] for d in range(3)), hgrid, center,
+                        rcoarse, origin=lo)
+    idx = np.nonzero(sub)
+    if idx[0].size == 0:
+        raise ValueError("no coarse point within rcoarse of the centre")
+    nl = tuple(int(idx[d].min()) + lo[d] for d in range(3))
+    nu = tuple(int(idx[d].max()) + lo[d] for d in range(3))
     n = tuple(nu[d] - nl[d] for d in range(3))
-    coarse = glob_c[nl[0]:nu[0] + 1, nl[1]:nu[1] + 1, nl[2]:nu[2] + 1]
+    coarse = sub[nl[0] - lo[0]:nu[0] - lo[0] + 1, nl[1] - lo[1]:nu[1] - lo[1] + 1,
+                 nl[2] - lo[2]:nu[2] - lo[2] + 1]
     fine = sphere_logrid(n, hgrid, center, rfine, origin=nl)
     fine &= coarse
     return region_from_grids("F", n, hgrid, coarse, fine), nl

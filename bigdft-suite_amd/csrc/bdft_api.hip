@@ -485,12 +485,13 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      const double* pot, const int64_t* pot_off, const int32_t* potdim,
                      const int32_t* ishift, const double* psi, const int64_t* psi_off,
                      double* hpsi, double* ekin, double* epot, int precision,
-                     bool own_ws, bdft_plan** out) {
+                     bool own_ws, bdft_plan** out, bool density = false) {
   if (!ctx || !out) return BDFT_ERR_ARG;
   *out = nullptr;
-  if (norb <= 0 || !lrs || !pot || !psi || !hpsi || !ekin || !epot)
+  if (norb <= 0 || !lrs || !psi || (!density && (!pot || !hpsi || !ekin || !epot)))
     return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
-  if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_BF16X4)
+  if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_BF16X4 ||
+      (density && precision > BDFT_PREC_FP32))
     return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
   const size_t esz = precision == BDFT_PREC_FP64 ? sizeof(double) : sizeof(float);
   const bool mfma = precision == BDFT_PREC_BF16X3 || precision == BDFT_PREC_BF16X4;
@@ -611,7 +612,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      mx, ctx->plane_lds);
   }
   {
-    bool plane = !ctx->force_lines && !mfma;
+    bool plane = !ctx->force_lines && !mfma && !density;
     size_t lpf = 0, lpb = 0;
     for (int i = 0; i < norb && plane; ++i) {
       const BoxDesc& b = box[i];
@@ -850,6 +851,106 @@ extern "C" int bdft_locham_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const
     ctx->cached_key.swap(key);
   }
   return run_plan(ctx->cached);
+}
+
+// ---------------------------------------------------------------------------
+// density: local_partial_density_OCL (interface_ocl.f90:908-1040) /
+// local_partial_density (sumrho.f90:377-469)
+// ---------------------------------------------------------------------------
+extern "C" int bdft_density_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* psi,
+                              const double* hfac, const int32_t* spin, int nspin, double* rho,
+                              int precision) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!lr || norb <= 0 || !psi || !hfac || !rho)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  if (nspin != 1 && nspin != 2) return set_err(ctx, BDFT_ERR_ARG, "nspin must be 1 or 2");
+  if (precision != BDFT_PREC_FP64 && precision != BDFT_PREC_FP32)
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                   "density: precision %d not available (FP64 or FP32)", precision);
+  std::vector<int32_t> sp((size_t)norb, 0);
+  if (spin)
+    for (int i = 0; i < norb; ++i) {
+      if (spin[i] < 0 || spin[i] >= nspin)
+        return set_err(ctx, BDFT_ERR_ARG, "orbital %d: spin index %d outside [0,%d)", i,
+                       spin[i], nspin);
+      sp[i] = spin[i];
+    }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::vector<const bdft_lr*> lrs((size_t)norb, lr);
+  bdft_plan* plan = nullptr;
+  int rc = make_plan(ctx, norb, lrs.data(), nullptr, nullptr, nullptr, nullptr, psi, nullptr,
+                     nullptr, nullptr, nullptr, precision, true, &plan, true);
+  if (rc) return rc;
+  DenArgs D{};
+  D.nisf = (long long)lr->L[0] * lr->L[1] * lr->L[2];
+  D.norb = norb;
+  D.nspin = nspin;
+  D.nxt = cdiv(lr->L[0], TW);
+  D.nbund = D.nxt * lr->L[1];
+  const int want = std::max(1, std::min(std::min(norb, 64), cdiv(4096, D.nbund)));
+  D.gsize = cdiv(norb, want);
+  D.ngroups = cdiv(norb, D.gsize);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t nb_h = sizeof(double) * norb, nb_s = sizeof(int32_t) * norb;
+  const size_t nb_p = sizeof(double) * (size_t)D.ngroups * nspin * (size_t)D.nisf;
+  DevBuf buf;
+  rc = dev_alloc(ctx, buf, al(nb_h) + al(nb_s) + al(nb_p));
+  if (rc) {
+    bdft_plan_destroy(plan);
+    return rc;
+  }
+  char* q = (char*)buf.p;
+  hipError_t e = hipMemcpy(q, hfac, nb_h, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(q + al(nb_h), sp.data(), nb_s, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    D.hfac = (const double*)q;
+    D.spin = (const int*)(q + al(nb_h));
+    D.part = (double*)(q + al(nb_h) + al(nb_s));
+    D.rho = rho;
+    e = launch_density(plan->B, plan->T, D, plan->maxN, plan->wtotal, plan->need_zero,
+                       precision == BDFT_PREC_FP32, ctx->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dev_free(buf);
+  bdft_plan_destroy(plan);
+  if (e != hipSuccess)
+    return set_err(ctx, BDFT_ERR_HIP, "density: %s", hipGetErrorString(e));
+  return BDFT_OK;
+}
+
+// host arrays in and out (the Fortran side passes host psi and rho_p, as
+// local_partial_density_OCL does)
+extern "C" int bdft_density(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* psi,
+                            const double* hfac, const int32_t* spin, int nspin, double* rho,
+                            int precision) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!lr || norb <= 0 || !psi || !hfac || !rho || (nspin != 1 && nspin != 2))
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer, norb <= 0 or nspin not 1 or 2");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t ncoef = (size_t)lr->nvctr_c + 7 * (size_t)lr->nvctr_f;
+  const size_t nrho = (size_t)lr->L[0] * lr->L[1] * lr->L[2] * nspin;
+  double *dpsi = nullptr, *drho = nullptr;
+  int rc = BDFT_OK;
+  hipError_t e = hipMalloc(&dpsi, ncoef * norb * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&drho, nrho * sizeof(double));
+  if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_NOMEM, "device buffers: %s", hipGetErrorString(e));
+  hipStream_t s = ctx->stream;
+  if (!rc) {
+    e = hipMemcpyAsync(dpsi, psi, ncoef * norb * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(drho, rho, nrho * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_HIP, "H2D copy: %s", hipGetErrorString(e));
+  }
+  if (!rc) rc = bdft_density_d(ctx, lr, norb, dpsi, hfac, spin, nspin, drho, precision);
+  if (!rc) {
+    e = hipMemcpyAsync(rho, drho, nrho * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_HIP, "D2H copy: %s", hipGetErrorString(e));
+  }
+  hipStreamSynchronize(s);
+  if (dpsi) hipFree(dpsi);
+  if (drho) hipFree(drho);
+  return rc;
 }
 
 extern "C" int bdft_locham_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* pot,

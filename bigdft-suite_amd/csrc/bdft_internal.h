@@ -229,7 +229,21 @@ struct MfmaLds {
   int fused;                          // 1: plane-fused sequence
 };
 
+// density: orbitals sharing one box are split into ngroups groups of gsize;
+// tile = group * nbund + z-line bundle; part[(g*nspin + s)*nisf + p] holds the
+// group's partial density, reduced in group order into rho
+struct DenArgs {
+  const double* hfac;   // per orbital, kwgt*occup/volume_element
+  const int* spin;      // per orbital, 0 or 1
+  double* part;
+  double* rho;          // nspin x nisf, accumulated
+  long long nisf;
+  int norb, nspin, ngroups, gsize, nbund, nxt;
+};
+
 size_t lds_bytes(int maxN, int elem);
+hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
+                          long long wtotal, bool need_zero, int fp32, hipStream_t s);
 void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s);
 void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, hipStream_t s);
 void launch_energy(const EnergyTables& et, const Batch& B, hipStream_t s);

@@ -162,7 +162,8 @@ __device__ __forceinline__ void mma_ident(const __bf16* __restrict__ xh,
 // The same product for up to MB line bundles at once (bundle j at xh + j*LS),
 // so that each table fragment is fetched once per K step for all of them:
 // acc[j] += T[o0..o0+31, :] X_j (orientation as in mma).
-constexpr int MB = 2;   // bundles per sweep in the plane kernels
+constexpr int MBF = 1;  // bundles per sweep: forward plane kernel
+constexpr int MBB = 2;  // bundles per sweep: backward plane kernel
 template <int NTERM, bool TABB, int NBMAX>
 __device__ __forceinline__ void mma_multi(const Batch& B, const OpTab& t, int o0,
                                           const __bf16* __restrict__ xh,
@@ -689,33 +690,21 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const int nobG = tx.Dp / 32;      // output blocks p1 of G_x
   float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   const float ksx = (float)bd.kscale[0], ksy = (float)bd.kscale[1];
-  // x kinetic -> K, G_x -> C
-  const int ch2 = task_chunks(nobK + nobG, nlb, NW);
-  for (int q = wv; q < (nobK + nobG) * ch2; q += NW) {
-    const Task tk = task_of(q, ch2, nlb);
-    const bool kin = tk.u < nobK;
-    const int ob = kin ? tk.u : tk.u - nobK;
-    const OpTab& t = kin ? kx : tx;
-    for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
-      const int nb = min(MB, tk.lb_end - lb0);
-      f32x16 acc[MB] = {};
-      mma_multi<NT4, true, MB>(B, t, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
-                           32 * XR, nb, acc);
-      const int n = 32 * ob + (lane & 31);
+  // G_x -> C
+  {
+    const int ch2 = task_chunks(nobG, nlb, NW);
+    for (int q = wv; q < nobG * ch2; q += NW) {
+      const Task tk = task_of(q, ch2, nlb);
+      const int ob = tk.u;
+      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBF) {
+        const int nb = min(MBF, tk.lb_end - lb0);
+        f32x16 acc[MBF] = {};
+        mma_multi<NT4, true, MBF>(B, tx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
+                                 32 * XR, nb, acc);
 #pragma unroll
-      for (int j = 0; j < MB; ++j) {
-        if (j >= nb) break;
-        const int lb = lb0 + j;
-        if (kin) {
-          if (n < W1) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              const int j2 = 32 * lb + drow(r, lane);
-              if (j2 < W2) K[n + W1 * j2] = acc[j][r] * ksx;
-            }
-          }
-        } else {
-          store_cols(acc[j], ch, cl, XC, n, 32 * lb);
+        for (int j = 0; j < MBF; ++j) {
+          if (j >= nb) break;
+          store_cols(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
         }
       }
     }
@@ -729,13 +718,13 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
     for (int q = wv; q < nob * ch3; q += NW) {
       const Task tk = task_of(q, ch3, nl1);
       const int ob = tk.u;
-      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
-        const int nb = min(MB, tk.lb_end - lb0);
-        f32x16 acc[MB] = {};
-        mma_multi<NT4, false, MB>(B, ty, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
+      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBF) {
+        const int nb = min(MBF, tk.lb_end - lb0);
+        f32x16 acc[MBF] = {};
+        mma_multi<NT4, false, MBF>(B, ty, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
                               32 * XC, nb, acc);
 #pragma unroll
-        for (int j = 0; j < MB; ++j) {
+        for (int j = 0; j < MBF; ++j) {
           if (j >= nb) break;
           const int p1 = 32 * (lb0 + j) + (lane & 31);
           if (p1 < L1) {
@@ -758,32 +747,20 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
     store_cols(acc, ch, cl, XC, 32 * nb + (lane & 31), 32 * lb);
   }
   __syncthreads();
-  // y kinetic: table rows = output block j2 (lb), lines = j1 blocks; K +=
-  const int ch5 = task_chunks(nlb, nobK, NW);
-  for (int q = wv; q < nlb * ch5; q += NW) {
-    const Task tk = task_of(q, ch5, nobK);
-    const int lb = tk.u;
-    for (int ob0 = tk.lb_begin; ob0 < tk.lb_end; ob0 += MB) {
-      const int nb = min(MB, tk.lb_end - ob0);
-      f32x16 acc[MB] = {};
-      mma_multi<NT4, false, MB>(B, ky, 32 * lb, ch + 32 * ob0 * XC, cl + 32 * ob0 * XC, XC,
-                            32 * XC, nb, acc);
+  // kinetic tiles (j2 block lb, j1 block ob): x part from R with the table as
+  // B operand, y part from W^T with the table as A operand; both land as
+  // rows j2, lanes j1, and K is stored once along x
+  for (int q = wv; q < nlb * nobK; q += NW) {
+    const int lb = q / nobK, ob = q - lb * nobK;
+    f32x16 ax = {}, ay = {};
+    mma<NT4, true>(B, kx, 32 * ob, rh + 32 * lb * XR, rl + 32 * lb * XR, XR, ax);
+    mma<NT4, false>(B, ky, 32 * lb, ch + 32 * ob * XC, cl + 32 * ob * XC, XC, ay);
+    const int j1 = 32 * ob + (lane & 31);
+    if (j1 < W1) {
 #pragma unroll
-      for (int j = 0; j < MB; ++j) {
-        if (j >= nb) break;
-        const int j1 = 32 * (ob0 + j) + (lane & 31);
-        if (j1 >= W1) continue;
-        float kold[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int j2 = 32 * lb + drow(r, lane);
-          kold[r] = j2 < W2 ? K[j1 + W1 * j2] : 0.0f;
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int j2 = 32 * lb + drow(r, lane);
-          if (j2 < W2) K[j1 + W1 * j2] = kold[r] + acc[j][r] * ksy;
-        }
+      for (int r = 0; r < 16; ++r) {
+        const int j2 = 32 * lb + drow(r, lane);
+        if (j2 < W2) K[j1 + W1 * j2] = ax[r] * ksx + ay[r] * ksy;
       }
     }
   }
@@ -848,13 +825,13 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
     for (int q = wv; q < nob * chx; q += NW) {
       const Task tk = task_of(q, chx, nlb);
       const int ob = tk.u;
-      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
-        const int nb = min(MB, tk.lb_end - lb0);
-        f32x16 acc[MB] = {};
-        mma_multi<NT4, true, MB>(B, sx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
+      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBB) {
+        const int nb = min(MBB, tk.lb_end - lb0);
+        f32x16 acc[MBB] = {};
+        mma_multi<NT4, true, MBB>(B, sx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
                              32 * XR, nb, acc);
 #pragma unroll
-        for (int j = 0; j < MB; ++j) {
+        for (int j = 0; j < MBB; ++j) {
           if (j >= nb) break;
           store_cols(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
         }
@@ -871,13 +848,13 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   for (int q = wv; q < nob * chy; q += NW) {
     const Task tk = task_of(q, chy, nl1);
     const int ob = tk.u;
-    for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MB) {
-      const int nb = min(MB, tk.lb_end - lb0);
-      f32x16 acc[MB] = {};
-      mma_multi<NT4, false, MB>(B, sy, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
+    for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBB) {
+      const int nb = min(MBB, tk.lb_end - lb0);
+      f32x16 acc[MBB] = {};
+      mma_multi<NT4, false, MBB>(B, sy, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
                                 32 * XC, nb, acc);
 #pragma unroll
-      for (int j = 0; j < MB; ++j) {
+      for (int j = 0; j < MBB; ++j) {
         if (j >= nb) break;
         const int j1 = 32 * (lb0 + j) + (lane & 31);
         if (j1 >= W1) continue;
@@ -1105,45 +1082,57 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
   double ep = 0.0;
-  // G_z, V: rows p3 < tg.Dp (zero beyond L3) -> R[j][line][p3]
+  // G_z, V: rows p3 < tg.Dp (zero beyond L3) -> R[j][line][p3].  The potential
+  // of bundle j+1 is read while bundle j is finished (bundle 0: during the
+  // matrix-core sweep).
+  auto load_pot = [&](int j, int ob, double (&vv)[16]) {
+    const int bi = ZG * lt + j;
+    const int p2 = bi / nxt;
+    const int p1 = (bi - p2 * nxt) * 32 + (lane & 31);
+    const int q1 = p1 - bd.ishift[0], q2 = p2 - bd.ishift[1];
+    const bool in12 = j < nb && p1 < L1 && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
+                      q2 < bd.potdim[1];
+    const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int p3 = 32 * ob + drow(r, lane);
+      const int q3 = p3 - bd.ishift[2];
+      vv[r] = (in12 && p3 < L3 && q3 >= 0 && q3 < bd.potdim[2]) ? pot[pz * q3] : 0.0;
+    }
+  };
+  auto finish = [&](int j, int ob, const f32x16& a, const double (&vv)[16]) {
+    const int line = lane & 31;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      bf16x4 h, l;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = 4 * g + u;
+        const double pr = (double)a[r];
+        ep = fma(vv[r] * pr, pr, ep);
+        __bf16 x, y;
+        split2((float)(vv[r] * pr), x, y);
+        h[u] = x;
+        l[u] = y;
+      }
+      const int o = (j * 32 + line) * RS + 32 * ob + 8 * g + 4 * (lane >> 5);
+      *(bf16x4*)(rh + o) = h;
+      *(bf16x4*)(rl + o) = l;
+    }
+  };
   for (int ob = wv; ob < tg.Dp / 32; ob += NTM / 64) {
+    double va[16], vb[16];
+    load_pot(0, ob, va);
     f32x16 acc[ZG] = {};
     mma_multi<NT4, false, ZG>(B, tg, 32 * ob, xh, xl, XS, 32 * XS, nb, acc);
-#pragma unroll
-    for (int j = 0; j < ZG; ++j) {
-      if (j >= nb) break;
-      const int bi = ZG * lt + j;
-      const int p2 = bi / nxt, line = lane & 31;
-      const int p1 = (bi - p2 * nxt) * 32 + line;
-      const int q1 = p1 - bd.ishift[0], q2 = p2 - bd.ishift[1];
-      const bool in12 = p1 < L1 && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
-                        q2 < bd.potdim[1];
-      const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
-      double vv[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int p3 = 32 * ob + drow(r, lane);
-        const int q3 = p3 - bd.ishift[2];
-        vv[r] = (in12 && p3 < L3 && q3 >= 0 && q3 < bd.potdim[2]) ? pot[pz * q3] : 0.0;
-      }
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        bf16x4 h, l;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int r = 4 * g + u;
-          const double pr = (double)acc[j][r];
-          ep = fma(vv[r] * pr, pr, ep);
-          __bf16 a, c;
-          split2((float)(vv[r] * pr), a, c);
-          h[u] = a;
-          l[u] = c;
-        }
-        const int o = (j * 32 + line) * RS + 32 * ob + 8 * g + 4 * (lane >> 5);
-        *(bf16x4*)(rh + o) = h;
-        *(bf16x4*)(rl + o) = l;
-      }
+    static_assert(ZG == 3, "the potential pipeline below assumes three bundles");
+    if (nb > 1) load_pot(1, ob, vb);
+    finish(0, ob, acc[0], va);
+    if (nb > 1) {
+      if (nb > 2) load_pot(2, ob, va);
+      finish(1, ob, acc[1], vb);
     }
+    if (nb > 2) finish(2, ob, acc[2], va);
   }
   const double eps = wave_block_sum(ep, red, NTM / 64);
   if (threadIdx.x == 0) part[tile] = eps;

@@ -744,6 +744,112 @@ __global__ __launch_bounds__(128) void k_energy(EnergyTables et, int nbox, doubl
 }
 
 // ---------------------------------------------------------------------------
+// density (local_partial_density, sumrho.f90:377-469; partial_density,
+// sumrho.f90:472-560): z lines of T2 = G_y G_x W for the orbitals of one
+// group in turn, psir = G_z line, and the group's partial density
+// sum hfac * psir^2 per spin kept in registers, written once.
+// ---------------------------------------------------------------------------
+template <typename T, int NSPIN>
+__global__ __launch_bounds__(NT) void k_zden(Batch B, DenArgs D) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
+  const int g = blockIdx.x / D.nbund;
+  const int bund = blockIdx.x - g * D.nbund;
+  const BoxDesc bd0 = B.box[0];
+  const int L1 = bd0.L[0], L2 = bd0.L[1], N3 = bd0.N[2], L3 = bd0.L[2], per = bd0.per;
+  const int j2 = bund / D.nxt;
+  const int j1 = (bund - j2 * D.nxt) * TW;
+  const int nl = min(TW, L1 - j1);
+  const long long pstride = (long long)L1 * L2;
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
+  const int NS = N3 + HL + HR;
+  const T* col = lds + tx;
+  double acc[NSPIN][3][8];
+#pragma unroll
+  for (int s = 0; s < NSPIN; ++s)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[s][m][r] = 0.0;
+  const int o0 = g * D.gsize, o1 = min(D.norb, o0 + D.gsize);
+  for (int o = o0; o < o1; ++o) {
+    const double hf = D.hfac[o];
+    const int sp = NSPIN == 2 ? D.spin[o] : 0;
+    const T* line = (const T*)B.T2 + B.box[o].t2_off + j1 + (long long)L1 * j2;
+    fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int p0 = 8 * (ty + m * TY);
+      if (p0 < L3) {
+        T gv[8];
+        grow8(col + grow_row0(p0, per) * LDW, col + (NS + grow_row0(p0, per)) * LDW, gv);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const double v = (double)gv[r];
+          if (NSPIN == 2 && sp == 1) acc[NSPIN - 1][m][r] += hf * (v * v);
+          else acc[0][m][r] += hf * (v * v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tx >= nl) return;
+#pragma unroll
+  for (int s = 0; s < NSPIN; ++s) {
+    double* dst = D.part + ((long long)g * D.nspin + s) * D.nisf + j1 + tx +
+                  (long long)L1 * j2;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int p0 = 8 * (ty + m * TY);
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (p0 + r < L3) dst[pstride * (p0 + r)] = acc[s][m][r];
+    }
+  }
+}
+
+// rho += sum of the group partials, groups in order
+__global__ __launch_bounds__(NT) void k_rho_reduce(DenArgs D) {
+  const long long n = (long long)D.nspin * D.nisf;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n;
+       i += (long long)gridDim.x * NT) {
+    const int s = (int)(i / D.nisf);
+    const long long p = i - (long long)s * D.nisf;
+    double a = 0.0;
+    for (int g = 0; g < D.ngroups; ++g) a += D.part[((long long)g * D.nspin + s) * D.nisf + p];
+    D.rho[i] += a;
+  }
+}
+
+template <typename T>
+static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs& D, int maxN,
+                             long long wtotal, bool need_zero, hipStream_t s) {
+  const size_t lds = lds_bytes(maxN, sizeof(T));
+  if (need_zero) {
+    const long long nb = (wtotal + NT - 1) / NT;
+    k_zero<T><<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>((T*)B.W, wtotal);
+  }
+  if (PT.ntiles[P_UNC] > 0) k_uncompress<T, false><<<PT.ntiles[P_UNC], NT, 0, s>>>(B, PT.t[P_UNC]);
+  if (PT.ntiles[P_XF] > 0)
+    k_xfwd<T><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], PT.part[0]);
+  if (PT.ntiles[P_YF] > 0) k_yfwd<T><<<PT.ntiles[P_YF], NT, lds, s>>>(B, PT.t[P_YF]);
+  const int nblk = D.ngroups * D.nbund;
+  if (D.nspin == 2) k_zden<T, 2><<<nblk, NT, lds, s>>>(B, D);
+  else k_zden<T, 1><<<nblk, NT, lds, s>>>(B, D);
+  const long long n = (long long)D.nspin * D.nisf;
+  const long long nb = (n + NT - 1) / NT;
+  k_rho_reduce<<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>(D);
+  return hipGetLastError();
+}
+
+hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
+                          long long wtotal, bool need_zero, int fp32, hipStream_t s) {
+  return fp32 ? launch_den<float>(B, T, D, maxN, wtotal, need_zero, s)
+              : launch_den<double>(B, T, D, maxN, wtotal, need_zero, s);
+}
+
+// ---------------------------------------------------------------------------
 // host-side launcher of the whole FP64 sequence
 // ---------------------------------------------------------------------------
 size_t lds_bytes(int maxN, int elem) {

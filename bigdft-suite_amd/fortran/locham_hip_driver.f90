@@ -12,7 +12,8 @@ program locham_hip_driver
   integer, dimension(:, :), allocatable :: keyg
   integer, dimension(:), allocatable :: keyv
   real(kind=8), dimension(:, :), allocatable :: psi, hpsi
-  real(kind=8), dimension(:), allocatable :: pot, occup
+  real(kind=8), dimension(:), allocatable :: pot, occup, spinsgn
+  real(kind=8), dimension(:, :), allocatable :: rho
   real(kind=8) :: ekin, epot, hsum
   character(len=32) :: arg
 
@@ -67,5 +68,19 @@ program locham_hip_driver
   print '(a,es25.17)', 'ekin_sum ', ekin
   print '(a,es25.17)', 'epot_sum ', epot
   print '(a,es25.17)', 'psi_hpsi ', hsum
+  ! density, alternating spins, accumulated on a 1e-20 prefill
+  allocate(rho(npot, 2), spinsgn(norb))
+  rho = 1.0d-20
+  do iorb = 1, norb
+     spinsgn(iorb) = 1.0d0 - 2.0d0 * mod(iorb - 1, 2)
+  end do
+  call local_partial_density_hip(ctx, lr, ncoef, norb, 2, 1.0d0, occup, spinsgn, &
+       0.225d0, 0.225d0, 0.225d0, psi, npot, rho, prec, ierr)
+  if (ierr /= 0) then
+     print '(a)', trim(bdft_error_message(ctx))
+     stop 'density failed'
+  end if
+  print '(a,es25.17)', 'rho_up_pot ', sum(rho(:, 1) * pot)
+  print '(a,es25.17)', 'rho_dn_sum ', sum(rho(:, 2))
   call free_gpu_hip(ctx, lr)
 end program locham_hip_driver

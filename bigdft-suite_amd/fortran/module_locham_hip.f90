@@ -13,7 +13,7 @@ module module_locham_hip
        BDFT_PREC_BF16X3 = 2, BDFT_PREC_BF16X4 = 3
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
-       free_gpu_hip, bdft_error_message
+       local_partial_density_hip, free_gpu_hip, bdft_error_message
 
   interface
      integer(c_int) function bdft_ctx_create(device, ctx) bind(C, name='bdft_ctx_create')
@@ -54,6 +54,15 @@ module module_locham_hip
        real(c_double), dimension(*), intent(inout) :: hpsi
        real(c_double), dimension(*), intent(out) :: ekin, epot
      end function bdft_local_hamiltonian
+     integer(c_int) function bdft_density(ctx, lr, norb, psi, hfac, spin, nspin, &
+          rho, precision) bind(C, name='bdft_density')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx, lr
+       integer(c_int), value :: norb, nspin, precision
+       real(c_double), dimension(*), intent(in) :: psi, hfac
+       integer(c_int32_t), dimension(*), intent(in) :: spin
+       real(c_double), dimension(*), intent(inout) :: rho
+     end function bdft_density
   end interface
 
 contains
@@ -139,6 +148,31 @@ contains
        epot_sum = epot_sum + kwgt * occup(iorb) * epot(iorb)
     end do
   end subroutine local_hamiltonian_hip
+
+  !> counterpart of local_partial_density_OCL (interface_ocl.f90:908-1040):
+  !! rho_p(:,:,:,isjmp) += kwgt*occup(iorb)/volume_element * psir(iorb)**2,
+  !! isjmp = 1 where spinsgn > 0, else 2 (sumrho.f90:425 and 515-519);
+  !! volume_element = hxh*hyh*hzh of the fine grid; rho_p accumulated
+  subroutine local_partial_density_hip(ctx, lr, ncoef, norbp, nspin, kwgt, occup, &
+       spinsgn, hxh, hyh, hzh, psi, nrho, rho_p, precision, ierr)
+    type(c_ptr), intent(in) :: ctx, lr
+    integer, intent(in) :: ncoef, norbp, nspin, nrho, precision
+    real(kind=8), intent(in) :: kwgt, hxh, hyh, hzh
+    real(kind=8), dimension(norbp), intent(in) :: occup, spinsgn
+    real(kind=8), dimension(ncoef, norbp), intent(in) :: psi
+    real(kind=8), dimension(nrho, nspin), intent(inout) :: rho_p
+    integer, intent(out) :: ierr
+    real(kind=8), dimension(norbp) :: hfac
+    integer(c_int32_t), dimension(norbp) :: spin
+    integer :: iorb
+    do iorb = 1, norbp
+       hfac(iorb) = kwgt * (occup(iorb) / (hxh * hyh * hzh))
+       spin(iorb) = 0
+       if (spinsgn(iorb) <= 0.0d0) spin(iorb) = 1
+    end do
+    ierr = bdft_density(ctx, lr, int(norbp, c_int), psi, hfac, spin, &
+         int(nspin, c_int), rho_p, int(precision, c_int))
+  end subroutine local_partial_density_hip
 
   !> counterpart of free_gpu_OCL (interface_ocl.f90:196-269)
   subroutine free_gpu_hip(ctx, lr)

@@ -112,6 +112,32 @@ int bdft_locham_batch_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
                         double *hpsi, double *ekin, double *epot,
                         int precision);
 
+/* Density of norb orbitals sharing one region (device pointers psi, rho):
+ * rho[s*n1i*n2i*n3i + p] += hfac[i] * psir_i(p)^2 for the orbitals i of
+ * spin s, psir_i = daub_to_isf(psi_i) on the region's fine ISF grid
+ * (bigdft/src/orbitals/daubisf.f90:13-80).  Replaces local_partial_density_OCL
+ * (interface_ocl.f90:908-1040, ocl_locden_generic per orbital) and the CPU
+ * local_partial_density / partial_density (bigdft/src/sumrho.f90:377-560,
+ * npsir = 1, no slab scattering).  hfac (host, norb doubles) is
+ * kwgt*occup/volume_element as the caller computes it (sumrho.f90:425);
+ * spin (host, norb ints in [0,nspin), or NULL for all 0) picks the density
+ * component as spinsgn does (sumrho.f90:515-519); nspin is 1 or 2.  rho is
+ * accumulated, not initialised (the OpenCL layer pre-fills it with 1e-20,
+ * interface_ocl.f90:957-960; here that is the caller's choice).  Orbitals are
+ * summed in a fixed order, so results do not depend on scheduling.
+ * precision: FP64 or FP32.  Returns after the density is complete. */
+int bdft_density_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
+                   const double *psi, const double *hfac, const int32_t *spin,
+                   int nspin, double *rho, int precision);
+
+/* bdft_density_d on host arrays psi (norb x ncoef) and rho (nspin x
+ * n1i*n2i*n3i, accumulated): the argument form of local_partial_density_OCL
+ * (interface_ocl.f90:908-921), which reads host psi and returns host rho_p.
+ * Copies in, computes, copies back; returns when rho is updated. */
+int bdft_density(bdft_ctx *ctx, const bdft_lr *lr, int norb, const double *psi,
+                 const double *hfac, const int32_t *spin, int nspin, double *rho,
+                 int precision);
+
 /* Plans: the same batch with fixed pointers, captured once into a hipGraph
  * and replayed per SCF iteration.  No counterpart in the reference (the
  * OpenCL path re-enqueues every kernel per orbital). */

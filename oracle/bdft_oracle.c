@@ -876,6 +876,66 @@ void oref_locham_many(char geocode, const int n[3], const int nfl[3],
   }
 }
 
+/* daub_to_isf (orbitals/daubisf.f90:13-80): psir on the fine ISF grid of the
+ * box (periodic: uncompress_per + convolut_magic_n_per, daubisf.f90:70-80;
+ * free: uncompress_forstandard + comb_grow_all, daubisf.f90:38-55) */
+void oref_daub_to_isf(char geocode, const int n[3], const int nfl[3],
+                      const int nfu[3], int nseg_c, int nvctr_c,
+                      const int32_t *keyg_c, const int32_t *keyv_c, int nseg_f,
+                      int nvctr_f, const int32_t *keyg_f,
+                      const int32_t *keyv_f, const double *psi, double *psir) {
+  const int n1 = n[0], n2 = n[1], n3 = n[2];
+  const double *psi_f = nvctr_f > 0 ? psi + nvctr_c : psi;
+  if (geocode == 'P') {
+    const size_t tot = (size_t)(2 * n1 + 2) * (2 * n2 + 2) * (2 * n3 + 2);
+    double *x_c = malloc(sizeof(double) * tot);
+    double *y_c = malloc(sizeof(double) * tot);
+    oref_uncompress_per(n1, n2, n3, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f,
+                        nvctr_f, keyg_f, keyv_f, psi, psi_f, x_c, psir);
+    oref_convolut_magic_n_per(2 * n1 + 1, 2 * n2 + 1, 2 * n3 + 1, x_c, psir,
+                              y_c);
+    free(x_c);
+    free(y_c);
+  } else {
+    const size_t nc = (size_t)(n1 + 1) * (n2 + 1) * (n3 + 1);
+    const size_t nf = (size_t)7 * (nfu[0] - nfl[0] + 1) *
+                      (nfu[1] - nfl[1] + 1) * (nfu[2] - nfl[2] + 1);
+    double *x_c = malloc(sizeof(double) * nc);
+    double *x_f = malloc(sizeof(double) * (nf ? nf : 1));
+    oref_uncompress_forstandard(n1, n2, n3, nfl, nfu, nseg_c, keyg_c, keyv_c,
+                                nseg_f, keyg_f, keyv_f, psi, psi_f, x_c, x_f);
+    oref_comb_grow_all(n1, n2, n3, nfl, nfu, x_c, x_f, psir);
+    free(x_c);
+    free(x_f);
+  }
+}
+
+/* local_partial_density (sumrho.f90:377-469) with partial_density
+ * (sumrho.f90:472-560) for npsir = 1 and no slab scattering: for each
+ * orbital, rho(:, spin) += hfac * psir^2, hfac = kwgt*occup/volume_element
+ * given by the caller (sumrho.f90:425); rho is accumulated. */
+void oref_density(char geocode, const int n[3], const int nfl[3],
+                  const int nfu[3], int nseg_c, int nvctr_c,
+                  const int32_t *keyg_c, const int32_t *keyv_c, int nseg_f,
+                  int nvctr_f, const int32_t *keyg_f, const int32_t *keyv_f,
+                  int norb, const double *psi, const double *hfac,
+                  const int *spin, double *rho) {
+  const int add = geocode == 'P' ? 2 : 31;
+  const size_t tot = (size_t)(2 * n[0] + add) * (2 * n[1] + add) *
+                     (2 * n[2] + add);
+  const size_t ncoef = (size_t)nvctr_c + 7 * (size_t)nvctr_f;
+  double *psir = malloc(sizeof(double) * tot);
+  for (int iorb = 0; iorb < norb; ++iorb) {
+    if (hfac[iorb] == 0.0) continue;
+    oref_daub_to_isf(geocode, n, nfl, nfu, nseg_c, nvctr_c, keyg_c, keyv_c,
+                     nseg_f, nvctr_f, keyg_f, keyv_f, psi + ncoef * iorb,
+                     psir);
+    double *r = rho + (spin ? (size_t)spin[iorb] * tot : 0);
+    for (size_t i = 0; i < tot; ++i) r[i] += hfac[iorb] * (psir[i] * psir[i]);
+  }
+  free(psir);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Descriptors, locregs/initlocregs.f90:1573-1686.                           */
 /* logrid is (0:n1,0:n2,0:n3) column-major, nonzero = point present.         */

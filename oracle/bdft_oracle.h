@@ -114,6 +114,19 @@ void oref_locham_many(char geocode, const int n[3], const int nfl[3],
                       const double *pot, int norb, const double *psi,
                       double *hpsi, double *ekin, double *epot);
 
+/* ---- density (orbitals/daubisf.f90:13-80, sumrho.f90:377-560) ---- */
+void oref_daub_to_isf(char geocode, const int n[3], const int nfl[3],
+                      const int nfu[3], int nseg_c, int nvctr_c,
+                      const int32_t *keyg_c, const int32_t *keyv_c, int nseg_f,
+                      int nvctr_f, const int32_t *keyg_f,
+                      const int32_t *keyv_f, const double *psi, double *psir);
+void oref_density(char geocode, const int n[3], const int nfl[3],
+                  const int nfu[3], int nseg_c, int nvctr_c,
+                  const int32_t *keyg_c, const int32_t *keyv_c, int nseg_f,
+                  int nvctr_f, const int32_t *keyg_f, const int32_t *keyv_f,
+                  int norb, const double *psi, const double *hfac,
+                  const int *spin, double *rho);
+
 /* ---- descriptors (locregs/initlocregs.f90:1573-1686) ---- */
 int oref_num_segkeys(int n1, int n2, int n3, int nl1, int nu1, int nl2,
                      int nu2, int nl3, int nu3, const uint8_t *logrid,

@@ -57,6 +57,12 @@ def _load():
     lib.oref_locham_many.argtypes = [
         c_char, _c_int3, _c_int3, _c_int3, _dp, c_int, c_int, _ip, _ip,
         c_int, c_int, _ip, _ip, _dp, c_int, _dp, _dp, _dp, _dp]
+    lib.oref_daub_to_isf.argtypes = [
+        c_char, _c_int3, _c_int3, _c_int3, c_int, c_int, _ip, _ip, c_int,
+        c_int, _ip, _ip, _dp, _dp]
+    lib.oref_density.argtypes = [
+        c_char, _c_int3, _c_int3, _c_int3, c_int, c_int, _ip, _ip, c_int,
+        c_int, _ip, _ip, c_int, _dp, _dp, _ip, _dp]
     lib.oref_num_segkeys.argtypes = [c_int] * 9 + [_u8, ctypes.POINTER(c_int),
                                                    ctypes.POINTER(c_int)]
     lib.oref_num_segkeys.restype = c_int
@@ -266,3 +272,42 @@ def locham(geocode, n, nfl, nfu, hgrid, keys, pot, psi, hpsi=None):
         keys.keyv_f, np.ascontiguousarray(pot, dtype=np.float64).ravel(),
         norb, psi, h, ek, ep)
     return h, ek, ep
+
+
+def _nfine(geocode, n):
+    add = 2 if geocode == "P" else 31
+    return tuple(2 * x + add for x in n)
+
+
+def daub_to_isf(geocode, n, nfl, nfu, keys, psi):
+    """psir on the fine ISF grid (x fastest) of one orbital."""
+    psi = np.ascontiguousarray(psi, dtype=np.float64).ravel()
+    nf = _nfine(geocode, n)
+    out = np.zeros(nf[0] * nf[1] * nf[2])
+    lib().oref_daub_to_isf(
+        geocode.encode(), _c_int3(*n), _c_int3(*nfl), _c_int3(*nfu),
+        keys.nseg_c, keys.nvctr_c, keys.keyg_c, keys.keyv_c, keys.nseg_f,
+        keys.nvctr_f, keys.keyg_f, keys.keyv_f, psi, out)
+    return out
+
+
+def density(geocode, n, nfl, nfu, keys, psi, hfac, spin=None, nspin=1,
+            rho=None):
+    """rho(:, spin) += hfac * psir^2 over the orbitals (rows of psi);
+    returns rho of shape (nspin, n1i*n2i*n3i)."""
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    if psi.ndim == 1:
+        psi = psi[None, :]
+    norb = psi.shape[0]
+    nf = _nfine(geocode, n)
+    tot = nf[0] * nf[1] * nf[2]
+    r = np.zeros((nspin, tot)) if rho is None else \
+        np.array(rho, dtype=np.float64, copy=True).reshape(nspin, tot)
+    sp = np.zeros(norb, dtype=np.int32) if spin is None else \
+        np.ascontiguousarray(spin, dtype=np.int32)
+    lib().oref_density(
+        geocode.encode(), _c_int3(*n), _c_int3(*nfl), _c_int3(*nfu),
+        keys.nseg_c, keys.nvctr_c, keys.keyg_c, keys.keyv_c, keys.nseg_f,
+        keys.nvctr_f, keys.keyg_f, keys.keyv_f, norb, psi,
+        np.ascontiguousarray(hfac, dtype=np.float64), sp, r)
+    return r

@@ -86,3 +86,25 @@ def test_sphere_region_local_box():
         r2 = (((i1 + nl[0]) * 0.4 - 8.1) ** 2 + ((i2 + nl[1]) * 0.4 - 7.3) ** 2 +
               ((i3 + nl[2]) * 0.4 - 9.2) ** 2)
         assert r2 <= 9.0 + 1e-12
+
+
+def test_sphere_region_matches_full_scan():
+    """the bounding-range evaluation selects exactly the points of a scan of
+    the whole global box"""
+    rng = np.random.default_rng(5)
+    n_glob, h = (30, 27, 33), (0.4, 0.37, 0.41)
+    for _ in range(6):
+        c = [rng.uniform(-1.0, n_glob[d] * h[d] + 1.0) for d in range(3)]
+        rc, rf = rng.uniform(1.0, 4.0), rng.uniform(0.5, 2.0)
+        full = sphere_logrid(n_glob, h, c, rc)
+        if not full.any():
+            continue
+        reg, nl = sphere_region(n_glob, h, c, rc, rf)
+        idx = np.nonzero(full)
+        assert nl == tuple(int(idx[d].min()) for d in range(3))
+        nu = tuple(int(idx[d].max()) for d in range(3))
+        assert reg.n == tuple(nu[d] - nl[d] for d in range(3))
+        assert reg.nvctr_c == int(full.sum())
+        box = full[nl[0]:nu[0] + 1, nl[1]:nu[1] + 1, nl[2]:nu[2] + 1]
+        fine = sphere_logrid(reg.n, h, c, rf, origin=nl) & box
+        assert reg.nvctr_f == int(fine.sum())

@@ -32,3 +32,11 @@ def test_fortran_driver_matches_oracle(n, norb):
     assert abs(vals["ekin_sum"] - (occup * ek).sum()) < 1e-12 * abs((occup * ek).sum())
     assert abs(vals["epot_sum"] - (occup * ep).sum()) < 1e-12 * abs((occup * ep).sum())
     assert abs(vals["psi_hpsi"] - (ek + ep).sum()) < 1e-11 * abs((ek + ep).sum())
+    # density: orbitals 1,3,.. spin up, 2,4,.. down; hfac = occup/0.225^3
+    hf = occup / 0.225 ** 3
+    sp = np.arange(norb, dtype=np.int32) % 2
+    rho = oracle.density("P", reg.n, reg.nfl, reg.nfu, okeys(reg), psi, hf, sp, nspin=2)
+    up = np.dot(rho[0], pot)
+    assert abs(vals["rho_up_pot"] - up) < 1e-12 * abs(up)
+    dn = rho[1].sum() + 1e-20 * reg.n_isf
+    assert abs(vals["rho_dn_sum"] - dn) < 1e-12 * abs(dn)
