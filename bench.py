@@ -57,7 +57,7 @@ PATH_STAGES = {
 
 def path_of(prec, plane_enabled, stages):
     """kernel sequence that ran, from the precision and the stages timed"""
-    if prec in ("bf16x3", "bf16x4"):
+    if prec in ("bf16x3", "bf16x4", "f16x3"):
         return "mfma_passes" if stages.get("x_grow_kin", 0.0) > 0.02 else "mfma"
     return "plane" if plane_enabled else "lines"
 
@@ -133,11 +133,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp32",
-                    choices=["fp64", "fp32", "bf16x3", "bf16x4"],
+    ap.add_argument("--precision", default="f16x3",
+                    choices=["fp64", "fp32", "bf16x3", "bf16x4", "f16x3"],
                     help="arithmetic of the headline run (fp32: FP32 working "
                          "grids; bf16x4: split-BF16 matrix-core path)")
-    ap.add_argument("--variants", default="fp64,fp32,bf16x4",
+    ap.add_argument("--variants", default="fp64,fp32,bf16x4,f16x3",
                     help="precisions timed beside the headline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
@@ -255,7 +255,7 @@ def main():
     ms_step, wall = head["ms_per_step"], head["wall_s"]
     avg = head["stages_ms"]
     esz = 8 if prec == bigdft_hip.PREC_FP64 else 4
-    if args.precision in ("bf16x3", "bf16x4"):
+    if args.precision in ("bf16x3", "bf16x4", "f16x3"):
         peak_c = PEAK_BF16_TFLOPS
     else:
         peak_c = PEAK_FP64_TFLOPS if esz == 8 else PEAK_FP32_TFLOPS
@@ -309,7 +309,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16",
-                  "bf16x4": "bf16"}[args.precision],
+                  "bf16x4": "bf16", "f16x3": "f16"}[args.precision],
         "data": "synthetic (seeded N(0,1) coefficients, 8-Gaussian potential)",
         "config": {"workload": cfg["workload"], "orbitals": norb,
                    "grid_pts": units, "precision": args.precision,

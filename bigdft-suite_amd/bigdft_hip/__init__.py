@@ -21,9 +21,9 @@ from .descriptors import Region, dense_region, sphere_region  # noqa: F401
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
 
-PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4, PREC_F16X3 = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
-              "bf16x4": PREC_BF16X4}
+              "bf16x4": PREC_BF16X4, "f16x3": PREC_F16X3}
 
 ERRORS = {1: "BDFT_ERR_ARG", 2: "BDFT_ERR_HIP", 3: "BDFT_ERR_DESC",
           4: "BDFT_ERR_NOMEM", 5: "BDFT_ERR_UNSUPPORTED"}
@@ -36,7 +36,7 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_local_hamiltonian", "bdft_uncompress_d",
            "bdft_compress_acc_d", "bdft_profile_enable", "bdft_profile_read",
            "bdft_stage_name", "bdft_ctx_info", "bdft_sync", "bdft_density_d",
-           "bdft_density")
+           "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d")
 
 
 class BigDFTError(RuntimeError):
@@ -66,6 +66,8 @@ def load_library(path=LIB_PATH):
         "bdft_locham_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
         "bdft_density_d": [P, P, I, P, P, P, I, P, I],
         "bdft_density": [P, P, I, P, P, P, I, P, I],
+        "bdft_psi_to_locreg_d": [P, P, I, P, P, P, P, P, P],
+        "bdft_locreg_to_psi_d": [P, P, I, P, P, P, P, P, P],
         "bdft_plan_create": [P, I, P, P, P, P, P, P, P, P, P, P, I,
                              ctypes.POINTER(P)],
         "bdft_plan_execute": [P],
@@ -256,6 +258,22 @@ class Context:
             hf.ctypes.data_as(ctypes.c_void_p),
             None if sp is None else sp.ctypes.data_as(ctypes.c_void_p),
             nspin, _ptr(rho), precision))
+
+    def _xfer(self, fn, glr, llrs, ns, src, soff, dst, doff):
+        arr = (ctypes.c_void_p * len(llrs))(*[l.handle.value for l in llrs])
+        nsa = np.ascontiguousarray(ns, dtype=np.int32).ravel()
+        so = None if soff is None else np.ascontiguousarray(soff, dtype=np.int64)
+        do = None if doff is None else np.ascontiguousarray(doff, dtype=np.int64)
+        self._check(fn(self.handle, glr.handle, len(llrs), arr, _ptr(nsa), _ptr(src),
+                       _ptr(so), _ptr(dst), _ptr(do)))
+
+    def psi_to_locreg(self, glr, llrs, ns, gpsi, lpsi, goff=None, loff=None):
+        """psi_to_locreg2: global orbitals -> local regions (device arrays)"""
+        self._xfer(lib().bdft_psi_to_locreg_d, glr, llrs, ns, gpsi, goff, lpsi, loff)
+
+    def locreg_to_psi(self, glr, llrs, ns, lpsi, gpsi, loff=None, goff=None):
+        """Lpsi_to_global2: local regions -> their points of global orbitals"""
+        self._xfer(lib().bdft_locreg_to_psi_d, glr, llrs, ns, lpsi, loff, gpsi, goff)
 
     def plan(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None, potdim=None,
              ishift=None, psi_off=None, precision=PREC_FP64):

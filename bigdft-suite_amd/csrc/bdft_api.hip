@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -349,6 +350,37 @@ static double bf2d(uint16_t h) {
   memcpy(&f, &u, 4);
   return (double)f;
 }
+// IEEE binary16 bits of v, round to nearest even (subnormals kept, overflow
+// to infinity)
+static uint16_t d2h(double v) {
+  const uint16_t sign = std::signbit(v) ? 0x8000 : 0;
+  const double a = std::fabs(v);
+  if (a != a) return 0x7e00;
+  if (a == 0.0) return sign;
+  if (a >= 65520.0) return sign | 0x7c00;
+  int e;
+  (void)std::frexp(a, &e);             // a = f 2^e, f in [0.5, 1)
+  if (e >= -13) {                      // a >= 2^-14: normal
+    const double q = std::ldexp(1.0, e - 11);
+    const double r = std::nearbyint(a / q) * q;
+    int er;
+    const double fr = std::frexp(r, &er);
+    const int E = er - 1 + 15;
+    const int mant = (int)std::ldexp(2.0 * fr - 1.0, 10);
+    return sign | (uint16_t)(E << 10) | (uint16_t)mant;
+  }
+  const double r = std::nearbyint(std::ldexp(a, 24));  // subnormal quanta of 2^-24
+  if (r >= 1024.0) return sign | (uint16_t)(1 << 10);
+  return sign | (uint16_t)r;
+}
+static double h2d(uint16_t h) {
+  const int E = (h >> 10) & 31, mant = h & 1023;
+  double v;
+  if (E == 0) v = std::ldexp((double)mant, -24);
+  else if (E == 31) v = mant ? NAN : INFINITY;
+  else v = std::ldexp(1.0 + mant / 1024.0, E - 15);
+  return (h & 0x8000) ? -v : v;
+}
 
 // Dense operator of one axis: op 0 = G (synthesis+magic, fil2 of
 // bigdft/src/orbitals/v.inc), 1 = G^T, 2 = wavelet-basis kinetic (unscaled;
@@ -409,9 +441,10 @@ struct TabSet {
   std::vector<OpTab> tabs;
   std::vector<uint16_t> frag;
   std::vector<uint32_t> mq;
-  std::map<std::tuple<int, int, int>, int> index;
-  int get(int op, int N, int per) {
-    const auto key = std::make_tuple(op, N, per);
+  std::map<std::tuple<int, int, int, int>, int> index;
+  // f16: fp16 pieces (hi, lo) instead of bf16 (hi, lo, lo2)
+  int get(int op, int N, int per, bool f16) {
+    const auto key = std::make_tuple(op, N, per, (int)f16);
     auto it = index.find(key);
     if (it != index.end()) return it->second;
     std::vector<double> M;
@@ -441,9 +474,16 @@ struct TabSet {
             const int r = 32 * q + (l & 31), c = 16 * s + 8 * (l >> 5) + u;
             if (r >= rows || c >= cols) continue;
             const double v = M[(size_t)r * cols + c];
-            const uint16_t h = f2bf((float)v);
-            const uint16_t l1 = f2bf((float)(v - bf2d(h)));
-            const uint16_t l2 = f2bf((float)(v - bf2d(h) - bf2d(l1)));
+            uint16_t h, l1, l2;
+            if (f16) {
+              h = d2h(v);
+              l1 = d2h(v - h2d(h));
+              l2 = 0;
+            } else {
+              h = f2bf((float)v);
+              l1 = f2bf((float)(v - bf2d(h)));
+              l2 = f2bf((float)(v - bf2d(h) - bf2d(l1)));
+            }
             frag[base + 0 * 512 + 8 * l + u] = h;
             frag[base + 1 * 512 + 8 * l + u] = l1;
             frag[base + 2 * 512 + 8 * l + u] = l2;
@@ -490,11 +530,12 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   *out = nullptr;
   if (norb <= 0 || !lrs || !psi || (!density && (!pot || !hpsi || !ekin || !epot)))
     return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
-  if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_BF16X4 ||
+  if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_F16X3 ||
       (density && precision > BDFT_PREC_FP32))
     return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
   const size_t esz = precision == BDFT_PREC_FP64 ? sizeof(double) : sizeof(float);
-  const bool mfma = precision == BDFT_PREC_BF16X3 || precision == BDFT_PREC_BF16X4;
+  const bool mfma = precision >= BDFT_PREC_BF16X3;
+  const bool f16 = precision == BDFT_PREC_F16X3;
   TabSet tabs;
   std::vector<BoxDesc> box((size_t)norb);
   long long wtot = 0, t1tot = 0, t2tot = 0;
@@ -535,9 +576,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     b.psi_off = psi_off ? psi_off[i] : ncoef * i;
     if (mfma)
       for (int d = 0; d < 3; ++d) {
-        b.opG[d] = tabs.get(0, b.N[d], b.per);
-        b.opS[d] = tabs.get(1, b.N[d], b.per);
-        b.opK[d] = tabs.get(2, b.N[d], b.per);
+        b.opG[d] = tabs.get(0, b.N[d], b.per, f16);
+        b.opS[d] = tabs.get(1, b.N[d], b.per, false);
+        b.opK[d] = tabs.get(2, b.N[d], b.per, f16);
       }
     wtot += 8LL * b.N[0] * b.N[1] * b.N[2];
     t1tot += (long long)b.L[0] * 4 * b.N[1] * b.N[2];
@@ -666,8 +707,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   }
   size_t nb_part = 0;
   for (int e = 0; e < 4; ++e) nb_part += sizeof(double) * T.ntiles[epass[e]];
+  const size_t nb_max = f16 ? sizeof(unsigned) * norb + sizeof(float) * T.ntiles[P_UNC] + 256 : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  int rc = dev_alloc(ctx, plan->meta, al(nb_box) + al(nb_st) + al(nb_part));
+  int rc = dev_alloc(ctx, plan->meta, al(nb_box) + al(nb_st) + al(nb_part) + al(nb_max));
   if (rc) {
     delete plan;
     return rc;
@@ -676,6 +718,10 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   BoxDesc* dbox = (BoxDesc*)m;
   int* dst = (int*)(m + al(nb_box));
   double* dpart = (double*)(m + al(nb_box) + al(nb_st));
+  plan->B.bmax = f16 ? (unsigned*)(m + al(nb_box) + al(nb_st) + al(nb_part)) : nullptr;
+  plan->B.tmax = f16 ? (float*)(m + al(nb_box) + al(nb_st) + al(nb_part) +
+                                al(sizeof(unsigned) * norb))
+                     : nullptr;
   hipError_t e1 = hipMemcpy(dbox, box.data(), nb_box, hipMemcpyHostToDevice);
   hipError_t e2 = starts.empty() ? hipSuccess
                                  : hipMemcpy(dst, starts.data(), nb_st, hipMemcpyHostToDevice);
@@ -743,7 +789,7 @@ static int run_plan(bdft_plan* plan) {
   }
   const bool mf = plan->precision >= BDFT_PREC_BF16X3;
   hipError_t e = mf ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal, plan->need_zero,
-                                  plan->precision == BDFT_PREC_BF16X4 ? 4 : 3, ctx->stream, ev)
+                                  plan->precision, ctx->stream, ev)
                      : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
                                      plan->need_zero, plan->precision == BDFT_PREC_FP32,
                                      ctx->stream, ev);
@@ -769,8 +815,7 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
   if (e == hipSuccess) {
     hipError_t el = plan->precision >= BDFT_PREC_BF16X3
                          ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal,
-                                       plan->need_zero,
-                                       plan->precision == BDFT_PREC_BF16X4 ? 4 : 3, cs, nullptr)
+                                       plan->need_zero, plan->precision, cs, nullptr)
                          : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
                                          plan->need_zero, plan->precision == BDFT_PREC_FP32,
                                          cs);
@@ -951,6 +996,103 @@ extern "C" int bdft_density(bdft_ctx* ctx, const bdft_lr* lr, int norb, const do
   if (dpsi) hipFree(dpsi);
   if (drho) hipFree(drho);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// region transfers: psi_to_locreg2 / Lpsi_to_global2
+// (locregs/locreg_operations.f90:2143-2318, 1665-1837)
+// ---------------------------------------------------------------------------
+static int xfer(bdft_ctx* ctx, const bdft_lr* glr, int norb, const bdft_lr* const* llrs,
+                const int32_t* ns, const double* src, const int64_t* soff, double* dst,
+                const int64_t* doff, bool to_local) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!glr || norb <= 0 || !llrs || !ns || !src || !dst)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  if (glr->ctx != ctx) return set_err(ctx, BDFT_ERR_ARG, "global region from another context");
+  if (glr->geocode != 'F')
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                   "region transfers are defined between free-BC regions "
+                   "(shift_locreg_indexes, locreg_operations.f90:2338-2343)");
+  const long long gnc = (long long)glr->nvctr_c + 7LL * glr->nvctr_f;
+  std::vector<XferDesc> d((size_t)norb);
+  long long lacc = 0;
+  long long tiles = 0;
+  for (int i = 0; i < norb; ++i) {
+    const bdft_lr* L = llrs[i];
+    if (!L || L->ctx != ctx)
+      return set_err(ctx, BDFT_ERR_ARG, "region %d missing or from another context", i);
+    if (L->geocode != 'F')
+      return set_err(ctx, BDFT_ERR_UNSUPPORTED, "region %d is not a free-BC region", i);
+    XferDesc& x = d[i];
+    x.cmap = L->d_cmap;
+    x.fmap = L->d_fmap;
+    x.nvctr_c = L->nvctr_c;
+    x.nvctr_f = L->nvctr_f;
+    x.m1 = L->n[0] + 1;
+    x.m2 = L->n[1] + 1;
+    for (int k = 0; k < 3; ++k) x.ns[k] = ns[3 * i + k];
+    const long long lnc = (long long)L->nvctr_c + 7LL * L->nvctr_f;
+    if (to_local) {
+      x.goff = soff ? soff[i] : gnc * i;
+      x.loff = doff ? doff[i] : lacc;
+    } else {
+      x.loff = soff ? soff[i] : lacc;
+      x.goff = doff ? doff[i] : gnc * i;
+    }
+    lacc += lnc;
+    x.tile0 = (int)tiles;
+    tiles += xfer_tiles(L->nvctr_c, L->nvctr_f);
+  }
+  if (tiles > 0x7fffffffLL) return set_err(ctx, BDFT_ERR_UNSUPPORTED, "batch too large");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t nb_d = sizeof(XferDesc) * norb;
+  DevBuf buf;
+  int rc = dev_alloc(ctx, buf, al(nb_d) + al(sizeof(int)));
+  if (rc) return rc;
+  char* q = (char*)buf.p;
+  int* miss = (int*)(q + al(nb_d));
+  hipError_t e = hipMemcpy(q, d.data(), nb_d, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemsetAsync(miss, 0, sizeof(int), ctx->stream);
+  int nmiss = 0;
+  if (e == hipSuccess) {
+    XferArgs A{};
+    A.d = (const XferDesc*)q;
+    A.nd = norb;
+    A.gcinv = glr->d_cinv;
+    A.gfinv = glr->d_finv;
+    A.g1 = glr->n[0] + 1;
+    A.g2 = glr->n[1] + 1;
+    A.g3 = glr->n[2] + 1;
+    A.gnvctr_c = glr->nvctr_c;
+    A.src = src;
+    A.dst = dst;
+    A.to_local = to_local ? 1 : 0;
+    A.miss = miss;
+    e = launch_xfer(A, (int)tiles, ctx->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&nmiss, miss, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dev_free(buf);
+  if (e != hipSuccess) return set_err(ctx, BDFT_ERR_HIP, "transfer: %s", hipGetErrorString(e));
+  if (nmiss > 0)
+    return set_err(ctx, BDFT_ERR_DESC,
+                   "%d local points have no counterpart in the global region", nmiss);
+  return BDFT_OK;
+}
+
+extern "C" int bdft_psi_to_locreg_d(bdft_ctx* ctx, const bdft_lr* glr, int norb,
+                                    const bdft_lr* const* llrs, const int32_t* ns,
+                                    const double* gpsi, const int64_t* goff, double* lpsi,
+                                    const int64_t* loff) {
+  return xfer(ctx, glr, norb, llrs, ns, gpsi, goff, lpsi, loff, true);
+}
+
+extern "C" int bdft_locreg_to_psi_d(bdft_ctx* ctx, const bdft_lr* glr, int norb,
+                                    const bdft_lr* const* llrs, const int32_t* ns,
+                                    const double* lpsi, const int64_t* loff, double* gpsi,
+                                    const int64_t* goff) {
+  return xfer(ctx, glr, norb, llrs, ns, lpsi, loff, gpsi, goff, false);
 }
 
 extern "C" int bdft_locham_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* pot,

@@ -196,6 +196,8 @@ struct Batch {
   const OpTab* tabs;    // matrix-core operator tables (device)
   const void* tfrag;    // operator fragments, matrix-core operand order
   const uint2* tmq;     // per 32-row block: {K-step mask, first fragment}
+  unsigned* bmax;       // per box: largest |psi| (float bits), fp16 scaling
+  float* tmax;          // per uncompress tile: largest |psi|
 };
 
 // Passes of one application; each has its own tile table.
@@ -241,11 +243,38 @@ struct DenArgs {
   int norb, nspin, ngroups, gsize, nbund, nxt;
 };
 
+// region transfers: one descriptor per local region, tiles of 256 points
+// (coarse then fine) starting at tile0
+struct XferDesc {
+  const int* cmap;      // local compressed -> local grid (0-based linear)
+  const int* fmap;
+  int nvctr_c, nvctr_f;
+  int m1, m2;           // local points per axis 1, 2 (n + 1)
+  int ns[3];            // local box origin in the global box
+  long long loff, goff; // offsets of the local and global orbitals
+  int tile0;
+};
+struct XferArgs {
+  const XferDesc* d;
+  int nd;
+  const int* gcinv;     // global grid -> compressed (null: identity)
+  const int* gfinv;
+  int g1, g2, g3;       // global points per axis
+  int gnvctr_c;
+  const double* src;
+  double* dst;
+  int to_local;         // 1: local = global(map), 0: global(map) = local
+  int* miss;            // local points with no global counterpart
+};
+hipError_t launch_xfer(const XferArgs& A, int ntiles, hipStream_t s);
+int xfer_tiles(int nvctr_c, int nvctr_f);
+
 size_t lds_bytes(int maxN, int elem);
 hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
                           long long wtotal, bool need_zero, int fp32, hipStream_t s);
 void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s);
-void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, hipStream_t s);
+void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, bool track_max,
+                             hipStream_t s);
 void launch_energy(const EnergyTables& et, const Batch& B, hipStream_t s);
 size_t mfma_lds_kin(int Wd, int Kp);
 size_t mfma_lds_grow(int W2, int L1, int Kp1, int Kp2);
@@ -257,7 +286,7 @@ size_t mfma_lds_zp(int KpG, int DpG);
 size_t mfma_lds_bwd(int W1, int L1, int L2);
 size_t mfma_attributes(size_t lds);
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
-                       long long wtotal, bool need_zero, int nterm, hipStream_t s,
+                       long long wtotal, bool need_zero, int precision, hipStream_t s,
                        hipEvent_t* ev);
 size_t plane_lds_fwd(int N1, int N2, int L1, int elem);
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem);

@@ -1,37 +1,81 @@
-// Split-BF16 matrix-core kernels of the mixed-precision local-Hamiltonian
-// apply (BDFT_PREC_BF16X3).
+// Matrix-core kernels of the mixed-precision local-Hamiltonian apply
+// (BDFT_PREC_BF16X3, BF16X4, F16X3).
 //
 // Every 1D operator of the path along an axis -- G (synthesis o magic,
 // combined filter fil2, growshrink.f90 comb_grow), G^T (comb_shrink) and the
 // wavelet-basis kinetic (ConvolkineticT, unscaled) -- is a fixed matrix T
 // (Dout x Din) for a given line length and boundary condition, with the
 // periodic wrap or the zero padding built in.  The host builds T once per
-// (operator, N, BC), pads it to 32-row blocks and 16-column K steps, splits
-// every entry into bf16 hi + lo, and records per 32-row block which K steps
-// hold non-zeros (the band).  A pass over a bundle of 32 lines stored in LDS
-// as bf16 hi/lo [line][k] then computes, per 32-row output block,
-//     D += T_hi X_hi + T_hi X_lo + T_lo X_hi      (FP32 accumulation)
-// with v_mfma_f32_32x32x16_bf16.  Lane l of a wave holds D rows
-// 8*(r/4) + 4*(l/32) + r%4 (r = 0..15) of line l%32, so results land with the
-// line index across lanes, which is both the coalesced direction in memory
-// and the k-contiguous layout the next pass along the other axis needs.
+// (operator, N, BC, piece format), splits every entry into 16-bit pieces and
+// stores, per 32-row block, only the 16-column K steps that hold non-zeros,
+// in matrix-core operand order.  A pass over bundles of 32 lines held in LDS
+// as 16-bit pieces [line][k] computes, per 32-row output block,
+//     D += T0 X0 + T0 X1 + T1 X0 (+ T2 X0)      (FP32 accumulation)
+// with v_mfma_f32_32x32x16_{bf16,f16}.  Lane l of a wave holds D rows
+// 8*(r/4) + 4*(l/32) + r%4 (r = 0..15) of column l%32; with the table as the
+// A operand the columns are lines, with the table as the B operand they are
+// outputs, which lets each pass choose where its results land (coalesced
+// along x in memory, or k-contiguous for the next pass in LDS).
 //
-// Sequence: uncompress (FP32 W) -> kinetic x, y, z (K, ekin) -> x-y plane
-// G_x G_y (T2) -> z lines G_z, V, G_z^T (in place, epot) -> x-y plane
-// G_y^T G_x^T + K, compress-accumulate into hpsi -> energies.
+// Plane-fused sequence: uncompress (FP32 W) -> k_mfwd (x-y plane: G_x, G_y
+// -> T2; x+y kinetic -> K) -> k_mkz (z kinetic, ekin) -> k_mzp (z lines:
+// G_z, V, epot, G_z^T) -> k_mbwd (x-y plane: G_x^T, G_y^T, + K, compress
+// into hpsi) -> energies.  A per-pass sequence covers boxes whose planes do
+// not fit in LDS.
 #include "bdft_internal.h"
+#include "../../include/bigdft_hip.h"
 
 namespace bdft {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+
 constexpr int NTM = 256;    // threads of the pencil kernels (4 waves)
 constexpr int NTMP = 512;   // threads of the plane kernels (8 waves)
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// Piece formats of the matrix-core passes (template argument M):
+//   3  : bf16 pieces, products T0 X0 + T0 X1 + T1 X0
+//   4  : bf16 pieces, and T2 X0 as well (filter kept to its third piece)
+//   16 : fp16 pieces of data scaled by a power of two per box (largest
+//        |value| below 2^12), products T0 X0 + T0 X1 + T1 X0; pieces of
+//        11 bits make each value and filter entry exact to 2^-22 relative
+//        (bf16: 2^-17).  fp16 values live in the same 16-bit LDS slots.
+template <int M>
 __device__ __forceinline__ void split2(float x, __bf16& hi, __bf16& lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
+  if (M == 16) {
+    const _Float16 h = (_Float16)x;
+    const _Float16 l = (_Float16)(x - (float)h);
+    hi = __builtin_bit_cast(__bf16, h);
+    lo = __builtin_bit_cast(__bf16, l);
+  } else {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+  }
+}
+
+template <int M>
+__device__ __forceinline__ f32x16 mf(const bf16x8& a, const bf16x8& b, const f32x16& c) {
+  if (M == 16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// scale of the data split into fp16 pieces: 2^(12 - e) with the box's
+// largest |psi| below 2^e (the grids derived from it stay within a few times
+// that); 1 for the bf16 formats
+template <int M>
+__device__ __forceinline__ float box_scale(const Batch& B, int b) {
+  if (M != 16) return 1.0f;
+  const float mx = __uint_as_float(B.bmax[b]);
+  if (!(mx > 0.0f)) return 1.0f;
+  int e;
+  (void)frexpf(mx, &e);
+  e = min(max(e, -100), 100);
+  return ldexpf(1.0f, 12 - e);
 }
 
 // D row held by lane in accumulator register r
@@ -64,17 +108,15 @@ __device__ __forceinline__ void mma_step(const bf16x8 (&f)[3], const __bf16* bh,
       const bf16x8 xh8 = *(const bf16x8*)(bh + j * LS + 16 * s);
       const bf16x8 xl8 = *(const bf16x8*)(bl + j * LS + 16 * s);
       if (TABB) {
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, f[0], acc[j], 0, 0, 0);
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl8, f[0], acc[j], 0, 0, 0);
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, f[1], acc[j], 0, 0, 0);
-        if (NTERM == 4)
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, f[2], acc[j], 0, 0, 0);
+        acc[j] = mf<NTERM>(xh8, f[0], acc[j]);
+        acc[j] = mf<NTERM>(xl8, f[0], acc[j]);
+        acc[j] = mf<NTERM>(xh8, f[1], acc[j]);
+        if (NTERM == 4) acc[j] = mf<NTERM>(xh8, f[2], acc[j]);
       } else {
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], xh8, acc[j], 0, 0, 0);
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[0], xl8, acc[j], 0, 0, 0);
-        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[1], xh8, acc[j], 0, 0, 0);
-        if (NTERM == 4)
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2], xh8, acc[j], 0, 0, 0);
+        acc[j] = mf<NTERM>(f[0], xh8, acc[j]);
+        acc[j] = mf<NTERM>(f[0], xl8, acc[j]);
+        acc[j] = mf<NTERM>(f[1], xh8, acc[j]);
+        if (NTERM == 4) acc[j] = mf<NTERM>(f[2], xh8, acc[j]);
       }
     }
   }
@@ -138,6 +180,7 @@ __device__ __forceinline__ void mfma_block(const Batch& B, const OpTab& t, int m
 // D = X I with X = the lines' columns n0..n0+31, so D[m = line][n = column]
 // holds hi + lo exactly (FP32), and lane l gets column n0 + l%32 of the 32
 // lines, four consecutive lines per register group.
+template <int M>
 __device__ __forceinline__ void mma_ident(const __bf16* __restrict__ xh,
                                           const __bf16* __restrict__ xl, int XS, int n0,
                                           f32x16& acc) {
@@ -146,22 +189,24 @@ __device__ __forceinline__ void mma_ident(const __bf16* __restrict__ xh,
   const int n = n0 + (lane & 31);
   const __bf16* ah = xh + (lane & 31) * XS + kh;
   const __bf16* al = xl + (lane & 31) * XS + kh;
+  const __bf16 one = M == 16 ? __builtin_bit_cast(__bf16, (_Float16)1.0f) : (__bf16)1.0f;
+  const __bf16 zero = (__bf16)0.0f;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int k0 = n0 + 16 * q;
     bf16x8 id;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) id[u] = (__bf16)(k0 + kh + u == n ? 1.0f : 0.0f);
+    for (int u = 0; u < 8; ++u) id[u] = k0 + kh + u == n ? one : zero;
     const bf16x8 xh8 = *(const bf16x8*)(ah + k0);
     const bf16x8 xl8 = *(const bf16x8*)(al + k0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8, id, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl8, id, acc, 0, 0, 0);
+    acc = mf<M>(xh8, id, acc);
+    acc = mf<M>(xl8, id, acc);
   }
 }
 
-// The same product for up to MB line bundles at once (bundle j at xh + j*LS),
-// so that each table fragment is fetched once per K step for all of them:
-// acc[j] += T[o0..o0+31, :] X_j (orientation as in mma).
+// The same product for up to NBMAX line bundles at once (bundle j at
+// xh + j*LS), so that each table fragment is fetched once per K step for all
+// of them: acc[j] += T[o0..o0+31, :] X_j (orientation as in mma).
 constexpr int MBF = 1;  // bundles per sweep: forward plane kernel
 constexpr int MBB = 2;  // bundles per sweep: backward plane kernel
 template <int NTERM, bool TABB, int NBMAX>
@@ -175,6 +220,7 @@ __device__ __forceinline__ void mma_multi(const Batch& B, const OpTab& t, int o0
 // Store the 16 results of a lane (one column n, rows m in four groups of four
 // consecutive m) into a bf16 hi/lo buffer laid out [n * XS + m], 8 bytes per
 // group and piece.
+template <int M>
 __device__ __forceinline__ void store_cols(const f32x16& acc, __bf16* __restrict__ yh,
                                            __bf16* __restrict__ yl, int XS, int n, int m0) {
   const int lane = threadIdx.x & 63;
@@ -184,7 +230,7 @@ __device__ __forceinline__ void store_cols(const f32x16& acc, __bf16* __restrict
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       __bf16 a, b;
-      split2(acc[4 * g + u], a, b);
+      split2<M>(acc[4 * g + u], a, b);
       h[u] = a;
       l[u] = b;
     }
@@ -249,6 +295,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
+  const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], W3 = 2 * bd.N[2];
   const int Wd = AX == 0 ? W1 : (AX == 1 ? W2 : W3);
   const OpTab t = B.tabs[bd.opK[AX]];
@@ -297,7 +344,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
         int l, k;
         if (AX == 0) { l = i / KpW; k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
         __bf16 h, lo;
-        split2(v[u], h, lo);
+        split2<NT4>(v[u] * sc, h, lo);
         xh[l * XS + k] = h;
         xl[l * XS + k] = lo;
         if (k < Wd) xf[l * XF + k] = v[u];
@@ -327,7 +374,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
       const int line = TB ? drow(r, lane) : (lane & 31);
       const int o = 32 * ob + (TB ? (lane & 31) : drow(r, lane));
       if (o < Wd && line < nl) {
-        const float v = acc[r] * ks;
+        const float v = acc[r] * (ks * isc);
         const float kn = v + kold[r];
         ek += (double)(TOTAL ? kn : v) * (double)xf[line * XF + o];
         K[base + line * lstride + o * pstride] = kn;
@@ -348,6 +395,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int j3 = tile - box_first_tile(tt, b);
+  const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], L1 = bd.L[0], L2 = bd.L[1];
   const OpTab tx = B.tabs[bd.opG[0]], ty = B.tabs[bd.opG[1]];
   const int XS1 = tx.Kp + 8, XS2 = ty.Kp + 8;
@@ -373,7 +421,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
       if (i < tot) {
         const int j2 = i / W1, j1 = i - j2 * W1;
         __bf16 h, lo;
-        split2(v[u], h, lo);
+        split2<NT4>(v[u] * sc, h, lo);
         rh[j2 * XS1 + j1] = h;
         rl[j2 * XS1 + j1] = lo;
       }
@@ -395,7 +443,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
         const int p1 = 32 * ob + drow(r, lane);
         if (p1 < L1 && j2 < W2) {
           __bf16 h, lo;
-          split2(acc[r], h, lo);
+          split2<NT4>(acc[r], h, lo);
           ch[p1 * XS2 + j2] = h;
           cl[p1 * XS2 + j2] = lo;
         }
@@ -415,7 +463,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int p2 = 32 * ob + drow(r, lane);
-        if (p1 < L1 && p2 < L2) dst[p1 + (long long)L1 * p2] = acc[r];
+        if (p1 < L1 && p2 < L2) dst[p1 + (long long)L1 * p2] = acc[r] * isc;
       }
     }
   }
@@ -424,7 +472,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // z lines of T2: G_z, V, epot, G_z^T, in place
 // ---------------------------------------------------------------------------
-template <int NT4>
+template <int NT4, int BM>
 __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
@@ -432,6 +480,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
+  const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int L1 = bd.L[0], L2 = bd.L[1], W3 = 2 * bd.N[2], L3 = bd.L[2];
   const OpTab tg = B.tabs[bd.opG[2]], tsh = B.tabs[bd.opS[2]];
   const int XS = tg.Kp + 8, RS = tsh.Kp + 8;
@@ -462,7 +511,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
       if (i < tot) {
         const int k = i >> 5, l = i & 31;
         __bf16 h, lo;
-        split2(v[u], h, lo);
+        split2<NT4>(v[u] * sc, h, lo);
         xh[l * XS + k] = h;
         xl[l * XS + k] = lo;
       }
@@ -488,10 +537,10 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
         const int q3 = p3 - bd.ishift[2];
         double v = 0.0;
         if (in12 && q3 >= 0 && q3 < bd.potdim[2]) v = pot[pz * q3];
-        const double pr = (double)acc[r];
+        const double pr = (double)(acc[r] * isc);
         ep = fma(v * pr, pr, ep);
         __bf16 h, lo;
-        split2((float)(v * pr), h, lo);
+        split2<BM>((float)(v * pr), h, lo);
         rh[line * RS + p3] = h;
         rl[line * RS + p3] = lo;
       }
@@ -502,7 +551,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
   __syncthreads();
   for (int ob = wv; ob < tsh.Dp / 32; ob += NTM / 64) {
     f32x16 acc = {};
-    mfma_block<NT4>(B, tsh, 32 * ob, rh, rl, RS, acc);
+    mfma_block<BM>(B, tsh, 32 * ob, rh, rl, RS, acc);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int o = 32 * ob + drow(r, lane);
@@ -514,7 +563,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
 // ---------------------------------------------------------------------------
 // x-y plane: out = G_x^T G_y^T T2 + K, compress-accumulate into hpsi
 // ---------------------------------------------------------------------------
-template <int NT4>
+template <int BM>
 __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x;
@@ -550,7 +599,7 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
       if (i < tot) {
         const int p2 = i / L1, p1 = i - p2 * L1;
         __bf16 h, lo;
-        split2(v[u], h, lo);
+        split2<BM>(v[u], h, lo);
         uh[p1 * XU + p2] = h;
         ul[p1 * XU + p2] = lo;
       }
@@ -564,14 +613,14 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
     for (int it = wv; it < nob * nlb; it += NTMP / 64) {
       const int ob = it % nob, lb = it / nob;
       f32x16 acc = {};
-      mfma_block<NT4>(B, sy, 32 * ob, uh + lb * 32 * XU, ul + lb * 32 * XU, XU, acc);
+      mfma_block<BM>(B, sy, 32 * ob, uh + lb * 32 * XU, ul + lb * 32 * XU, XU, acc);
       const int p1 = 32 * lb + (lane & 31);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int o2 = 32 * ob + drow(r, lane);
         if (o2 < W2 && p1 < L1) {
           __bf16 h, lo;
-          split2(acc[r], h, lo);
+          split2<BM>(acc[r], h, lo);
           bh[o2 * XB + p1] = h;
           bl[o2 * XB + p1] = lo;
         }
@@ -584,7 +633,7 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
     for (int it = wv; it < nob * nlb; it += NTMP / 64) {
       const int ob = it % nob, lb = it / nob;
       f32x16 acc = {};
-      mfma_block<NT4>(B, sx, 32 * ob, bh + lb * 32 * XB, bl + lb * 32 * XB, XB, acc);
+      mfma_block<BM>(B, sx, 32 * ob, bh + lb * 32 * XB, bl + lb * 32 * XB, XB, acc);
       const int j2 = 32 * lb + (lane & 31);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -642,6 +691,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int j3 = tile - box_first_tile(tt, b);
+  const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], L1 = bd.L[0], L2 = bd.L[1];
   const OpTab tx = B.tabs[bd.opG[0]], ty = B.tabs[bd.opG[1]];
   const OpTab kx = B.tabs[bd.opK[0]], ky = B.tabs[bd.opK[1]];
@@ -673,8 +723,8 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
         if (i < tot) {
           const int j2 = i / hp, j1 = 2 * (i - j2 * hp);
           __bf16 h0, l0, h1, l1;
-          split2(v[u].x, h0, l0);
-          split2(v[u].y, h1, l1);
+          split2<NT4>(v[u].x * sc, h0, l0);
+          split2<NT4>(v[u].y * sc, h1, l1);
           *(bf16x2*)(rh + j2 * XR + j1) = bf16x2{h0, h1};
           *(bf16x2*)(rl + j2 * XR + j1) = bf16x2{l0, l1};
         }
@@ -689,7 +739,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const int nobK = KX / 32;         // output blocks j1 of the kinetic
   const int nobG = tx.Dp / 32;      // output blocks p1 of G_x
   float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-  const float ksx = (float)bd.kscale[0], ksy = (float)bd.kscale[1];
+  const float ksx = (float)bd.kscale[0] * isc, ksy = (float)bd.kscale[1] * isc;
   // G_x -> C
   {
     const int ch2 = task_chunks(nobG, nlb, NW);
@@ -704,7 +754,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
 #pragma unroll
         for (int j = 0; j < MBF; ++j) {
           if (j >= nb) break;
-          store_cols(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
+          store_cols<NT4>(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
         }
       }
     }
@@ -731,7 +781,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
               const int p2 = 32 * ob + drow(r, lane);
-              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r];
+              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r] * isc;
             }
           }
         }
@@ -743,8 +793,8 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   for (int q = wv; q < nlb * nobK; q += NW) {
     const int lb = q / nobK, nb = q - lb * nobK;
     f32x16 acc = {};
-    mma_ident(rh + 32 * lb * XR, rl + 32 * lb * XR, XR, 32 * nb, acc);
-    store_cols(acc, ch, cl, XC, 32 * nb + (lane & 31), 32 * lb);
+    mma_ident<NT4>(rh + 32 * lb * XR, rl + 32 * lb * XR, XR, 32 * nb, acc);
+    store_cols<NT4>(acc, ch, cl, XC, 32 * nb + (lane & 31), 32 * lb);
   }
   __syncthreads();
   // kinetic tiles (j2 block lb, j1 block ob): x part from R with the table as
@@ -774,7 +824,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
 // lanes along j1 for the K reads and the compress.  One wave per output
 // block, sweeping all line bundles.
 // ---------------------------------------------------------------------------
-template <int NT4>
+template <int BM>
 __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x;
@@ -808,7 +858,7 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
         if (i < tot) {
           const int p2 = i / KX, p1 = i - p2 * KX;
           __bf16 h, l;
-          split2(v[u], h, l);
+          split2<BM>(v[u], h, l);
           rh[p2 * XR + p1] = h;
           rl[p2 * XR + p1] = l;
         }
@@ -828,12 +878,12 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
       for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBB) {
         const int nb = min(MBB, tk.lb_end - lb0);
         f32x16 acc[MBB] = {};
-        mma_multi<NT4, true, MBB>(B, sx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
+        mma_multi<BM, true, MBB>(B, sx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
                              32 * XR, nb, acc);
 #pragma unroll
         for (int j = 0; j < MBB; ++j) {
           if (j >= nb) break;
-          store_cols(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
+          store_cols<BM>(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
         }
       }
     }
@@ -851,7 +901,7 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
     for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBB) {
       const int nb = min(MBB, tk.lb_end - lb0);
       f32x16 acc[MBB] = {};
-      mma_multi<NT4, false, MBB>(B, sy, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
+      mma_multi<BM, false, MBB>(B, sy, 32 * ob, ch + 32 * lb0 * XC, cl + 32 * lb0 * XC, XC,
                                 32 * XC, nb, acc);
 #pragma unroll
       for (int j = 0; j < MBB; ++j) {
@@ -921,6 +971,7 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
+  const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], W3 = 2 * bd.N[2];
   const OpTab t = B.tabs[bd.opK[2]];
   const int XS = t.Kp + 8, XF = W3 + 1, KpW = t.Kp;
@@ -957,10 +1008,10 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
         const int k = rem >> 4, l = 2 * (rem & 15);
         __bf16 h, lo;
         const int o = (j * 32 + l) * XS + k;
-        split2(v[u].x, h, lo);
+        split2<NT4>(v[u].x * sc, h, lo);
         xh[o] = h;
         xl[o] = lo;
-        split2(v[u].y, h, lo);
+        split2<NT4>(v[u].y * sc, h, lo);
         xh[o + XS] = h;
         xl[o + XS] = lo;
         if (k < W3) {
@@ -973,7 +1024,7 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const float ks = (float)bd.kscale[2];
+  const float ks = (float)bd.kscale[2] * isc;
   float* K = (float*)B.K;
   double ek = 0.0;
   for (int ob = wv; ob < t.Dp / 32; ob += NTM / 64) {
@@ -1014,7 +1065,7 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
 }
 
 // z lines of T2, ZG bundles per work-group: G_z, V (epot), G_z^T in place.
-template <int NT4>
+template <int NT4, int BM>
 __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
@@ -1022,6 +1073,7 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
+  const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int L1 = bd.L[0], L2 = bd.L[1], W3 = 2 * bd.N[2], L3 = bd.L[2];
   const OpTab tg = B.tabs[bd.opG[2]], tsh = B.tabs[bd.opS[2]];
   const int XS = tg.Kp + 8, KpG = tg.Kp;
@@ -1066,11 +1118,11 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
         const int l = pairs ? 2 * (rem & 15) : (rem & 31);
         const int o = (j * 32 + l) * XS + k;
         __bf16 h, lo;
-        split2(v[u].x, h, lo);
+        split2<NT4>(v[u].x * sc, h, lo);
         xh[o] = h;
         xl[o] = lo;
         if (pairs) {
-          split2(v[u].y, h, lo);
+          split2<NT4>(v[u].y * sc, h, lo);
           xh[o + XS] = h;
           xl[o + XS] = lo;
         }
@@ -1108,10 +1160,10 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int r = 4 * g + u;
-        const double pr = (double)a[r];
+        const double pr = (double)(a[r] * isc);
         ep = fma(vv[r] * pr, pr, ep);
         __bf16 x, y;
-        split2((float)(vv[r] * pr), x, y);
+        split2<BM>((float)(vv[r] * pr), x, y);
         h[u] = x;
         l[u] = y;
       }
@@ -1140,7 +1192,7 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
   // G_z^T back to the wavelet z lines, in place
   for (int ob = wv; ob < tsh.Dp / 32; ob += NTM / 64) {
     f32x16 acc[ZG] = {};
-    mma_multi<NT4, false, ZG>(B, tsh, 32 * ob, rh, rl, RS, 32 * RS, nb, acc);
+    mma_multi<BM, false, ZG>(B, tsh, 32 * ob, rh, rl, RS, 32 * RS, nb, acc);
 #pragma unroll
     for (int j = 0; j < ZG; ++j) {
       if (j >= nb) break;
@@ -1193,14 +1245,14 @@ size_t mfma_lds_bwd(int W1, int L1, int L2) {
   return (size_t)4 * nR * (KX + 8) + (size_t)4 * nC * (nR + 8);
 }
 
-template <int NT4>
+template <int FM, int BM>
 static void set_attr(size_t lds) {
-  const void* k[] = {(const void*)k_mkin<0, NT4, false>, (const void*)k_mkin<1, NT4, false>,
-                     (const void*)k_mkin<2, NT4, false>, (const void*)k_mkin<2, NT4, true>,
-                     (const void*)k_mgrow_plane<NT4>, (const void*)k_mzpot<NT4>,
-                     (const void*)k_mshrink_plane<NT4>, (const void*)k_mfwd<NT4>,
-                     (const void*)k_mbwd<NT4>, (const void*)k_mkz<NT4>,
-                     (const void*)k_mzp<NT4>};
+  const void* k[] = {(const void*)k_mkin<0, FM, false>, (const void*)k_mkin<1, FM, false>,
+                     (const void*)k_mkin<2, FM, false>, (const void*)k_mkin<2, FM, true>,
+                     (const void*)k_mgrow_plane<FM>, (const void*)k_mzpot<FM, BM>,
+                     (const void*)k_mshrink_plane<BM>, (const void*)k_mfwd<FM>,
+                     (const void*)k_mbwd<BM>, (const void*)k_mkz<FM>,
+                     (const void*)k_mzp<FM, BM>};
   for (const void* f : k)
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
@@ -1208,12 +1260,15 @@ static void set_attr(size_t lds) {
 }
 
 size_t mfma_attributes(size_t lds) {
-  set_attr<3>(lds);
-  set_attr<4>(lds);
+  set_attr<3, 3>(lds);
+  set_attr<4, 4>(lds);
+  set_attr<16, 4>(lds);
   return lds;
 }
 
-template <int NT4>
+// FM: format of the forward transforms and the kinetic; BM: format of the
+// transposed transforms
+template <int FM, int BM>
 static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const MfmaLds& L,
                                 long long wtotal, bool need_zero, hipStream_t s,
                                 hipEvent_t* ev) {
@@ -1227,25 +1282,26 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
   }
   mark(S_ZERO, 1);
   mark(S_UNC, 0);
-  if (PT.ntiles[P_UNC] > 0) launch_uncompress_float(B, PT.t[P_UNC], PT.ntiles[P_UNC], s);
+  if (PT.ntiles[P_UNC] > 0)
+    launch_uncompress_float(B, PT.t[P_UNC], PT.ntiles[P_UNC], FM == 16, s);
   mark(S_UNC, 1);
   EnergyTables et;
   if (L.fused) {
     mark(S_PF, 0);
     if (PT.ntiles[P_PF] > 0)
-      k_mfwd<NT4><<<PT.ntiles[P_PF], NTMP, L.fwd, s>>>(B, PT.t[P_PF]);
+      k_mfwd<FM><<<PT.ntiles[P_PF], NTMP, L.fwd, s>>>(B, PT.t[P_PF]);
     mark(S_PF, 1);
     mark(S_KZ, 0);
     if (PT.ntiles[P_KZG] > 0)
-      k_mkz<NT4><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
+      k_mkz<FM><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
     mark(S_KZ, 1);
     mark(S_ZP, 0);
     if (PT.ntiles[P_ZPG] > 0)
-      k_mzp<NT4><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
+      k_mzp<FM, BM><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
     mark(S_ZP, 1);
     mark(S_PB, 0);
     if (PT.ntiles[P_PB] > 0)
-      k_mbwd<NT4><<<PT.ntiles[P_PB], NTMP, L.bwd, s>>>(B, PT.t[P_PB]);
+      k_mbwd<BM><<<PT.ntiles[P_PB], NTMP, L.bwd, s>>>(B, PT.t[P_PB]);
     mark(S_PB, 1);
     et.t[0] = PT.t[P_KZG];
     et.p[0] = PT.part[2];
@@ -1256,30 +1312,30 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
   } else {
     mark(S_XF, 0);
     if (PT.ntiles[P_XF] > 0)
-      k_mkin<0, NT4, false><<<PT.ntiles[P_XF], NTM, L.kin[0], s>>>(B, PT.t[P_XF],
+      k_mkin<0, FM, false><<<PT.ntiles[P_XF], NTM, L.kin[0], s>>>(B, PT.t[P_XF],
                                                                    PT.part[0]);
     mark(S_XF, 1);
     mark(S_KY, 0);
     if (PT.ntiles[P_KY] > 0)
-      k_mkin<1, NT4, false><<<PT.ntiles[P_KY], NTM, L.kin[1], s>>>(B, PT.t[P_KY],
+      k_mkin<1, FM, false><<<PT.ntiles[P_KY], NTM, L.kin[1], s>>>(B, PT.t[P_KY],
                                                                    PT.part[1]);
     mark(S_KY, 1);
     mark(S_KZ, 0);
     if (PT.ntiles[P_KZ] > 0)
-      k_mkin<2, NT4, false><<<PT.ntiles[P_KZ], NTM, L.kin[2], s>>>(B, PT.t[P_KZ],
+      k_mkin<2, FM, false><<<PT.ntiles[P_KZ], NTM, L.kin[2], s>>>(B, PT.t[P_KZ],
                                                                    PT.part[2]);
     mark(S_KZ, 1);
     mark(S_PF, 0);
     if (PT.ntiles[P_PF] > 0)
-      k_mgrow_plane<NT4><<<PT.ntiles[P_PF], NTMP, L.grow, s>>>(B, PT.t[P_PF]);
+      k_mgrow_plane<FM><<<PT.ntiles[P_PF], NTMP, L.grow, s>>>(B, PT.t[P_PF]);
     mark(S_PF, 1);
     mark(S_ZP, 0);
     if (PT.ntiles[P_ZP] > 0)
-      k_mzpot<NT4><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);
+      k_mzpot<FM, BM><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);
     mark(S_ZP, 1);
     mark(S_PB, 0);
     if (PT.ntiles[P_PB] > 0)
-      k_mshrink_plane<NT4><<<PT.ntiles[P_PB], NTMP, L.shrink, s>>>(B, PT.t[P_PB]);
+      k_mshrink_plane<BM><<<PT.ntiles[P_PB], NTMP, L.shrink, s>>>(B, PT.t[P_PB]);
     mark(S_PB, 1);
     et.t[0] = PT.t[P_XF];
     et.p[0] = PT.part[0];
@@ -1312,10 +1368,13 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
 }
 
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
-                       long long wtotal, bool need_zero, int nterm, hipStream_t s,
+                       long long wtotal, bool need_zero, int precision, hipStream_t s,
                        hipEvent_t* ev) {
-  return nterm == 4 ? launch_mfma_t<4>(B, PT, L, wtotal, need_zero, s, ev)
-                    : launch_mfma_t<3>(B, PT, L, wtotal, need_zero, s, ev);
+  switch (precision) {
+    case BDFT_PREC_BF16X3: return launch_mfma_t<3, 3>(B, PT, L, wtotal, need_zero, s, ev);
+    case BDFT_PREC_BF16X4: return launch_mfma_t<4, 4>(B, PT, L, wtotal, need_zero, s, ev);
+    default: return launch_mfma_t<16, 4>(B, PT, L, wtotal, need_zero, s, ev);
+  }
 }
 
 }  // namespace bdft

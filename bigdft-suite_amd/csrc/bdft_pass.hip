@@ -948,8 +948,75 @@ hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
 void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s) {
   k_zero<float><<<nblocks, NT, 0, s>>>(p, n);
 }
-void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, hipStream_t s) {
-  k_uncompress<float, false><<<ntiles, NT, 0, s>>>(B, t);
+// uncompress into the FP32 grid and record each tile's largest |psi| in
+// tmax[tile]; k_box_max then reduces the tiles of each box into bmax[box]
+__global__ __launch_bounds__(NT) void k_uncompress_max(Batch B, TileTable tt, float* tmax) {
+  __shared__ float red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  float* W = (float*)B.W + bd.w_off;
+  float mx = 0.0f;
+  if (lt < ctiles) {
+    const int k = lt * CPT + threadIdx.x;
+    if (k < bd.nvctr_c) {
+      const float v = (float)B.psi[bd.psi_off + k];
+      W[psig_index(bd, bd.cmap[k], 0, 0, 0)] = v;
+      mx = fabsf(v);
+    }
+  } else {
+    const int k = (lt - ctiles) * CPT + threadIdx.x;
+    if (k < bd.nvctr_f) {
+      const int lin = bd.fmap[k];
+      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
+#pragma unroll
+      for (int t = 1; t <= 7; ++t) {
+        const float v = (float)B.psi[base + t - 1];
+        W[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)] = v;
+        mx = fmaxf(mx, fabsf(v));
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int w = 1; w < NT / 64; ++w) m = fmaxf(m, red[w]);
+    tmax[tile] = m;
+  }
+}
+
+// one block per box: bmax[box] = max over the box's tiles (as float bits)
+__global__ __launch_bounds__(NT) void k_box_max(Batch B, TileTable tt, const float* tmax) {
+  __shared__ float red[NT / 64];
+  const int b = blockIdx.x;
+  const int t0 = box_first_tile(tt, b);
+  const int t1 = tt.uniform > 0 ? t0 + tt.uniform : tt.start[b + 1];
+  float mx = 0.0f;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += NT) mx = fmaxf(mx, tmax[t]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int w = 1; w < NT / 64; ++w) m = fmaxf(m, red[w]);
+    B.bmax[b] = __float_as_uint(m);
+  }
+}
+
+void launch_uncompress_float(const Batch& B, const TileTable& t, int ntiles, bool track_max,
+                             hipStream_t s) {
+  if (track_max) {
+    k_uncompress_max<<<ntiles, NT, 0, s>>>(B, t, B.tmax);
+    k_box_max<<<B.nbox, NT, 0, s>>>(B, t, B.tmax);
+  } else {
+    k_uncompress<float, false><<<ntiles, NT, 0, s>>>(B, t);
+  }
 }
 void launch_energy(const EnergyTables& et, const Batch& B, hipStream_t s) {
   k_energy<<<B.nbox, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);

@@ -10,7 +10,7 @@ module module_locham_hip
   private
 
   integer, parameter, public :: BDFT_PREC_FP64 = 0, BDFT_PREC_FP32 = 1, &
-       BDFT_PREC_BF16X3 = 2, BDFT_PREC_BF16X4 = 3
+       BDFT_PREC_BF16X3 = 2, BDFT_PREC_BF16X4 = 3, BDFT_PREC_F16X3 = 4
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
        local_partial_density_hip, free_gpu_hip, bdft_error_message

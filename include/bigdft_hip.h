@@ -54,8 +54,12 @@ enum {
   BDFT_PREC_FP32 = 1,    /* FP32 MFMA passes, FP64 energies and accumulate */
   BDFT_PREC_BF16X3 = 2,  /* split-BF16 MFMA, FP32 accumulation: filter and data
                             split hi+lo, products hi*hi + hi*lo + lo*hi */
-  BDFT_PREC_BF16X4 = 3   /* as BF16X3 plus the third filter piece times the data
+  BDFT_PREC_BF16X4 = 3,  /* as BF16X3 plus the third filter piece times the data
                             hi part (removes the filter's fixed 2^-16 error) */
+  BDFT_PREC_F16X3 = 4    /* forward transforms and kinetic on fp16 pieces of the
+                            data scaled by a power of two per box (three
+                            products, 2^-22 representation), transposed
+                            transforms as BF16X4; FP32 accumulation */
 };
 
 /* Context on one device.  Replaces init_acceleration_OCL
@@ -137,6 +141,29 @@ int bdft_density_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
 int bdft_density(bdft_ctx *ctx, const bdft_lr *lr, int norb, const double *psi,
                  const double *hfac, const int32_t *spin, int nspin, double *rho,
                  int precision);
+
+/* Global -> local compressed transfer for norb orbitals (device pointers):
+ * orbital i of the global region glr (at gpsi + goff[i], default
+ * i*(nvctr_c+7*nvctr_f) of glr) into local region llrs[i] whose box starts at
+ * ns[3i..3i+2] in glr's box (at lpsi + loff[i], default consecutive).
+ * Replaces psi_to_locreg2 (locregs/locreg_operations.f90:2143-2318):
+ * lpsi is set to the global values at the local points, zero where a local
+ * point is not in glr.  Free-BC regions only, as shift_locreg_indexes
+ * requires (:2338-2343).  Returns when done. */
+int bdft_psi_to_locreg_d(bdft_ctx *ctx, const bdft_lr *glr, int norb,
+                         const bdft_lr *const *llrs, const int32_t *ns,
+                         const double *gpsi, const int64_t *goff, double *lpsi,
+                         const int64_t *loff);
+
+/* Local -> global: the values of each local orbital written at its points
+ * of the global orbital (other global values untouched).  Replaces
+ * Lpsi_to_global2 (locreg_operations.f90:1665-1837, nspin = 1); where the
+ * reference stops because a local point is not in the global region
+ * (:1827-1831) this returns BDFT_ERR_DESC.  Returns when done. */
+int bdft_locreg_to_psi_d(bdft_ctx *ctx, const bdft_lr *glr, int norb,
+                         const bdft_lr *const *llrs, const int32_t *ns,
+                         const double *lpsi, const int64_t *loff, double *gpsi,
+                         const int64_t *goff);
 
 /* Plans: the same batch with fixed pointers, captured once into a hipGraph
  * and replayed per SCF iteration.  No counterpart in the reference (the

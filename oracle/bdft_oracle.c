@@ -937,6 +937,140 @@ void oref_density(char geocode, const int n[3], const int nfl[3],
 }
 
 /* ------------------------------------------------------------------------ */
+/* Transfers between a localisation region and the global region (free BC),  */
+/* locregs/locreg_operations.f90.  Local keys are in the local box (0:nl);   */
+/* the local box starts at ns in the global box (0:ng).                      */
+/* ------------------------------------------------------------------------ */
+
+/* shift_locreg_indexes (locreg_operations.f90:2321-2380): local segment
+ * bounds as 1-based linear indices of the global box */
+static void shift_keys(const int nl[3], const int ng[3], const int ns[3],
+                       int nseg, const int32_t *keyg_l, int32_t *mask) {
+  const long long pl = (long long)(nl[0] + 1) * (nl[1] + 1);
+  for (int s = 0; s < 2 * nseg; ++s) {
+    long long t = keyg_l[s] - 1;
+    const long long z = t / pl;
+    t -= z * pl;
+    const long long y = t / (nl[0] + 1);
+    const long long x = t - y * (nl[0] + 1);
+    mask[s] = (int32_t)((z + ns[2]) * (ng[0] + 1) * (long long)(ng[1] + 1) +
+                        (y + ns[1]) * (ng[0] + 1) + (x + ns[0]) + 1);
+  }
+}
+
+/* psi_to_locreg2 (locreg_operations.f90:2143-2318): lpsi = 0, then every
+ * element of a local segment that overlaps a global segment is copied from
+ * gpsi.  The local index advances with each copied element, as in the
+ * reference (correct when every local point lies in the global region). */
+void oref_psi_to_locreg2(const int nl[3], const int ng[3], const int ns[3],
+                         int nseg_lc, int nseg_lf, int nvctr_lc, int nvctr_lf,
+                         const int32_t *keyg_l, const int32_t *keyv_l,
+                         int nseg_gc, int nseg_gf, int nvctr_gc,
+                         const int32_t *keyg_g, const int32_t *keyv_g,
+                         const double *gpsi, double *lpsi) {
+  const int nseg = nseg_lc + nseg_lf;
+  int32_t *mask = malloc(sizeof(int32_t) * 2 * (nseg ? nseg : 1));
+  shift_keys(nl, ng, ns, nseg, keyg_l, mask);
+  memset(lpsi, 0, sizeof(double) * ((size_t)nvctr_lc + 7 * (size_t)nvctr_lf));
+  int isegstart = 0;
+  for (int il = 0; il < nseg_lc; ++il) {
+    const int lmin = mask[2 * il], lmax = mask[2 * il + 1];
+    int istart = keyv_l[il] - 1;
+    for (int ig = isegstart; ig < nseg_gc; ++ig) {
+      const int gmin = keyg_g[2 * ig], gmax = keyg_g[2 * ig + 1];
+      if (lmin > gmax) isegstart = ig;
+      if (gmin > lmax) break;
+      if (lmin > gmax || lmax < gmin) continue;
+      const int offset = lmin - gmin > 0 ? lmin - gmin : 0;
+      const int length = (lmax < gmax ? lmax : gmax) - (lmin > gmin ? lmin : gmin);
+      for (int ix = 0; ix <= length; ++ix) {
+        istart += 1;
+        lpsi[istart - 1] = gpsi[keyv_g[ig] + offset + ix - 1];
+      }
+    }
+  }
+  const int start = nvctr_lc, gstart = nvctr_gc;
+  isegstart = nseg_gc;
+  for (int il = nseg_lc; il < nseg; ++il) {
+    const int lmin = mask[2 * il], lmax = mask[2 * il + 1];
+    int istart = keyv_l[il] - 1;
+    for (int ig = isegstart; ig < nseg_gc + nseg_gf; ++ig) {
+      const int gmin = keyg_g[2 * ig], gmax = keyg_g[2 * ig + 1];
+      if (lmin > gmax) isegstart = ig;
+      if (gmin > lmax) break;
+      if (lmin > gmax || lmax < gmin) continue;
+      const int offset = lmin - gmin > 0 ? lmin - gmin : 0;
+      const int length = (lmax < gmax ? lmax : gmax) - (lmin > gmin ? lmin : gmin);
+      for (int ix = 0; ix <= length; ++ix) {
+        istart += 1;
+        for (int igrid = 1; igrid <= 7; ++igrid)
+          lpsi[start + (size_t)(istart - 1) * 7 + igrid - 1] =
+              gpsi[gstart + (size_t)(keyv_g[ig] + offset + ix - 1) * 7 + igrid - 1];
+      }
+    }
+  }
+  free(mask);
+}
+
+/* Lpsi_to_global2 (locreg_operations.f90:1665-1837), nspin = 1: the
+ * elements of the local segments (in global coordinates) are written into
+ * psi where they overlap global segments; psi is otherwise left as it is.
+ * Returns the number of local points placed (the reference stops when it
+ * differs from nvctr_c + nvctr_f). */
+int oref_lpsi_to_global2(const int nl[3], const int ng[3], const int ns[3],
+                         int nseg_lc, int nseg_lf, int nvctr_lc, int nvctr_lf,
+                         const int32_t *keyg_l, const int32_t *keyv_l,
+                         int nseg_gc, int nseg_gf, int nvctr_gc,
+                         const int32_t *keyg_g, const int32_t *keyv_g,
+                         const double *lpsi, double *psi) {
+  (void)nvctr_lf;
+  const int nseg = nseg_lc + nseg_lf;
+  int32_t *mask = malloc(sizeof(int32_t) * 2 * (nseg ? nseg : 1));
+  shift_keys(nl, ng, ns, nseg, keyg_l, mask);
+  int icheck = 0, isegstart = 0;
+  for (int il = 0; il < nseg_lc; ++il) {
+    const int lmin = mask[2 * il], lmax = mask[2 * il + 1];
+    int istart = keyv_l[il] - 1;
+    for (int ig = isegstart; ig < nseg_gc; ++ig) {
+      const int gmin = keyg_g[2 * ig], gmax = keyg_g[2 * ig + 1];
+      if (lmin > gmax) isegstart = ig;
+      if (gmin > lmax) break;
+      if (lmin > gmax) continue;
+      const int offset = lmin - gmin > 0 ? lmin - gmin : 0;
+      const int length = (lmax < gmax ? lmax : gmax) - (lmin > gmin ? lmin : gmin);
+      icheck += length + 1;
+      for (int ix = 0; ix <= length; ++ix) {
+        istart += 1;
+        psi[keyv_g[ig] + offset + ix - 1] = lpsi[istart - 1];
+      }
+    }
+  }
+  const int start = nvctr_lc, gstart = nvctr_gc;
+  isegstart = nseg_gc;
+  for (int il = nseg_lc; il < nseg; ++il) {
+    const int lmin = mask[2 * il], lmax = mask[2 * il + 1];
+    int istart = keyv_l[il] - 1;
+    for (int ig = isegstart; ig < nseg_gc + nseg_gf; ++ig) {
+      const int gmin = keyg_g[2 * ig], gmax = keyg_g[2 * ig + 1];
+      if (lmin > gmax) isegstart = ig;
+      if (gmin > lmax) break;
+      if (lmin > gmax) continue;
+      const int offset = lmin - gmin > 0 ? lmin - gmin : 0;
+      const int length = (lmax < gmax ? lmax : gmax) - (lmin > gmin ? lmin : gmin);
+      icheck += length + 1;
+      for (int ix = 0; ix <= length; ++ix) {
+        istart += 1;
+        for (int igrid = 1; igrid <= 7; ++igrid)
+          psi[gstart + (size_t)(keyv_g[ig] + offset + ix - 1) * 7 + igrid - 1] =
+              lpsi[start + (size_t)(istart - 1) * 7 + igrid - 1];
+      }
+    }
+  }
+  free(mask);
+  return icheck;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Descriptors, locregs/initlocregs.f90:1573-1686.                           */
 /* logrid is (0:n1,0:n2,0:n3) column-major, nonzero = point present.         */
 /* ------------------------------------------------------------------------ */

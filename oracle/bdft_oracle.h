@@ -127,6 +127,20 @@ void oref_density(char geocode, const int n[3], const int nfl[3],
                   int norb, const double *psi, const double *hfac,
                   const int *spin, double *rho);
 
+/* ---- transfers (locregs/locreg_operations.f90:1665-1837, 2143-2380) ---- */
+void oref_psi_to_locreg2(const int nl[3], const int ng[3], const int ns[3],
+                         int nseg_lc, int nseg_lf, int nvctr_lc, int nvctr_lf,
+                         const int32_t *keyg_l, const int32_t *keyv_l,
+                         int nseg_gc, int nseg_gf, int nvctr_gc,
+                         const int32_t *keyg_g, const int32_t *keyv_g,
+                         const double *gpsi, double *lpsi);
+int oref_lpsi_to_global2(const int nl[3], const int ng[3], const int ns[3],
+                         int nseg_lc, int nseg_lf, int nvctr_lc, int nvctr_lf,
+                         const int32_t *keyg_l, const int32_t *keyv_l,
+                         int nseg_gc, int nseg_gf, int nvctr_gc,
+                         const int32_t *keyg_g, const int32_t *keyv_g,
+                         const double *lpsi, double *psi);
+
 /* ---- descriptors (locregs/initlocregs.f90:1573-1686) ---- */
 int oref_num_segkeys(int n1, int n2, int n3, int nl1, int nu1, int nl2,
                      int nu2, int nl3, int nu3, const uint8_t *logrid,

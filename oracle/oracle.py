@@ -63,6 +63,10 @@ def _load():
     lib.oref_density.argtypes = [
         c_char, _c_int3, _c_int3, _c_int3, c_int, c_int, _ip, _ip, c_int,
         c_int, _ip, _ip, c_int, _dp, _dp, _ip, _dp]
+    for name in ("oref_psi_to_locreg2", "oref_lpsi_to_global2"):
+        getattr(lib, name).argtypes = [_c_int3, _c_int3, _c_int3] + [c_int] * 4 + \
+            [_ip, _ip] + [c_int] * 3 + [_ip, _ip, _dp, _dp]
+    lib.oref_lpsi_to_global2.restype = c_int
     lib.oref_num_segkeys.argtypes = [c_int] * 9 + [_u8, ctypes.POINTER(c_int),
                                                    ctypes.POINTER(c_int)]
     lib.oref_num_segkeys.restype = c_int
@@ -311,3 +315,34 @@ def density(geocode, n, nfl, nfu, keys, psi, hfac, spin=None, nspin=1,
         keys.nvctr_f, keys.keyg_f, keys.keyv_f, norb, psi,
         np.ascontiguousarray(hfac, dtype=np.float64), sp, r)
     return r
+
+
+def _cat_keys(keys):
+    kg = np.concatenate([keys.keyg_c, keys.keyg_f]).astype(np.int32)
+    kv = np.concatenate([keys.keyv_c, keys.keyv_f]).astype(np.int32)
+    return np.ascontiguousarray(kg), np.ascontiguousarray(kv)
+
+
+def psi_to_locreg2(nl, ng, ns, lkeys, gkeys, gpsi):
+    """global compressed orbital -> local compressed orbital (free BC)."""
+    kgl, kvl = _cat_keys(lkeys)
+    kgg, kvg = _cat_keys(gkeys)
+    lpsi = np.zeros(lkeys.nvctr_c + 7 * lkeys.nvctr_f)
+    lib().oref_psi_to_locreg2(
+        _c_int3(*nl), _c_int3(*ng), _c_int3(*ns), lkeys.nseg_c, lkeys.nseg_f,
+        lkeys.nvctr_c, lkeys.nvctr_f, kgl, kvl, gkeys.nseg_c, gkeys.nseg_f,
+        gkeys.nvctr_c, kgg, kvg, np.ascontiguousarray(gpsi, dtype=np.float64), lpsi)
+    return lpsi
+
+
+def lpsi_to_global2(nl, ng, ns, lkeys, gkeys, lpsi, psi):
+    """local compressed orbital written into a copy of the global one;
+    returns (psi, number of local points placed)."""
+    kgl, kvl = _cat_keys(lkeys)
+    kgg, kvg = _cat_keys(gkeys)
+    out = np.array(psi, dtype=np.float64, copy=True)
+    n = lib().oref_lpsi_to_global2(
+        _c_int3(*nl), _c_int3(*ng), _c_int3(*ns), lkeys.nseg_c, lkeys.nseg_f,
+        lkeys.nvctr_c, lkeys.nvctr_f, kgl, kvl, gkeys.nseg_c, gkeys.nseg_f,
+        gkeys.nvctr_c, kgg, kvg, np.ascontiguousarray(lpsi, dtype=np.float64), out)
+    return out, n

@@ -421,3 +421,27 @@ def test_bf16x3_three_products(ctx):
     assert np.abs(hg - ho).max() < BF_H * np.abs(ho).max()
     assert np.all(np.abs(eg - eo) < 2e-5 * np.abs(eo))
     assert np.all(np.abs(pg - po) < 2e-5 * np.abs(po))
+
+
+# split-FP16 forward transforms and kinetic (power-of-two scaling per box),
+# split-BF16 transposed transforms: same bars as the four-product BF16 form
+@pytest.mark.parametrize("seq", ["plane", "lines"])
+@pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
+                                        ("F", (15, 13, 17), 3), ("F", (39, 39, 39), 5),
+                                        ("P", (47, 20, 9), 6)])
+def test_f16x3(ctx, ctx_lines, seq, geo, n, seed):
+    reg = dense_region(geo, n, (0.45, 0.4, 0.5))
+    print(f"f16x3 case {seq} {geo} {n}", flush=True)
+    psi = random_psi(reg, 2, 1500 + seed) * 1e-3
+    pot = random_pot(reg, 1600 + seed)
+    c = ctx if seq == "plane" else ctx_lines
+    hg, eg, pg = run_gpu(c, reg, pot, psi, precision=4)
+    ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                               okeys(reg), pot, psi)
+    eh = np.abs(hg - ho).max() / np.abs(ho).max()
+    ee = np.max(np.abs(eg - eo) / np.abs(eo))
+    ep_ = np.max(np.abs(pg - po) / np.abs(po))
+    print(f"f16x3 {seq} {geo} {n}: hpsi {eh:.2e} ekin {ee:.2e} epot {ep_:.2e}",
+          flush=True)
+    assert eh < BF_H
+    assert ee < BF_E and ep_ < BF_E

@@ -1,0 +1,76 @@
+"""Region transfers through the C ABI vs the oracle: bit-exact."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import okeys
+from test_oracle_transfers import setup
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import bigdft_hip
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    c = bigdft_hip.Context(0)
+    yield c
+    c.close()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_transfers_batch_bit_exact(ctx):
+    cases = [setup(s) for s in (1, 2, 3)]
+    glob = cases[0][0]
+    ng = cases[0][2]
+    g = ctx.region(glob)
+    locs = [ctx.region(c[1]) for c in cases]
+    ns = np.array([c[4] for c in cases], dtype=np.int32)
+    rng = np.random.default_rng(7)
+    gpsi = rng.standard_normal((3, glob.ncoef))
+    lnc = [c[1].ncoef for c in cases]
+    loff = np.concatenate([[0], np.cumsum(lnc)[:-1]])
+    lpsi = torch.full((sum(lnc),), np.nan, dtype=torch.float64, device="cuda")
+    ctx.psi_to_locreg(g, locs, ns, dev(gpsi), lpsi)
+    got = lpsi.cpu().numpy()
+    for i, (gl, lo, _, nl, nsi) in enumerate(cases):
+        exp = oracle.psi_to_locreg2(nl, ng, nsi, okeys(lo), okeys(gl), gpsi[i])
+        assert np.array_equal(got[loff[i]:loff[i] + lnc[i]], exp)
+    # local -> global onto a global array holding other values
+    lsrc = rng.standard_normal(sum(lnc))
+    base = rng.standard_normal((3, glob.ncoef))
+    gout = dev(base)
+    ctx.locreg_to_psi(g, locs, ns, dev(lsrc), gout)
+    got = gout.cpu().numpy()
+    for i, (gl, lo, _, nl, nsi) in enumerate(cases):
+        exp, n = oracle.lpsi_to_global2(nl, ng, nsi, okeys(lo), okeys(gl),
+                                        lsrc[loff[i]:loff[i] + lnc[i]], base[i])
+        assert n == lo.nvctr_c + lo.nvctr_f
+        assert np.array_equal(got[i], exp)
+    for l in locs:
+        l.close()
+    g.close()
+
+
+def test_locreg_to_psi_rejects_points_outside(ctx):
+    import bigdft_hip
+    from bigdft_hip.descriptors import region_from_grids
+    glob, loc, ng, nl, ns = setup(4)
+    full = np.ones(tuple(x + 1 for x in nl), dtype=bool)
+    loc2 = region_from_grids("F", nl, loc.hgrid, full, np.zeros_like(full))
+    g, l = ctx.region(glob), ctx.region(loc2)
+    gpsi = torch.zeros(glob.ncoef, dtype=torch.float64, device="cuda")
+    lp = dev(np.ones(loc2.ncoef))
+    with pytest.raises(bigdft_hip.BigDFTError, match="DESC"):
+        ctx.locreg_to_psi(g, [l], np.array(ns, dtype=np.int32), lp, gpsi)
+    # the other direction fills the missing points with zeros
+    out = torch.full((loc2.ncoef,), np.nan, dtype=torch.float64, device="cuda")
+    ctx.psi_to_locreg(g, [l], np.array(ns, dtype=np.int32),
+                      dev(np.arange(glob.ncoef, dtype=np.float64)), out)
+    assert not torch.isnan(out).any()
+    l.close()
+    g.close()
