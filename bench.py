@@ -62,6 +62,25 @@ def path_of(prec, plane_enabled, stages):
     return "plane" if plane_enabled else "lines"
 
 
+def measured_traffic(precision, config, stage):
+    """HBM bytes per launch of a stage from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a
+    device copy of known size), or None when there is no summary"""
+    if config != "c2":
+        return None
+    path = os.path.join(ROOT, "profiles", f"rocprof_{precision}_periodic48_256orb.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if "```json" not in text:
+        return None
+    blob = text.split("```json", 1)[1].split("```", 1)[0]
+    try:
+        return json.loads(blob)["bytes_per_launch"].get(stage)
+    except (ValueError, KeyError):
+        return None
+
+
 def roofline(path, avg, units, esz, peak_c):
     """roofline entry of the dominant (longest) kernel stage"""
     tab = PATH_STAGES[path]
@@ -260,6 +279,11 @@ def main():
     else:
         peak_c = PEAK_FP64_TFLOPS if esz == 8 else PEAK_FP32_TFLOPS
     roof = roofline(head["path"], avg, units, esz, peak_c)
+    roof["traffic"] = measured_traffic(args.precision, args.config, roof["kernel"])
+    if roof["traffic"] is not None:
+        roof["traffic_bytes_per_pt"] = roof["traffic"] / units
+        roof["traffic_source"] = (f"profiles/rocprof_{args.precision}_periodic48_256orb.md "
+                                  "(FETCH_SIZE+WRITE_SIZE per dispatch, copy-calibrated)")
 
     # ---- parity on a sample + CPU baseline (oracle, host cores) ----
     parity, cpu = None, None
