@@ -36,7 +36,8 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_local_hamiltonian", "bdft_uncompress_d",
            "bdft_compress_acc_d", "bdft_profile_enable", "bdft_profile_read",
            "bdft_stage_name", "bdft_ctx_info", "bdft_sync", "bdft_density_d",
-           "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d")
+           "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d",
+           "bdft_bounds_layout", "bdft_locreg_bounds_d", "bdft_locreg_bounds")
 
 
 class BigDFTError(RuntimeError):
@@ -79,6 +80,9 @@ def load_library(path=LIB_PATH):
         "bdft_profile_enable": [P, I],
         "bdft_profile_read": [P, I, P, P, I],
         "bdft_ctx_info": [P, P, P],
+        "bdft_bounds_layout": [I3, I3, I3, P, P],
+        "bdft_locreg_bounds_d": [P, P, P],
+        "bdft_locreg_bounds": [P, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -275,6 +279,21 @@ class Context:
         """Lpsi_to_global2: local regions -> their points of global orbitals"""
         self._xfer(lib().bdft_locreg_to_psi_d, glr, llrs, ns, lpsi, loff, gpsi, goff)
 
+    def locreg_bounds(self, lr, out=None):
+        """locreg_bounds: the 17 convolution-bounds arrays of a free-BC
+        region in one int32 buffer (layout: bounds_layout).  out: a device
+        int32 tensor of bounds_layout(...)[0][17] elements (filled in stream
+        order), else a numpy array is returned."""
+        r = lr.region
+        off, _ = bounds_layout(r.n, r.nfl, r.nfu)
+        if out is not None:
+            assert out.dtype == torch_int32() and out.numel() >= off[17]
+            self._check(lib().bdft_locreg_bounds_d(self.handle, lr.handle, _ptr(out)))
+            return out
+        buf = np.zeros(int(off[17]), dtype=np.int32)
+        self._check(lib().bdft_locreg_bounds(self.handle, lr.handle, _ptr(buf)))
+        return buf
+
     def plan(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None, potdim=None,
              ishift=None, psi_off=None, precision=PREC_FP64):
         arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
@@ -353,3 +372,28 @@ def local_partial_density(ctx, lr, occup, psi, rho, kwgt=1.0, spinsgn=None,
         if nspin == 1 and spin.any():
             raise ValueError("spin-down orbitals need nspin = 2")
     ctx.density(lr, psi, hfac, rho, spin=spin, nspin=nspin, precision=precision)
+
+
+BOUNDS_NAMES = ("ibyz_c", "ibxz_c", "ibxy_c", "ibyz_f", "ibxz_f", "ibxy_f",
+                "ibzxx_c", "ibxxyy_c", "ibyz_ff", "ibzxx_f", "ibxxyy_f",
+                "ibzzx_c", "ibyyzz_c", "ibxy_ff", "ibzzx_f", "ibyyzz_f",
+                "ibyyzz_r")
+
+
+def bounds_layout(n, nfl, nfu):
+    """(offset[18], dims[17, 4]) of the convolution-bounds buffer
+    (bdft_bounds_layout): array k = BOUNDS_NAMES[k] starts at offset[k] and
+    is the Fortran array ib(2, lo1:hi1, lo2:hi2), dims[k] = (lo1, hi1, lo2,
+    hi2)."""
+    I3 = ctypes.c_int * 3
+    off = np.zeros(18, dtype=np.int64)
+    dims = np.zeros(68, dtype=np.int32)
+    rc = lib().bdft_bounds_layout(I3(*n), I3(*nfl), I3(*nfu), _ptr(off), _ptr(dims))
+    if rc:
+        raise BigDFTError(rc, "nfl > nfu or n < 0")
+    return off, dims.reshape(17, 4)
+
+
+def torch_int32():
+    import torch
+    return torch.int32

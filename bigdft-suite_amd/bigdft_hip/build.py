@@ -7,7 +7,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libbigdft_hip.so")
 SOURCES = ["bdft_api.hip", "bdft_pass.hip", "bdft_plane.hip", "bdft_mfma.hip",
-           "bdft_xfer.hip"]
+           "bdft_xfer.hip", "bdft_bounds.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

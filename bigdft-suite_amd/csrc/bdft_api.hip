@@ -1095,6 +1095,78 @@ extern "C" int bdft_locreg_to_psi_d(bdft_ctx* ctx, const bdft_lr* glr, int norb,
   return xfer(ctx, glr, norb, llrs, ns, lpsi, loff, gpsi, goff, false);
 }
 
+// ---------------------------------------------------------------------------
+// convolution bounds (locregs/bounds.f90:555-613, locreg_bounds)
+// ---------------------------------------------------------------------------
+extern "C" int bdft_bounds_layout(const int n[3], const int nfl[3], const int nfu[3],
+                                  int64_t offset[18], int32_t dims[68]) {
+  if (!n || !nfl || !nfu || !offset || !dims) return BDFT_ERR_ARG;
+  return bounds_layout(n, nfl, nfu, offset, dims) ? BDFT_ERR_ARG : BDFT_OK;
+}
+
+static int bounds_check(bdft_ctx* ctx, const bdft_lr* lr, int64_t* total) {
+  if (!lr) return set_err(ctx, BDFT_ERR_ARG, "null region");
+  if (lr->ctx != ctx) return set_err(ctx, BDFT_ERR_ARG, "region from another context");
+  if (lr->geocode != 'F')
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED, "convolution bounds are defined for free BC");
+  for (int d = 0; d < 3; ++d) {
+    if (lr->n[d] > BOUNDS_MAXN)
+      return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                     "n%d=%d: bounds need n <= %d (the 1000 sentinel)", d + 1, lr->n[d],
+                     BOUNDS_MAXN);
+    if (lr->nfl[d] < 0 || lr->nfl[d] > lr->nfu[d] || lr->nfu[d] > lr->n[d])
+      return set_err(ctx, BDFT_ERR_DESC, "fine box [%d,%d] outside 0..%d", lr->nfl[d],
+                     lr->nfu[d], lr->n[d]);
+  }
+  int64_t off[BOUNDS_NARR + 1];
+  int32_t dims[4 * BOUNDS_NARR];
+  bounds_layout(lr->n, lr->nfl, lr->nfu, off, dims);
+  *total = off[BOUNDS_NARR];
+  return BDFT_OK;
+}
+
+extern "C" int bdft_locreg_bounds_d(bdft_ctx* ctx, const bdft_lr* lr, int32_t* bounds) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!bounds) return set_err(ctx, BDFT_ERR_ARG, "null output");
+  int64_t total = 0;
+  int rc = bounds_check(ctx, lr, &total);
+  if (rc) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  BoundsArgs A{};
+  for (int d = 0; d < 3; ++d) {
+    A.n[d] = lr->n[d];
+    A.nfl[d] = lr->nfl[d];
+    A.nfu[d] = lr->nfu[d];
+  }
+  A.cinv = lr->d_cinv;
+  A.finv = lr->d_finv;
+  A.out = bounds;
+  HIPCHK(ctx, launch_bounds(A, ctx->stream));
+  return BDFT_OK;
+}
+
+extern "C" int bdft_locreg_bounds(bdft_ctx* ctx, const bdft_lr* lr, int32_t* bounds) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!bounds) return set_err(ctx, BDFT_ERR_ARG, "null output");
+  int64_t total = 0;
+  int rc = bounds_check(ctx, lr, &total);
+  if (rc) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DevBuf buf;
+  rc = dev_alloc(ctx, buf, sizeof(int32_t) * (size_t)total);
+  if (rc) return rc;
+  rc = bdft_locreg_bounds_d(ctx, lr, (int32_t*)buf.p);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(bounds, buf.p, sizeof(int32_t) * (size_t)total,
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_HIP, "D2H copy: %s", hipGetErrorString(e));
+  }
+  hipStreamSynchronize(ctx->stream);
+  dev_free(buf);
+  return rc;
+}
+
 extern "C" int bdft_locham_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* pot,
                              const double* psi, double* hpsi, double* ekin, double* epot,
                              int precision) {

@@ -269,6 +269,19 @@ struct XferArgs {
 hipError_t launch_xfer(const XferArgs& A, int ntiles, hipStream_t s);
 int xfer_tiles(int nvctr_c, int nvctr_f);
 
+// convolution bounds of a free-BC region (bdft_bounds.hip)
+constexpr int BOUNDS_NARR = 17;
+constexpr int BOUNDS_MAXN = 484;  // values stay below the 1000 sentinel
+int bounds_layout(const int n[3], const int nfl[3], const int nfu[3], int64_t off[18],
+                  int32_t dims[68]);
+struct BoundsArgs {
+  int n[3], nfl[3], nfu[3];
+  const int* cinv;      // grid point -> coarse coefficient (null: all present)
+  const int* finv;      // grid point -> fine coefficient (null: all present)
+  int32_t* out;         // bounds_layout buffer
+};
+hipError_t launch_bounds(const BoundsArgs& A, hipStream_t s);
+
 size_t lds_bytes(int maxN, int elem);
 hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
                           long long wtotal, bool need_zero, int fp32, hipStream_t s);

@@ -13,7 +13,8 @@ module module_locham_hip
        BDFT_PREC_BF16X3 = 2, BDFT_PREC_BF16X4 = 3, BDFT_PREC_F16X3 = 4
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
-       local_partial_density_hip, free_gpu_hip, bdft_error_message
+       local_partial_density_hip, free_gpu_hip, bdft_error_message, &
+       bdft_bounds_layout, bdft_locreg_bounds
 
   interface
      integer(c_int) function bdft_ctx_create(device, ctx) bind(C, name='bdft_ctx_create')
@@ -63,6 +64,21 @@ module module_locham_hip
        integer(c_int32_t), dimension(*), intent(in) :: spin
        real(c_double), dimension(*), intent(inout) :: rho
      end function bdft_density
+     !> offsets and index bounds of the 17 convolution-bounds arrays
+     integer(c_int) function bdft_bounds_layout(n, nfl, nfu, offset, dims) &
+          bind(C, name='bdft_bounds_layout')
+       import :: c_int, c_int32_t, c_int64_t
+       integer(c_int), dimension(3), intent(in) :: n, nfl, nfu
+       integer(c_int64_t), dimension(18), intent(out) :: offset
+       integer(c_int32_t), dimension(68), intent(out) :: dims
+     end function bdft_bounds_layout
+     !> locreg_bounds (bounds.f90:555-613) into one host buffer
+     integer(c_int) function bdft_locreg_bounds(ctx, lr, bounds) &
+          bind(C, name='bdft_locreg_bounds')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx, lr
+       integer(c_int32_t), dimension(*), intent(out) :: bounds
+     end function bdft_locreg_bounds
   end interface
 
 contains

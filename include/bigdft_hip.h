@@ -165,6 +165,26 @@ int bdft_locreg_to_psi_d(bdft_ctx *ctx, const bdft_lr *glr, int norb,
                          const double *lpsi, const int64_t *loff, double *gpsi,
                          const int64_t *goff);
 
+/* Convolution bounds of a free-BC region: the 17 arrays of
+ * convolutions_bounds (bigdft/src/locregs/bounds.f90:44-50) in one int32
+ * buffer, in the order kb%ibyz_c ibxz_c ibxy_c ibyz_f ibxz_f ibxy_f,
+ * gb%ibzxx_c ibxxyy_c ibyz_ff ibzxx_f ibxxyy_f, sb%ibzzx_c ibyyzz_c ibxy_ff
+ * ibzzx_f ibyyzz_f, ibyyzz_r.  Array k starts at offset[k] (offset[17] is
+ * the total) and is the Fortran array ib(2, lo1:hi1, lo2:hi2), column-major,
+ * with dims[4k..4k+3] = lo1, hi1, lo2, hi2.  Pure host function. */
+int bdft_bounds_layout(const int n[3], const int nfl[3], const int nfu[3],
+                       int64_t offset[18], int32_t dims[68]);
+/* Replaces locreg_bounds (bounds.f90:555-613; called through
+ * ensure_locreg_bounds, bigdft/src/locregs/locregs.f90:664-676): the arrays
+ * computed on the device from the region's keys into a device buffer of
+ * offset[17] int32 (stream-ordered).  Values as the reference's, including
+ * ibyz_c squared into 2x2 blocks (make_ib_c squares it in place).  Free BC
+ * only, n <= 484 per axis (the reference's 1000 sentinel must stay above
+ * every index). */
+int bdft_locreg_bounds_d(bdft_ctx *ctx, const bdft_lr *lr, int32_t *bounds);
+/* The same into a host buffer; returns when done. */
+int bdft_locreg_bounds(bdft_ctx *ctx, const bdft_lr *lr, int32_t *bounds);
+
 /* Plans: the same batch with fixed pointers, captured once into a hipGraph
  * and replayed per SCF iteration.  No counterpart in the reference (the
  * OpenCL path re-enqueues every kernel per orbital). */

@@ -1138,3 +1138,282 @@ int oref_segkeys(int n1, int n2, int n3, int nl1, int nu1, int nl2, int nu2,
     }
   return nend != nsrt ? 1 : 0;
 }
+
+/* ======================================================================
+ * Convolution bounds, locregs/bounds.f90:555-1123.
+ *
+ * Restated with the reference's own mechanism: a flat logical work grid is
+ * filled from one array's intervals (ib_to_logrid_*) and read back, with
+ * its axes taken in a rotated order, as the first/last true index of every
+ * row (ib_from_logrid_*).  The rotation is only a reinterpretation of the
+ * flat grid's extents, as in the reference; the tests check every stage
+ * against a direct per-axis formulation.
+ * ====================================================================== */
+
+/* Fortran ib(2, lo1:hi1, lo2:hi2) addressing */
+#define IB(a, k, i, j, lo1, n1_, lo2) \
+  (a)[(k) + 2 * (((i) - (lo1)) + (size_t)(n1_) * ((j) - (lo2)))]
+
+int oref_bounds_layout(const int n[3], const int nfl[3], const int nfu[3],
+                       int64_t off[18], int32_t dims[68]) {
+  for (int d = 0; d < 3; ++d)
+    if (nfl[d] > nfu[d]) return 1;
+  const int n1 = n[0], n2 = n[1], n3 = n[2];
+  const int l1 = nfl[0], u1 = nfu[0], l2 = nfl[1], u2 = nfu[1], l3 = nfl[2],
+            u3 = nfu[2];
+  /* bounds.f90:572-601 (allocation extents, same order as the types) */
+  const int32_t d[17][4] = {
+      {0, n2, 0, n3},                                  /* ibyz_c   */
+      {0, n1, 0, n3},                                  /* ibxz_c   */
+      {0, n1, 0, n2},                                  /* ibxy_c   */
+      {0, n2, 0, n3},                                  /* ibyz_f   */
+      {0, n1, 0, n3},                                  /* ibxz_f   */
+      {0, n1, 0, n2},                                  /* ibxy_f   */
+      {0, n3, -14, 2 * n1 + 16},                       /* ibzxx_c  */
+      {-14, 2 * n1 + 16, -14, 2 * n2 + 16},            /* ibxxyy_c */
+      {l2, u2, l3, u3},                                /* ibyz_ff  */
+      {l3, u3, 2 * l1 - 14, 2 * u1 + 16},              /* ibzxx_f  */
+      {2 * l1 - 14, 2 * u1 + 16, 2 * l2 - 14, 2 * u2 + 16}, /* ibxxyy_f */
+      {-14, 2 * n3 + 16, 0, n1},                       /* ibzzx_c  */
+      {-14, 2 * n2 + 16, -14, 2 * n3 + 16},            /* ibyyzz_c */
+      {l1, u1, l2, u2},                                /* ibxy_ff  */
+      {2 * l3 - 14, 2 * u3 + 16, l1, u1},              /* ibzzx_f  */
+      {2 * l2 - 14, 2 * u2 + 16, 2 * l3 - 14, 2 * u3 + 16}, /* ibyyzz_f */
+      {-14, 2 * n2 + 16, -14, 2 * n3 + 16}};           /* ibyyzz_r */
+  int64_t o = 0;
+  for (int k = 0; k < 17; ++k) {
+    off[k] = o;
+    for (int j = 0; j < 4; ++j) dims[4 * k + j] = d[k][j];
+    o += 2 * (int64_t)(d[k][1] - d[k][0] + 1) * (d[k][3] - d[k][2] + 1);
+  }
+  off[17] = o;
+  return 0;
+}
+
+/* make_bounds, bounds.f90:690-775 */
+static void ob_make_bounds(int n1, int n2, int n3, const uint8_t *lg,
+                           int32_t *ibyz, int32_t *ibxz, int32_t *ibxy) {
+#define LG(i1, i2, i3) lg[(i1) + (size_t)(n1 + 1) * ((i2) + (size_t)(n2 + 1) * (i3))]
+  for (int i3 = 0; i3 <= n3; ++i3)
+    for (int i2 = 0; i2 <= n2; ++i2) {
+      int a = 1000, b = -1000;
+      for (int i1 = 0; i1 <= n1; ++i1)
+        if (LG(i1, i2, i3)) { a = i1; break; }
+      for (int i1 = n1; i1 >= 0; --i1)
+        if (LG(i1, i2, i3)) { b = i1; break; }
+      IB(ibyz, 0, i2, i3, 0, n2 + 1, 0) = a;
+      IB(ibyz, 1, i2, i3, 0, n2 + 1, 0) = b;
+    }
+  for (int i3 = 0; i3 <= n3; ++i3)
+    for (int i1 = 0; i1 <= n1; ++i1) {
+      int a = 1000, b = -1000;
+      for (int i2 = 0; i2 <= n2; ++i2)
+        if (LG(i1, i2, i3)) { a = i2; break; }
+      for (int i2 = n2; i2 >= 0; --i2)
+        if (LG(i1, i2, i3)) { b = i2; break; }
+      IB(ibxz, 0, i1, i3, 0, n1 + 1, 0) = a;
+      IB(ibxz, 1, i1, i3, 0, n1 + 1, 0) = b;
+    }
+  for (int i2 = 0; i2 <= n2; ++i2)
+    for (int i1 = 0; i1 <= n1; ++i1) {
+      int a = 1000, b = -1000;
+      for (int i3 = 0; i3 <= n3; ++i3)
+        if (LG(i1, i2, i3)) { a = i3; break; }
+      for (int i3 = n3; i3 >= 0; --i3)
+        if (LG(i1, i2, i3)) { b = i3; break; }
+      IB(ibxy, 0, i1, i2, 0, n1 + 1, 0) = a;
+      IB(ibxy, 1, i1, i2, 0, n1 + 1, 0) = b;
+    }
+#undef LG
+}
+
+/* ib_to_logrid_inv, bounds.f90:895-915: logrid(-14+2nfl:2nfu+16, ndat) */
+static void ob_to_logrid_inv(const int32_t *ib, uint8_t *lg, int nfl, int nfu,
+                             long ndat) {
+  const long w = 2L * (nfu - nfl) + 31;
+  memset(lg, 0, (size_t)(w * ndat));
+  for (long l = 0; l < ndat; ++l)
+    for (int i = 2 * ib[2 * l] - 14; i <= 2 * ib[2 * l + 1] + 16; ++i)
+      lg[(i - (2 * nfl - 14)) + w * l] = 1;
+}
+
+/* ib_from_logrid_inv, bounds.f90:919-950: logrid(ndat, ml1:mu1) */
+static void ob_from_logrid_inv(int32_t *ib, const uint8_t *lg, int ml1,
+                               int mu1, long ndat) {
+  for (long i = 0; i < ndat; ++i) {
+    ib[2 * i] = 1000;
+    ib[2 * i + 1] = -1000;
+    for (int i1 = ml1; i1 <= mu1; ++i1)
+      if (lg[i + ndat * (i1 - ml1)]) { ib[2 * i] = i1; break; }
+    for (int i1 = mu1; i1 >= ml1; --i1)
+      if (lg[i + ndat * (i1 - ml1)]) { ib[2 * i + 1] = i1; break; }
+  }
+}
+
+/* ib_to_logrid_rot, bounds.f90:1010-1028: logrid(ndat, -14+2nfl:2nfu+16) */
+static void ob_to_logrid_rot(const int32_t *ib, uint8_t *lg, int nfl, int nfu,
+                             long ndat) {
+  const long w = 2L * (nfu - nfl) + 31;
+  memset(lg, 0, (size_t)(w * ndat));
+  for (long l = 0; l < ndat; ++l)
+    for (int i = 2 * ib[2 * l] - 14; i <= 2 * ib[2 * l + 1] + 16; ++i)
+      lg[l + ndat * (i - (2 * nfl - 14))] = 1;
+}
+
+/* ib_from_logrid, bounds.f90:1032-1063: logrid(ml1:mu1, ndat) */
+static void ob_from_logrid(int32_t *ib, const uint8_t *lg, int ml1, int mu1,
+                           long ndat) {
+  const long w = mu1 - ml1 + 1;
+  for (long i = 0; i < ndat; ++i) {
+    ib[2 * i] = 1000;
+    ib[2 * i + 1] = -1000;
+    for (int i1 = ml1; i1 <= mu1; ++i1)
+      if (lg[(i1 - ml1) + w * i]) { ib[2 * i] = i1; break; }
+    for (int i1 = mu1; i1 >= ml1; --i1)
+      if (lg[(i1 - ml1) + w * i]) { ib[2 * i + 1] = i1; break; }
+  }
+}
+
+/* squares, bounds.f90:1097-1123: ib(2, 0:n2, 0:n3) made of 2x2 blocks */
+static void ob_squares(int32_t *ib, int n2, int n3) {
+  if (n2 == 0 || n3 == 0) return;
+  for (int i3 = 0; i3 <= (n3 - 1) / 2; ++i3) {
+    const int j3 = 2 * i3;
+    for (int i2 = 0; i2 <= (n2 - 1) / 2; ++i2) {
+      const int j2 = 2 * i2;
+      int32_t *p[4] = {&IB(ib, 0, j2, j3, 0, n2 + 1, 0),
+                       &IB(ib, 0, j2 + 1, j3, 0, n2 + 1, 0),
+                       &IB(ib, 0, j2, j3 + 1, 0, n2 + 1, 0),
+                       &IB(ib, 0, j2 + 1, j3 + 1, 0, n2 + 1, 0)};
+      int mn = p[0][0], mx = p[0][1];
+      for (int q = 1; q < 4; ++q) {
+        if (p[q][0] < mn) mn = p[q][0];
+        if (p[q][1] > mx) mx = p[q][1];
+      }
+      for (int q = 0; q < 4; ++q) {
+        p[q][0] = mn;
+        p[q][1] = mx;
+      }
+    }
+  }
+}
+
+/* squares_1d, bounds.f90:1069-1091: ib(2, nfl2:nfu2, nfl3:nfu3) made of
+ * pairs along the first axis (C division truncates like Fortran's) */
+static void ob_squares_1d(int32_t *ib, int nfl2, int nfu2, int nfl3, int nfu3) {
+  const int w = nfu2 - nfl2 + 1;
+  for (int i3 = nfl3; i3 <= nfu3; ++i3)
+    for (int i2 = nfl2 / 2; i2 <= (nfu2 - 1) / 2; ++i2) {
+      const int j2 = 2 * i2;
+      int32_t *a = &IB(ib, 0, j2, i3, nfl2, w, nfl3);
+      int32_t *b = &IB(ib, 0, j2 + 1, i3, nfl2, w, nfl3);
+      const int mn = a[0] < b[0] ? a[0] : b[0];
+      const int mx = a[1] > b[1] ? a[1] : b[1];
+      a[0] = b[0] = mn;
+      a[1] = b[1] = mx;
+    }
+}
+
+/* make_ib_inv, bounds.f90:868-891 */
+static void ob_make_ib_inv(uint8_t *lg, const int32_t *ibxy, int32_t *ibzzx,
+                           int32_t *ibyyzz, int l1, int u1, int l2, int u2,
+                           int l3, int u3) {
+  long nt = (long)(u1 - l1 + 1) * (u2 - l2 + 1);
+  ob_to_logrid_inv(ibxy, lg, l3, u3, nt);
+  nt = (2L * (u3 - l3) + 31) * (u1 - l1 + 1);
+  ob_from_logrid_inv(ibzzx, lg, l2, u2, nt);
+  ob_to_logrid_inv(ibzzx, lg, l2, u2, nt);
+  nt = (2L * (u2 - l2) + 31) * (2L * (u3 - l3) + 31);
+  ob_from_logrid_inv(ibyyzz, lg, l1, u1, nt);
+}
+
+/* make_ib_c, bounds.f90:954-980 (squares ibyz in place, as the reference) */
+static void ob_make_ib_c(uint8_t *lg, int32_t *ibyz, int32_t *ibzxx,
+                         int32_t *ibxxyy, int n1, int n2, int n3) {
+  ob_squares(ibyz, n2, n3);
+  long nt = (long)(n2 + 1) * (n3 + 1);
+  ob_to_logrid_rot(ibyz, lg, 0, n1, nt);
+  nt = (long)(n3 + 1) * (2 * n1 + 31);
+  ob_from_logrid(ibzxx, lg, 0, n2, nt);
+  ob_squares(ibzxx, n3, 2 * n1 + 30);
+  ob_to_logrid_rot(ibzxx, lg, 0, n2, nt);
+  nt = (long)(2 * n1 + 31) * (2 * n2 + 31);
+  ob_from_logrid(ibxxyy, lg, 0, n3, nt);
+  ob_squares(ibxxyy, 2 * n1 + 30, 2 * n2 + 30);
+}
+
+/* make_ib, bounds.f90:984-1006 */
+static void ob_make_ib(uint8_t *lg, const int32_t *ibyz, int32_t *ibzxx,
+                       int32_t *ibxxyy, int l1, int u1, int l2, int u2, int l3,
+                       int u3) {
+  long nt = (long)(u2 - l2 + 1) * (u3 - l3 + 1);
+  ob_to_logrid_rot(ibyz, lg, l1, u1, nt);
+  nt = (long)(u3 - l3 + 1) * (2 * (u1 - l1) + 31);
+  ob_from_logrid(ibzxx, lg, l2, u2, nt);
+  ob_to_logrid_rot(ibzxx, lg, l2, u2, nt);
+  nt = (long)(2 * (u1 - l1) + 31) * (2 * (u2 - l2) + 31);
+  ob_from_logrid(ibxxyy, lg, l3, u3, nt);
+}
+
+/* wfd_to_logrids (bounds.f90:616-688), make_bounds, make_all_ib
+ * (bounds.f90:777-865) as called by locreg_bounds (bounds.f90:555-613) */
+int oref_locreg_bounds(const int n[3], const int nfl[3], const int nfu[3],
+                       int nseg_c, const int32_t *keyg_c, int nseg_f,
+                       const int32_t *keyg_f, int32_t *out) {
+  int64_t off[18];
+  int32_t dims[68];
+  if (oref_bounds_layout(n, nfl, nfu, off, dims)) return 1;
+  const int n1 = n[0], n2 = n[1], n3 = n[2];
+  const size_t M = (size_t)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  uint8_t *lgc = calloc(M, 1), *lgf = calloc(M, 1);
+  for (int f = 0; f < 2; ++f) {
+    const int nseg = f ? nseg_f : nseg_c;
+    const int32_t *kg = f ? keyg_f : keyg_c;
+    uint8_t *lg = f ? lgf : lgc;
+    for (int s = 0; s < nseg; ++s) {
+      const long j0 = kg[2 * s] - 1, len = kg[2 * s + 1] - kg[2 * s];
+      for (long i = 0; i <= len; ++i) lg[j0 + i] = 1;
+    }
+  }
+  int32_t *A[17];
+  for (int k = 0; k < 17; ++k) A[k] = out + off[k];
+  ob_make_bounds(n1, n2, n3, lgc, A[0], A[1], A[2]);
+  ob_make_bounds(n1, n2, n3, lgf, A[3], A[4], A[5]);
+  free(lgc);
+  free(lgf);
+  const int l1 = nfl[0], u1 = nfu[0], l2 = nfl[1], u2 = nfu[1], l3 = nfl[2],
+            u3 = nfu[2];
+  uint8_t *big = malloc((size_t)(2 * n1 + 31) * (2 * n2 + 31) * (2 * n3 + 31));
+  /* shrink: ibxy_ff = ibxy_f on the fine box */
+  for (int i2 = l2; i2 <= u2; ++i2)
+    for (int i1 = l1; i1 <= u1; ++i1)
+      for (int k = 0; k < 2; ++k)
+        IB(A[13], k, i1, i2, l1, u1 - l1 + 1, l2) = IB(A[5], k, i1, i2, 0, n1 + 1, 0);
+  ob_make_ib_inv(big, A[2], A[11], A[12], 0, n1, 0, n2, 0, n3);
+  ob_make_ib_inv(big, A[13], A[14], A[15], l1, u1, l2, u2, l3, u3);
+  /* real space */
+  const int w2 = 2 * n2 + 31;
+  for (int i3 = -14; i3 <= 2 * n3 + 16; ++i3)
+    for (int i2 = -14; i2 <= 2 * n2 + 16; ++i2) {
+      const int a = IB(A[12], 0, i2, i3, -14, w2, -14);
+      if (a != 1000) {
+        IB(A[16], 0, i2, i3, -14, w2, -14) = 2 * a;
+        IB(A[16], 1, i2, i3, -14, w2, -14) = 2 * IB(A[12], 1, i2, i3, -14, w2, -14) + 30;
+      } else {
+        IB(A[16], 0, i2, i3, -14, w2, -14) = 1000;
+        IB(A[16], 1, i2, i3, -14, w2, -14) = -1000;
+      }
+    }
+  ob_squares(A[16], 2 * n2 + 30, 2 * n3 + 30);
+  /* grow: ibyz_ff = ibyz_f on the fine box */
+  for (int i2 = l2; i2 <= u2; ++i2)
+    for (int i3 = l3; i3 <= u3; ++i3)
+      for (int k = 0; k < 2; ++k)
+        IB(A[8], k, i2, i3, l2, u2 - l2 + 1, l3) = IB(A[3], k, i2, i3, 0, n2 + 1, 0);
+  ob_make_ib_c(big, A[0], A[6], A[7], n1, n2, n3);
+  ob_make_ib(big, A[8], A[9], A[10], l1, u1, l2, u2, l3, u3);
+  ob_squares_1d(A[10], 2 * l1 - 14, 2 * u1 + 16, 2 * l2 - 14, 2 * u2 + 16);
+  free(big);
+  return 0;
+}
+#undef IB

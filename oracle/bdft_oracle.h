@@ -149,6 +149,19 @@ int oref_segkeys(int n1, int n2, int n3, int nl1, int nu1, int nl2, int nu2,
                  int nl3, int nu3, const uint8_t *logrid, int mseg,
                  int32_t *keyg, int32_t *keyv);
 
+/* ---- convolution bounds (locregs/bounds.f90:555-1123) ----
+ * The 17 integer arrays of convolutions_bounds in one int32 buffer, in the
+ * order kb (ibyz_c ibxz_c ibxy_c ibyz_f ibxz_f ibxy_f), gb (ibzxx_c ibxxyy_c
+ * ibyz_ff ibzxx_f ibxxyy_f), sb (ibzzx_c ibyyzz_c ibxy_ff ibzzx_f ibyyzz_f),
+ * ibyyzz_r; each is the Fortran array ib(2, lo1:hi1, lo2:hi2) column-major.
+ * layout: off[k] = first element of array k (off[17] = total), dims[4k..4k+3]
+ * = lo1, hi1, lo2, hi2.  Returns nonzero when nfl > nfu on an axis. */
+int oref_bounds_layout(const int n[3], const int nfl[3], const int nfu[3],
+                       int64_t off[18], int32_t dims[68]);
+int oref_locreg_bounds(const int n[3], const int nfl[3], const int nfu[3],
+                       int nseg_c, const int32_t *keyg_c, int nseg_f,
+                       const int32_t *keyg_f, int32_t *out);
+
 int oref_num_threads(void);
 
 #ifdef __cplusplus

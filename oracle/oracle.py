@@ -73,6 +73,12 @@ def _load():
     lib.oref_segkeys.argtypes = [c_int] * 9 + [_u8, c_int, _ip, _ip]
     lib.oref_segkeys.restype = c_int
     lib.oref_num_threads.restype = c_int
+    _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    lib.oref_bounds_layout.argtypes = [_c_int3, _c_int3, _c_int3, _i64, _ip]
+    lib.oref_bounds_layout.restype = c_int
+    lib.oref_locreg_bounds.argtypes = [_c_int3, _c_int3, _c_int3, c_int, _ip,
+                                       c_int, _ip, _ip]
+    lib.oref_locreg_bounds.restype = c_int
     return lib
 
 
@@ -346,3 +352,45 @@ def lpsi_to_global2(nl, ng, ns, lkeys, gkeys, lpsi, psi):
         lkeys.nvctr_c, lkeys.nvctr_f, kgl, kvl, gkeys.nseg_c, gkeys.nseg_f,
         gkeys.nvctr_c, kgg, kvg, np.ascontiguousarray(lpsi, dtype=np.float64), out)
     return out, n
+
+
+BOUNDS_NAMES = ("ibyz_c", "ibxz_c", "ibxy_c", "ibyz_f", "ibxz_f", "ibxy_f",
+                "ibzxx_c", "ibxxyy_c", "ibyz_ff", "ibzxx_f", "ibxxyy_f",
+                "ibzzx_c", "ibyyzz_c", "ibxy_ff", "ibzzx_f", "ibyyzz_f",
+                "ibyyzz_r")
+
+
+def bounds_layout(n, nfl, nfu):
+    """(off[18], dims[17, 4]) of the convolution-bounds buffer: array k
+    starts at off[k] and is ib(2, lo1:hi1, lo2:hi2) with dims[k] =
+    (lo1, hi1, lo2, hi2)."""
+    off = np.zeros(18, dtype=np.int64)
+    dims = np.zeros(68, dtype=np.int32)
+    if lib().oref_bounds_layout(_c_int3(*n), _c_int3(*nfl), _c_int3(*nfu), off, dims):
+        raise ValueError("nfl > nfu")
+    return off, dims.reshape(17, 4)
+
+
+def unpack_bounds(buf, n, nfl, nfu):
+    """dict name -> (ib, lo1, lo2); ib[k, i - lo1, j - lo2] is the
+    reference's ib(k+1, i, j)"""
+    off, dims = bounds_layout(n, nfl, nfu)
+    out = {}
+    for k, name in enumerate(BOUNDS_NAMES):
+        lo1, hi1, lo2, hi2 = (int(x) for x in dims[k])
+        a = np.asarray(buf[off[k]:off[k + 1]])
+        a = a.reshape(hi2 - lo2 + 1, hi1 - lo1 + 1, 2).transpose(2, 1, 0)
+        out[name] = (a, lo1, lo2)
+    return out
+
+
+def locreg_bounds(n, nfl, nfu, keys):
+    """convolution bounds of a free-BC region (flat int32 buffer)"""
+    off, _ = bounds_layout(n, nfl, nfu)
+    buf = np.zeros(int(off[17]), dtype=np.int32)
+    kc = np.ascontiguousarray(keys.keyg_c, dtype=np.int32)
+    kf = np.ascontiguousarray(keys.keyg_f, dtype=np.int32)
+    if lib().oref_locreg_bounds(_c_int3(*n), _c_int3(*nfl), _c_int3(*nfu),
+                                keys.nseg_c, kc, keys.nseg_f, kf, buf):
+        raise ValueError("nfl > nfu")
+    return buf

@@ -44,3 +44,17 @@ def test_null_arguments_are_errors():
     assert lib.bdft_plan_execute(None) == 1
     assert lib.bdft_lr_destroy(None) == 1
     assert lib.bdft_ctx_create(0, None) == 1
+
+
+@pytest.mark.parametrize("n,nfl,nfu", [((14, 17, 12), (3, 5, 2), (10, 12, 9)),
+                                       ((7, 0, 5), (0, 0, 0), (7, 0, 5)),
+                                       ((40, 33, 21), (17, 16, 9), (18, 16, 11))])
+def test_bounds_layout_matches_oracle(n, nfl, nfu):
+    """the library's bounds layout (a host function) against the oracle's"""
+    import numpy as np
+    import oracle
+    off, dims = bigdft_hip.bounds_layout(n, nfl, nfu)
+    o_off, o_dims = oracle.bounds_layout(n, nfl, nfu)
+    assert np.array_equal(off, o_off) and np.array_equal(dims, o_dims)
+    with pytest.raises(bigdft_hip.BigDFTError):
+        bigdft_hip.bounds_layout(n, (5, 0, 0), (4, 0, 0))
