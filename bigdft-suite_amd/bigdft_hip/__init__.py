@@ -37,7 +37,8 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_compress_acc_d", "bdft_profile_enable", "bdft_profile_read",
            "bdft_stage_name", "bdft_ctx_info", "bdft_sync", "bdft_density_d",
            "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d",
-           "bdft_bounds_layout", "bdft_locreg_bounds_d", "bdft_locreg_bounds")
+           "bdft_bounds_layout", "bdft_locreg_bounds_d", "bdft_locreg_bounds",
+           "bdft_locham_conf_batch_d")
 
 
 class BigDFTError(RuntimeError):
@@ -65,6 +66,7 @@ def load_library(path=LIB_PATH):
         "bdft_lr_info": [P, I3, ctypes.POINTER(ctypes.c_int64)],
         "bdft_locham_d": [P, P, I, P, P, P, P, P, I],
         "bdft_locham_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
+        "bdft_locham_conf_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, P, I],
         "bdft_density_d": [P, P, I, P, P, P, I, P, I],
         "bdft_density": [P, P, I, P, P, P, I, P, I],
         "bdft_psi_to_locreg_d": [P, P, I, P, P, P, P, P, P],
@@ -242,13 +244,23 @@ class Context:
 
     def locham_batch(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None,
                      potdim=None, ishift=None, psi_off=None,
-                     precision=PREC_FP64):
+                     precision=PREC_FP64, conf=None):
+        """conf: None or one CONFPOT_DTYPE record per orbital (host)"""
         arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
                                                psi_off)
-        self._check(lib().bdft_locham_batch_d(
+        if conf is None:
+            self._check(lib().bdft_locham_batch_d(
+                self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
+                _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
+                _ptr(epot), precision))
+            return
+        c = np.ascontiguousarray(conf, dtype=CONFPOT_DTYPE).ravel()
+        if c.size != len(lrs):
+            raise ValueError("one confinement record per orbital")
+        self._check(lib().bdft_locham_conf_batch_d(
             self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
             _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
-            _ptr(epot), precision))
+            _ptr(epot), _ptr(c), precision))
 
     def density(self, lr, psi, hfac, rho, spin=None, nspin=1, precision=PREC_FP64):
         """rho[s] += hfac[i] psir_i^2 over the orbitals (rows of the device
@@ -372,6 +384,21 @@ def local_partial_density(ctx, lr, occup, psi, rho, kwgt=1.0, spinsgn=None,
         if nspin == 1 and spin.any():
             raise ValueError("spin-down orbitals need nspin = 2")
     ctx.density(lr, psi, hfac, rho, spin=spin, nspin=nspin, precision=precision)
+
+
+# bdft_confpot (confpot_data, bigdft/src/locregs/locreg_operations.f90:23-30)
+CONFPOT_DTYPE = np.dtype([("potorder", np.int32), ("ioffset", np.int32, 3),
+                          ("prefac", np.float64), ("hh", np.float64, 3),
+                          ("rxyzConf", np.float64, 3), ("damping", np.float64)],
+                         align=True)
+
+
+def confpot(potorder, prefac, hh, center, ioffset, damping=1.0):
+    """one bdft_confpot record"""
+    c = np.zeros(1, dtype=CONFPOT_DTYPE)
+    c["potorder"], c["prefac"], c["damping"] = potorder, prefac, damping
+    c["hh"], c["rxyzConf"], c["ioffset"] = hh, center, ioffset
+    return c
 
 
 BOUNDS_NAMES = ("ibyz_c", "ibxz_c", "ibxy_c", "ibyz_f", "ibxz_f", "ibxy_f",

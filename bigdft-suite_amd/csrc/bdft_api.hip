@@ -525,7 +525,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      const double* pot, const int64_t* pot_off, const int32_t* potdim,
                      const int32_t* ishift, const double* psi, const int64_t* psi_off,
                      double* hpsi, double* ekin, double* epot, int precision,
-                     bool own_ws, bdft_plan** out, bool density = false) {
+                     bool own_ws, bdft_plan** out, bool density = false,
+                     const bdft_confpot* conf = nullptr) {
   if (!ctx || !out) return BDFT_ERR_ARG;
   *out = nullptr;
   if (norb <= 0 || !lrs || !psi || (!density && (!pot || !hpsi || !ekin || !epot)))
@@ -562,6 +563,19 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                        lr->N[d], d + 1, MAXN);
     }
     b.per = lr->geocode == 'P';
+    if (conf && conf[i].potorder != 0) {
+      const bdft_confpot& c = conf[i];
+      if (c.potorder < 0)
+        return set_err(ctx, BDFT_ERR_ARG, "orbital %d: negative confinement order", i);
+      b.conf_on = 1;
+      b.conf_half = c.potorder / 2;
+      b.conf_prefac = c.prefac;
+      for (int d = 0; d < 3; ++d) {
+        b.conf_ioff[d] = c.ioffset[d];
+        b.conf_hh[d] = c.hh[d];
+        b.conf_r[d] = c.rxyzConf[d];
+      }
+    }
     b.nvctr_c = lr->nvctr_c;
     b.nvctr_f = lr->nvctr_f;
     b.cmap = lr->d_cmap;
@@ -854,11 +868,11 @@ extern "C" int bdft_plan_destroy(bdft_plan* plan) {
 
 // one-shot batch: reuses the context's last plan when called again with the
 // same arguments (the common case of an SCF loop)
-extern "C" int bdft_locham_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
-                                   const double* pot, const int64_t* pot_off,
-                                   const int32_t* potdim, const int32_t* ishift,
-                                   const double* psi, const int64_t* psi_off, double* hpsi,
-                                   double* ekin, double* epot, int precision) {
+static int locham_batch(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                        const double* pot, const int64_t* pot_off, const int32_t* potdim,
+                        const int32_t* ishift, const double* psi, const int64_t* psi_off,
+                        double* hpsi, double* ekin, double* epot,
+                        const bdft_confpot* conf, int precision) {
   if (!ctx) return BDFT_ERR_ARG;
   if (norb <= 0 || !lrs) return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
   std::vector<uintptr_t> key;
@@ -880,7 +894,13 @@ extern "C" int bdft_locham_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const
       if (potdim) key.push_back((uintptr_t)(uint32_t)potdim[3 * i + d]);
       if (ishift) key.push_back((uintptr_t)(uint32_t)ishift[3 * i + d]);
     }
+    if (conf) {  // the confinement parameters, bit for bit
+      const uintptr_t* w = (const uintptr_t*)&conf[i];
+      for (size_t k = 0; k < sizeof(bdft_confpot) / sizeof(uintptr_t); ++k)
+        key.push_back(w[k]);
+    }
   }
+  key.push_back((uintptr_t)(conf != nullptr));
   if (!ctx->cached || key != ctx->cached_key) {
     if (ctx->cached) {
       HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -890,12 +910,31 @@ extern "C" int bdft_locham_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const
     }
     bdft_plan* plan = nullptr;
     int rc = make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
-                       epot, precision, false, &plan);
+                       epot, precision, false, &plan, false, conf);
     if (rc) return rc;
     ctx->cached = plan;
     ctx->cached_key.swap(key);
   }
   return run_plan(ctx->cached);
+}
+
+extern "C" int bdft_locham_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                                   const double* pot, const int64_t* pot_off,
+                                   const int32_t* potdim, const int32_t* ishift,
+                                   const double* psi, const int64_t* psi_off, double* hpsi,
+                                   double* ekin, double* epot, int precision) {
+  return locham_batch(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
+                      epot, nullptr, precision);
+}
+
+extern "C" int bdft_locham_conf_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                                        const double* pot, const int64_t* pot_off,
+                                        const int32_t* potdim, const int32_t* ishift,
+                                        const double* psi, const int64_t* psi_off,
+                                        double* hpsi, double* ekin, double* epot,
+                                        const bdft_confpot* conf, int precision) {
+  return locham_batch(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
+                      epot, conf, precision);
 }
 
 // ---------------------------------------------------------------------------

@@ -45,7 +45,26 @@ struct BoxDesc {
   const int* finv;      // grid point -> fine point index or -1
                         // (null: identity, every point present in key order)
   int opG[3], opS[3], opK[3];  // operator tables per axis (matrix-core path)
+  // confining potential prefac * r2^conf_half added to V (conf_on = 0: none);
+  // r2 = sum_d (conf_hh_d (p_d + 1 + conf_ioff_d) - conf_r_d)^2 at the fine
+  // point p of the box (0-based)
+  int conf_on, conf_half;
+  int conf_ioff[3];
+  double conf_prefac;
+  double conf_hh[3];
+  double conf_r[3];
 };
+
+// the confining term at fine point (p1, p2, p3); callers test bd.conf_on
+__device__ __forceinline__ double conf_axis2(const BoxDesc& bd, int d, int p) {
+  const double x = bd.conf_hh[d] * (double)(p + 1 + bd.conf_ioff[d]) - bd.conf_r[d];
+  return x * x;
+}
+__device__ __forceinline__ double conf_term(const BoxDesc& bd, double r2) {
+  double t = 1.0;
+  for (int k = 0; k < bd.conf_half; ++k) t *= r2;
+  return bd.conf_prefac * t;
+}
 
 // A 1D operator as a padded matrix for the matrix-core path: rows = outputs,
 // cols = inputs, split in bf16 hi/lo; mask bit s of 32-row block q set when

@@ -526,6 +526,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
                     q2 < bd.potdim[1];
   const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
   const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + line) + conf_axis2(bd, 1, j2) : 0.0;
   double ep = 0.0;
   for (int ob = wv; ob < tg.Dp / 32; ob += NTM / 64) {
     f32x16 acc = {};
@@ -536,7 +537,10 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
       if (p3 < L3) {
         const int q3 = p3 - bd.ishift[2];
         double v = 0.0;
-        if (in12 && q3 >= 0 && q3 < bd.potdim[2]) v = pot[pz * q3];
+        if (in12 && q3 >= 0 && q3 < bd.potdim[2]) {
+          v = pot[pz * q3];
+          if (bd.conf_on) v += conf_term(bd, cxy + conf_axis2(bd, 2, p3));
+        }
         const double pr = (double)(acc[r] * isc);
         ep = fma(v * pr, pr, ep);
         __bf16 h, lo;
@@ -1150,6 +1154,16 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
       const int p3 = 32 * ob + drow(r, lane);
       const int q3 = p3 - bd.ishift[2];
       vv[r] = (in12 && p3 < L3 && q3 >= 0 && q3 < bd.potdim[2]) ? pot[pz * q3] : 0.0;
+    }
+    if (bd.conf_on) {
+      const double cxy = conf_axis2(bd, 0, p1) + conf_axis2(bd, 1, p2);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int p3 = 32 * ob + drow(r, lane);
+        const int q3 = p3 - bd.ishift[2];
+        if (in12 && p3 < L3 && q3 >= 0 && q3 < bd.potdim[2])
+          vv[r] += conf_term(bd, cxy + conf_axis2(bd, 2, p3));
+      }
     }
   };
   auto finish = [&](int j, int ob, const f32x16& a, const double (&vv)[16]) {

@@ -556,6 +556,7 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
                     q2 < bd.potdim[1];
   const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
   const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
   T g[3][8];
   double ep = 0.0;
 #pragma unroll
@@ -568,7 +569,10 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
       for (int r = 0; r < 8; ++r) {
         const int q3 = p0 + r - bd.ishift[2];
         double v = 0.0;
-        if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) v = pot[pz * q3];
+        if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) {
+          v = pot[pz * q3];
+          if (bd.conf_on) v += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
+        }
         const double pr = (double)g[m][r];
         ep = fma(v * pr, pr, ep);
         g[m][r] = (T)(v * pr);

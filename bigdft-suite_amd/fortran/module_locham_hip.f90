@@ -14,7 +14,18 @@ module module_locham_hip
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
        local_partial_density_hip, free_gpu_hip, bdft_error_message, &
-       bdft_bounds_layout, bdft_locreg_bounds
+       bdft_bounds_layout, bdft_locreg_bounds, bdft_locham_conf_batch_d
+
+  !> confpot_data (locregs/locreg_operations.f90:23-30) as the C struct
+  !! bdft_confpot
+  type, bind(C), public :: bdft_confpot
+     integer(c_int) :: potorder
+     integer(c_int), dimension(3) :: ioffset
+     real(c_double) :: prefac
+     real(c_double), dimension(3) :: hh
+     real(c_double), dimension(3) :: rxyzConf
+     real(c_double) :: damping
+  end type bdft_confpot
 
   interface
      integer(c_int) function bdft_ctx_create(device, ctx) bind(C, name='bdft_ctx_create')
@@ -64,6 +75,17 @@ module module_locham_hip
        integer(c_int32_t), dimension(*), intent(in) :: spin
        real(c_double), dimension(*), intent(inout) :: rho
      end function bdft_density
+     !> batched Hamiltonian with one confining potential per orbital
+     !! (device arrays as c_ptr; conf on the host)
+     integer(c_int) function bdft_locham_conf_batch_d(ctx, norb, lrs, pot, pot_off, &
+          potdim, ishift, psi, psi_off, hpsi, ekin, epot, conf, precision) &
+          bind(C, name='bdft_locham_conf_batch_d')
+       import :: c_int, c_ptr, bdft_confpot
+       type(c_ptr), value :: ctx, lrs, pot, pot_off, potdim, ishift, psi, psi_off, &
+            hpsi, ekin, epot
+       integer(c_int), value :: norb, precision
+       type(bdft_confpot), dimension(*), intent(in) :: conf
+     end function bdft_locham_conf_batch_d
      !> offsets and index bounds of the 17 convolution-bounds arrays
      integer(c_int) function bdft_bounds_layout(n, nfl, nfu, offset, dims) &
           bind(C, name='bdft_bounds_layout')

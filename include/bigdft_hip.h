@@ -39,6 +39,22 @@ typedef struct bdft_ctx bdft_ctx;
 typedef struct bdft_lr bdft_lr;
 typedef struct bdft_plan bdft_plan;
 
+/* Confining potential of one orbital: the fields of confpot_data
+ * (bigdft/src/locregs/locreg_operations.f90:23-30).  The local potential
+ * becomes V + prefac * r2^(potorder/2) (integer division), r2 the squared
+ * distance of the ISF point hh*(i + ioffset) (1-based index i of the
+ * region's fine grid) from rxyzConf, wherever the potential box covers the
+ * region (apply_potential_lr_conf*, bigdft/src/orbitals/potential.f90:
+ * 203-560).  potorder 0: no confinement.  damping is not used here. */
+typedef struct bdft_confpot {
+  int potorder;
+  int ioffset[3];
+  double prefac;
+  double hh[3];
+  double rxyzConf[3];
+  double damping;
+} bdft_confpot;
+
 enum {
   BDFT_OK = 0,
   BDFT_ERR_ARG = 1,         /* null handle, bad size or geocode */
@@ -115,6 +131,17 @@ int bdft_locham_batch_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
                         const double *psi, const int64_t *psi_off,
                         double *hpsi, double *ekin, double *epot,
                         int precision);
+/* The same with a confining potential per orbital (conf[norb], host
+ * array; the ket loop passes psi%confdata to psir_to_vpsi,
+ * modules/orbital_basis.f90:830-831; the linear-scaling Hamiltonian of
+ * the reference runs this on the CPU only).  epot includes the
+ * confinement energy, as the reference's does. */
+int bdft_locham_conf_batch_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
+                             const double *pot, const int64_t *pot_off,
+                             const int32_t *potdim, const int32_t *ishift,
+                             const double *psi, const int64_t *psi_off,
+                             double *hpsi, double *ekin, double *epot,
+                             const bdft_confpot *conf, int precision);
 
 /* Density of norb orbitals sharing one region (device pointers psi, rho):
  * rho[s*n1i*n2i*n3i + p] += hfac[i] * psir_i(p)^2 for the orbitals i of

@@ -571,6 +571,63 @@ void oref_apply_potential_lr(int n1i, int n2i, int n3i, int n1ip, int n2ip,
   *epot = ep;
 }
 
+/* apply_potential_lr_conf_nobounds (potential.f90:362-560), nspinor = 1:
+ * as oref_apply_potential_lr with the confining term
+ * prefac * r2^(potorder/2), r2 = sum_d (hh_d (i_d + ioffset_d) - rxyzConf_d)^2
+ * (1-based ISF index i_d of the wavefunction box), added to the potential
+ * where the potential box and the wavefunction box overlap (:541-556).  The
+ * free-BC variant apply_potential_lr_conf (:203-360) restricts the product
+ * to the ibyyzz_r rows, outside of which psir is zero: same result. */
+void oref_apply_potential_lr_conf(int n1i, int n2i, int n3i, int n1ip, int n2ip,
+                                  int n3ip, const int ishift[3], double *psir,
+                                  const double *pot, const oref_confpot *cd,
+                                  double *epot) {
+  if (!cd || cd->potorder == 0) {
+    oref_apply_potential_lr(n1i, n2i, n3i, n1ip, n2ip, n3ip, ishift, psir, pot, epot);
+    return;
+  }
+  const int i3s = ishift[2] + 1 > 1 ? ishift[2] + 1 : 1;
+  const int i3e = n3ip + ishift[2] < n3i ? n3ip + ishift[2] : n3i;
+  const int i2s = ishift[1] + 1 > 1 ? ishift[1] + 1 : 1;
+  const int i2e = n2ip + ishift[1] < n2i ? n2ip + ishift[1] : n2i;
+  const int i1s = ishift[0] + 1 > 1 ? ishift[0] + 1 : 1;
+  const int i1e = n1ip + ishift[0] < n1i ? n1ip + ishift[0] : n1i;
+  const int half = cd->potorder / 2;
+  double ep = 0.0;
+#pragma omp parallel for reduction(+ : ep) schedule(static)
+  for (int i3 = 1; i3 <= n3i; ++i3) {
+    const double z = cd->hh[2] * (double)(i3 + cd->ioffset[2]) - cd->rxyzConf[2];
+    const double z2 = z * z;
+    for (int i2 = 1; i2 <= n2i; ++i2) {
+      const double y = cd->hh[1] * (double)(i2 + cd->ioffset[1]) - cd->rxyzConf[1];
+      const double y2 = y * y;
+      double *row = psir + (size_t)n1i * ((i2 - 1) + (size_t)n2i * (i3 - 1));
+      const int in23 = (i3 >= i3s && i3 <= i3e && i2 >= i2s && i2 <= i2e);
+      for (int i1 = 1; i1 <= n1i; ++i1) {
+        if (in23 && i1 >= i1s && i1 <= i1e) {
+          const double x = cd->hh[0] * (double)(i1 + cd->ioffset[0]) - cd->rxyzConf[0];
+          const double r2 = x * x + y2 + z2;
+          double rp = 1.0;
+          for (int k = 0; k < half; ++k) rp *= r2;
+          const double cp = cd->prefac * rp;
+          const double psir1 = row[i1 - 1];
+          const double pot1 =
+              pot[(i1 - ishift[0] - 1) +
+                  (size_t)n1ip * ((i2 - ishift[1] - 1) +
+                                  (size_t)n2ip * (i3 - ishift[2] - 1))] +
+              cp;
+          const double tt11 = pot1 * psir1;
+          ep += tt11 * psir1;
+          row[i1 - 1] = tt11;
+        } else {
+          row[i1 - 1] = 0.0;
+        }
+      }
+    }
+  }
+  *epot = ep;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Free boundary conditions.                                                 */
 /* ------------------------------------------------------------------------ */
@@ -775,12 +832,12 @@ void oref_convolkinetic_T(int n1, int n2, int n3, const int nfl[3],
 /* periodic branch: daub_to_isf_locham (daubisf.f90:297-312) ->
  * psir_to_vpsi (locreg_operations.f90:2502-2590) -> isf_to_daub_kinetic P
  * (locreg_operations.f90:2839-2865) */
-void oref_locham_per(const int n[3], const double hgrid[3], int nseg_c,
-                     int nvctr_c, const int32_t *keyg_c, const int32_t *keyv_c,
-                     int nseg_f, int nvctr_f, const int32_t *keyg_f,
-                     const int32_t *keyv_f, const double *pot,
-                     const double *psi, double *hpsi, double *ekin,
-                     double *epot) {
+static void locham_per(const int n[3], const double hgrid[3], int nseg_c,
+                       int nvctr_c, const int32_t *keyg_c, const int32_t *keyv_c,
+                       int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                       const int32_t *keyv_f, const double *pot,
+                       const double *psi, double *hpsi, double *ekin,
+                       double *epot, const oref_confpot *cd) {
   const int n1 = n[0], n2 = n[1], n3 = n[2];
   const int n1i = 2 * n1 + 2, n2i = 2 * n2 + 2, n3i = 2 * n3 + 2;
   const size_t tot = (size_t)n1i * n2i * n3i;
@@ -796,8 +853,8 @@ void oref_locham_per(const int n[3], const double hgrid[3], int nseg_c,
   oref_convolut_magic_n_per(n1i - 1, n2i - 1, n3i - 1, x_c, psir, y_c);
   /* psir_to_vpsi: apply_potential_lr_nobounds, ishift = 0 */
   const int ishift[3] = {0, 0, 0};
-  oref_apply_potential_lr(n1i, n2i, n3i, n1i, n2i, n3i, ishift, psir, pot,
-                          epot);
+  oref_apply_potential_lr_conf(n1i, n2i, n3i, n1i, n2i, n3i, ishift, psir, pot,
+                               cd, epot);
   /* isf_to_daub_kinetic, periodic: hgridh = hgrid/2 */
   const double hh[3] = {0.5 * hgrid[0], 0.5 * hgrid[1], 0.5 * hgrid[2]};
   double kst[3];
@@ -817,13 +874,13 @@ void oref_locham_per(const int n[3], const double hgrid[3], int nseg_c,
  * -> apply_potential_lr_bounds (potential.f90:1049-1248) -> comb_shrink +
  * ConvolkineticT + compress_and_accumulate_standard
  * (locreg_operations.f90:2642-2675) */
-void oref_locham_free(const int n[3], const int nfl[3], const int nfu[3],
-                      const double hgrid[3], int nseg_c, int nvctr_c,
-                      const int32_t *keyg_c, const int32_t *keyv_c,
-                      int nseg_f, int nvctr_f, const int32_t *keyg_f,
-                      const int32_t *keyv_f, const double *pot,
-                      const double *psi, double *hpsi, double *ekin,
-                      double *epot) {
+static void locham_free(const int n[3], const int nfl[3], const int nfu[3],
+                        const double hgrid[3], int nseg_c, int nvctr_c,
+                        const int32_t *keyg_c, const int32_t *keyv_c,
+                        int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                        const int32_t *keyv_f, const double *pot,
+                        const double *psi, double *hpsi, double *ekin,
+                        double *epot, const oref_confpot *cd) {
   const int n1 = n[0], n2 = n[1], n3 = n[2];
   const int n1i = 2 * n1 + 31, n2i = 2 * n2 + 31, n3i = 2 * n3 + 31;
   const size_t nc = (size_t)(n1 + 1) * (n2 + 1) * (n3 + 1);
@@ -840,8 +897,8 @@ void oref_locham_free(const int n[3], const int nfl[3], const int nfu[3],
                               nseg_f, keyg_f, keyv_f, psi, psi_f, x_c, x_f);
   oref_comb_grow_all(n1, n2, n3, nfl, nfu, x_c, x_f, psir);
   const int ishift[3] = {0, 0, 0};
-  oref_apply_potential_lr(n1i, n2i, n3i, n1i, n2i, n3i, ishift, psir, pot,
-                          epot);
+  oref_apply_potential_lr_conf(n1i, n2i, n3i, n1i, n2i, n3i, ishift, psir, pot,
+                               cd, epot);
   oref_comb_shrink(n1, n2, n3, nfl, nfu, psir, y_c, y_f);
   oref_convolkinetic_T(n1, n2, n3, nfl, nfu, hgrid, x_c, x_f, y_c, y_f, ekin);
   oref_compress_and_accumulate_standard(n1, n2, n3, nfl, nfu, nseg_c, keyg_c,
@@ -854,7 +911,52 @@ void oref_locham_free(const int n[3], const int nfl[3], const int nfu[3],
   free(psir);
 }
 
-/* the ket loop of LocalHamiltonianApplication, hpsiortho.f90:945-958 */
+void oref_locham_per(const int n[3], const double hgrid[3], int nseg_c,
+                     int nvctr_c, const int32_t *keyg_c, const int32_t *keyv_c,
+                     int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                     const int32_t *keyv_f, const double *pot,
+                     const double *psi, double *hpsi, double *ekin,
+                     double *epot) {
+  locham_per(n, hgrid, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f, nvctr_f, keyg_f,
+             keyv_f, pot, psi, hpsi, ekin, epot, NULL);
+}
+
+void oref_locham_free(const int n[3], const int nfl[3], const int nfu[3],
+                      const double hgrid[3], int nseg_c, int nvctr_c,
+                      const int32_t *keyg_c, const int32_t *keyv_c,
+                      int nseg_f, int nvctr_f, const int32_t *keyg_f,
+                      const int32_t *keyv_f, const double *pot,
+                      const double *psi, double *hpsi, double *ekin,
+                      double *epot) {
+  locham_free(n, nfl, nfu, hgrid, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f,
+              nvctr_f, keyg_f, keyv_f, pot, psi, hpsi, ekin, epot, NULL);
+}
+
+/* the ket loop of LocalHamiltonianApplication, hpsiortho.f90:945-958, with
+ * the orbital's confdata passed to psir_to_vpsi (orbital_basis.f90:830-831);
+ * conf = NULL or potorder = 0: no confinement */
+void oref_locham_many_conf(char geocode, const int n[3], const int nfl[3],
+                           const int nfu[3], const double hgrid[3], int nseg_c,
+                           int nvctr_c, const int32_t *keyg_c,
+                           const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                           const int32_t *keyg_f, const int32_t *keyv_f,
+                           const double *pot, int norb, const double *psi,
+                           double *hpsi, double *ekin, double *epot,
+                           const oref_confpot *conf) {
+  const size_t ncoef = (size_t)nvctr_c + 7 * (size_t)nvctr_f;
+  for (int iorb = 0; iorb < norb; ++iorb) {
+    const oref_confpot *cd = conf ? conf + iorb : NULL;
+    if (geocode == 'P')
+      locham_per(n, hgrid, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f, nvctr_f,
+                 keyg_f, keyv_f, pot, psi + ncoef * iorb, hpsi + ncoef * iorb,
+                 ekin + iorb, epot + iorb, cd);
+    else
+      locham_free(n, nfl, nfu, hgrid, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f,
+                  nvctr_f, keyg_f, keyv_f, pot, psi + ncoef * iorb,
+                  hpsi + ncoef * iorb, ekin + iorb, epot + iorb, cd);
+  }
+}
+
 void oref_locham_many(char geocode, const int n[3], const int nfl[3],
                       const int nfu[3], const double hgrid[3], int nseg_c,
                       int nvctr_c, const int32_t *keyg_c,
@@ -862,18 +964,9 @@ void oref_locham_many(char geocode, const int n[3], const int nfl[3],
                       const int32_t *keyg_f, const int32_t *keyv_f,
                       const double *pot, int norb, const double *psi,
                       double *hpsi, double *ekin, double *epot) {
-  const size_t ncoef = (size_t)nvctr_c + 7 * (size_t)nvctr_f;
-  for (int iorb = 0; iorb < norb; ++iorb) {
-    if (geocode == 'P')
-      oref_locham_per(n, hgrid, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f,
-                      nvctr_f, keyg_f, keyv_f, pot, psi + ncoef * iorb,
-                      hpsi + ncoef * iorb, ekin + iorb, epot + iorb);
-    else
-      oref_locham_free(n, nfl, nfu, hgrid, nseg_c, nvctr_c, keyg_c, keyv_c,
-                       nseg_f, nvctr_f, keyg_f, keyv_f, pot,
-                       psi + ncoef * iorb, hpsi + ncoef * iorb, ekin + iorb,
-                       epot + iorb);
-  }
+  oref_locham_many_conf(geocode, n, nfl, nfu, hgrid, nseg_c, nvctr_c, keyg_c,
+                        keyv_c, nseg_f, nvctr_f, keyg_f, keyv_f, pot, norb, psi,
+                        hpsi, ekin, epot, NULL);
 }
 
 /* daub_to_isf (orbitals/daubisf.f90:13-80): psir on the fine ISF grid of the

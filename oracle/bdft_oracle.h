@@ -18,6 +18,16 @@
 extern "C" {
 #endif
 
+/* confpot_data (bigdft/src/locregs/locreg_operations.f90:23-30) */
+typedef struct {
+  int potorder;       /* 0: no confinement */
+  int ioffset[3];     /* ISF offset of the box in the global region */
+  double prefac;
+  double hh[3];       /* ISF grid spacings (hgrid/2) */
+  double rxyzConf[3]; /* centre, global coordinates */
+  double damping;     /* not used by the local Hamiltonian */
+} oref_confpot;
+
 /* ---- filter tables (copied values; see bdft_oracle.c for the citation) ---- */
 void oref_filters(double ch[16], double cg[16], double magic[16],
                   double kin_a[15], double kin_c[29], double kin_e[15],
@@ -74,6 +84,11 @@ void oref_compress_and_accumulate_standard(
 void oref_apply_potential_lr(int n1i, int n2i, int n3i, int n1ip, int n2ip,
                              int n3ip, const int ishift[3], double *psir,
                              const double *pot, double *epot);
+/* with the confining term of confdata (NULL or potorder 0: none) */
+void oref_apply_potential_lr_conf(int n1i, int n2i, int n3i, int n1ip, int n2ip,
+                                  int n3ip, const int ishift[3], double *psir,
+                                  const double *pot, const oref_confpot *cd,
+                                  double *epot);
 
 /* ---- free boundary conditions (growshrink.f90, convolut_ib_optim.f90) ---- */
 void oref_comb_grow_all(int n1, int n2, int n3, const int nfl[3],
@@ -113,6 +128,15 @@ void oref_locham_many(char geocode, const int n[3], const int nfl[3],
                       const int32_t *keyg_f, const int32_t *keyv_f,
                       const double *pot, int norb, const double *psi,
                       double *hpsi, double *ekin, double *epot);
+/* the same with one confdata per orbital (NULL: none) */
+void oref_locham_many_conf(char geocode, const int n[3], const int nfl[3],
+                           const int nfu[3], const double hgrid[3], int nseg_c,
+                           int nvctr_c, const int32_t *keyg_c,
+                           const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                           const int32_t *keyg_f, const int32_t *keyv_f,
+                           const double *pot, int norb, const double *psi,
+                           double *hpsi, double *ekin, double *epot,
+                           const oref_confpot *conf);
 
 /* ---- density (orbitals/daubisf.f90:13-80, sumrho.f90:377-560) ---- */
 void oref_daub_to_isf(char geocode, const int n[3], const int nfl[3],

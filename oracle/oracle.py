@@ -73,6 +73,10 @@ def _load():
     lib.oref_segkeys.argtypes = [c_int] * 9 + [_u8, c_int, _ip, _ip]
     lib.oref_segkeys.restype = c_int
     lib.oref_num_threads.restype = c_int
+    lib.oref_locham_many_conf.argtypes = lib.oref_locham_many.argtypes + [
+        ctypes.c_void_p]
+    lib.oref_apply_potential_lr_conf.argtypes = [c_int] * 6 + [
+        _c_int3, _dp, _dp, ctypes.c_void_p, _dp]
     _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
     lib.oref_bounds_layout.argtypes = [_c_int3, _c_int3, _c_int3, _i64, _ip]
     lib.oref_bounds_layout.restype = c_int
@@ -262,11 +266,39 @@ def apply_potential(psir, pot, dims, potdims=None, ishift=(0, 0, 0)):
     return p, ep[0]
 
 
-def locham(geocode, n, nfl, nfu, hgrid, keys, pot, psi, hpsi=None):
+# confpot_data (locregs/locreg_operations.f90:23-30), C layout of oref_confpot
+CONFPOT_DTYPE = np.dtype([("potorder", np.int32), ("ioffset", np.int32, 3),
+                          ("prefac", np.float64), ("hh", np.float64, 3),
+                          ("rxyzConf", np.float64, 3), ("damping", np.float64)],
+                         align=True)
+
+
+def confpot(potorder, prefac, hh, center, ioffset, damping=1.0):
+    c = np.zeros(1, dtype=CONFPOT_DTYPE)
+    c["potorder"], c["prefac"], c["damping"] = potorder, prefac, damping
+    c["hh"], c["rxyzConf"], c["ioffset"] = hh, center, ioffset
+    return c
+
+
+def apply_potential_conf(psir, pot, dims, conf, potdims=None, ishift=(0, 0, 0)):
+    """apply_potential_lr_conf_nobounds on one psir (returns psir', epot)"""
+    potdims = potdims or dims
+    p = np.array(psir, dtype=np.float64, copy=True).ravel()
+    ep = np.zeros(1)
+    c = np.ascontiguousarray(conf, dtype=CONFPOT_DTYPE)
+    lib().oref_apply_potential_lr_conf(dims[0], dims[1], dims[2], potdims[0],
+                                       potdims[1], potdims[2], _c_int3(*ishift), p,
+                                       np.ascontiguousarray(pot, dtype=np.float64).ravel(),
+                                       c.ctypes.data, ep)
+    return p, ep[0]
+
+
+def locham(geocode, n, nfl, nfu, hgrid, keys, pot, psi, hpsi=None, conf=None):
     """Local Hamiltonian of norb orbitals sharing one locreg.
 
     psi: (norb, ncoef) C-contiguous; returns (hpsi, ekin[norb], epot[norb]).
-    hpsi is accumulated onto the given array (zeros if None)."""
+    hpsi is accumulated onto the given array (zeros if None).  conf: None or
+    norb CONFPOT_DTYPE records (the orbitals' confinement)."""
     psi = np.ascontiguousarray(psi, dtype=np.float64)
     if psi.ndim == 1:
         psi = psi[None, :]
@@ -274,13 +306,18 @@ def locham(geocode, n, nfl, nfu, hgrid, keys, pot, psi, hpsi=None):
     h = np.zeros_like(psi) if hpsi is None else \
         np.array(hpsi, dtype=np.float64, copy=True).reshape(psi.shape)
     ek, ep = np.zeros(norb), np.zeros(norb)
-    lib().oref_locham_many(
+    fn, extra = lib().oref_locham_many, []
+    if conf is not None:
+        conf = np.ascontiguousarray(conf, dtype=CONFPOT_DTYPE).ravel()
+        assert conf.size == norb
+        fn, extra = lib().oref_locham_many_conf, [conf.ctypes.data]
+    fn(
         geocode.encode() if isinstance(geocode, str) else geocode,
         _c_int3(*n), _c_int3(*nfl), _c_int3(*nfu),
         np.asarray(hgrid, dtype=np.float64), keys.nseg_c, keys.nvctr_c,
         keys.keyg_c, keys.keyv_c, keys.nseg_f, keys.nvctr_f, keys.keyg_f,
         keys.keyv_f, np.ascontiguousarray(pot, dtype=np.float64).ravel(),
-        norb, psi, h, ek, ep)
+        norb, psi, h, ek, ep, *extra)
     return h, ek, ep
 
 
