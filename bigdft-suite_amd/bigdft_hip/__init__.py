@@ -38,7 +38,7 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_stage_name", "bdft_ctx_info", "bdft_sync", "bdft_density_d",
            "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d",
            "bdft_bounds_layout", "bdft_locreg_bounds_d", "bdft_locreg_bounds",
-           "bdft_locham_conf_batch_d")
+           "bdft_locham_conf_batch_d", "bdft_precond_cg_d", "bdft_precondition")
 
 
 class BigDFTError(RuntimeError):
@@ -83,6 +83,8 @@ def load_library(path=LIB_PATH):
         "bdft_profile_read": [P, I, P, P, I],
         "bdft_ctx_info": [P, P, P],
         "bdft_bounds_layout": [I3, I3, I3, P, P],
+        "bdft_precond_cg_d": [P, I, P, P, P, I, P, P],
+        "bdft_precondition": [P, P, I, P, I, P, P],
         "bdft_locreg_bounds_d": [P, P, P],
         "bdft_locreg_bounds": [P, P, P],
     }
@@ -290,6 +292,25 @@ class Context:
     def locreg_to_psi(self, glr, llrs, ns, lpsi, gpsi, loff=None, goff=None):
         """Lpsi_to_global2: local regions -> their points of global orbitals"""
         self._xfer(lib().bdft_locreg_to_psi_d, glr, llrs, ns, lpsi, loff, gpsi, goff)
+
+    def precond_cg(self, lrs, cprecr, ncong, x, b, psi_off=None):
+        """ocl_preconditioner_generic_k over a batch: x (device, in/out) <-
+        S y after ncong CG steps on S (T + c) S y = b from the guess x"""
+        arr = (ctypes.c_void_p * len(lrs))(*[l.handle.value for l in lrs])
+        c = np.ascontiguousarray(cprecr, dtype=np.float64)
+        if c.size != len(lrs):
+            raise ValueError("one cprecr per orbital")
+        so = None if psi_off is None else np.ascontiguousarray(psi_off, dtype=np.int64)
+        self._check(lib().bdft_precond_cg_d(self.handle, len(lrs), arr, _ptr(so), _ptr(c),
+                                            int(ncong), _ptr(x), _ptr(b)))
+
+    def precondition_host(self, lr, cprecr, ncong, x, b):
+        """host arrays (norb, ncoef); x is overwritten"""
+        assert x.flags["C_CONTIGUOUS"] and x.dtype == np.float64
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        c = np.ascontiguousarray(cprecr, dtype=np.float64)
+        self._check(lib().bdft_precondition(self.handle, lr.handle, c.size, _ptr(c),
+                                            int(ncong), _ptr(x), _ptr(bb)))
 
     def locreg_bounds(self, lr, out=None):
         """locreg_bounds: the 17 convolution-bounds arrays of a free-BC

@@ -1135,6 +1135,109 @@ extern "C" int bdft_locreg_to_psi_d(bdft_ctx* ctx, const bdft_lr* glr, int norb,
 }
 
 // ---------------------------------------------------------------------------
+// preconditioner: the GPU part of preconditionall_OCL (interface_ocl.f90:
+// 622-880), ocl_preconditioner_generic_k (OpenCL/Preconditioner.c:245-358)
+// ---------------------------------------------------------------------------
+extern "C" int bdft_precond_cg_d(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                                 const int64_t* psi_off, const double* cprecr, int ncong,
+                                 double* x, const double* b) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (norb <= 0 || !lrs || !cprecr || !x || !b)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  if (ncong < 0 || ncong > 10000) return set_err(ctx, BDFT_ERR_ARG, "ncong %d", ncong);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  long long ntot = 0;
+  std::vector<double> pc((size_t)norb * 9);
+  for (int i = 0; i < norb; ++i) {
+    const bdft_lr* lr = lrs[i];
+    if (!lr) return set_err(ctx, BDFT_ERR_ARG, "orbital %d: null region", i);
+    const long long nc = (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
+    const long long off = psi_off ? (long long)psi_off[i] : nc * i;  // as make_plan
+    if (off < 0) return set_err(ctx, BDFT_ERR_ARG, "negative offset");
+    ntot = std::max(ntot, off + nc);
+    const double c = cprecr[i];
+    if (!(c >= 0.0)) return set_err(ctx, BDFT_ERR_ARG, "orbital %d: cprecr %g", i, c);
+    // scale_psi_d / uncompress_scale_d (OpenCL/Uncompress.c:377-470), h = hgrid/2
+    double hh[3];
+    for (int d = 0; d < 3; ++d) {
+      const double h = 0.5 * lr->hgrid[d];
+      hh[d] = 0.125 / (h * h);
+    }
+    pc[9 * i] = c;
+    for (int t = 0; t < 8; ++t) {
+      double v = c;
+      for (int d = 0; d < 3; ++d) v += ((t >> d) & 1 ? 24.8758460293923314 : 3.55369228991319019) * hh[d];
+      pc[9 * i + 1 + t] = 1.0 / std::sqrt(v);
+    }
+  }
+  bdft_plan* plan = nullptr;
+  int rc = make_plan(ctx, norb, lrs, nullptr, nullptr, nullptr, nullptr, x, psi_off, nullptr,
+                     nullptr, nullptr, BDFT_PREC_FP64, true, &plan, true);
+  if (rc) return rc;
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t nvec = sizeof(double) * (size_t)ntot;
+  const size_t npc = sizeof(double) * pc.size(), nsc = sizeof(double) * 4 * (size_t)norb;
+  const size_t npart = sizeof(double) * std::max(1, plan->T.ntiles[P_UNC]);
+  DevBuf buf;
+  rc = dev_alloc(ctx, buf, 3 * al(nvec) + al(npc) + al(nsc) + al(npart));
+  if (rc) {
+    bdft_plan_destroy(plan);
+    return rc;
+  }
+  char* q = (char*)buf.p;
+  PcArgs A{};
+  A.r = (double*)q;
+  A.d = (double*)(q + al(nvec));
+  A.q = (double*)(q + 2 * al(nvec));
+  A.pc = (const double*)(q + 3 * al(nvec));
+  A.sc = (double*)(q + 3 * al(nvec) + al(npc));
+  A.part = (double*)(q + 3 * al(nvec) + al(npc) + al(nsc));
+  A.x = x;
+  A.b = b;
+  A.ncong = ncong;
+  hipError_t e = hipMemcpyAsync((void*)A.pc, pc.data(), npc, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_precond(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero, A,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipStreamSynchronize(ctx->stream);
+  dev_free(buf);
+  bdft_plan_destroy(plan);
+  if (e != hipSuccess) return set_err(ctx, BDFT_ERR_HIP, "preconditioner: %s", hipGetErrorString(e));
+  return BDFT_OK;
+}
+
+// host arrays: norb orbitals of one region (the per-orbital transfers of
+// preconditionall_OCL, interface_ocl.f90:769-843, done once for the batch)
+extern "C" int bdft_precondition(bdft_ctx* ctx, const bdft_lr* lr, int norb,
+                                 const double* cprecr, int ncong, double* x, const double* b) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (!lr || norb <= 0 || !x || !b || !cprecr)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t n = ((size_t)lr->nvctr_c + 7 * (size_t)lr->nvctr_f) * norb;
+  DevBuf buf;
+  int rc = dev_alloc(ctx, buf, 2 * sizeof(double) * std::max<size_t>(n, 1));
+  if (rc) return rc;
+  double* dx = (double*)buf.p;
+  double* db = dx + std::max<size_t>(n, 1);
+  hipError_t e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dev_free(buf);
+    return set_err(ctx, BDFT_ERR_HIP, "H2D copy: %s", hipGetErrorString(e));
+  }
+  std::vector<const bdft_lr*> lrs((size_t)norb, lr);
+  rc = bdft_precond_cg_d(ctx, norb, lrs.data(), nullptr, cprecr, ncong, dx, db);
+  if (!rc) {
+    e = hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_HIP, "D2H copy: %s", hipGetErrorString(e));
+  }
+  dev_free(buf);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // convolution bounds (locregs/bounds.f90:555-613, locreg_bounds)
 // ---------------------------------------------------------------------------
 extern "C" int bdft_bounds_layout(const int n[3], const int nfl[3], const int nfu[3],

@@ -288,6 +288,21 @@ struct XferArgs {
 hipError_t launch_xfer(const XferArgs& A, int ntiles, hipStream_t s);
 int xfer_tiles(int nvctr_c, int nvctr_f);
 
+// conjugate-gradient preconditioner, apply = S (K + c) S (bdft_pass.hip)
+struct PcArgs {
+  const double* pc;  // per box: c, scal[0..7] (9 doubles)
+  double* x;         // solution, in/out
+  const double* b;   // right-hand side
+  double* r;         // residual
+  double* d;         // search direction
+  double* q;         // A d
+  double* part;      // per compressed tile partial sums
+  double* sc;        // per box: ns, alpha, beta, unused
+  int ncong;
+};
+hipError_t launch_precond(const Batch& B, const PassTables& T, int maxN, long long wtotal,
+                          bool need_zero, const PcArgs& A, hipStream_t s);
+
 // convolution bounds of a free-BC region (bdft_bounds.hip)
 constexpr int BOUNDS_NARR = 17;
 constexpr int BOUNDS_MAXN = 484;  // values stay below the 1000 sentinel

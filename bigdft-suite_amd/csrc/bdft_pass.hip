@@ -1026,4 +1026,259 @@ void launch_energy(const EnergyTables& et, const Batch& B, hipStream_t s) {
   k_energy<<<B.nbox, 128, 0, s>>>(et, B.nbox, B.ekin, B.epot);
 }
 
+
+// ---------------------------------------------------------------------------
+// Conjugate-gradient preconditioner: ocl_preconditioner_generic_k
+// (bigdft/src/OpenCL/Preconditioner.c:245-358), real case, batched over
+// orbitals.  apply_hp (:198-243) = S Ana (K + c) Syn S on the compressed
+// vector equals S (K~ + c) S with K~ the wavelet-basis kinetic, as
+// Ana Syn = I on the box: W = S v scattered, K = c W + K~_x W (k_pc_kinx),
+// K += K~_y W, K~_z W (k_kin<double, 1/2>), q = S K gathered.  The CG
+// scalars stay on the device (per box ns, alpha, beta), reduced from
+// per-tile partials in a fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double pc_scal(const PcArgs& A, int b, int t) {
+  return A.pc[9 * b + 1 + t];
+}
+
+// W <- S v at the region's points; update: d <- r + beta d first, v = d
+__global__ __launch_bounds__(NT) void k_pc_scatter(Batch B, TileTable tt, PcArgs A,
+                                                   const double* src, int update) {
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  double* W = (double*)B.W + bd.w_off;
+  const double beta = update ? A.sc[4 * b + 2] : 0.0;
+  auto val = [&](long long i) {
+    if (!update) return src[i];
+    const double v = fma(beta, A.d[i], A.r[i]);
+    A.d[i] = v;
+    return v;
+  };
+  if (lt < ctiles) {
+    const int k = lt * CPT + threadIdx.x;
+    if (k < bd.nvctr_c)
+      W[psig_index(bd, bd.cmap[k], 0, 0, 0)] = pc_scal(A, b, 0) * val(bd.psi_off + k);
+  } else {
+    const int k = (lt - ctiles) * CPT + threadIdx.x;
+    if (k < bd.nvctr_f) {
+      const int lin = bd.fmap[k];
+      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
+#pragma unroll
+      for (int t = 1; t <= 7; ++t)
+        W[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)] =
+            pc_scal(A, b, t) * val(base + t - 1);
+    }
+  }
+}
+
+// K = c W + K~_x W along x lines (the line loading of k_xfwd)
+__global__ __launch_bounds__(NT) void k_pc_kinx(Batch B, TileTable tt, PcArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* lds = (double*)smem;
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int N1 = bd.N[0], per = bd.per;
+  const int W1 = 2 * N1;
+  const int nlines = 4 * bd.N[1] * bd.N[2];
+  const int l0 = lt * TW;
+  const int nl = min(TW, nlines - l0);
+  const double* src = (const double*)B.W + bd.w_off + (long long)l0 * W1;
+  fill_wav<true>(lds, src, W1, 1, nl, N1, per);
+  __syncthreads();
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
+  const int NS = N1 + HL + HR;
+  const double* col = lds + tx;
+  const double ks = bd.kscale[0], c = A.pc[9 * b];
+  double kc[2][4], kd[2][4];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int i0 = 4 * (ty + m * TY);
+    if (i0 < N1) {
+      kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, kc[m], kd[m]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = min(i0 + r, N1 - 1);
+        kc[m][r] = fma(c, col[(i + HL) * LDW], kc[m][r] * ks);
+        kd[m][r] = fma(c, col[(NS + i + HL) * LDW], kd[m][r] * ks);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const int i0 = 4 * (ty + m * TY);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N1) {
+        lds[(i0 + r) * LDW + tx] = kc[m][r];
+        lds[(N1 + i0 + r) * LDW + tx] = kd[m][r];
+      }
+  }
+  __syncthreads();
+  double* kdst = (double*)B.K + bd.w_off + (long long)l0 * W1;
+  const float inv = 1.0f / (float)W1;
+  for (int idx = threadIdx.x; idx < nl * W1; idx += NT) {
+    const int l = qdiv(idx, W1, inv), p = idx - l * W1;
+    kdst[idx] = lds[p * LDW + l];
+  }
+}
+
+// v = S K at the region's points.  init: r = b - v, d = r, partial |r|^2;
+// else q = v, partial q.d
+__global__ __launch_bounds__(NT) void k_pc_gather(Batch B, TileTable tt, PcArgs A, int init) {
+  __shared__ double red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  const double* K = (const double*)B.K + bd.w_off;
+  double acc = 0.0;
+  auto put = [&](long long i, double v) {
+    if (init) {
+      const double r = A.b[i] - v;
+      A.r[i] = r;
+      A.d[i] = r;
+      acc = fma(r, r, acc);
+    } else {
+      A.q[i] = v;
+      acc = fma(v, A.d[i], acc);
+    }
+  };
+  if (lt < ctiles) {
+    const int k = lt * CPT + threadIdx.x;
+    if (k < bd.nvctr_c)
+      put(bd.psi_off + k, pc_scal(A, b, 0) * K[psig_index(bd, bd.cmap[k], 0, 0, 0)]);
+  } else {
+    const int k = (lt - ctiles) * CPT + threadIdx.x;
+    if (k < bd.nvctr_f) {
+      const int lin = bd.fmap[k];
+      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
+#pragma unroll
+      for (int t = 1; t <= 7; ++t)
+        put(base + t - 1,
+            pc_scal(A, b, t) * K[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)]);
+    }
+  }
+  const double sum = block_sum(acc, red);
+  if (threadIdx.x == 0) A.part[tile] = sum;
+}
+
+// x += alpha d; unless last: r -= alpha q, partial |r|^2
+__global__ __launch_bounds__(NT) void k_pc_update(Batch B, TileTable tt, PcArgs A, int last) {
+  __shared__ double red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  const double alpha = A.sc[4 * b + 1];
+  double acc = 0.0;
+  auto upd = [&](long long i) {
+    A.x[i] = fma(alpha, A.d[i], A.x[i]);
+    if (!last) {
+      const double r = fma(-alpha, A.q[i], A.r[i]);
+      A.r[i] = r;
+      acc = fma(r, r, acc);
+    }
+  };
+  if (lt < ctiles) {
+    const int k = lt * CPT + threadIdx.x;
+    if (k < bd.nvctr_c) upd(bd.psi_off + k);
+  } else {
+    const int k = (lt - ctiles) * CPT + threadIdx.x;
+    if (k < bd.nvctr_f) {
+      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
+#pragma unroll
+      for (int t = 0; t < 7; ++t) upd(base + t);
+    }
+  }
+  if (!last) {
+    const double sum = block_sum(acc, red);
+    if (threadIdx.x == 0) A.part[tile] = sum;
+  }
+}
+
+// x <- S x (scale_psi, OpenCL/Uncompress.c:377-470)
+__global__ __launch_bounds__(NT) void k_pc_final(Batch B, TileTable tt, PcArgs A) {
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  if (lt < ctiles) {
+    const int k = lt * CPT + threadIdx.x;
+    if (k < bd.nvctr_c) A.x[bd.psi_off + k] *= pc_scal(A, b, 0);
+  } else {
+    const int k = (lt - ctiles) * CPT + threadIdx.x;
+    if (k < bd.nvctr_f) {
+      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
+#pragma unroll
+      for (int t = 1; t <= 7; ++t) A.x[base + t - 1] *= pc_scal(A, b, t);
+    }
+  }
+}
+
+// one block per box: s = sum of the box's tile partials (fixed order).
+// mode 0: ns = s; 1: alpha = ns / s; 2: beta = s / ns, ns = s.  A zero
+// denominator gives 0 (the reference divides: 0/0 for a zero orbital).
+__global__ __launch_bounds__(NT) void k_pc_scalars(TileTable tt, PcArgs A, int mode) {
+  __shared__ double red[NT / 64];
+  const int b = blockIdx.x;
+  const int t0 = box_first_tile(tt, b);
+  const int t1 = tt.uniform > 0 ? t0 + tt.uniform : tt.start[b + 1];
+  double acc = 0.0;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += NT) acc += A.part[t];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    double* sc = A.sc + 4 * b;
+    if (mode == 0) {
+      sc[0] = s;
+    } else if (mode == 1) {
+      sc[1] = s != 0.0 ? sc[0] / s : 0.0;
+    } else {
+      sc[2] = sc[0] != 0.0 ? s / sc[0] : 0.0;
+      sc[0] = s;
+    }
+  }
+}
+
+hipError_t launch_precond(const Batch& B, const PassTables& PT, int maxN, long long wtotal,
+                          bool need_zero, const PcArgs& A, hipStream_t s) {
+  const size_t lds = lds_bytes(maxN, sizeof(double));
+  const int nu = PT.ntiles[P_UNC];
+  const TileTable& tu = PT.t[P_UNC];
+  if (need_zero) {
+    const long long nb = (wtotal + NT - 1) / NT;
+    k_zero<double><<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>((double*)B.W, wtotal);
+  }
+  auto apply = [&](const double* src, int update) {
+    if (nu > 0) k_pc_scatter<<<nu, NT, 0, s>>>(B, tu, A, src, update);
+    if (PT.ntiles[P_XF] > 0) k_pc_kinx<<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], A);
+    if (PT.ntiles[P_KY] > 0)
+      k_kin<double, 1><<<PT.ntiles[P_KY], NT, lds, s>>>(B, PT.t[P_KY], PT.part[1]);
+    if (PT.ntiles[P_KZ] > 0)
+      k_kin<double, 2><<<PT.ntiles[P_KZ], NT, lds, s>>>(B, PT.t[P_KZ], PT.part[2]);
+  };
+  if (nu == 0) return hipGetLastError();
+  apply(A.x, 0);
+  k_pc_gather<<<nu, NT, 0, s>>>(B, tu, A, 1);
+  k_pc_scalars<<<B.nbox, NT, 0, s>>>(tu, A, 0);
+  for (int it = 0; it < A.ncong; ++it) {
+    apply(A.d, it > 0);
+    k_pc_gather<<<nu, NT, 0, s>>>(B, tu, A, 0);
+    k_pc_scalars<<<B.nbox, NT, 0, s>>>(tu, A, 1);
+    const int last = it == A.ncong - 1;
+    k_pc_update<<<nu, NT, 0, s>>>(B, tu, A, last);
+    if (!last) k_pc_scalars<<<B.nbox, NT, 0, s>>>(tu, A, 2);
+  }
+  k_pc_final<<<nu, NT, 0, s>>>(B, tu, A);
+  return hipGetLastError();
+}
+
 }  // namespace bdft

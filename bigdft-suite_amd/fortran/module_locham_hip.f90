@@ -14,7 +14,8 @@ module module_locham_hip
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
        local_partial_density_hip, free_gpu_hip, bdft_error_message, &
-       bdft_bounds_layout, bdft_locreg_bounds, bdft_locham_conf_batch_d
+       bdft_bounds_layout, bdft_locreg_bounds, bdft_locham_conf_batch_d, &
+       bdft_precondition
 
   !> confpot_data (locregs/locreg_operations.f90:23-30) as the C struct
   !! bdft_confpot
@@ -86,6 +87,18 @@ module module_locham_hip
        integer(c_int), value :: norb, precision
        type(bdft_confpot), dimension(*), intent(in) :: conf
      end function bdft_locham_conf_batch_d
+     !> device part of preconditionall_OCL (interface_ocl.f90:622-880) for
+     !! norb orbitals of one region: x (the scaled guess left by
+     !! precondition_preconditioner) <- S y after ncong CG steps; b the
+     !! scaled right-hand side; cprecr from cprecr_from_eval
+     integer(c_int) function bdft_precondition(ctx, lr, norb, cprecr, ncong, x, b) &
+          bind(C, name='bdft_precondition')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, lr
+       integer(c_int), value :: norb, ncong
+       real(c_double), dimension(*), intent(in) :: cprecr, b
+       real(c_double), dimension(*), intent(inout) :: x
+     end function bdft_precondition
      !> offsets and index bounds of the 17 convolution-bounds arrays
      integer(c_int) function bdft_bounds_layout(n, nfl, nfu, offset, dims) &
           bind(C, name='bdft_bounds_layout')

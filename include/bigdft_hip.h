@@ -192,6 +192,26 @@ int bdft_locreg_to_psi_d(bdft_ctx *ctx, const bdft_lr *glr, int norb,
                          const double *lpsi, const int64_t *loff, double *gpsi,
                          const int64_t *goff);
 
+/* Preconditioner, the device part of preconditionall_OCL
+ * (bigdft/src/interface_ocl.f90:622-880): ocl_preconditioner_generic_k
+ * (bigdft/src/OpenCL/Preconditioner.c:245-358), real wavefunctions,
+ * batched over orbitals.  For orbital i (region lrs[i], coefficients at
+ * psi_off[i] of x and b; NULL psi_off: consecutive), ncong conjugate-
+ * gradient steps on S (T + cprecr[i]) S y = b from the guess x, then
+ * x = S y; T is the kinetic operator of the region (periodic, or free),
+ * S the diagonal scaling 1/sqrt(sum_d {A2|B2}/(2 hgrid_d^2) + cprecr) of
+ * each component type (OpenCL/Uncompress.c:377-470).  x and b are what
+ * precondition_preconditioner (wfn_opt/precond.f90:673) leaves on the
+ * host (the scaled guess and the scaled right-hand side); x, b device
+ * pointers, cprecr host (cprecr_from_eval, precond.f90:472).  b is not
+ * modified.  Returns when done. */
+int bdft_precond_cg_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
+                      const int64_t *psi_off, const double *cprecr, int ncong,
+                      double *x, const double *b);
+/* The same for norb consecutive orbitals of one region, host arrays. */
+int bdft_precondition(bdft_ctx *ctx, const bdft_lr *lr, int norb,
+                      const double *cprecr, int ncong, double *x, const double *b);
+
 /* Convolution bounds of a free-BC region: the 17 arrays of
  * convolutions_bounds (bigdft/src/locregs/bounds.f90:44-50) in one int32
  * buffer, in the order kb%ibyz_c ibxz_c ibxy_c ibyz_f ibxz_f ibxy_f,

@@ -1510,3 +1510,200 @@ int oref_locreg_bounds(const int n[3], const int nfl[3], const int nfu[3],
   return 0;
 }
 #undef IB
+
+/* ======================================================================
+ * Preconditioner: the GPU part of preconditionall_OCL
+ * (bigdft/src/interface_ocl.f90:622-880), ocl_preconditioner_generic_k
+ * (bigdft/src/OpenCL/Preconditioner.c:245-358), real wavefunctions.
+ *
+ * apply_hp (Preconditioner.c:198-243): y = S Ana (K + c) Syn S x on the
+ * compressed vector, S the diagonal scaling of each of the 8 component
+ * types (scale_psi/uncompress_scale/compress_scale, OpenCL/Uncompress.c:
+ * 377-470: scal_t = 1/sqrt(sum_d (bit_d(t) ? B2 : A2) / (8 h_d^2) + c),
+ * h = hgrid/2), Syn/Ana the 3D wavelet synthesis/analysis (periodic, or
+ * grown by 7 coarse points per side and shrunk back for free BC:
+ * syn_self_d_generic / ana_self_d_generic, OpenCL/Wavelet.c:166-314), K
+ * the fine-grid kinetic (kinetic_d_generic, OpenCL/Kinetic.c:240) with
+ * spacing h and zero outside the grid for free BC.
+ * ====================================================================== */
+#define PREC_A2 3.55369228991319019
+#define PREC_B2 24.8758460293923314
+
+void oref_precond_scal(const double hgrid[3], double c, double scal[8]) {
+  const double h1 = 0.5 * hgrid[0], h2 = 0.5 * hgrid[1], h3 = 0.5 * hgrid[2];
+  const double hh[3] = {0.125 / (h1 * h1), 0.125 / (h2 * h2), 0.125 / (h3 * h3)};
+  for (int t = 0; t < 8; ++t) {
+    double s = c;
+    for (int d = 0; d < 3; ++d) s += ((t >> d) & 1 ? PREC_B2 : PREC_A2) * hh[d];
+    scal[t] = 1.0 / sqrt(s);
+  }
+}
+
+/* free-BC synthesis of one axis with the full support: x(0:2n+1, ndat),
+ * c in 0:n, d in n+1:2n+1 -> y(ndat, 0:2n+14-1), fine index j-7 at row j:
+ * y(j) = sum_k ch(j-2k) c(k) + cg(j-2k) d(k), j-2k in -7..8 */
+static void syngrow_rot(int n, long ndat, const double *x, double *y) {
+  const int np1 = n + 1, m = 2 * n + 16;
+#pragma omp parallel for schedule(static)
+  for (long jd = 0; jd < ndat; ++jd) {
+    const double *xj = x + (size_t)(2 * np1) * jd;
+    for (int r = 0; r < m; ++r) {
+      const int j = r - 7;
+      double s = 0.0;
+      for (int k = 0; k < np1; ++k) {
+        const int l = j - 2 * k;
+        if (l >= -7 && l <= 8) s += CHf(l) * xj[k] + CGf(l) * xj[np1 + k];
+      }
+      y[jd + (size_t)ndat * r] = s;
+    }
+  }
+}
+
+/* its adjoint, the analysis restricted to the box: y(0:2n+15, ndat) (row j
+ * = fine index j-7) -> x(ndat, 0:2n+1): c(i) = sum_l ch(l) y(2i+l) */
+static void anashrink_rot(int n, long ndat, const double *y, double *x) {
+  const int np1 = n + 1, m = 2 * n + 16;
+#pragma omp parallel for schedule(static)
+  for (long jd = 0; jd < ndat; ++jd) {
+    const double *yj = y + (size_t)m * jd;
+    for (int i = 0; i < np1; ++i) {
+      double ci = 0.0, di = 0.0;
+      for (int l = -7; l <= 8; ++l) {
+        const double v = yj[2 * i + l + 7];
+        ci += CHf(l) * v;
+        di += CGf(l) * v;
+      }
+      x[jd + (size_t)ndat * i] = ci;
+      x[jd + (size_t)ndat * (np1 + i)] = di;
+    }
+  }
+}
+
+/* y += K x on an m1 x m2 x m3 fine grid, periodic or zero outside */
+static void kinetic_fine(int m1, int m2, int m3, int per, const double h[3],
+                         const double *x, double *y) {
+  double fil[3][29];
+  for (int d = 0; d < 3; ++d) {
+    const double scale = -0.5 / (h[d] * h[d]);
+    for (int l = 0; l <= 14; ++l) fil[d][14 + l] = fil[d][14 - l] = KIN_A[l] * scale;
+  }
+  const int m[3] = {m1, m2, m3};
+  const size_t st[3] = {1, (size_t)m1, (size_t)m1 * m2};
+#pragma omp parallel for schedule(static)
+  for (int i3 = 0; i3 < m3; ++i3)
+    for (int i2 = 0; i2 < m2; ++i2)
+      for (int i1 = 0; i1 < m1; ++i1) {
+        const int ii[3] = {i1, i2, i3};
+        const size_t p = i1 + st[1] * i2 + st[2] * i3;
+        double tt = 0.0;
+        for (int d = 0; d < 3; ++d)
+          for (int l = -14; l <= 14; ++l) {
+            int q = ii[d] + l;
+            if (per) q = pmod(q, m[d]);
+            else if (q < 0 || q >= m[d]) continue;
+            tt += fil[d][l + 14] * x[p + st[d] * (size_t)(q - ii[d])];
+          }
+        y[p] += tt;
+      }
+}
+
+/* apply_hp_generic_k (Preconditioner.c:198-243), nspinor = 1 */
+void oref_precond_apply(char geocode, const int n[3], const double hgrid[3],
+                        double c, int nseg_c, int nvctr_c, const int32_t *keyg_c,
+                        const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                        const int32_t *keyg_f, const int32_t *keyv_f,
+                        const double *x, double *y) {
+  const int n1 = n[0], n2 = n[1], n3 = n[2], per = geocode == 'P';
+  double scal[8];
+  oref_precond_scal(hgrid, c, scal);
+  const size_t ncoef = (size_t)nvctr_c + 7 * (size_t)nvctr_f;
+  double *xs = malloc(sizeof(double) * (ncoef ? ncoef : 1));
+  for (size_t k = 0; k < (size_t)nvctr_c; ++k) xs[k] = scal[0] * x[k];
+  for (size_t k = 0; k < (size_t)nvctr_f; ++k)
+    for (int t = 0; t < 7; ++t)
+      xs[nvctr_c + 7 * k + t] = scal[t + 1] * x[nvctr_c + 7 * k + t];
+  const int e = per ? 2 : 16;  /* fine points per axis: 2(n+1) or 2(n+1)+14 */
+  const int m1 = 2 * n1 + e, m2 = 2 * n2 + e, m3 = 2 * n3 + e;
+  const size_t nw = (size_t)8 * (n1 + 1) * (n2 + 1) * (n3 + 1);
+  const size_t ng = (size_t)m1 * m2 * m3;
+  double *psig = malloc(sizeof(double) * nw);
+  double *a = malloc(sizeof(double) * ng), *b = malloc(sizeof(double) * ng);
+  oref_uncompress_psig(n1, n2, n3, nseg_c, keyg_c, keyv_c, nseg_f, keyg_f, keyv_f, xs,
+                       xs + nvctr_c, psig);
+  if (per) {
+    memcpy(a, psig, sizeof(double) * nw);
+    oref_synthese_per_self(n1, n2, n3, a, b);            /* result in b */
+  } else {
+    syngrow_rot(n1, (long)(2 * n2 + 2) * (2 * n3 + 2), psig, a);
+    syngrow_rot(n2, (long)(2 * n3 + 2) * m1, a, b);
+    syngrow_rot(n3, (long)m1 * m2, b, a);
+    memcpy(b, a, sizeof(double) * ng);
+  }
+  const double hf[3] = {0.5 * hgrid[0], 0.5 * hgrid[1], 0.5 * hgrid[2]};
+  for (size_t i = 0; i < ng; ++i) a[i] = c * b[i];
+  kinetic_fine(m1, m2, m3, per, hf, b, a);               /* a = (c + K) b */
+  if (per) {
+    oref_analyse_per_self(n1, n2, n3, a, psig);           /* result in psig */
+  } else {
+    anashrink_rot(n1, (long)m2 * m3, a, b);
+    anashrink_rot(n2, (long)m3 * (2 * n1 + 2), b, a);
+    anashrink_rot(n3, (long)(2 * n1 + 2) * (2 * n2 + 2), a, psig);
+  }
+  memset(y, 0, sizeof(double) * ncoef);
+  oref_compress_and_accumulate_mixed(n1, n2, n3, nseg_c, keyg_c, keyv_c, nseg_f, keyg_f,
+                                     keyv_f, psig, y, y + nvctr_c);
+  for (size_t k = 0; k < (size_t)nvctr_c; ++k) y[k] *= scal[0];
+  for (size_t k = 0; k < (size_t)nvctr_f; ++k)
+    for (int t = 0; t < 7; ++t) y[nvctr_c + 7 * k + t] *= scal[t + 1];
+  free(xs);
+  free(psig);
+  free(a);
+  free(b);
+}
+
+static double vdot(size_t n, const double *a, const double *b) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* ocl_preconditioner_generic_k (Preconditioner.c:245-358), nspinor = 1:
+ * x (the guess from precondition_preconditioner) is overwritten by
+ * S x_ncong; b is the scaled right-hand side (overwritten as work, as the
+ * reference does with psi_c_b) */
+void oref_precond_cg(char geocode, const int n[3], const double hgrid[3], double c,
+                     int ncong, int nseg_c, int nvctr_c, const int32_t *keyg_c,
+                     const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                     const int32_t *keyg_f, const int32_t *keyv_f, double *x,
+                     double *b) {
+  const size_t nc = (size_t)nvctr_c + 7 * (size_t)nvctr_f;
+  double *r = malloc(sizeof(double) * (nc ? nc : 1));
+  double *d = malloc(sizeof(double) * (nc ? nc : 1));
+#define APPLY(in, out)                                                             \
+  oref_precond_apply(geocode, n, hgrid, c, nseg_c, nvctr_c, keyg_c, keyv_c, nseg_f, \
+                     nvctr_f, keyg_f, keyv_f, in, out)
+  APPLY(x, d);
+  for (size_t i = 0; i < nc; ++i) r[i] = b[i] - d[i];
+  memcpy(d, r, sizeof(double) * nc);
+  double ns = vdot(nc, r, r);
+  for (int it = 0; it < ncong; ++it) {
+    APPLY(d, b);
+    const double alpha = ns / vdot(nc, d, b);
+    for (size_t i = 0; i < nc; ++i) x[i] += alpha * d[i];
+    if (it != ncong - 1) {
+      for (size_t i = 0; i < nc; ++i) r[i] -= alpha * b[i];
+      const double ns2 = vdot(nc, r, r);
+      const double beta = ns2 / ns;
+      for (size_t i = 0; i < nc; ++i) d[i] = beta * d[i] + r[i];
+      ns = ns2;
+    }
+  }
+#undef APPLY
+  double scal[8];
+  oref_precond_scal(hgrid, c, scal);
+  for (size_t k = 0; k < (size_t)nvctr_c; ++k) x[k] *= scal[0];
+  for (size_t k = 0; k < (size_t)nvctr_f; ++k)
+    for (int t = 0; t < 7; ++t) x[nvctr_c + 7 * k + t] *= scal[t + 1];
+  free(r);
+  free(d);
+}

@@ -186,6 +186,21 @@ int oref_locreg_bounds(const int n[3], const int nfl[3], const int nfu[3],
                        int nseg_c, const int32_t *keyg_c, int nseg_f,
                        const int32_t *keyg_f, int32_t *out);
 
+/* ---- preconditioner, GPU part of preconditionall_OCL (interface_ocl.f90:
+ * 622-880): ocl_preconditioner_generic_k (OpenCL/Preconditioner.c:245-358),
+ * real case.  hgrid is the coarse spacing (the OpenCL code takes hgrid/2). */
+void oref_precond_scal(const double hgrid[3], double c, double scal[8]);
+void oref_precond_apply(char geocode, const int n[3], const double hgrid[3],
+                        double c, int nseg_c, int nvctr_c, const int32_t *keyg_c,
+                        const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                        const int32_t *keyg_f, const int32_t *keyv_f,
+                        const double *x, double *y);
+void oref_precond_cg(char geocode, const int n[3], const double hgrid[3], double c,
+                     int ncong, int nseg_c, int nvctr_c, const int32_t *keyg_c,
+                     const int32_t *keyv_c, int nseg_f, int nvctr_f,
+                     const int32_t *keyg_f, const int32_t *keyv_f, double *x,
+                     double *b);
+
 int oref_num_threads(void);
 
 #ifdef __cplusplus

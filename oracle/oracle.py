@@ -73,6 +73,11 @@ def _load():
     lib.oref_segkeys.argtypes = [c_int] * 9 + [_u8, c_int, _ip, _ip]
     lib.oref_segkeys.restype = c_int
     lib.oref_num_threads.restype = c_int
+    lib.oref_precond_scal.argtypes = [_dp, ctypes.c_double, _dp]
+    lib.oref_precond_apply.argtypes = [c_char, _c_int3, _dp, ctypes.c_double, c_int,
+                                       c_int, _ip, _ip, c_int, c_int, _ip, _ip, _dp, _dp]
+    lib.oref_precond_cg.argtypes = [c_char, _c_int3, _dp, ctypes.c_double, c_int, c_int,
+                                    c_int, _ip, _ip, c_int, c_int, _ip, _ip, _dp, _dp]
     lib.oref_locham_many_conf.argtypes = lib.oref_locham_many.argtypes + [
         ctypes.c_void_p]
     lib.oref_apply_potential_lr_conf.argtypes = [c_int] * 6 + [
@@ -431,3 +436,31 @@ def locreg_bounds(n, nfl, nfu, keys):
                                 keys.nseg_c, kc, keys.nseg_f, kf, buf):
         raise ValueError("nfl > nfu")
     return buf
+
+
+def precond_scal(hgrid, c):
+    """the 8 diagonal scalings (type t = s1 + 2 s2 + 4 s3)"""
+    out = np.zeros(8)
+    lib().oref_precond_scal(np.asarray(hgrid, dtype=np.float64), float(c), out)
+    return out
+
+
+def precond_apply(geocode, n, hgrid, c, keys, x):
+    """y = S Ana (K + c) Syn S x (apply_hp of the OpenCL preconditioner)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    lib().oref_precond_apply(geocode.encode(), _c_int3(*n), np.asarray(hgrid, dtype=np.float64),
+                             float(c), keys.nseg_c, keys.nvctr_c, keys.keyg_c, keys.keyv_c,
+                             keys.nseg_f, keys.nvctr_f, keys.keyg_f, keys.keyv_f, x, y)
+    return y
+
+
+def precond_cg(geocode, n, hgrid, c, ncong, keys, x, b):
+    """ocl_preconditioner_generic_k: returns the scaled CG solution"""
+    x = np.array(x, dtype=np.float64, copy=True)
+    b = np.array(b, dtype=np.float64, copy=True)
+    lib().oref_precond_cg(geocode.encode(), _c_int3(*n), np.asarray(hgrid, dtype=np.float64),
+                          float(c), int(ncong), keys.nseg_c, keys.nvctr_c, keys.keyg_c,
+                          keys.keyv_c, keys.nseg_f, keys.nvctr_f, keys.keyg_f, keys.keyv_f,
+                          x, b)
+    return x
