@@ -146,6 +146,79 @@ def build_batch(cfg, seed, device):
                 units=reg.n_isf * norb)
 
 
+# Algorithmic FP64 bytes per coefficient and CG step of the preconditioner
+# (dense keys: one working-grid point per coefficient): d = r + beta d and
+# W = S d (32), K = c W + K_x W (16), K += K_y W, K_z W (24 + 24), q = S K and
+# q.d (24), x += alpha d, r -= alpha q, |r|^2 (48).
+PC_BYTES_PER_COEF_STEP = 168
+
+
+def bench_precond(args, cfg, batch, ctx, lrs, stream):
+    """one step = one preconditioner call over the batch: the initial
+    residual, ncong CG steps and the final scaling"""
+    import numpy as np
+    import torch
+    norb = len(lrs)
+    b = batch["psi"].clone()
+    x = 0.1 * batch["psi"]
+    cpr = np.full(norb, 0.5)
+    ncoef = int(batch["ncoef"].sum())
+    for _ in range(args.warmup):
+        ctx.precond_cg(lrs, cpr, args.ncong, x, b, psi_off=batch["psi_off"])
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(args.steps):
+        ctx.precond_cg(lrs, cpr, args.ncong, x, b, psi_off=batch["psi_off"])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = t0.elapsed_time(t1) / args.steps
+    applies = args.ncong + 1
+    units = batch["units"]
+    gbs = PC_BYTES_PER_COEF_STEP * ncoef * applies / (ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+            "kernel": "whole call (scatter, kinetic x/y/z, gather, update per CG step)",
+            "bytes_per_coef_step": PC_BYTES_PER_COEF_STEP}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        regions = batch["regions"]
+        r = regions[0]
+        keys = oracle.Keys(r.nseg_c, r.nvctr_c, r.keyg_c, r.keyv_c, r.nseg_f, r.nvctr_f,
+                           r.keyg_f, r.keyv_f)
+        nc = int(batch["ncoef"][0])
+        xb = x[:nc].cpu().numpy()
+        bb = b[:nc].cpu().numpy()
+        done, ts = 0, 0.0
+        while done < norb and (done < 1 or ts < args.cpu_seconds):
+            tt = time.perf_counter()
+            oracle.precond_cg(r.geocode, r.n, r.hgrid, 0.5, args.ncong, keys, xb, bb)
+            ts += time.perf_counter() - tt
+            done += 1
+        cpu = {"value": r.n_isf * done * applies / ts,
+               "unit": "grid-pt*orbital*CG-step/s", "cores": oracle.num_threads(),
+               "kind": "port", "sample": f"{done} orbital(s) of the batch, ncong "
+               f"{args.ncong}, oracle (C+OpenMP restatement of the OpenCL "
+               f"preconditioner), {ts:.1f} s"}
+    out = {
+        "metric": "grid-pt*orbital*CG-step/s of the preconditioner "
+                  "(ocl_preconditioner_generic_k), FP64",
+        "value": units * applies / (ms * 1e-3), "unit": "grid-pt*orbital*CG-step/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) right-hand sides, guess 0.1 b)",
+        "config": {"workload": "preconditioner on " + cfg["workload"], "orbitals": norb,
+                   "ncong": args.ncong, "cprecr": 0.5, "wall_s": wall},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +235,11 @@ def parse():
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", default="locham", choices=["locham", "precond"],
+                    help="locham: the local Hamiltonian (the headline); precond: the "
+                         "conjugate-gradient preconditioner on the same batch")
+    ap.add_argument("--ncong", type=int, default=6,
+                    help="CG steps of the preconditioner workload")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (the "
                          "known-size kernel that scales profiler byte counters)")
@@ -205,6 +283,12 @@ def main():
     lrs = [lr_of[id(r)] for r in regions]
     bargs = dict(pot_off=batch["pot_off"], potdim=batch["potdim"],
                  ishift=batch["ishift"], psi_off=batch["psi_off"])
+    if args.workload == "precond":
+        bench_precond(args, cfg, batch, ctx, lrs, stream)
+        for lr in lr_of.values():
+            lr.close()
+        ctx.close()
+        return
 
     def timed(prec):
         """graph-replay timing, then per-stage HIP-event timing, then one

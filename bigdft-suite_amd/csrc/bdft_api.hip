@@ -31,6 +31,10 @@ struct bdft_ctx {
   // cache of the last one-shot plan
   bdft_plan* cached = nullptr;
   std::vector<uintptr_t> cached_key;
+  // preconditioner: plan of the last batch geometry and its vectors
+  bdft_plan* pc_plan = nullptr;
+  std::vector<uintptr_t> pc_key;
+  DevBuf pc_buf;
   // stage profiling (non-graph launches)
   bool profile = false;
   std::vector<hipEvent_t> ev_free;                 // reusable events
@@ -152,6 +156,8 @@ extern "C" int bdft_ctx_destroy(bdft_ctx* ctx) {
     for (auto x : set) hipEventDestroy(x);
   for (auto x : ctx->ev_free) hipEventDestroy(x);
   if (ctx->cached) bdft_plan_destroy(ctx->cached);
+  if (ctx->pc_plan) bdft_plan_destroy(ctx->pc_plan);
+  dev_free(ctx->pc_buf);
   dev_free(ctx->ws);
   if (ctx->own) hipStreamDestroy(ctx->own);
   delete ctx;
@@ -317,6 +323,11 @@ extern "C" int bdft_lr_destroy(bdft_lr* lr) {
       bdft_plan_destroy(lr->ctx->cached);
       lr->ctx->cached = nullptr;
       lr->ctx->cached_key.clear();
+    }
+    if (lr->ctx->pc_plan) {
+      bdft_plan_destroy(lr->ctx->pc_plan);
+      lr->ctx->pc_plan = nullptr;
+      lr->ctx->pc_key.clear();
     }
   }
   if (lr->d_cmap) hipFree(lr->d_cmap);
@@ -1170,20 +1181,39 @@ extern "C" int bdft_precond_cg_d(bdft_ctx* ctx, int norb, const bdft_lr* const* 
       pc[9 * i + 1 + t] = 1.0 / std::sqrt(v);
     }
   }
-  bdft_plan* plan = nullptr;
-  int rc = make_plan(ctx, norb, lrs, nullptr, nullptr, nullptr, nullptr, x, psi_off, nullptr,
-                     nullptr, nullptr, BDFT_PREC_FP64, true, &plan, true);
-  if (rc) return rc;
+  // the plan (descriptors, tile tables, W and K) depends on the geometry
+  // only: reused while the regions and offsets stay the same
+  std::vector<uintptr_t> key;
+  key.reserve(2 + 2 * (size_t)norb);
+  key.push_back((uintptr_t)norb);
+  key.push_back((uintptr_t)(psi_off != nullptr));
+  for (int i = 0; i < norb; ++i) {
+    key.push_back((uintptr_t)lrs[i]);
+    if (psi_off) key.push_back((uintptr_t)psi_off[i]);
+  }
+  if (!ctx->pc_plan || key != ctx->pc_key) {
+    if (ctx->pc_plan) {
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      bdft_plan_destroy(ctx->pc_plan);
+      ctx->pc_plan = nullptr;
+      ctx->pc_key.clear();
+    }
+    bdft_plan* np = nullptr;
+    int rc0 = make_plan(ctx, norb, lrs, nullptr, nullptr, nullptr, nullptr, x, psi_off,
+                        nullptr, nullptr, nullptr, BDFT_PREC_FP64, true, &np, true);
+    if (rc0) return rc0;
+    ctx->pc_plan = np;
+    ctx->pc_key.swap(key);
+  }
+  bdft_plan* plan = ctx->pc_plan;
   auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t nvec = sizeof(double) * (size_t)ntot;
   const size_t npc = sizeof(double) * pc.size(), nsc = sizeof(double) * 4 * (size_t)norb;
   const size_t npart = sizeof(double) * std::max(1, plan->T.ntiles[P_UNC]);
-  DevBuf buf;
-  rc = dev_alloc(ctx, buf, 3 * al(nvec) + al(npc) + al(nsc) + al(npart));
-  if (rc) {
-    bdft_plan_destroy(plan);
-    return rc;
-  }
+  DevBuf& buf = ctx->pc_buf;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the buffer may be reallocated
+  int rc = dev_alloc(ctx, buf, 3 * al(nvec) + al(npc) + al(nsc) + al(npart));
+  if (rc) return rc;
   char* q = (char*)buf.p;
   PcArgs A{};
   A.r = (double*)q;
@@ -1201,8 +1231,6 @@ extern "C" int bdft_precond_cg_d(bdft_ctx* ctx, int norb, const bdft_lr* const* 
                        ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   hipStreamSynchronize(ctx->stream);
-  dev_free(buf);
-  bdft_plan_destroy(plan);
   if (e != hipSuccess) return set_err(ctx, BDFT_ERR_HIP, "preconditioner: %s", hipGetErrorString(e));
   return BDFT_OK;
 }
