@@ -513,7 +513,7 @@ struct TabSet {
 // ---------------------------------------------------------------------------
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-static int box_tiles(const BoxDesc& b, int pass) {
+static int box_tiles(const BoxDesc& b, int pass, int zg) {
   const int N1 = b.N[0], N2 = b.N[1], N3 = b.N[2], L1 = b.L[0], L2 = b.L[1];
   switch (pass) {
     case P_UNC: return cdiv(b.nvctr_c, 256) + cdiv(b.nvctr_f, 256);
@@ -526,8 +526,8 @@ static int box_tiles(const BoxDesc& b, int pass) {
     case P_ZP: return cdiv(L1, TW) * L2;
     case P_PF:
     case P_PB: return 2 * N3;
-    case P_KZG: return cdiv((long long)cdiv(2 * N1, TW) * 2 * N2, ZGROUP);
-    case P_ZPG: return cdiv((long long)cdiv(L1, TW) * L2, ZGROUP);
+    case P_KZG: return cdiv((long long)cdiv(2 * N1, TW) * 2 * N2, zg);
+    case P_ZPG: return cdiv((long long)cdiv(L1, TW) * L2, zg);
   }
   return 0;
 }
@@ -610,16 +610,30 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     t2tot += (long long)b.L[0] * b.L[1] * 2 * b.N[2];
     if (!lr->dense) need_zero = true;
   }
+  // bundles per z-line tile of the matrix-core sequence: three, unless a
+  // z-line kernel would then take more than half of a CU's LDS (one
+  // work-group per CU); two halve the waste of a partial last tile too
+  int zg = ZGROUP;
+  if (mfma) {
+    const OpTab* tb = tabs.tabs.data();
+    size_t mx = 0;
+    for (int i = 0; i < norb; ++i) {
+      const BoxDesc& b = box[i];
+      mx = std::max(mx, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp, ZGROUP));
+      mx = std::max(mx, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp, ZGROUP));
+    }
+    if (mx > ctx->plane_lds / 2) zg = 2;
+  }
   // tile tables
   std::vector<int> starts;
   PassTables T{};
   int start_off[P_COUNT];
   for (int p = 0; p < P_COUNT; ++p) {
-    const int t0 = box_tiles(box[0], p);
+    const int t0 = box_tiles(box[0], p, zg);
     bool uni = true;
     long long tot = 0;
     for (int i = 0; i < norb; ++i) {
-      const int t = box_tiles(box[i], p);
+      const int t = box_tiles(box[i], p, zg);
       uni = uni && t == t0;
       tot += t;
     }
@@ -633,7 +647,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       int acc = 0;
       for (int i = 0; i < norb; ++i) {
         starts.push_back(acc);
-        acc += box_tiles(box[i], p);
+        acc += box_tiles(box[i], p, zg);
       }
       starts.push_back(acc);
     }
@@ -642,6 +656,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // least 18 coarse points (single wrap of the in-plane stencils) and both
   // plane kernels must fit in LDS
   MfmaLds ml{};
+  ml.zg = zg;
   if (mfma) {
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
@@ -659,8 +674,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       const OpTab* tb = tabs.tabs.data();
       ml.fwd = std::max(ml.fwd, mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]));
       ml.bwd = std::max(ml.bwd, mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]));
-      ml.kz = std::max(ml.kz, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp));
-      ml.zp = std::max(ml.zp, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp));
+      ml.kz = std::max(ml.kz, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp, zg));
+      ml.zp = std::max(ml.zp, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp, zg));
     }
     // plane-fused sequence when both plane kernels fit; the per-pass one
     // otherwise (or when line passes are forced)
@@ -1437,7 +1452,7 @@ static int transfer(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* ps
     B.psi = psi;
     B.hpsi = hpsi;
     B.W = psig;
-    TileTable t{nullptr, norb, box_tiles(box[0], P_UNC)};
+    TileTable t{nullptr, norb, box_tiles(box[0], P_UNC, ZGROUP)};
     const long long nt = (long long)t.uniform * norb;
     if (t.uniform == 0 || nt > 0x7fffffffLL) {
       dev_free(meta);

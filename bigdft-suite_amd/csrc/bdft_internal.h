@@ -55,6 +55,15 @@ struct BoxDesc {
   double conf_r[3];
 };
 
+// a / b for 0 <= a < 2^24 and 0 < b with inv = 1/b: one float product and
+// one correction step each way instead of an integer division
+__device__ __forceinline__ int qdiv(int a, int b, float inv) {
+  int q = (int)((float)a * inv);
+  if ((q + 1) * b <= a) ++q;
+  if (q * b > a) --q;
+  return q;
+}
+
 // the confining term at fine point (p1, p2, p3); callers test bd.conf_on
 __device__ __forceinline__ double conf_axis2(const BoxDesc& bd, int d, int p) {
   const double x = bd.conf_hh[d] * (double)(p + 1 + bd.conf_ioff[d]) - bd.conf_r[d];
@@ -220,11 +229,11 @@ struct Batch {
 };
 
 // Passes of one application; each has its own tile table.
-// P_KZG, P_ZPG: z-line passes of the matrix-core sequence, ZGROUP bundles of
-// 32 lines per tile
+// P_KZG, P_ZPG: z-line passes of the matrix-core sequence, zg (2 or 3)
+// bundles of 32 lines per tile, chosen per plan (MfmaLds::zg)
 enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_PF, P_PB, P_KZG, P_ZPG,
             P_COUNT };
-constexpr int ZGROUP = 3;
+constexpr int ZGROUP = 3;   // bundles per z-line tile unless LDS asks for 2
 
 struct PassTables {
   TileTable t[P_COUNT];
@@ -248,6 +257,7 @@ struct MfmaLds {
   size_t kin[3], grow, zpot, shrink;  // per-pass sequence
   size_t fwd, bwd, kz, zp;            // plane-fused sequence
   int fused;                          // 1: plane-fused sequence
+  int zg;                             // bundles per z-line tile (2 or 3)
 };
 
 // density: orbitals sharing one box are split into ngroups groups of gsize;
@@ -328,8 +338,8 @@ size_t mfma_lds_grow(int W2, int L1, int Kp1, int Kp2);
 size_t mfma_lds_zpot(int KpG, int KpS);
 size_t mfma_lds_shrink(int L1, int W1, int W2, int KpU, int KpB);
 size_t mfma_lds_fwd(int W1, int W2, int L1);
-size_t mfma_lds_kz(int W3, int Kp);
-size_t mfma_lds_zp(int KpG, int DpG);
+size_t mfma_lds_kz(int W3, int Kp, int zg);
+size_t mfma_lds_zp(int KpG, int DpG, int zg);
 size_t mfma_lds_bwd(int W1, int L1, int L2);
 size_t mfma_attributes(size_t lds);
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,

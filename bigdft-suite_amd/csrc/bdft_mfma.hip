@@ -327,6 +327,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
   // are zero
   const float* W = (const float*)B.W;
   const int KpW = t.Kp;
+  const float ikp = 1.0f / (float)KpW;
   const int tot = 32 * KpW;
   for (int i0 = threadIdx.x; i0 < tot; i0 += NTM * 8) {
     float v[8];
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u * NTM;
       int l, k;
-      if (AX == 0) { l = i / KpW; k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
+      if (AX == 0) { l = qdiv(i, KpW, ikp); k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
       v[u] = (i < tot && l < nl && k < Wd) ? W[base + l * lstride + k * pstride] : 0.0f;
     }
 #pragma unroll
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
       const int i = i0 + u * NTM;
       if (i < tot) {
         int l, k;
-        if (AX == 0) { l = i / KpW; k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
+        if (AX == 0) { l = qdiv(i, KpW, ikp); k = i - l * KpW; } else { k = i >> 5; l = i & 31; }
         __bf16 h, lo;
         split2<NT4>(v[u] * sc, h, lo);
         xh[l * XS + k] = h;
@@ -408,6 +409,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
   __syncthreads();
   const float* src = (const float*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tot = W1 * W2;
+  const float iw1 = 1.0f / (float)W1;
   for (int i0 = threadIdx.x; i0 < tot; i0 += NTMP * 8) {
     float v[8];
 #pragma unroll
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u * NTMP;
       if (i < tot) {
-        const int j2 = i / W1, j1 = i - j2 * W1;
+        const int j2 = qdiv(i, W1, iw1), j1 = i - j2 * W1;
         __bf16 h, lo;
         split2<NT4>(v[u] * sc, h, lo);
         rh[j2 * XS1 + j1] = h;
@@ -590,6 +592,7 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
   __syncthreads();
   const float* src = (const float*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
   const int tot = L1 * L2;
+  const float il1 = 1.0f / (float)L1;
   for (int i0 = threadIdx.x; i0 < tot; i0 += NTMP * 8) {
     float v[8];
 #pragma unroll
@@ -601,7 +604,7 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + u * NTMP;
       if (i < tot) {
-        const int p2 = i / L1, p1 = i - p2 * L1;
+        const int p2 = qdiv(i, L1, il1), p1 = i - p2 * L1;
         __bf16 h, lo;
         split2<BM>(v[u], h, lo);
         uh[p1 * XU + p2] = h;
@@ -711,13 +714,16 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
     const float2* src =
         (const float2*)((const float*)B.W + bd.w_off + (long long)W1 * W2 * j3);
     const int hp = KX / 2;
+    const float ihp = 1.0f / (float)hp;
     const int tot = nR * hp;
     for (int i0 = threadIdx.x; i0 < tot; i0 += NTMP * 8) {
       float2 v[8];
+      int jj[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int i = i0 + u * NTMP;
-        const int j2 = i / hp, j1 = 2 * (i - j2 * hp);
+        const int j2 = qdiv(i, hp, ihp), j1 = 2 * (i - j2 * hp);
+        jj[u] = j2 * XR + j1;
         v[u] = (i < tot && j2 < W2 && j1 < W1) ? src[(j2 * W1 + j1) >> 1]
                                                 : make_float2(0.0f, 0.0f);
       }
@@ -725,12 +731,11 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
       for (int u = 0; u < 8; ++u) {
         const int i = i0 + u * NTMP;
         if (i < tot) {
-          const int j2 = i / hp, j1 = 2 * (i - j2 * hp);
           __bf16 h0, l0, h1, l1;
           split2<NT4>(v[u].x * sc, h0, l0);
           split2<NT4>(v[u].y * sc, h1, l1);
-          *(bf16x2*)(rh + j2 * XR + j1) = bf16x2{h0, h1};
-          *(bf16x2*)(rl + j2 * XR + j1) = bf16x2{l0, l1};
+          *(bf16x2*)(rh + jj[u]) = bf16x2{h0, h1};
+          *(bf16x2*)(rl + jj[u]) = bf16x2{l0, l1};
         }
       }
     }
@@ -848,23 +853,25 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   {
     const float* src = (const float*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
     const int tot = nR * KX;
+    const float ikx = 1.0f / (float)KX;
     for (int i0 = threadIdx.x; i0 < tot; i0 += NTMP * 8) {
       float v[8];
+      int oo[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int i = i0 + u * NTMP;
-        const int p2 = i / KX, p1 = i - p2 * KX;
+        const int p2 = qdiv(i, KX, ikx), p1 = i - p2 * KX;
+        oo[u] = p2 * XR + p1;
         v[u] = (i < tot && p2 < L2 && p1 < L1) ? src[p1 + L1 * p2] : 0.0f;
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int i = i0 + u * NTMP;
         if (i < tot) {
-          const int p2 = i / KX, p1 = i - p2 * KX;
           __bf16 h, l;
           split2<BM>(v[u], h, l);
-          rh[p2 * XR + p1] = h;
-          rl[p2 * XR + p1] = l;
+          rh[oo[u]] = h;
+          rl[oo[u]] = l;
         }
       }
     }
@@ -930,8 +937,9 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   double* hc = B.hpsi + bd.psi_off;
   double* hf = B.hpsi + bd.psi_off + bd.nvctr_c;
   const int npl = N1 * N2;
+  const float in1 = 1.0f / (float)N1;
   for (int i = threadIdx.x; i < npl; i += NTMP) {
-    const int i2 = i / N1, i1 = i - i2 * N1;
+    const int i2 = qdiv(i, N1, in1), i1 = i - i2 * N1;
     const int lin = i + npl * i3;
     const int ci = bd.cinv ? bd.cinv[lin] : lin;
     const int fi = bd.finv ? bd.finv[lin] : lin;
@@ -964,10 +972,91 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
 // once for all of them).  Kinetic z: K += Ktilde_z W, and the tile's share of
 // ekin = sum W * K (K complete after this pass).
 // ---------------------------------------------------------------------------
-constexpr int ZG = ZGROUP;
 constexpr int ZU = 24;   // loads in flight per thread in the z-line loaders
 
-template <int NT4>
+// Stages ZG bundles of 32 consecutive-x z lines of a float grid into LDS as
+// split pieces [j][line][k] for k < Kp (rows nrows <= k < Kp are zero), and
+// with KEEPF the float values of rows k < nrows in xf [j][line][k].  src
+// points at (x 0, y line 0, z 0) of the box (x stride 1, y stride rowstride,
+// z stride pstride); bundle bi = ZG*lt + j holds x lines 32*(bi % nxt)..+31
+// of y line bi / nxt, valid for x < nvalid.  Thread t keeps the lines
+// LPT*(t % TPR) + (0..LPT-1) and two consecutive rows per unrolled step, so
+// bundle and row follow from the step index with no division, and both rows
+// go to LDS in one 32-bit store per piece.  Every load of the thread is in
+// flight before the first store.  Requires Kp <= 128 (n <= 64).
+template <int NT4, bool PAIRS, bool KEEPF, int ZG>
+__device__ __forceinline__ void stage_zlines(const float* __restrict__ src, int rowstride,
+                                             long long pstride, int nxt, int lt, int nb,
+                                             int nvalid, int nrows, int Kp, float sc,
+                                             __bf16* xh, __bf16* xl, int XS, float* xf,
+                                             int XF) {
+  constexpr int LPT = PAIRS ? 2 : 1;       // x lines per thread
+  constexpr int TPR = 32 / LPT;            // threads per row of a bundle
+  constexpr int RPS = 2 * (NTM / TPR);     // rows per step
+  constexpr int SPB = 128 / RPS;           // steps per bundle
+  constexpr int NS = ZG * SPB;
+  const int t = threadIdx.x;
+  const int l = LPT * (t % TPR);
+  const int kq = 2 * (t / TPR);
+  long long base[ZG];
+  int x0[ZG];
+#pragma unroll
+  for (int j = 0; j < ZG; ++j) {
+    const int bi = ZG * lt + j;
+    const int y = bi / nxt;
+    x0[j] = (bi - y * nxt) * 32;
+    base[j] = x0[j] + (long long)rowstride * y;
+  }
+  float v[NS][2][LPT];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int j = u / SPB;
+    const int k = kq + RPS * (u % SPB);
+    const bool ok = j < nb && x0[j] + l < nvalid && k < nrows;
+    const float* a = src + base[j] + l + k * pstride;
+    if (PAIRS) {
+      float2 p = make_float2(0.0f, 0.0f), q = p;
+      if (ok) {
+        p = *(const float2*)a;
+        q = *(const float2*)(a + pstride);
+      }
+      v[u][0][0] = p.x;
+      v[u][1][0] = q.x;
+      v[u][0][LPT - 1] = p.y;
+      v[u][1][LPT - 1] = q.y;
+    } else {
+      v[u][0][0] = ok ? a[0] : 0.0f;
+      v[u][1][0] = ok ? a[pstride] : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int j = u / SPB;
+    const int k = kq + RPS * (u % SPB);
+    if (j < nb && k < Kp) {
+#pragma unroll
+      for (int i = 0; i < LPT; ++i) {
+        const int o = (j * 32 + l + i) * XS + k;   // k and XS even: 4-byte aligned
+        bf16x2 hh, ll;
+        __bf16 h, lo;
+        split2<NT4>(v[u][0][i] * sc, h, lo);
+        hh[0] = h;
+        ll[0] = lo;
+        split2<NT4>(v[u][1][i] * sc, h, lo);
+        hh[1] = h;
+        ll[1] = lo;
+        *(bf16x2*)(xh + o) = hh;
+        *(bf16x2*)(xl + o) = ll;
+        if (KEEPF && k < nrows) {
+          xf[(j * 32 + l + i) * XF + k] = v[u][0][i];
+          xf[(j * 32 + l + i) * XF + k + 1] = v[u][1][i];
+        }
+      }
+    }
+  }
+}
+
+template <int NT4, int ZG>
 __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
@@ -986,45 +1075,9 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   __bf16* xh = (__bf16*)smem;          // [j][line][k]
   __bf16* xl = xh + ZG * 32 * XS;
   float* xf = (float*)(xl + ZG * 32 * XS);  // [j][line][k], k < W3
-  // z lines of 32 consecutive x: pairs of x-neighbours per load (W1 even,
-  // bundle starts at multiples of 32), all loads of a thread in flight at once
-  const float* W = (const float*)B.W;
-  const int per = 16 * KpW;         // pairs per bundle
-  const int tot = nb * per;
-  for (int i0 = threadIdx.x; i0 < tot; i0 += NTM * ZU) {
-    float2 v[ZU];
-#pragma unroll
-    for (int u = 0; u < ZU; ++u) {
-      const int i = i0 + u * NTM;
-      const int j = i / per, rem = i - j * per;
-      const int k = rem >> 4, l = 2 * (rem & 15);
-      const int bi = ZG * lt + j;
-      const int jo = bi / nxt, j1 = (bi - jo * nxt) * 32 + l;
-      v[u] = (i < tot && j1 < W1 && k < W3)
-                 ? *(const float2*)(W + bd.w_off + j1 + (long long)W1 * jo + k * pstride)
-                 : make_float2(0.0f, 0.0f);
-    }
-#pragma unroll
-    for (int u = 0; u < ZU; ++u) {
-      const int i = i0 + u * NTM;
-      if (i < tot) {
-        const int j = i / per, rem = i - j * per;
-        const int k = rem >> 4, l = 2 * (rem & 15);
-        __bf16 h, lo;
-        const int o = (j * 32 + l) * XS + k;
-        split2<NT4>(v[u].x * sc, h, lo);
-        xh[o] = h;
-        xl[o] = lo;
-        split2<NT4>(v[u].y * sc, h, lo);
-        xh[o + XS] = h;
-        xl[o + XS] = lo;
-        if (k < W3) {
-          xf[(j * 32 + l) * XF + k] = v[u].x;
-          xf[(j * 32 + l + 1) * XF + k] = v[u].y;
-        }
-      }
-    }
-  }
+  // z lines of 32 consecutive x (W1 even: pairs of x-neighbours per load)
+  stage_zlines<NT4, true, true, ZG>((const float*)B.W + bd.w_off, W1, pstride, nxt, lt, nb, W1,
+                                    W3, KpW, sc, xh, xl, XS, xf, XF);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1069,8 +1122,8 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
 }
 
 // z lines of T2, ZG bundles per work-group: G_z, V (epot), G_z^T in place.
-template <int NT4, int BM>
-__global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part) {
+template <int NT4, int BM, int ZG>
+__global__ __launch_bounds__(NTM, 2) void k_mzp(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
   const int tile = blockIdx.x;
@@ -1092,47 +1145,12 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
   __bf16* rh = xl + ZG * 32 * XS;
   __bf16* rl = rh + ZG * 32 * RS;
   // z lines of 32 consecutive x; pairs of x-neighbours when L1 is even
-  const bool pairs = (L1 & 1) == 0;
-  const int ve = pairs ? 2 : 1;
-  const int per = (32 / ve) * KpG;
-  const int tot = nb * per;
-  for (int i0 = threadIdx.x; i0 < tot; i0 += NTM * ZU) {
-    float2 v[ZU];
-#pragma unroll
-    for (int u = 0; u < ZU; ++u) {
-      const int i = i0 + u * NTM;
-      const int j = i / per, rem = i - j * per;
-      const int k = pairs ? rem >> 4 : rem >> 5;
-      const int l = pairs ? 2 * (rem & 15) : (rem & 31);
-      const int bi = ZG * lt + j;
-      const int p2 = bi / nxt, p1 = (bi - p2 * nxt) * 32 + l;
-      v[u] = make_float2(0.0f, 0.0f);
-      if (i < tot && p1 < L1 && k < W3) {
-        const float* a = T2 + p1 + (long long)L1 * p2 + k * pstride;
-        if (pairs) v[u] = *(const float2*)a;
-        else v[u].x = *a;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < ZU; ++u) {
-      const int i = i0 + u * NTM;
-      if (i < tot) {
-        const int j = i / per, rem = i - j * per;
-        const int k = pairs ? rem >> 4 : rem >> 5;
-        const int l = pairs ? 2 * (rem & 15) : (rem & 31);
-        const int o = (j * 32 + l) * XS + k;
-        __bf16 h, lo;
-        split2<NT4>(v[u].x * sc, h, lo);
-        xh[o] = h;
-        xl[o] = lo;
-        if (pairs) {
-          split2<NT4>(v[u].y * sc, h, lo);
-          xh[o + XS] = h;
-          xl[o + XS] = lo;
-        }
-      }
-    }
-  }
+  if ((L1 & 1) == 0)
+    stage_zlines<NT4, true, false, ZG>(T2, L1, pstride, nxt, lt, nb, L1, W3, KpG, sc, xh, xl,
+                                       XS, nullptr, 0);
+  else
+    stage_zlines<NT4, false, false, ZG>(T2, L1, pstride, nxt, lt, nb, L1, W3, KpG, sc, xh, xl,
+                                        XS, nullptr, 0);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1191,14 +1209,18 @@ __global__ __launch_bounds__(NTM) void k_mzp(Batch B, TileTable tt, double* part
     load_pot(0, ob, va);
     f32x16 acc[ZG] = {};
     mma_multi<NT4, false, ZG>(B, tg, 32 * ob, xh, xl, XS, 32 * XS, nb, acc);
-    static_assert(ZG == 3, "the potential pipeline below assumes three bundles");
+    static_assert(ZG == 2 || ZG == 3, "the potential pipeline below handles two or three bundles");
     if (nb > 1) load_pot(1, ob, vb);
     finish(0, ob, acc[0], va);
     if (nb > 1) {
-      if (nb > 2) load_pot(2, ob, va);
+      if constexpr (ZG > 2) {
+        if (nb > 2) load_pot(2, ob, va);
+      }
       finish(1, ob, acc[1], vb);
     }
-    if (nb > 2) finish(2, ob, acc[2], va);
+    if constexpr (ZG > 2) {
+      if (nb > 2) finish(2, ob, acc[2], va);
+    }
   }
   const double eps = wave_block_sum(ep, red, NTM / 64);
   if (threadIdx.x == 0) part[tile] = eps;
@@ -1248,11 +1270,11 @@ size_t mfma_lds_fwd(int W1, int W2, int L1) {
   const int nC = rup_h(L1, 32) > KX ? rup_h(L1, 32) : KX;
   return (size_t)4 * nR * (KX + 8) + (size_t)4 * nC * (nR + 8);
 }
-size_t mfma_lds_kz(int W3, int Kp) {
-  return (size_t)ZGROUP * ((size_t)2 * 32 * (Kp + 8) * 2 + (size_t)32 * (W3 + 1) * 4);
+size_t mfma_lds_kz(int W3, int Kp, int zg) {
+  return (size_t)zg * ((size_t)2 * 32 * (Kp + 8) * 2 + (size_t)32 * (W3 + 1) * 4);
 }
-size_t mfma_lds_zp(int KpG, int DpG) {
-  return (size_t)ZGROUP * ((size_t)2 * 32 * (KpG + 8) * 2 + (size_t)2 * 32 * (DpG + 8) * 2);
+size_t mfma_lds_zp(int KpG, int DpG, int zg) {
+  return (size_t)zg * ((size_t)2 * 32 * (KpG + 8) * 2 + (size_t)2 * 32 * (DpG + 8) * 2);
 }
 size_t mfma_lds_bwd(int W1, int L1, int L2) {
   const int nR = rup_h(L2, 32), KX = rup_h(L1, 32), nC = rup_h(W1, 32);
@@ -1265,8 +1287,9 @@ static void set_attr(size_t lds) {
                      (const void*)k_mkin<2, FM, false>, (const void*)k_mkin<2, FM, true>,
                      (const void*)k_mgrow_plane<FM>, (const void*)k_mzpot<FM, BM>,
                      (const void*)k_mshrink_plane<BM>, (const void*)k_mfwd<FM>,
-                     (const void*)k_mbwd<BM>, (const void*)k_mkz<FM>,
-                     (const void*)k_mzp<FM, BM>};
+                     (const void*)k_mbwd<BM>, (const void*)k_mkz<FM, 2>,
+                     (const void*)k_mkz<FM, 3>, (const void*)k_mzp<FM, BM, 2>,
+                     (const void*)k_mzp<FM, BM, 3>};
   for (const void* f : k)
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
@@ -1306,12 +1329,20 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       k_mfwd<FM><<<PT.ntiles[P_PF], NTMP, L.fwd, s>>>(B, PT.t[P_PF]);
     mark(S_PF, 1);
     mark(S_KZ, 0);
-    if (PT.ntiles[P_KZG] > 0)
-      k_mkz<FM><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
+    if (PT.ntiles[P_KZG] > 0) {
+      if (L.zg == 2)
+        k_mkz<FM, 2><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
+      else
+        k_mkz<FM, 3><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
+    }
     mark(S_KZ, 1);
     mark(S_ZP, 0);
-    if (PT.ntiles[P_ZPG] > 0)
-      k_mzp<FM, BM><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
+    if (PT.ntiles[P_ZPG] > 0) {
+      if (L.zg == 2)
+        k_mzp<FM, BM, 2><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
+      else
+        k_mzp<FM, BM, 3><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
+    }
     mark(S_ZP, 1);
     mark(S_PB, 0);
     if (PT.ntiles[P_PB] > 0)

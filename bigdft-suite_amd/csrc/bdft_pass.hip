@@ -37,12 +37,6 @@ __device__ __forceinline__ int wrap(int i, int n) {
 }
 
 // a / b for 0 <= a < 2^22 with a float reciprocal, corrected to exact
-__device__ __forceinline__ int qdiv(int a, int b, float inv) {
-  int q = (int)((float)a * inv);
-  if ((q + 1) * b <= a) ++q;
-  if (q * b > a) --q;
-  return q;
-}
 
 // ---------------------------------------------------------------------------
 // LDS staging.  Layout: lds[row * LDW + line].
