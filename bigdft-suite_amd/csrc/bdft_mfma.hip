@@ -1084,6 +1084,39 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   const float ks = (float)bd.kscale[2] * isc;
   float* K = (float*)B.K;
   double ek = 0.0;
+  const int nobk = t.Dp / 32;
+  if (2 * nobk <= NTM / 64 && nb > 1) {
+    // at most half as many output blocks as waves: one (block, bundle) pair
+    // per step, so every wave works (each reads the table fragments for its
+    // own bundle; with three blocks for four waves sharing them is faster)
+    for (int it = wv; it < nobk * nb; it += NTM / 64) {
+      const int ob = it / nb, j = it - ob * nb;
+      const int bi = ZG * lt + j;
+      const int jo = bi / nxt, j1 = (bi - jo * nxt) * 32 + (lane & 31);
+      const bool okj = j1 < W1;
+      const long long basej = bd.w_off + j1 + (long long)W1 * jo;
+      float kold[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = 32 * ob + drow(r, lane);
+        kold[r] = (okj && o < W3) ? K[basej + o * pstride] : 0.0f;
+      }
+      f32x16 acc[1] = {};
+      mma_multi<NT4, false, 1>(B, t, 32 * ob, xh + j * 32 * XS, xl + j * 32 * XS, XS,
+                               32 * XS, 1, acc);
+      if (okj) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int o = 32 * ob + drow(r, lane);
+          if (o < W3) {
+            const float kn = kold[r] + acc[0][r] * ks;
+            ek += (double)kn * (double)xf[(j * 32 + (lane & 31)) * XF + o];
+            K[basej + o * pstride] = kn;
+          }
+        }
+      }
+    }
+  } else
   for (int ob = wv; ob < t.Dp / 32; ob += NTM / 64) {
     // the K values this wave updates, read while the matrix cores run
     float kold[ZG][16];
@@ -1204,6 +1237,19 @@ __global__ __launch_bounds__(NTM, 2) void k_mzp(Batch B, TileTable tt, double* p
       *(bf16x4*)(rl + o) = l;
     }
   };
+  const int nob1 = tg.Dp / 32;
+  if (nob1 < NTM / 64 && nb > 1) {
+    // fewer output blocks than waves: one (block, bundle) pair per step
+    for (int it = wv; it < nob1 * nb; it += NTM / 64) {
+      const int ob = it / nb, j = it - ob * nb;
+      double va[16];
+      load_pot(j, ob, va);
+      f32x16 acc[1] = {};
+      mma_multi<NT4, false, 1>(B, tg, 32 * ob, xh + j * 32 * XS, xl + j * 32 * XS, XS,
+                               32 * XS, 1, acc);
+      finish(j, ob, acc[0], va);
+    }
+  } else
   for (int ob = wv; ob < tg.Dp / 32; ob += NTM / 64) {
     double va[16], vb[16];
     load_pot(0, ob, va);
@@ -1226,6 +1272,26 @@ __global__ __launch_bounds__(NTM, 2) void k_mzp(Batch B, TileTable tt, double* p
   if (threadIdx.x == 0) part[tile] = eps;
   __syncthreads();
   // G_z^T back to the wavelet z lines, in place
+  const int nob2 = tsh.Dp / 32;
+  if (nob2 < NTM / 64 && nb > 1) {
+    for (int it = wv; it < nob2 * nb; it += NTM / 64) {
+      const int ob = it / nb, j = it - ob * nb;
+      f32x16 acc[1] = {};
+      mma_multi<BM, false, 1>(B, tsh, 32 * ob, rh + j * 32 * RS, rl + j * 32 * RS, RS,
+                              32 * RS, 1, acc);
+      const int bi = ZG * lt + j;
+      const int p2 = bi / nxt;
+      const int p1 = (bi - p2 * nxt) * 32 + (lane & 31);
+      if (p1 < L1) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int o = 32 * ob + drow(r, lane);
+          if (o < W3) T2[p1 + (long long)L1 * p2 + o * pstride] = acc[0][r];
+        }
+      }
+    }
+    return;
+  }
   for (int ob = wv; ob < tsh.Dp / 32; ob += NTM / 64) {
     f32x16 acc[ZG] = {};
     mma_multi<BM, false, ZG>(B, tsh, 32 * ob, rh, rl, RS, 32 * RS, nb, acc);
