@@ -1035,36 +1035,66 @@ __device__ __forceinline__ double pc_scal(const PcArgs& A, int b, int t) {
   return A.pc[9 * b + 1 + t];
 }
 
+// Element range of compressed tile lt of a box: CPT coarse coefficients, or
+// the 7 CPT contiguous coefficients of CPT fine points (the vector side of
+// the preconditioner is read and written through these ranges, coalesced;
+// the 7 components of a fine point go through LDS to the thread that owns
+// the point's grid positions)
+struct PcRange {
+  long long e0;   // first vector element
+  int ne;         // elements
+  int k0;         // first point
+  int fine;
+};
+__device__ __forceinline__ PcRange pc_range(const BoxDesc& bd, int lt) {
+  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  PcRange R;
+  R.fine = lt >= ctiles;
+  if (!R.fine) {
+    R.k0 = lt * CPT;
+    R.e0 = bd.psi_off + R.k0;
+    R.ne = min(CPT, bd.nvctr_c - R.k0);
+  } else {
+    R.k0 = (lt - ctiles) * CPT;
+    R.e0 = bd.psi_off + bd.nvctr_c + 7LL * R.k0;
+    R.ne = 7 * min(CPT, bd.nvctr_f - R.k0);
+  }
+  return R;
+}
+
 // W <- S v at the region's points; update: d <- r + beta d first, v = d
 __global__ __launch_bounds__(NT) void k_pc_scatter(Batch B, TileTable tt, PcArgs A,
                                                    const double* src, int update) {
+  __shared__ double st[7 * CPT];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
-  const int lt = tile - box_first_tile(tt, b);
-  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
   double* W = (double*)B.W + bd.w_off;
   const double beta = update ? A.sc[4 * b + 2] : 0.0;
-  auto val = [&](long long i) {
-    if (!update) return src[i];
-    const double v = fma(beta, A.d[i], A.r[i]);
-    A.d[i] = v;
-    return v;
-  };
-  if (lt < ctiles) {
-    const int k = lt * CPT + threadIdx.x;
-    if (k < bd.nvctr_c)
-      W[psig_index(bd, bd.cmap[k], 0, 0, 0)] = pc_scal(A, b, 0) * val(bd.psi_off + k);
-  } else {
-    const int k = (lt - ctiles) * CPT + threadIdx.x;
-    if (k < bd.nvctr_f) {
-      const int lin = bd.fmap[k];
-      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
-#pragma unroll
-      for (int t = 1; t <= 7; ++t)
-        W[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)] =
-            pc_scal(A, b, t) * val(base + t - 1);
+  // the tile's vector elements, coalesced (d updated in place)
+  for (int e = threadIdx.x; e < R.ne; e += NT) {
+    const long long i = R.e0 + e;
+    double v;
+    if (update) {
+      v = fma(beta, A.d[i], A.r[i]);
+      A.d[i] = v;
+    } else {
+      v = src[i];
     }
+    st[e] = v;
+  }
+  __syncthreads();
+  const int k = R.k0 + threadIdx.x;
+  if (!R.fine) {
+    if (threadIdx.x < R.ne)
+      W[psig_index(bd, bd.cmap[k], 0, 0, 0)] = pc_scal(A, b, 0) * st[threadIdx.x];
+  } else if (7 * (int)threadIdx.x < R.ne) {
+    const int lin = bd.fmap[k];
+#pragma unroll
+    for (int t = 1; t <= 7; ++t)
+      W[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)] =
+          pc_scal(A, b, t) * st[7 * threadIdx.x + t - 1];
   }
 }
 
@@ -1126,14 +1156,28 @@ __global__ __launch_bounds__(NT) void k_pc_kinx(Batch B, TileTable tt, PcArgs A)
 // else q = v, partial q.d
 __global__ __launch_bounds__(NT) void k_pc_gather(Batch B, TileTable tt, PcArgs A, int init) {
   __shared__ double red[NT / 64];
+  __shared__ double st[7 * CPT];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
-  const int lt = tile - box_first_tile(tt, b);
-  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
   const double* K = (const double*)B.K + bd.w_off;
+  const int k = R.k0 + threadIdx.x;
+  if (!R.fine) {
+    if (threadIdx.x < R.ne)
+      st[threadIdx.x] = pc_scal(A, b, 0) * K[psig_index(bd, bd.cmap[k], 0, 0, 0)];
+  } else if (7 * (int)threadIdx.x < R.ne) {
+    const int lin = bd.fmap[k];
+#pragma unroll
+    for (int t = 1; t <= 7; ++t)
+      st[7 * threadIdx.x + t - 1] =
+          pc_scal(A, b, t) * K[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)];
+  }
+  __syncthreads();
   double acc = 0.0;
-  auto put = [&](long long i, double v) {
+  for (int e = threadIdx.x; e < R.ne; e += NT) {
+    const long long i = R.e0 + e;
+    const double v = st[e];
     if (init) {
       const double r = A.b[i] - v;
       A.r[i] = r;
@@ -1143,53 +1187,28 @@ __global__ __launch_bounds__(NT) void k_pc_gather(Batch B, TileTable tt, PcArgs 
       A.q[i] = v;
       acc = fma(v, A.d[i], acc);
     }
-  };
-  if (lt < ctiles) {
-    const int k = lt * CPT + threadIdx.x;
-    if (k < bd.nvctr_c)
-      put(bd.psi_off + k, pc_scal(A, b, 0) * K[psig_index(bd, bd.cmap[k], 0, 0, 0)]);
-  } else {
-    const int k = (lt - ctiles) * CPT + threadIdx.x;
-    if (k < bd.nvctr_f) {
-      const int lin = bd.fmap[k];
-      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
-#pragma unroll
-      for (int t = 1; t <= 7; ++t)
-        put(base + t - 1,
-            pc_scal(A, b, t) * K[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)]);
-    }
   }
   const double sum = block_sum(acc, red);
   if (threadIdx.x == 0) A.part[tile] = sum;
 }
 
-// x += alpha d; unless last: r -= alpha q, partial |r|^2
+// x += alpha d; unless last: r -= alpha q, partial |r|^2 (elementwise over
+// the tile's contiguous range)
 __global__ __launch_bounds__(NT) void k_pc_update(Batch B, TileTable tt, PcArgs A, int last) {
   __shared__ double red[NT / 64];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
-  const int lt = tile - box_first_tile(tt, b);
-  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
+  const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
   const double alpha = A.sc[4 * b + 1];
   double acc = 0.0;
-  auto upd = [&](long long i) {
+  for (int e = threadIdx.x; e < R.ne; e += NT) {
+    const long long i = R.e0 + e;
     A.x[i] = fma(alpha, A.d[i], A.x[i]);
     if (!last) {
       const double r = fma(-alpha, A.q[i], A.r[i]);
       A.r[i] = r;
       acc = fma(r, r, acc);
-    }
-  };
-  if (lt < ctiles) {
-    const int k = lt * CPT + threadIdx.x;
-    if (k < bd.nvctr_c) upd(bd.psi_off + k);
-  } else {
-    const int k = (lt - ctiles) * CPT + threadIdx.x;
-    if (k < bd.nvctr_f) {
-      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
-#pragma unroll
-      for (int t = 0; t < 7; ++t) upd(base + t);
     }
   }
   if (!last) {
@@ -1203,19 +1222,9 @@ __global__ __launch_bounds__(NT) void k_pc_final(Batch B, TileTable tt, PcArgs A
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
-  const int lt = tile - box_first_tile(tt, b);
-  const int ctiles = (bd.nvctr_c + CPT - 1) / CPT;
-  if (lt < ctiles) {
-    const int k = lt * CPT + threadIdx.x;
-    if (k < bd.nvctr_c) A.x[bd.psi_off + k] *= pc_scal(A, b, 0);
-  } else {
-    const int k = (lt - ctiles) * CPT + threadIdx.x;
-    if (k < bd.nvctr_f) {
-      const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
-#pragma unroll
-      for (int t = 1; t <= 7; ++t) A.x[base + t - 1] *= pc_scal(A, b, t);
-    }
-  }
+  const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
+  for (int e = threadIdx.x; e < R.ne; e += NT)
+    A.x[R.e0 + e] *= pc_scal(A, b, R.fine ? 1 + e % 7 : 0);
 }
 
 // one block per box: s = sum of the box's tile partials (fixed order).
