@@ -692,6 +692,55 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      "offers %zu",
                      mx, ctx->plane_lds);
   }
+  // plane-fused matrix-core kernels: the launch reserves the largest box's
+  // LDS for every work-group; boxes whose plane kernels fit in half a CU's
+  // LDS get launches of their own (two work-groups per CU), the others a
+  // second pair of launches
+  int g_off[4] = {-1, -1, -1, -1};  // small starts, small map, large starts, large map
+  if (mfma && ml.fused) {
+    const size_t half = ctx->plane_lds / 2;
+    std::vector<int> sm, lg;
+    size_t fs = 0, bs = 0, fl = 0, bl = 0;
+    for (int i = 0; i < norb; ++i) {
+      const BoxDesc& b = box[i];
+      const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]);
+      const size_t w = mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]);
+      if (std::max(f, w) <= half) {
+        sm.push_back(i);
+        fs = std::max(fs, f);
+        bs = std::max(bs, w);
+      } else {
+        lg.push_back(i);
+        fl = std::max(fl, f);
+        bl = std::max(bl, w);
+      }
+    }
+    if (!sm.empty() && !lg.empty()) {
+      auto add = [&](const std::vector<int>& g, int& so, int& mo) {
+        so = (int)starts.size();
+        int acc = 0;
+        for (int i : g) {
+          starts.push_back(acc);
+          acc += box_tiles(box[i], P_PF, zg);
+        }
+        starts.push_back(acc);
+        mo = (int)starts.size();
+        for (int i : g) starts.push_back(i);
+        return acc;
+      };
+      T.ntiles[P_PF] = T.ntiles[P_PB] = add(sm, g_off[0], g_off[1]);
+      T.ntiles_pf2 = T.ntiles_pb2 = add(lg, g_off[2], g_off[3]);
+      for (int p : {P_PF, P_PB}) {
+        T.t[p].nbox = (int)sm.size();
+        T.t[p].uniform = 0;
+      }
+      T.pf2.nbox = T.pb2.nbox = (int)lg.size();
+      ml.fwd = fs;
+      ml.bwd = bs;
+      ml.fwd2 = fl;
+      ml.bwd2 = bl;
+    }
+  }
   {
     bool plane = !ctx->force_lines && !mfma && !density;
     size_t lpf = 0, lpb = 0;
@@ -771,6 +820,14 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     return set_err(ctx, BDFT_ERR_HIP, "descriptor upload failed");
   }
   for (int p = 0; p < P_COUNT; ++p) T.t[p].start = start_off[p] >= 0 ? dst + start_off[p] : nullptr;
+  if (g_off[0] >= 0) {
+    for (int p : {P_PF, P_PB}) {
+      T.t[p].start = dst + g_off[0];
+      T.t[p].bmap = dst + g_off[1];
+    }
+    T.pf2.start = T.pb2.start = dst + g_off[2];
+    T.pf2.bmap = T.pb2.bmap = dst + g_off[3];
+  }
   {
     double* q = dpart;
     for (int e = 0; e < 4; ++e) {

@@ -91,6 +91,7 @@ struct TileTable {
   const int* start;     // nbox + 1 cumulative tile counts (device)
   int nbox;
   int uniform;          // >0: every box has exactly this many tiles
+  const int* bmap;      // null, or the batch index of each of the nbox boxes
 };
 
 __device__ __forceinline__ int find_box(const TileTable& tt, int tile) {
@@ -101,6 +102,10 @@ __device__ __forceinline__ int find_box(const TileTable& tt, int tile) {
     if (tt.start[mid] <= tile) lo = mid; else hi = mid;
   }
   return lo;
+}
+// batch index of the table's box lb (tables over a subset of the batch)
+__device__ __forceinline__ int table_box(const TileTable& tt, int lb) {
+  return tt.bmap ? tt.bmap[lb] : lb;
 }
 __device__ __forceinline__ int box_first_tile(const TileTable& tt, int b) {
   return tt.uniform > 0 ? b * tt.uniform : tt.start[b];
@@ -241,6 +246,10 @@ struct PassTables {
   double* part[4];       // energy partials: xfwd|plane_fwd, kin_y, kin_z, zpot
   int plane;             // 1: plane-fused sequence, 0: line passes
   size_t lds_pf, lds_pb; // LDS bytes of the plane kernels
+  // matrix-core plane kernels: boxes needing more than half a CU's LDS get
+  // launches of their own (t[P_PF], t[P_PB] then cover the other boxes)
+  TileTable pf2, pb2;
+  int ntiles_pf2, ntiles_pb2;
 };
 
 // Launch stages in order; profiling records one event pair per stage.
@@ -258,6 +267,7 @@ struct MfmaLds {
   size_t fwd, bwd, kz, zp;            // plane-fused sequence
   int fused;                          // 1: plane-fused sequence
   int zg;                             // bundles per z-line tile (2 or 3)
+  size_t fwd2, bwd2;                  // plane kernels of the large-box launches
 };
 
 // density: orbitals sharing one box are split into ngroups groups of gsize;

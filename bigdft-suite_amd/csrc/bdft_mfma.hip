@@ -695,9 +695,10 @@ template <int NT4>
 __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x;
-  const int b = find_box(tt, tile);
+  const int lb = find_box(tt, tile);
+  const int b = table_box(tt, lb);
   const BoxDesc bd = B.box[b];
-  const int j3 = tile - box_first_tile(tt, b);
+  const int j3 = tile - box_first_tile(tt, lb);
   const float sc = box_scale<NT4>(B, b), isc = 1.0f / sc;
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], L1 = bd.L[0], L2 = bd.L[1];
   const OpTab tx = B.tabs[bd.opG[0]], ty = B.tabs[bd.opG[1]];
@@ -837,9 +838,10 @@ template <int BM>
 __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x;
-  const int b = find_box(tt, tile);
+  const int lb = find_box(tt, tile);
+  const int b = table_box(tt, lb);
   const BoxDesc bd = B.box[b];
-  const int j3 = tile - box_first_tile(tt, b);
+  const int j3 = tile - box_first_tile(tt, lb);
   const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], L1 = bd.L[0], L2 = bd.L[1];
   const int W1 = 2 * N1, W2 = 2 * N2;
   const OpTab sx = B.tabs[bd.opS[0]], sy = B.tabs[bd.opS[1]];
@@ -1393,6 +1395,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     mark(S_PF, 0);
     if (PT.ntiles[P_PF] > 0)
       k_mfwd<FM><<<PT.ntiles[P_PF], NTMP, L.fwd, s>>>(B, PT.t[P_PF]);
+    if (PT.ntiles_pf2 > 0) k_mfwd<FM><<<PT.ntiles_pf2, NTMP, L.fwd2, s>>>(B, PT.pf2);
     mark(S_PF, 1);
     mark(S_KZ, 0);
     if (PT.ntiles[P_KZG] > 0) {
@@ -1413,6 +1416,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     mark(S_PB, 0);
     if (PT.ntiles[P_PB] > 0)
       k_mbwd<BM><<<PT.ntiles[P_PB], NTMP, L.bwd, s>>>(B, PT.t[P_PB]);
+    if (PT.ntiles_pb2 > 0) k_mbwd<BM><<<PT.ntiles_pb2, NTMP, L.bwd2, s>>>(B, PT.pb2);
     mark(S_PB, 1);
     et.t[0] = PT.t[P_KZG];
     et.p[0] = PT.part[2];
