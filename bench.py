@@ -219,6 +219,169 @@ def bench_precond(args, cfg, batch, ctx, lrs, stream):
     print(json.dumps(out))
 
 
+def timed_calls(fn, steps, warmup, stream):
+    """ms per call of fn over steps calls after warmup, with events on the
+    library's stream, and the wall time"""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(steps):
+        fn()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps, time.perf_counter() - w0
+
+
+def oracle_keys(oracle, reg):
+    return oracle.Keys(reg.nseg_c, reg.nvctr_c, reg.keyg_c, reg.keyv_c, reg.nseg_f,
+                       reg.nvctr_f, reg.keyg_f, reg.keyv_f)
+
+
+def bench_density(args, cfg, batch, ctx, lrs, stream):
+    """one step = rho += hfac psir^2 over every orbital of the batch (one
+    region shared by all orbitals: the dense configurations)"""
+    import numpy as np
+    import torch
+    import bigdft_hip
+    if len({id(r) for r in batch["regions"]}) != 1:
+        raise SystemExit("the density workload needs one shared region (c2 or c3)")
+    norb = len(lrs)
+    reg = batch["regions"][0]
+    prec = bigdft_hip.PREC_FP32 if args.precision == "fp32" else bigdft_hip.PREC_FP64
+    rho = torch.zeros((1, reg.n_isf), dtype=torch.float64, device="cuda")
+    hf = np.full(norb, 2.0)
+    psi = batch["psi"]
+    ms, wall = timed_calls(lambda: ctx.density(lrs[0], psi, hf, rho, precision=prec),
+                           args.steps, args.warmup, stream)
+    units = batch["units"]
+    # algorithmic cost per grid point and orbital: psi read once (8 B: one
+    # coefficient per fine point on a dense box) and rho read and written
+    # once per call (16 B over the orbitals); three grow passes (synthesis +
+    # magic filter, 64 flops each) and hfac*psir^2 (2)
+    esz = 8 if prec == bigdft_hip.PREC_FP64 else 4
+    bpp = 8 + 16 / norb
+    fpp = 3 * 64 + 2
+    t = ms * 1e-3
+    gbs, tfs = bpp * units / t / 1e9, fpp * units / t / 1e12
+    peak_c = 78.6 if esz == 8 else 157.3
+    if tfs / peak_c >= gbs / PEAK_HBM_GBS:
+        roof = {"bound": "mfma", "achieved": tfs, "peak": peak_c,
+                "unit": "TFLOP/s", "frac": tfs / peak_c, "hbm_GBps": gbs}
+    else:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": gbs / PEAK_HBM_GBS, "flop_TFLOPs": tfs}
+    roof.update({"traffic": None, "bytes_per_pt": bpp, "flops_per_pt": fpp,
+                 "kernel": "whole call (uncompress, x and y grow, z grow + hfac psir^2, "
+                           "group reduction)"})
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        keys = oracle_keys(oracle, reg)
+        nc = int(batch["ncoef"][0])
+        done, ts = 0, 0.0
+        while done < norb and (done < 1 or ts < args.cpu_seconds):
+            p = psi[done * nc:(done + 1) * nc].cpu().numpy()[None, :]
+            tt = time.perf_counter()
+            oracle.density(reg.geocode, reg.n, reg.nfl, reg.nfu, keys, p, [2.0])
+            ts += time.perf_counter() - tt
+            done += 1
+        cpu = {"value": reg.n_isf * done / ts, "unit": "grid-pt*orbital/s",
+               "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"{done} orbital(s), oracle density, {ts:.1f} s"}
+    print(json.dumps({
+        "metric": "grid-pt*orbital/s of the density accumulation (local_partial_density)",
+        "value": units / (ms * 1e-3), "unit": "grid-pt*orbital/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if esz == 8 else "f32", "data": "synthetic (seeded N(0,1) coefficients)",
+        "config": {"workload": "density on " + cfg["workload"], "orbitals": norb, "wall_s": wall},
+        "roofline": roof,
+        "cpu_baseline": cpu}))
+
+
+def bench_transfers(args):
+    """one step = psi_to_locreg2 of every localisation region of the
+    compressed-sphere batch out of its own global orbital, then
+    Lpsi_to_global2 back (bit-exact copies, no arithmetic)"""
+    import numpy as np
+    import torch
+    import bigdft_hip
+    from bigdft_hip import workloads
+    from bigdft_hip.descriptors import dense_region
+    norb = 256
+    sb = workloads.sphere_batch(norb, args.seed, device="cuda")
+    ng = (95, 95, 95)
+    glob = dense_region("F", ng, (0.4, 0.4, 0.4))
+    ctx = bigdft_hip.Context(0)
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
+    g = ctx.region(glob)
+    cache = {}
+    lrs = [cache.setdefault(id(r), ctx.region(r)) for r in sb["regions"]]
+    ns = np.array(sb["nl"], dtype=np.int32).reshape(-1, 3)
+    gnc = glob.ncoef
+    gpsi = torch.randn(norb * gnc, dtype=torch.float64, device="cuda")
+    goff = np.arange(norb, dtype=np.int64) * gnc
+    lpsi = torch.empty_like(sb["psi"])
+    loff = sb["psi_off"]
+    ms_g, _ = timed_calls(lambda: ctx.psi_to_locreg(g, lrs, ns, gpsi, lpsi, goff=goff, loff=loff),
+                          args.steps, args.warmup, stream)
+    ms_l, _ = timed_calls(lambda: ctx.locreg_to_psi(g, lrs, ns, lpsi, gpsi, loff=loff, goff=goff),
+                          args.steps, args.warmup, stream)
+    nloc = int(sb["ncoef"].sum())
+    npts = int(sum(r.nvctr_c + r.nvctr_f for r in sb["regions"]))
+    # bytes per direction: every local coefficient read once and written
+    # once (16 B), plus per point the local map and the global inverse-map
+    # entry (8 B; the dense global region needs no inverse map: 4 B)
+    byts = 16 * nloc + 4 * npts
+    ms = ms_g + ms_l
+    gbs = 2 * byts / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        gk = oracle_keys(oracle, glob)
+        done, ts, moved = 0, 0.0, 0
+        while done < norb and (done < 1 or ts < args.cpu_seconds):
+            r = sb["regions"][done]
+            gp = gpsi[done * gnc:(done + 1) * gnc].cpu().numpy()
+            lk = oracle_keys(oracle, r)
+            tt = time.perf_counter()
+            lp = oracle.psi_to_locreg2(r.n, ng, sb["nl"][done], lk, gk, gp)
+            oracle.lpsi_to_global2(r.n, ng, sb["nl"][done], lk, gk, lp, gp)
+            ts += time.perf_counter() - tt
+            moved += 2 * r.ncoef
+            done += 1
+        cpu = {"value": moved / ts, "unit": "coefficient/s", "cores": 1, "kind": "port",
+               "sample": f"{done} orbital(s) there and back, oracle psi_to_locreg2 + "
+                         f"lpsi_to_global2, {ts:.1f} s"}
+    print(json.dumps({
+        "metric": "compressed coefficients/s moved between global and localisation "
+                  "regions (psi_to_locreg2 + Lpsi_to_global2)",
+        "value": 2 * nloc / (ms * 1e-3), "unit": "coefficient/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "ms_to_local": ms_g, "ms_to_global": ms_l, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded spheres, N(0,1) global orbitals)",
+        "config": {"workload": f"{norb} compressed spheres (coarse r 7, fine r 3.5 bohr) of a "
+                               "96^3 free global box, one global orbital each",
+                   "local_coefficients": nloc},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_xfer",
+                     "bytes_per_call": byts},
+        "cpu_baseline": cpu}))
+    for l in cache.values():
+        l.close()
+    g.close()
+    ctx.close()
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,9 +398,12 @@ def parse():
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", default="locham", choices=["locham", "precond"],
+    ap.add_argument("--workload", default="locham",
+                    choices=["locham", "precond", "density", "transfers"],
                     help="locham: the local Hamiltonian (the headline); precond: the "
-                         "conjugate-gradient preconditioner on the same batch")
+                         "conjugate-gradient preconditioner on the same batch; density: "
+                         "rho += occ psir^2 over the batch; transfers: global <-> "
+                         "localisation-region copies of compressed orbitals")
     ap.add_argument("--ncong", type=int, default=6,
                     help="CG steps of the preconditioner workload")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -253,6 +419,9 @@ def main():
               "batch shards by orbital across GPUs but no multi-GPU run is "
               "part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "transfers":
+        bench_transfers(args)
+        return
     import numpy as np
     import torch
 
@@ -283,8 +452,9 @@ def main():
     lrs = [lr_of[id(r)] for r in regions]
     bargs = dict(pot_off=batch["pot_off"], potdim=batch["potdim"],
                  ishift=batch["ishift"], psi_off=batch["psi_off"])
-    if args.workload == "precond":
-        bench_precond(args, cfg, batch, ctx, lrs, stream)
+    if args.workload in ("precond", "density"):
+        (bench_precond if args.workload == "precond" else bench_density)(
+            args, cfg, batch, ctx, lrs, stream)
         for lr in lr_of.values():
             lr.close()
         ctx.close()

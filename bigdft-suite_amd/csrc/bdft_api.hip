@@ -35,6 +35,13 @@ struct bdft_ctx {
   bdft_plan* pc_plan = nullptr;
   std::vector<uintptr_t> pc_key;
   DevBuf pc_buf;
+  // density: plan of the last (region, orbital count, precision) and its
+  // per-group partial sums
+  bdft_plan* den_plan = nullptr;
+  std::vector<uintptr_t> den_key;
+  DevBuf den_buf;
+  // region transfers: descriptors and the miss counter
+  DevBuf xfer_buf;
   // stage profiling (non-graph launches)
   bool profile = false;
   std::vector<hipEvent_t> ev_free;                 // reusable events
@@ -158,6 +165,9 @@ extern "C" int bdft_ctx_destroy(bdft_ctx* ctx) {
   if (ctx->cached) bdft_plan_destroy(ctx->cached);
   if (ctx->pc_plan) bdft_plan_destroy(ctx->pc_plan);
   dev_free(ctx->pc_buf);
+  if (ctx->den_plan) bdft_plan_destroy(ctx->den_plan);
+  dev_free(ctx->den_buf);
+  dev_free(ctx->xfer_buf);
   dev_free(ctx->ws);
   if (ctx->own) hipStreamDestroy(ctx->own);
   delete ctx;
@@ -328,6 +338,11 @@ extern "C" int bdft_lr_destroy(bdft_lr* lr) {
       bdft_plan_destroy(lr->ctx->pc_plan);
       lr->ctx->pc_plan = nullptr;
       lr->ctx->pc_key.clear();
+    }
+    if (lr->ctx->den_plan) {
+      bdft_plan_destroy(lr->ctx->den_plan);
+      lr->ctx->den_plan = nullptr;
+      lr->ctx->den_key.clear();
     }
   }
   if (lr->d_cmap) hipFree(lr->d_cmap);
@@ -1043,11 +1058,27 @@ extern "C" int bdft_density_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const 
       sp[i] = spin[i];
     }
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  std::vector<const bdft_lr*> lrs((size_t)norb, lr);
-  bdft_plan* plan = nullptr;
-  int rc = make_plan(ctx, norb, lrs.data(), nullptr, nullptr, nullptr, nullptr, psi, nullptr,
-                     nullptr, nullptr, nullptr, precision, true, &plan, true);
-  if (rc) return rc;
+  // the plan (tile tables and work grids) depends on the region, the
+  // orbital count and the precision only: reused while they stay the same,
+  // with the orbitals' address set per call
+  const std::vector<uintptr_t> key{(uintptr_t)lr, (uintptr_t)norb, (uintptr_t)precision};
+  if (!ctx->den_plan || key != ctx->den_key) {
+    if (ctx->den_plan) {
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      bdft_plan_destroy(ctx->den_plan);
+      ctx->den_plan = nullptr;
+      ctx->den_key.clear();
+    }
+    std::vector<const bdft_lr*> lrs((size_t)norb, lr);
+    bdft_plan* np = nullptr;
+    int rc0 = make_plan(ctx, norb, lrs.data(), nullptr, nullptr, nullptr, nullptr, psi, nullptr,
+                        nullptr, nullptr, nullptr, precision, true, &np, true);
+    if (rc0) return rc0;
+    ctx->den_plan = np;
+    ctx->den_key = key;
+  }
+  bdft_plan* plan = ctx->den_plan;
+  plan->B.psi = psi;
   DenArgs D{};
   D.nisf = (long long)lr->L[0] * lr->L[1] * lr->L[2];
   D.norb = norb;
@@ -1060,12 +1091,10 @@ extern "C" int bdft_density_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const 
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t nb_h = sizeof(double) * norb, nb_s = sizeof(int32_t) * norb;
   const size_t nb_p = sizeof(double) * (size_t)D.ngroups * nspin * (size_t)D.nisf;
-  DevBuf buf;
-  rc = dev_alloc(ctx, buf, al(nb_h) + al(nb_s) + al(nb_p));
-  if (rc) {
-    bdft_plan_destroy(plan);
-    return rc;
-  }
+  DevBuf& buf = ctx->den_buf;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the buffer may be reallocated
+  int rc = dev_alloc(ctx, buf, al(nb_h) + al(nb_s) + al(nb_p));
+  if (rc) return rc;
   char* q = (char*)buf.p;
   hipError_t e = hipMemcpy(q, hfac, nb_h, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(q + al(nb_h), sp.data(), nb_s, hipMemcpyHostToDevice);
@@ -1078,8 +1107,6 @@ extern "C" int bdft_density_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const 
                        precision == BDFT_PREC_FP32, ctx->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  dev_free(buf);
-  bdft_plan_destroy(plan);
   if (e != hipSuccess)
     return set_err(ctx, BDFT_ERR_HIP, "density: %s", hipGetErrorString(e));
   return BDFT_OK;
@@ -1145,6 +1172,8 @@ static int xfer(bdft_ctx* ctx, const bdft_lr* glr, int norb, const bdft_lr* cons
       return set_err(ctx, BDFT_ERR_ARG, "region %d missing or from another context", i);
     if (L->geocode != 'F')
       return set_err(ctx, BDFT_ERR_UNSUPPORTED, "region %d is not a free-BC region", i);
+    if (7LL * L->nvctr_f > 0x7fffffffLL)
+      return set_err(ctx, BDFT_ERR_UNSUPPORTED, "region %d: too many fine points", i);
     XferDesc& x = d[i];
     x.cmap = L->d_cmap;
     x.fmap = L->d_fmap;
@@ -1169,13 +1198,15 @@ static int xfer(bdft_ctx* ctx, const bdft_lr* glr, int norb, const bdft_lr* cons
   HIPCHK(ctx, hipSetDevice(ctx->device));
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t nb_d = sizeof(XferDesc) * norb;
-  DevBuf buf;
+  DevBuf& buf = ctx->xfer_buf;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the buffer may be reallocated
   int rc = dev_alloc(ctx, buf, al(nb_d) + al(sizeof(int)));
   if (rc) return rc;
   char* q = (char*)buf.p;
   int* miss = (int*)(q + al(nb_d));
   hipError_t e = hipMemcpy(q, d.data(), nb_d, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemsetAsync(miss, 0, sizeof(int), ctx->stream);
+  // only scatters into the global orbital count missing points
+  if (e == hipSuccess && !to_local) e = hipMemsetAsync(miss, 0, sizeof(int), ctx->stream);
   int nmiss = 0;
   if (e == hipSuccess) {
     XferArgs A{};
@@ -1193,9 +1224,9 @@ static int xfer(bdft_ctx* ctx, const bdft_lr* glr, int norb, const bdft_lr* cons
     A.miss = miss;
     e = launch_xfer(A, (int)tiles, ctx->stream);
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(&nmiss, miss, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && !to_local)
+    e = hipMemcpyAsync(&nmiss, miss, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  dev_free(buf);
   if (e != hipSuccess) return set_err(ctx, BDFT_ERR_HIP, "transfer: %s", hipGetErrorString(e));
   if (nmiss > 0)
     return set_err(ctx, BDFT_ERR_DESC,

@@ -306,7 +306,7 @@ struct XferArgs {
   int* miss;            // local points with no global counterpart
 };
 hipError_t launch_xfer(const XferArgs& A, int ntiles, hipStream_t s);
-int xfer_tiles(int nvctr_c, int nvctr_f);
+long long xfer_tiles(int nvctr_c, int nvctr_f);
 
 // conjugate-gradient preconditioner, apply = S (K + c) S (bdft_pass.hip)
 struct PcArgs {

@@ -18,7 +18,13 @@
 
 namespace bdft {
 
-constexpr int NTP = 512;  // threads per plane work-group
+// threads per plane work-group: an FP64 plane fills most of a CU's LDS, so
+// one work-group per CU runs and it takes 12 waves; FP32 planes run two
+// work-groups of 8 waves per CU
+template <typename T>
+struct PlaneNT {
+  static constexpr int v = sizeof(T) == 8 ? 768 : 512;
+};
 
 template <typename T>
 struct PLine {
@@ -143,6 +149,7 @@ __device__ __forceinline__ void shrink4p(PLine<T> ln, int N, int L, int i0, int 
   }
 }
 
+template <int NW>
 __device__ __forceinline__ double block_sum_p(double v, double* red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -152,7 +159,7 @@ __device__ __forceinline__ double block_sum_p(double v, double* red) {
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < NTP / 64; ++i) s += red[i];
+    for (int i = 0; i < NW; ++i) s += red[i];
   return s;
 }
 
@@ -184,7 +191,9 @@ __device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, i
 // forward plane kernel
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NTP) void k_plane_fwd(Batch B, TileTable tt, double* part) {
+__global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable tt,
+                                                             double* part) {
+  constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTP / 64];
   const int tile = blockIdx.x;
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(NTP) void k_plane_fwd(Batch B, TileTable tt, double
         ek += (double)vc * (double)A[a] + (double)vd * (double)A[d];
       }
   }
-  const double eks = block_sum_p(ek, red);
+  const double eks = block_sum_p<NTP / 64>(ek, red);
   if (tid == 0) part[tile] = eks;
   __syncthreads();
   // K plane -> memory (contiguous)
@@ -303,7 +312,8 @@ __global__ __launch_bounds__(NTP) void k_plane_fwd(Batch B, TileTable tt, double
 // backward plane kernel with compress-accumulate
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NTP) void k_plane_bwd(Batch B, TileTable tt) {
+__global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable tt) {
+  constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x;
   const int b = find_box(tt, tile);
@@ -445,13 +455,13 @@ size_t plane_attributes() {
 template <typename T>
 hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                             double* part, hipStream_t s) {
-  k_plane_fwd<T><<<ntiles, NTP, lds, s>>>(B, t, part);
+  k_plane_fwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t, part);
   return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                             hipStream_t s) {
-  k_plane_bwd<T><<<ntiles, NTP, lds, s>>>(B, t);
+  k_plane_bwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t);
   return hipGetLastError();
 }
 template hipError_t launch_plane_fwd<double>(const Batch&, const TileTable&, int, size_t,

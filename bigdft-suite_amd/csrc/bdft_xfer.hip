@@ -3,15 +3,17 @@
 // and Lpsi_to_global2 (locreg_operations.f90:1665-1837).  The reference walks
 // the local segments (shifted into the global box, shift_locreg_indexes,
 // :2321-2380) against the global segments; here every local compressed point
-// is one thread: local grid point (region map) + box offset -> global grid
-// point -> global compressed index (inverse map of the global region), then
-// one gather or scatter of 1 (coarse) or 7 (fine) values.  Same elements,
-// same values, no floating-point arithmetic.
+// is located through the region map (local grid point + box offset -> global
+// grid point -> global compressed index through the inverse map of the
+// global region), and every coefficient is one thread: a coarse point moves
+// one value, a fine point's seven values are moved by seven consecutive
+// threads, so loads and stores of a wave cover contiguous ranges of both
+// orbitals.  Same elements, same values, no floating-point arithmetic.
 #include "bdft_internal.h"
 
 namespace bdft {
 
-constexpr int XT = 256;   // points per tile
+constexpr int XT = 256;   // coefficients per tile
 
 __device__ __forceinline__ int xfer_find(const XferArgs& A, int tile) {
   int lo = 0, hi = A.nd - 1;
@@ -29,8 +31,11 @@ __global__ __launch_bounds__(XT) void k_xfer(XferArgs A) {
   const int lt = tile - D.tile0;
   const int ctiles = (D.nvctr_c + XT - 1) / XT;
   const bool fine = lt >= ctiles;
-  const int k = (fine ? lt - ctiles : lt) * XT + threadIdx.x;
-  if (k >= (fine ? D.nvctr_f : D.nvctr_c)) return;
+  // coefficient e of the coarse or the fine part; fine: point e / 7, wavelet t
+  const int e = (fine ? lt - ctiles : lt) * XT + threadIdx.x;
+  if (e >= (fine ? 7 * D.nvctr_f : D.nvctr_c)) return;
+  const int k = fine ? e / 7 : e;
+  const int t = fine ? e - 7 * k : 0;
   const int lin = fine ? D.fmap[k] : D.cmap[k];
   const int i1 = lin % D.m1, r = lin / D.m1;
   const int i2 = r % D.m2, i3 = r / D.m2;
@@ -41,26 +46,14 @@ __global__ __launch_bounds__(XT) void k_xfer(XferArgs A) {
     const int* inv = fine ? A.gfinv : A.gcinv;
     gi = inv ? inv[g] : g;
   }
-  if (!fine) {
-    if (A.to_local) {
-      A.dst[D.loff + k] = gi >= 0 ? A.src[D.goff + gi] : 0.0;
-    } else if (gi >= 0) {
-      A.dst[D.goff + gi] = A.src[D.loff + k];
-    } else {
-      atomicAdd(A.miss, 1);
-    }
-  } else {
-    const long long l0 = D.loff + D.nvctr_c + 7LL * k;
-    const long long g0 = D.goff + A.gnvctr_c + 7LL * gi;
-    if (A.to_local) {
-#pragma unroll
-      for (int t = 0; t < 7; ++t) A.dst[l0 + t] = gi >= 0 ? A.src[g0 + t] : 0.0;
-    } else if (gi >= 0) {
-#pragma unroll
-      for (int t = 0; t < 7; ++t) A.dst[g0 + t] = A.src[l0 + t];
-    } else {
-      atomicAdd(A.miss, 1);
-    }
+  const long long li = fine ? D.loff + D.nvctr_c + e : D.loff + k;
+  const long long gx = fine ? D.goff + A.gnvctr_c + 7LL * gi + t : D.goff + gi;
+  if (A.to_local) {
+    A.dst[li] = gi >= 0 ? A.src[gx] : 0.0;
+  } else if (gi >= 0) {
+    A.dst[gx] = A.src[li];
+  } else if (t == 0) {   // one count per missing point
+    atomicAdd(A.miss, 1);
   }
 }
 
@@ -69,8 +62,8 @@ hipError_t launch_xfer(const XferArgs& A, int ntiles, hipStream_t s) {
   return hipGetLastError();
 }
 
-int xfer_tiles(int nvctr_c, int nvctr_f) {
-  return (nvctr_c + XT - 1) / XT + (nvctr_f + XT - 1) / XT;
+long long xfer_tiles(int nvctr_c, int nvctr_f) {
+  return ((long long)nvctr_c + XT - 1) / XT + (7LL * nvctr_f + XT - 1) / XT;
 }
 
 }  // namespace bdft

@@ -93,3 +93,22 @@ def test_density_argument_errors(ctx):
     with pytest.raises(bigdft_hip.BigDFTError, match="ARG"):
         ctx.density(lr, psi, [1.0, 1.0], rho, nspin=3)
     lr.close()
+
+
+def test_density_reuses_its_plan_across_calls(ctx):
+    """repeated calls on one region keep the plan: new orbital arrays,
+    another orbital count and another precision in between give the
+    oracle's densities each time"""
+    reg = dense_region("F", (13, 15, 11), (0.45, 0.4, 0.5))
+    lr = ctx.region(reg)
+    ok = okeys(reg)
+    for k, (norb, prec) in enumerate([(4, 0), (4, 0), (2, 0), (4, 1), (4, 0)]):
+        psi = random_psi(reg, norb, 300 + k)
+        hf = np.linspace(0.5, 2.0, norb)
+        ro = oracle.density(reg.geocode, reg.n, reg.nfl, reg.nfu, ok, psi, hf)
+        rho = torch.zeros((1, reg.n_isf), dtype=torch.float64, device="cuda")
+        ctx.density(lr, dev(psi), hf, rho, precision=prec)
+        err = np.abs(rho.cpu().numpy() - ro).max() / np.abs(ro).max()
+        print(f"  call {k}: norb {norb} precision {prec}: {err:.2e}", flush=True)
+        assert err < (1e-12 if prec == 0 else 1e-6)
+    lr.close()
