@@ -552,7 +552,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      const int32_t* ishift, const double* psi, const int64_t* psi_off,
                      double* hpsi, double* ekin, double* epot, int precision,
                      bool own_ws, bdft_plan** out, bool density = false,
-                     const bdft_confpot* conf = nullptr) {
+                     const bdft_confpot* conf = nullptr, bool grow_plane = false) {
   if (!ctx || !out) return BDFT_ERR_ARG;
   *out = nullptr;
   if (norb <= 0 || !lrs || !psi || (!density && (!pot || !hpsi || !ekin || !epot)))
@@ -757,7 +757,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     }
   }
   {
-    bool plane = !ctx->force_lines && !mfma && !density;
+    // density plans (grow_plane) use the forward plane kernel without its
+    // kinetic part; the preconditioner's plans keep the line passes
+    bool plane = !ctx->force_lines && !mfma && (!density || grow_plane);
     size_t lpf = 0, lpb = 0;
     for (int i = 0; i < norb && plane; ++i) {
       const BoxDesc& b = box[i];
@@ -1072,7 +1074,7 @@ extern "C" int bdft_density_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const 
     std::vector<const bdft_lr*> lrs((size_t)norb, lr);
     bdft_plan* np = nullptr;
     int rc0 = make_plan(ctx, norb, lrs.data(), nullptr, nullptr, nullptr, nullptr, psi, nullptr,
-                        nullptr, nullptr, nullptr, precision, true, &np, true);
+                        nullptr, nullptr, nullptr, precision, true, &np, true, nullptr, true);
     if (rc0) return rc0;
     ctx->den_plan = np;
     ctx->den_key = key;

@@ -362,6 +362,9 @@ template <typename T>
 hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                             double* part, hipStream_t s);
 template <typename T>
+hipError_t launch_plane_grow(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                             hipStream_t s);
+template <typename T>
 hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                             hipStream_t s);
 hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,

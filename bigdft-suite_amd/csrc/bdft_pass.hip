@@ -829,9 +829,17 @@ static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs
     k_zero<T><<<(unsigned)(nb < 65536 ? nb : 65536), NT, 0, s>>>((T*)B.W, wtotal);
   }
   if (PT.ntiles[P_UNC] > 0) k_uncompress<T, false><<<PT.ntiles[P_UNC], NT, 0, s>>>(B, PT.t[P_UNC]);
-  if (PT.ntiles[P_XF] > 0)
-    k_xfwd<T><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], PT.part[0]);
-  if (PT.ntiles[P_YF] > 0) k_yfwd<T><<<PT.ntiles[P_YF], NT, lds, s>>>(B, PT.t[P_YF]);
+  if (PT.plane) {
+    // x and y grow of each plane in one pass through LDS
+    if (PT.ntiles[P_PF] > 0) {
+      const hipError_t e = launch_plane_grow<T>(B, PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pf, s);
+      if (e != hipSuccess) return e;
+    }
+  } else {
+    if (PT.ntiles[P_XF] > 0)
+      k_xfwd<T><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], PT.part[0]);
+    if (PT.ntiles[P_YF] > 0) k_yfwd<T><<<PT.ntiles[P_YF], NT, lds, s>>>(B, PT.t[P_YF]);
+  }
   const int nblk = D.ngroups * D.nbund;
   if (D.nspin == 2) k_zden<T, 2><<<nblk, NT, lds, s>>>(B, D);
   else k_zden<T, 1><<<nblk, NT, lds, s>>>(B, D);

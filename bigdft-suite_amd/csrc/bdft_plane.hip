@@ -69,8 +69,9 @@ __device__ __forceinline__ void grow8p(PLine<T> ln, int N, int k0, int per, T o[
 }
 
 // wavelet-basis kinetic (unscaled) on a wavelet line: 4 coarse outputs
-// i0..i0+3 (c and d) from k = i0-14 .. i0+17.
-template <typename T, bool INTERIOR>
+// i0..i0+3 (c and d) from k = i0-14 .. i0+17.  WRAP1: periodic line with
+// N >= 18, where one conditional shift wraps every k of the stencil.
+template <typename T, bool INTERIOR, bool WRAP1 = false>
 __device__ __forceinline__ void kin4p(PLine<T> ln, int N, int i0, int per, T oc[4],
                                       T od[4]) {
 #pragma unroll
@@ -85,6 +86,10 @@ __device__ __forceinline__ void kin4p(PLine<T> ln, int N, int i0, int per, T oc[
     if (INTERIOR) {
       c = ln.p[k * ln.es];
       d = ln.p[(N + k) * ln.es];
+    } else if (WRAP1) {
+      const int kw = k < 0 ? k + N : (k >= N ? k - N : k);
+      c = ln.p[kw * ln.es];
+      d = ln.p[(N + kw) * ln.es];
     } else if (per) {
       int kw = k;
       while (kw < 0) kw += N;
@@ -177,6 +182,7 @@ template <typename T>
 __device__ __forceinline__ void kin4_any(PLine<T> ln, int N, int i0, int per, T oc[4],
                                          T od[4]) {
   if (i0 - 14 >= 0 && i0 + 17 < N) kin4p<T, true>(ln, N, i0, per, oc, od);
+  else if (per && N >= 18) kin4p<T, false, true>(ln, N, i0, per, oc, od);
   else kin4p<T, false>(ln, N, i0, per, oc, od);
 }
 template <typename T>
@@ -188,9 +194,10 @@ __device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, i
 }
 
 // ---------------------------------------------------------------------------
-// forward plane kernel
+// forward plane kernel; KIN = false: T2 only (the density's x and y grow,
+// daub_to_isf in sumrho.f90:377-469), no kinetic, no K, no ekin
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool KIN = true>
 __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable tt,
                                                              double* part) {
   constexpr int NTP = PlaneNT<T>::v;
@@ -232,6 +239,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
     }
   }
   __syncthreads();
+  if constexpr (KIN) {
   // K_x: rows j2, chunks of 4 coarse x positions -> C
   const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1];
   const int cx = (N1 + 3) >> 2;
@@ -282,6 +290,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
     }
   }
   __syncthreads();
+  }
   // G_x: rows j2 -> C as (W2 rows) x (L1 cols), row stride RB
   const int gx = (L1 + 7) >> 3;
   for (int it = tid; it < W2 * gx; it += NTP) {
@@ -432,6 +441,9 @@ hipError_t plane_setup(int bytes) {
   hipError_t e = hipFuncSetAttribute((const void*)k_plane_fwd<T>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_plane_fwd<T, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_plane_bwd<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   return e;
@@ -459,6 +471,12 @@ hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size
   return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_plane_grow(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                             hipStream_t s) {
+  k_plane_fwd<T, false><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t, nullptr);
+  return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                             hipStream_t s) {
   k_plane_bwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t);
@@ -468,6 +486,10 @@ template hipError_t launch_plane_fwd<double>(const Batch&, const TileTable&, int
                                              double*, hipStream_t);
 template hipError_t launch_plane_fwd<float>(const Batch&, const TileTable&, int, size_t,
                                             double*, hipStream_t);
+template hipError_t launch_plane_grow<double>(const Batch&, const TileTable&, int, size_t,
+                                              hipStream_t);
+template hipError_t launch_plane_grow<float>(const Batch&, const TileTable&, int, size_t,
+                                             hipStream_t);
 template hipError_t launch_plane_bwd<double>(const Batch&, const TileTable&, int, size_t,
                                              hipStream_t);
 template hipError_t launch_plane_bwd<float>(const Batch&, const TileTable&, int, size_t,

@@ -112,3 +112,34 @@ def test_density_reuses_its_plan_across_calls(ctx):
         print(f"  call {k}: norb {norb} precision {prec}: {err:.2e}", flush=True)
         assert err < (1e-12 if prec == 0 else 1e-6)
     lr.close()
+
+
+@pytest.fixture(scope="module")
+def ctx_lines():
+    import os
+
+    import bigdft_hip
+    os.environ["BDFT_LINE_PASSES"] = "1"
+    try:
+        c = bigdft_hip.Context(0)
+    finally:
+        del os.environ["BDFT_LINE_PASSES"]
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name,reg", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_density_line_and_plane_grow_agree(ctx, ctx_lines, name, reg):
+    """the plane grow (default) and the x/y line passes give the oracle's
+    density on the same orbitals"""
+    psi = random_psi(reg, 3, 91)
+    hf = np.array([2.0, 1.5, 0.5])
+    ro = oracle.density(reg.geocode, reg.n, reg.nfl, reg.nfu, okeys(reg), psi, hf)
+    for c in (ctx, ctx_lines):
+        lr = c.region(reg)
+        rho = torch.zeros((1, reg.n_isf), dtype=torch.float64, device="cuda")
+        c.density(lr, dev(psi), hf, rho)
+        lr.close()
+        err = np.abs(rho.cpu().numpy() - ro).max() / np.abs(ro).max()
+        print(f"  {name}: {err:.2e}", flush=True)
+        assert err < 1e-12
