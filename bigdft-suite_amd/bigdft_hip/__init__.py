@@ -157,12 +157,29 @@ class LocReg:
             pass
 
 
+def _stream_ordered(method):
+    """Device-array entry points run on the context's stream: it first waits
+    for the work already queued on torch's current stream (tensors the
+    caller just allocated or filled), and torch's current stream then waits
+    for the call, so results read through torch see it complete."""
+    def wrapped(self, *args, **kwargs):
+        ctx = self if isinstance(self, Context) else self.ctx
+        ctx._enter()
+        try:
+            return method(self, *args, **kwargs)
+        finally:
+            ctx._leave()
+    wrapped.__name__, wrapped.__doc__ = method.__name__, method.__doc__
+    return wrapped
+
+
 class Plan:
     """A captured batch (bdft_plan); execute() replays its hipGraph."""
 
     def __init__(self, ctx, handle, keep):
         self.ctx, self.handle, self._keep = ctx, handle, keep
 
+    @_stream_ordered
     def execute(self):
         self.ctx._check(lib().bdft_plan_execute(self.handle))
 
@@ -187,6 +204,34 @@ class Context:
         if rc:
             raise BigDFTError(rc, f"cannot create a context on device {device}")
         self.handle = h
+        self.device = device
+        # the torch stream the context runs on (default priority), so that
+        # its work can be ordered against torch's
+        import torch
+        self._stream = torch.cuda.Stream(device=device)
+        self._check(lib().bdft_ctx_set_stream(self.handle,
+                                              ctypes.c_void_p(self._stream.cuda_stream)))
+
+    def _events(self):
+        import torch
+        if getattr(self, "_ev", None) is None:   # two events, reused by every call
+            with torch.cuda.device(self.device):
+                self._ev = (torch.cuda.Event(), torch.cuda.Event())
+        return torch, self._ev
+
+    def _enter(self):
+        torch, ev = self._events()
+        cur = torch.cuda.current_stream(self.device)
+        if cur != self._stream:
+            ev[0].record(cur)
+            self._stream.wait_event(ev[0])
+
+    def _leave(self):
+        torch, ev = self._events()
+        cur = torch.cuda.current_stream(self.device)
+        if cur != self._stream:
+            ev[1].record(self._stream)
+            cur.wait_event(ev[1])
 
     def _check(self, rc):
         if rc:
@@ -197,9 +242,13 @@ class Context:
         return LocReg(self, region)
 
     def set_stream(self, stream):
-        """Run on a torch/HIP stream (object with .cuda_stream) or None."""
-        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
-        self._check(lib().bdft_ctx_set_stream(self.handle, s))
+        """Run on a torch stream (None: a new stream of the context's own)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.Stream(device=self.device)
+        self._check(lib().bdft_ctx_set_stream(self.handle,
+                                              ctypes.c_void_p(stream.cuda_stream)))
+        self._stream = stream
 
     def sync(self):
         self._check(lib().bdft_sync(self.handle))
@@ -229,6 +278,7 @@ class Context:
         return out, cnt.value
 
     # ---- device-pointer entry points (torch tensors) ----
+    @_stream_ordered
     def locham(self, lr, pot, psi, hpsi, ekin, epot, precision=PREC_FP64):
         norb = ekin.numel()
         self._check(lib().bdft_locham_d(self.handle, lr.handle, norb,
@@ -244,6 +294,7 @@ class Context:
         return (arr, a(pot_off, np.int64), a(potdim, np.int32),
                 a(ishift, np.int32), a(psi_off, np.int64))
 
+    @_stream_ordered
     def locham_batch(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None,
                      potdim=None, ishift=None, psi_off=None,
                      precision=PREC_FP64, conf=None):
@@ -264,6 +315,7 @@ class Context:
             _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
             _ptr(epot), _ptr(c), precision))
 
+    @_stream_ordered
     def density(self, lr, psi, hfac, rho, spin=None, nspin=1, precision=PREC_FP64):
         """rho[s] += hfac[i] psir_i^2 over the orbitals (rows of the device
         array psi) of spin s; hfac, spin: host sequences; rho: device,
@@ -277,6 +329,7 @@ class Context:
             None if sp is None else sp.ctypes.data_as(ctypes.c_void_p),
             nspin, _ptr(rho), precision))
 
+    @_stream_ordered
     def _xfer(self, fn, glr, llrs, ns, src, soff, dst, doff):
         arr = (ctypes.c_void_p * len(llrs))(*[l.handle.value for l in llrs])
         nsa = np.ascontiguousarray(ns, dtype=np.int32).ravel()
@@ -293,6 +346,7 @@ class Context:
         """Lpsi_to_global2: local regions -> their points of global orbitals"""
         self._xfer(lib().bdft_locreg_to_psi_d, glr, llrs, ns, lpsi, loff, gpsi, goff)
 
+    @_stream_ordered
     def precond_cg(self, lrs, cprecr, ncong, x, b, psi_off=None):
         """ocl_preconditioner_generic_k over a batch: x (device, in/out) <-
         S y after ncong CG steps on S (T + c) S y = b from the guess x"""
@@ -321,12 +375,17 @@ class Context:
         off, _ = bounds_layout(r.n, r.nfl, r.nfu)
         if out is not None:
             assert out.dtype == torch_int32() and out.numel() >= off[17]
-            self._check(lib().bdft_locreg_bounds_d(self.handle, lr.handle, _ptr(out)))
+            self._enter()
+            try:
+                self._check(lib().bdft_locreg_bounds_d(self.handle, lr.handle, _ptr(out)))
+            finally:
+                self._leave()
             return out
         buf = np.zeros(int(off[17]), dtype=np.int32)
         self._check(lib().bdft_locreg_bounds(self.handle, lr.handle, _ptr(buf)))
         return buf
 
+    @_stream_ordered
     def plan(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None, potdim=None,
              ishift=None, psi_off=None, precision=PREC_FP64):
         arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
@@ -338,10 +397,12 @@ class Context:
             _ptr(epot), precision, ctypes.byref(h)))
         return Plan(self, h, (lrs, pot, psi, hpsi, ekin, epot))
 
+    @_stream_ordered
     def uncompress(self, lr, norb, psi, psig):
         self._check(lib().bdft_uncompress_d(self.handle, lr.handle, norb,
                                             _ptr(psi), _ptr(psig)))
 
+    @_stream_ordered
     def compress_acc(self, lr, norb, psig, hpsi):
         self._check(lib().bdft_compress_acc_d(self.handle, lr.handle, norb,
                                               _ptr(psig), _ptr(hpsi)))

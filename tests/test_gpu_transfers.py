@@ -74,3 +74,46 @@ def test_locreg_to_psi_rejects_points_outside(ctx):
     assert not torch.isnan(out).any()
     l.close()
     g.close()
+
+
+def test_transfers_sphere_batch_round_trip(ctx):
+    """48 compressed spheres of a 64^3 free global box (tiles of several
+    thousand coefficients, coarse and fine parts of many sizes): local
+    orbitals equal the oracle's psi_to_locreg2, and writing them back into
+    a zero global orbital restores exactly the points each sphere holds"""
+    from bigdft_hip import workloads
+    from bigdft_hip.descriptors import dense_region
+    sb = workloads.sphere_batch(48, 5, n_glob=63, device="cuda")
+    regs = sb["regions"]
+    ng = (63, 63, 63)
+    glob = dense_region("F", ng, (0.4,) * 3)
+    g = ctx.region(glob)
+    cache = {}
+    lrs = [cache.setdefault(id(r), ctx.region(r)) for r in regs]
+    ns = np.array(sb["nl"], dtype=np.int32).reshape(-1, 3)
+    rng = np.random.default_rng(8)
+    gh = rng.standard_normal((len(regs), glob.ncoef))
+    goff = np.arange(len(regs), dtype=np.int64) * glob.ncoef
+    loff = sb["psi_off"]
+    lp = torch.full_like(sb["psi"], np.nan)
+    ctx.psi_to_locreg(g, lrs, ns, dev(gh.ravel()), lp, goff=goff, loff=loff)
+    lh = lp.cpu().numpy()
+    gk = okeys(glob)
+    for i in rng.choice(len(regs), 5, replace=False):
+        r = regs[i]
+        exp = oracle.psi_to_locreg2(r.n, ng, sb["nl"][i], okeys(r), gk, gh[i])
+        assert np.array_equal(lh[loff[i]:loff[i] + r.ncoef], exp)
+    back = torch.zeros(len(regs) * glob.ncoef, dtype=torch.float64, device="cuda")
+    ctx.locreg_to_psi(g, lrs, ns, lp, back, loff=loff, goff=goff)
+    bh = back.cpu().numpy().reshape(len(regs), -1)
+    for i in range(len(regs)):
+        nz = bh[i] != 0.0
+        assert np.array_equal(bh[i][nz], gh[i][nz])
+        assert nz.sum() == regs[i].ncoef
+    # the same batch again (descriptors reused) on other values
+    lp2 = torch.full_like(sb["psi"], np.nan)
+    ctx.psi_to_locreg(g, lrs, ns, dev(2.0 * gh.ravel()), lp2, goff=goff, loff=loff)
+    assert np.array_equal(lp2.cpu().numpy(), 2.0 * lh)
+    for l in cache.values():
+        l.close()
+    g.close()
