@@ -321,6 +321,8 @@ def bench_transfers(args):
     ctx = bigdft_hip.Context(0)
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
+    torch.cuda.synchronize()
+    torch.cuda.set_stream(stream)   # one stream for torch and the library
     g = ctx.region(glob)
     cache = {}
     lrs = [cache.setdefault(id(r), ctx.region(r)) for r in sb["regions"]]
@@ -445,6 +447,8 @@ def main():
     ctx = bigdft_hip.Context(0)
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
+    torch.cuda.synchronize()
+    torch.cuda.set_stream(stream)   # one stream for torch and the library
     lr_of = {}
     for r in regions:
         if id(r) not in lr_of:
