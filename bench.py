@@ -408,6 +408,9 @@ def parse():
                          "localisation-region copies of compressed orbitals")
     ap.add_argument("--ncong", type=int, default=6,
                     help="CG steps of the preconditioner workload")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the automatic launch options instead of timing the "
+                         "candidates on the batch first (bigdft_hip.autotune)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (the "
                          "known-size kernel that scales profiler byte counters)")
@@ -464,9 +467,22 @@ def main():
         ctx.close()
         return
 
+    tune_cache = None
+    if args.autotune:
+        import tempfile
+
+        from bigdft_hip import autotune
+        tune_cache = autotune.TuneCache(os.path.join(tempfile.mkdtemp(), "tune.json"))
+    chosen = {}
+
     def timed(prec):
-        """graph-replay timing, then per-stage HIP-event timing, then one
+        """launch options (autotuned on this batch unless --no-autotune),
+        graph-replay timing, then per-stage HIP-event timing, then one
         clean application for the error analysis"""
+        if tune_cache is not None:
+            _, opts, _ = autotune.tune(ctx, lrs, batch["pot"], psi, prec, cache=tune_cache,
+                                       **bargs)
+            chosen[prec] = opts
         h = torch.zeros_like(psi)
         ek = torch.zeros(norb, dtype=torch.float64, device=dev)
         ep = torch.zeros_like(ek)
@@ -497,6 +513,12 @@ def main():
         return ms, wall, {k: v / nl for k, v in stages.items()}, h, ek, ep
 
     _, plane_enabled = ctx.info()
+
+    def plane_seq(prec):
+        """plane-fused sequence allowed for the run of this precision"""
+        o = chosen.get(prec)
+        return plane_enabled if o is None else o.get("line_passes", 0) == 0
+
     ms64, wall64, st64, h64, ek64, ep64 = timed(bigdft_hip.PREC_FP64)
     hmax = h64.abs().max()
 
@@ -524,8 +546,9 @@ def main():
             err = errors(hm, ekm, epm)
             del hm
         variants[name] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
+                          "launch_options": chosen.get(bigdft_hip.PRECISIONS[name]),
                           "wall_s": round(wall, 4), "path":
-                          path_of(name, plane_enabled, avg),
+                          path_of(name, plane_seq(bigdft_hip.PRECISIONS[name]), avg),
                           "stages_ms": {k: round(v, 4) for k, v in avg.items()},
                           "error_vs_fp64": err}
     head = variants[args.precision]
@@ -595,7 +618,8 @@ def main():
         "data": "synthetic (seeded N(0,1) coefficients, 8-Gaussian potential)",
         "config": {"workload": cfg["workload"], "orbitals": norb,
                    "grid_pts": units, "precision": args.precision,
-                   "graph": True, "wall_s": wall},
+                   "graph": True, "launch_options": head["launch_options"],
+                   "wall_s": wall},
         "roofline": roof,
         "speedup_vs_fp64": ms64 / ms_step,
         "error_vs_fp64": head["error_vs_fp64"],

@@ -22,6 +22,8 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4, PREC_F16X3 = 0, 1, 2, 3, 4
+# launch options (BDFT_OPT_* of bigdft_hip.h)
+OPTIONS = {"line_passes": 1, "zgroup": 2}
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
               "bf16x4": PREC_BF16X4, "f16x3": PREC_F16X3}
 
@@ -38,7 +40,8 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_stage_name", "bdft_ctx_info", "bdft_sync", "bdft_density_d",
            "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d",
            "bdft_bounds_layout", "bdft_locreg_bounds_d", "bdft_locreg_bounds",
-           "bdft_locham_conf_batch_d", "bdft_precond_cg_d", "bdft_precondition")
+           "bdft_locham_conf_batch_d", "bdft_precond_cg_d", "bdft_precondition",
+           "bdft_ctx_set_option", "bdft_ctx_get_option")
 
 
 class BigDFTError(RuntimeError):
@@ -82,6 +85,8 @@ def load_library(path=LIB_PATH):
         "bdft_profile_enable": [P, I],
         "bdft_profile_read": [P, I, P, P, I],
         "bdft_ctx_info": [P, P, P],
+        "bdft_ctx_set_option": [P, I, I],
+        "bdft_ctx_get_option": [P, I, P],
         "bdft_bounds_layout": [I3, I3, I3, P, P],
         "bdft_precond_cg_d": [P, I, P, P, P, I, P, P],
         "bdft_precondition": [P, P, I, P, I, P, P],
@@ -259,6 +264,16 @@ class Context:
         self._check(lib().bdft_ctx_info(self.handle, ctypes.byref(lds),
                                         ctypes.byref(pl)))
         return lds.value, bool(pl.value)
+
+    def set_option(self, name, value):
+        """launch option (OPTIONS: "line_passes" 0/1, "zgroup" 0/2/3); cached
+        plans of the context are dropped when a value changes"""
+        self._check(lib().bdft_ctx_set_option(self.handle, OPTIONS[name], int(value)))
+
+    def get_option(self, name):
+        v = ctypes.c_int()
+        self._check(lib().bdft_ctx_get_option(self.handle, OPTIONS[name], ctypes.byref(v)))
+        return v.value
 
     def profile(self, on=True):
         self._check(lib().bdft_profile_enable(self.handle, int(on)))

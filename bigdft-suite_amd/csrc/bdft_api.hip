@@ -51,6 +51,7 @@ struct bdft_ctx {
   double prof_ms[S_COUNT] = {0};
   long long prof_n = 0;
   bool force_lines = false;  // disable the plane-fused sequence
+  int opt_zg = 0;            // z-line bundles per work-group (0: automatic)
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
 };
 
@@ -639,7 +640,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       mx = std::max(mx, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp, ZGROUP));
       mx = std::max(mx, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp, ZGROUP));
     }
-    if (mx > ctx->plane_lds / 2) zg = 2;
+    if (ctx->opt_zg == 2 || (ctx->opt_zg == 0 && mx > ctx->plane_lds / 2) ||
+        mx > ctx->plane_lds)
+      zg = 2;
   }
   // tile tables
   std::vector<int> starts;
@@ -1613,6 +1616,51 @@ extern "C" int bdft_profile_read(bdft_ctx* ctx, int nstage, double* ms, int64_t*
     ctx->prof_n = 0;
   }
   return BDFT_OK;
+}
+
+static void drop_cached_plans(bdft_ctx* ctx) {
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->cached) bdft_plan_destroy(ctx->cached);
+  if (ctx->pc_plan) bdft_plan_destroy(ctx->pc_plan);
+  if (ctx->den_plan) bdft_plan_destroy(ctx->den_plan);
+  ctx->cached = ctx->pc_plan = ctx->den_plan = nullptr;
+  ctx->cached_key.clear();
+  ctx->pc_key.clear();
+  ctx->den_key.clear();
+}
+
+extern "C" int bdft_ctx_set_option(bdft_ctx* ctx, int option, int value) {
+  if (!ctx) return BDFT_ERR_ARG;
+  switch (option) {
+    case BDFT_OPT_LINE_PASSES:
+      if (value != 0 && value != 1)
+        return set_err(ctx, BDFT_ERR_ARG, "line passes: 0 or 1 (got %d)", value);
+      if (ctx->force_lines != (value == 1)) {
+        drop_cached_plans(ctx);
+        ctx->force_lines = value == 1;
+      }
+      return BDFT_OK;
+    case BDFT_OPT_ZGROUP:
+      if (value != 0 && value != 2 && value != 3)
+        return set_err(ctx, BDFT_ERR_ARG, "z-line bundles per work-group: 0, 2 or 3 (got %d)",
+                       value);
+      if (ctx->opt_zg != value) {
+        drop_cached_plans(ctx);
+        ctx->opt_zg = value;
+      }
+      return BDFT_OK;
+  }
+  return set_err(ctx, BDFT_ERR_ARG, "unknown option %d", option);
+}
+
+extern "C" int bdft_ctx_get_option(const bdft_ctx* ctx, int option, int* value) {
+  if (!ctx || !value) return BDFT_ERR_ARG;
+  switch (option) {
+    case BDFT_OPT_LINE_PASSES: *value = ctx->force_lines ? 1 : 0; return BDFT_OK;
+    case BDFT_OPT_ZGROUP: *value = ctx->opt_zg; return BDFT_OK;
+  }
+  return BDFT_ERR_ARG;
 }
 
 extern "C" int bdft_ctx_info(const bdft_ctx* ctx, int64_t* lds_per_block, int* plane_enabled) {

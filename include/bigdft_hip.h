@@ -279,6 +279,23 @@ const char *bdft_stage_name(int stage);
 int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
                   int *plane_enabled);
 
+/* Launch options of the context.  No counterpart in the reference (its
+ * OpenCL kernels have fixed work-group shapes); these are the choices the
+ * autotuner (bigdft_hip/autotune.py) times per box shape and caches.
+ *   BDFT_OPT_LINE_PASSES  0: plane-fused sequences where they fit (default),
+ *                         1: line passes (FP64/FP32) and per-pass
+ *                            matrix-core kernels for every batch
+ *   BDFT_OPT_ZGROUP       matrix-core z-line bundles of 32 lines per
+ *                         work-group: 0 automatic (3 unless the z kernels
+ *                         would then take more than half a CU's LDS), 2, 3
+ *                         (3 falls back to 2 where it does not fit)
+ * A change drops the context's cached plans; plans already created by
+ * bdft_plan_create keep the options they were made with. */
+#define BDFT_OPT_LINE_PASSES 1
+#define BDFT_OPT_ZGROUP 2
+int bdft_ctx_set_option(bdft_ctx *ctx, int option, int value);
+int bdft_ctx_get_option(const bdft_ctx *ctx, int option, int *value);
+
 /* Wait for all work of the context.  Replaces ocl_finish in
  * finish_hamiltonian_OCL (interface_ocl.f90:599). */
 int bdft_sync(bdft_ctx *ctx);

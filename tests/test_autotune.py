@@ -1,0 +1,73 @@
+"""Autotuner host logic (no GPU): cache keys, candidate lists, the JSON
+table, and the choice of the fastest candidate with a stand-in timer."""
+import json
+
+import pytest
+
+import bigdft_hip
+from bigdft_hip import autotune
+from bigdft_hip.descriptors import dense_region
+
+
+class FakeCtx:
+    def __init__(self):
+        self.opts = {"line_passes": 0, "zgroup": 0}
+        self.history = []
+
+    def set_option(self, k, v):
+        assert k in bigdft_hip.OPTIONS
+        self.opts[k] = v
+        self.history.append((k, v))
+
+
+class FakeLr:
+    def __init__(self, region):
+        self.region = region
+
+
+def test_shape_keys():
+    a = dense_region("P", (47, 47, 47), (0.35,) * 3)
+    b = dense_region("F", (20, 31, 17), (0.4,) * 3)
+    c = dense_region("F", (25, 18, 19), (0.4,) * 3)
+    assert autotune.shape_key([a] * 4, bigdft_hip.PREC_F16X3) == "P:47x47x47:f16x3"
+    assert autotune.shape_key([b, c], bigdft_hip.PREC_FP64) == "F:max25x31x19:mixed:fp64"
+    assert autotune.shape_key([a, b], bigdft_hip.PREC_FP32).startswith("mixed:max47x47x47")
+
+
+def test_candidates():
+    assert len(autotune.candidates(bigdft_hip.PREC_FP64)) == 2
+    c = autotune.candidates(bigdft_hip.PREC_F16X3)
+    assert len(c) == 6 and c[0] == {"line_passes": 0, "zgroup": 0}
+    assert {x["zgroup"] for x in c} == {0, 2, 3}
+
+
+def test_tune_picks_the_fastest_and_caches(tmp_path):
+    path = tmp_path / "t.json"
+    reg = dense_region("P", (23, 23, 23), (0.45,) * 3)
+    lrs = [FakeLr(reg)] * 3
+    cost = {(0, 0): 5.0, (0, 2): 4.0, (0, 3): 4.5, (1, 0): 9.0, (1, 2): 8.0, (1, 3): 8.5}
+
+    def timer(ctx, lrs_, pot, psi, prec, kw):
+        return cost[(ctx.opts["line_passes"], ctx.opts["zgroup"])]
+    ctx = FakeCtx()
+    key, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_BF16X4,
+                                     cache=autotune.TuneCache(str(path)), timer=timer)
+    assert key == "P:23x23x23:bf16x4" and opts == {"line_passes": 0, "zgroup": 2}
+    assert ctx.opts == opts and len(times) == 6
+    table = json.loads(path.read_text())
+    assert table[key]["options"] == opts and min(table[key]["ms"].values()) == 4.0
+    # a second tuner reads the table and times nothing
+    ctx2 = FakeCtx()
+
+    def no_timer(*a):
+        raise AssertionError("the cache should answer")
+    _, opts2, times2 = autotune.tune(ctx2, lrs, None, None, bigdft_hip.PREC_BF16X4,
+                                     cache=autotune.TuneCache(str(path)), timer=no_timer)
+    assert opts2 == opts and times2 is None and ctx2.opts == opts
+
+
+def test_bad_cache_file(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text("[1, 2]")
+    with pytest.raises(ValueError):
+        autotune.TuneCache(str(p))
