@@ -52,6 +52,12 @@ program locham_hip_driver
 
   call init_acceleration_hip(0, ctx, ierr)
   if (ierr /= 0) stop 'context creation failed'
+  ! optional 4th argument 1: line passes for every batch (a launch option)
+  if (command_argument_count() >= 4) then
+     call get_command_argument(4, arg); read(arg, *) i
+     ierr = bdft_ctx_set_option(ctx, BDFT_OPT_LINE_PASSES, int(i, c_int))
+     if (ierr /= 0) stop 'option not accepted'
+  end if
   call allocate_data_hip(ctx, 'P', n, n, n, [0, 0, 0], [n, n, n], &
        [0.45d0, 0.45d0, 0.45d0], nseg, nvctr, nseg, nvctr, keyg, keyv, lr, ierr)
   if (ierr /= 0) then

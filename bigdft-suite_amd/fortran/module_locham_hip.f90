@@ -12,10 +12,14 @@ module module_locham_hip
   integer, parameter, public :: BDFT_PREC_FP64 = 0, BDFT_PREC_FP32 = 1, &
        BDFT_PREC_BF16X3 = 2, BDFT_PREC_BF16X4 = 3, BDFT_PREC_F16X3 = 4
 
+  !> launch options (BDFT_OPT_* of bigdft_hip.h), e.g. a choice cached by
+  !! the autotuner
+  integer, parameter, public :: BDFT_OPT_LINE_PASSES = 1, BDFT_OPT_ZGROUP = 2
+
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
        local_partial_density_hip, free_gpu_hip, bdft_error_message, &
        bdft_bounds_layout, bdft_locreg_bounds, bdft_locham_conf_batch_d, &
-       bdft_precondition
+       bdft_precondition, bdft_ctx_set_option
 
   !> confpot_data (locregs/locreg_operations.f90:23-30) as the C struct
   !! bdft_confpot
@@ -38,6 +42,12 @@ module module_locham_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function bdft_ctx_destroy
+     integer(c_int) function bdft_ctx_set_option(ctx, option, value) &
+          bind(C, name='bdft_ctx_set_option')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: option, value
+     end function bdft_ctx_set_option
      type(c_ptr) function bdft_last_error(ctx) bind(C, name='bdft_last_error')
        import :: c_ptr
        type(c_ptr), value :: ctx

@@ -16,9 +16,9 @@ DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no amdflang)")
-@pytest.mark.parametrize("n,norb", [(11, 3), (23, 2)])
-def test_fortran_driver_matches_oracle(n, norb):
-    r = subprocess.run([DRIVER, str(n), str(norb), "0"], capture_output=True,
+@pytest.mark.parametrize("n,norb,lines", [(11, 3, 0), (23, 2, 0), (23, 2, 1)])
+def test_fortran_driver_matches_oracle(n, norb, lines):
+    r = subprocess.run([DRIVER, str(n), str(norb), "0", str(lines)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     vals = {l.split()[0]: float(l.split()[1]) for l in r.stdout.splitlines() if l.strip()}
