@@ -52,6 +52,7 @@ struct bdft_ctx {
   long long prof_n = 0;
   bool force_lines = false;  // disable the plane-fused sequence
   int opt_zg = 0;            // z-line bundles per work-group (0: automatic)
+  int opt_zpot = 0;          // matrix-core z potential pass: 0 auto, 1 MFMA, 2 FP32
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
 };
 
@@ -700,6 +701,14 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     // plane-fused sequence when both plane kernels fit; the per-pass one
     // otherwise (or when line passes are forced)
     ml.fused = !ctx->force_lines && std::max(ml.fwd, ml.bwd) <= ctx->plane_lds;
+    // z potential pass of the fused sequence: the FP32 line kernel where it
+    // applies (every extent within the line kernels' range), unless the
+    // matrix cores are asked for
+    bool line_ok = true;
+    for (int i = 0; i < norb; ++i)
+      for (int d = 0; d < 3; ++d) line_ok = line_ok && box[i].N[d] <= MAXN;
+    ml.zvalu = ml.fused && line_ok && ctx->opt_zpot != 1;
+    ml.maxN = maxN;
     size_t mx = std::max(ml.kin[2], ml.zpot);
     if (ml.fused)
       mx = std::max(std::max(ml.kz, ml.zp), std::max(ml.fwd, ml.bwd));
@@ -1641,6 +1650,14 @@ extern "C" int bdft_ctx_set_option(bdft_ctx* ctx, int option, int value) {
         ctx->force_lines = value == 1;
       }
       return BDFT_OK;
+    case BDFT_OPT_ZPOT:
+      if (value < 0 || value > 2)
+        return set_err(ctx, BDFT_ERR_ARG, "z potential pass: 0, 1 or 2 (got %d)", value);
+      if (ctx->opt_zpot != value) {
+        drop_cached_plans(ctx);
+        ctx->opt_zpot = value;
+      }
+      return BDFT_OK;
     case BDFT_OPT_ZGROUP:
       if (value != 0 && value != 2 && value != 3)
         return set_err(ctx, BDFT_ERR_ARG, "z-line bundles per work-group: 0, 2 or 3 (got %d)",
@@ -1659,6 +1676,7 @@ extern "C" int bdft_ctx_get_option(const bdft_ctx* ctx, int option, int* value) 
   switch (option) {
     case BDFT_OPT_LINE_PASSES: *value = ctx->force_lines ? 1 : 0; return BDFT_OK;
     case BDFT_OPT_ZGROUP: *value = ctx->opt_zg; return BDFT_OK;
+    case BDFT_OPT_ZPOT: *value = ctx->opt_zpot; return BDFT_OK;
   }
   return BDFT_ERR_ARG;
 }

@@ -268,6 +268,8 @@ struct MfmaLds {
   int fused;                          // 1: plane-fused sequence
   int zg;                             // bundles per z-line tile (2 or 3)
   size_t fwd2, bwd2;                  // plane kernels of the large-box launches
+  int zvalu;                          // 1: z potential pass by the FP32 line kernel
+  int maxN;                           // largest coarse extent (line-kernel LDS)
 };
 
 // density: orbitals sharing one box are split into ngroups groups of gsize;
@@ -352,6 +354,10 @@ size_t mfma_lds_kz(int W3, int Kp, int zg);
 size_t mfma_lds_zp(int KpG, int DpG, int zg);
 size_t mfma_lds_bwd(int W1, int L1, int L2);
 size_t mfma_attributes(size_t lds);
+// the FP32 z-line potential pass (k_zpot<float>) over P_ZP tiles, in place
+// on T2, epot partials per tile
+hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
+                           double* part, hipStream_t s);
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
                        long long wtotal, bool need_zero, int precision, hipStream_t s,
                        hipEvent_t* ev);

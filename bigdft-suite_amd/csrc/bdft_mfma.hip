@@ -1406,7 +1406,13 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     }
     mark(S_KZ, 1);
     mark(S_ZP, 0);
-    if (PT.ntiles[P_ZPG] > 0) {
+    if (L.zvalu) {
+      // G_z, V, G_z^T in FP32 on the vector units (the pass where they beat
+      // the split-precision matrix-core form); T2 is unscaled FP32 in both
+      const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
+                                           PT.part[3], s);
+      if (e != hipSuccess) return e;
+    } else if (PT.ntiles[P_ZPG] > 0) {
       if (L.zg == 2)
         k_mzp<FM, BM, 2><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
       else
@@ -1420,7 +1426,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     mark(S_PB, 1);
     et.t[0] = PT.t[P_KZG];
     et.p[0] = PT.part[2];
-    et.t[1] = PT.t[P_ZPG];
+    et.t[1] = L.zvalu ? PT.t[P_ZP] : PT.t[P_ZPG];
     et.p[1] = PT.part[3];
     et.nk = 1;
     et.n = 2;

@@ -849,6 +849,12 @@ static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs
   return hipGetLastError();
 }
 
+hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
+                           double* part, hipStream_t s) {
+  if (ntiles > 0) k_zpot<float><<<ntiles, NT, lds_bytes(maxN, sizeof(float)), s>>>(B, t, part);
+  return hipGetLastError();
+}
+
 hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
                           long long wtotal, bool need_zero, int fp32, hipStream_t s) {
   return fp32 ? launch_den<float>(B, T, D, maxN, wtotal, need_zero, s)

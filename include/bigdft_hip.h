@@ -289,10 +289,17 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
  *                         work-group: 0 automatic (3 unless the z kernels
  *                         would then take more than half a CU's LDS), 2, 3
  *                         (3 falls back to 2 where it does not fit)
+ *   BDFT_OPT_ZPOT         z potential pass (G_z, V, G_z^T) of the
+ *                         matrix-core precisions: 0 automatic (the FP32
+ *                         vector-unit line kernel where every box extent is
+ *                         at most 64 coarse points, else the matrix cores),
+ *                         1 matrix cores (split precision), 2 FP32 vector
+ *                         units where they apply
  * A change drops the context's cached plans; plans already created by
  * bdft_plan_create keep the options they were made with. */
 #define BDFT_OPT_LINE_PASSES 1
 #define BDFT_OPT_ZGROUP 2
+#define BDFT_OPT_ZPOT 3
 int bdft_ctx_set_option(bdft_ctx *ctx, int option, int value);
 int bdft_ctx_get_option(const bdft_ctx *ctx, int option, int *value);
 

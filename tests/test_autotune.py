@@ -11,7 +11,7 @@ from bigdft_hip.descriptors import dense_region
 
 class FakeCtx:
     def __init__(self):
-        self.opts = {"line_passes": 0, "zgroup": 0}
+        self.opts = {"line_passes": 0, "zgroup": 0, "zpot": 0}
         self.history = []
 
     def set_option(self, k, v):
@@ -37,23 +37,25 @@ def test_shape_keys():
 def test_candidates():
     assert len(autotune.candidates(bigdft_hip.PREC_FP64)) == 2
     c = autotune.candidates(bigdft_hip.PREC_F16X3)
-    assert len(c) == 6 and c[0] == {"line_passes": 0, "zgroup": 0}
-    assert {x["zgroup"] for x in c} == {0, 2, 3}
+    assert len(c) == 7 and c[0] == {"line_passes": 0, "zgroup": 0, "zpot": 0}
+    assert {x["zgroup"] for x in c} == {0, 2, 3} and {x["zpot"] for x in c} == {0, 1}
+    assert sum(x["line_passes"] for x in c) == 1
 
 
 def test_tune_picks_the_fastest_and_caches(tmp_path):
     path = tmp_path / "t.json"
     reg = dense_region("P", (23, 23, 23), (0.45,) * 3)
     lrs = [FakeLr(reg)] * 3
-    cost = {(0, 0): 5.0, (0, 2): 4.0, (0, 3): 4.5, (1, 0): 9.0, (1, 2): 8.0, (1, 3): 8.5}
+    cost = {(0, 0, 0): 5.0, (0, 2, 0): 4.0, (0, 3, 0): 4.5, (0, 0, 1): 6.0, (0, 2, 1): 5.5,
+            (0, 3, 1): 5.2, (1, 0, 0): 9.0}
 
     def timer(ctx, lrs_, pot, psi, prec, kw):
-        return cost[(ctx.opts["line_passes"], ctx.opts["zgroup"])]
+        return cost[(ctx.opts["line_passes"], ctx.opts["zgroup"], ctx.opts["zpot"])]
     ctx = FakeCtx()
     key, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_BF16X4,
                                      cache=autotune.TuneCache(str(path)), timer=timer)
-    assert key == "P:23x23x23:bf16x4" and opts == {"line_passes": 0, "zgroup": 2}
-    assert ctx.opts == opts and len(times) == 6
+    assert key == "P:23x23x23:bf16x4" and opts == {"line_passes": 0, "zgroup": 2, "zpot": 0}
+    assert ctx.opts == opts and len(times) == 7
     table = json.loads(path.read_text())
     assert table[key]["options"] == opts and min(table[key]["ms"].values()) == 4.0
     # a second tuner reads the table and times nothing

@@ -358,18 +358,24 @@ BF_E = 1e-6
 BF_H = 1e-4
 
 
-@pytest.mark.parametrize("seq", ["plane", "lines"])
+@pytest.mark.parametrize("seq", ["plane", "plane-mzp", "lines"])
 @pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
                                         ("F", (15, 13, 17), 3), ("P", (31, 19, 24), 4),
                                         ("F", (39, 39, 39), 5), ("P", (47, 20, 9), 6)])
 def test_bf16x4(ctx, ctx_lines, seq, geo, n, seed):
-    """plane: the plane-fused matrix-core sequence; lines: the per-pass one"""
+    """plane: the plane-fused matrix-core sequence with the automatic z
+    potential pass (FP32 line kernel on these boxes); plane-mzp: with the
+    matrix-core z potential pass; lines: the per-pass sequence"""
     reg = dense_region(geo, n, (0.45, 0.4, 0.5))
     print(f"bf16x4 case {seq} {geo} {n}", flush=True)
     psi = random_psi(reg, 2, 1000 + seed)
     pot = random_pot(reg, 1100 + seed)
-    c = ctx if seq == "plane" else ctx_lines
-    hg, eg, pg = run_gpu(c, reg, pot, psi, precision=3)
+    c = ctx_lines if seq == "lines" else ctx
+    c.set_option("zpot", 1 if seq == "plane-mzp" else 0)
+    try:
+        hg, eg, pg = run_gpu(c, reg, pot, psi, precision=3)
+    finally:
+        c.set_option("zpot", 0)
     ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
                                okeys(reg), pot, psi)
     eh = np.abs(hg - ho).max() / np.abs(ho).max()
@@ -425,17 +431,23 @@ def test_bf16x3_three_products(ctx):
 
 # split-FP16 forward transforms and kinetic (power-of-two scaling per box),
 # split-BF16 transposed transforms: same bars as the four-product BF16 form
-@pytest.mark.parametrize("seq", ["plane", "lines"])
+@pytest.mark.parametrize("seq", ["plane", "plane-mzp", "lines"])
 @pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
                                         ("F", (15, 13, 17), 3), ("F", (39, 39, 39), 5),
                                         ("P", (47, 20, 9), 6)])
 def test_f16x3(ctx, ctx_lines, seq, geo, n, seed):
+    """plane: the fused sequence with the automatic z potential pass (FP32
+    line kernel on these boxes); plane-mzp: with the matrix-core one"""
     reg = dense_region(geo, n, (0.45, 0.4, 0.5))
     print(f"f16x3 case {seq} {geo} {n}", flush=True)
     psi = random_psi(reg, 2, 1500 + seed) * 1e-3
     pot = random_pot(reg, 1600 + seed)
-    c = ctx if seq == "plane" else ctx_lines
-    hg, eg, pg = run_gpu(c, reg, pot, psi, precision=4)
+    c = ctx_lines if seq == "lines" else ctx
+    c.set_option("zpot", 1 if seq == "plane-mzp" else 0)
+    try:
+        hg, eg, pg = run_gpu(c, reg, pot, psi, precision=4)
+    finally:
+        c.set_option("zpot", 0)
     ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
                                okeys(reg), pot, psi)
     eh = np.abs(hg - ho).max() / np.abs(ho).max()
