@@ -13,8 +13,8 @@ fastest per shape key in a JSON cache:
 
 Candidates: the plane-fused and the line/per-pass sequences, and for the
 matrix-core precisions in the plane-fused sequence 2 or 3 z-line bundles per
-work-group and the z potential pass on the matrix cores or on the FP32
-vector units.  Timing runs
+work-group, and the z potential pass and the backward plane pass each on the
+matrix cores or on the FP32 vector units.  Timing runs
 replay a captured plan into scratch outputs, so the caller's hpsi, ekin and
 epot are not touched.  The cache file is plain JSON ({key: {"options": {...},
 "ms": {candidate: ms}}}); its default location is
@@ -51,12 +51,13 @@ def candidates(precision):
     """option sets tried for a precision, the automatic choice first (the
     bundling and the z potential unit only matter in the plane-fused
     matrix-core sequence)"""
-    auto = {"line_passes": 0, "zgroup": 0, "zpot": 0}
+    auto = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0}
     lines = dict(auto, line_passes=1)
     if PREC_NAMES.get(precision) not in MATRIX_CORE:
         return [auto, lines]
-    fused = [dict(auto, zgroup=z, zpot=p) for p in (0, 1) for z in (0, 2, 3)]
-    return fused + [lines]
+    fused = [dict(auto, zgroup=z, zpot=p, bwd=w) for p in (0, 1) for z in (0, 2, 3)
+             for w in (1, 2)]
+    return [auto] + fused + [lines]
 
 
 def cand_name(opts):

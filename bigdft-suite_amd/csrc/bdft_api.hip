@@ -53,6 +53,7 @@ struct bdft_ctx {
   bool force_lines = false;  // disable the plane-fused sequence
   int opt_zg = 0;            // z-line bundles per work-group (0: automatic)
   int opt_zpot = 0;          // matrix-core z potential pass: 0 auto, 1 MFMA, 2 FP32
+  int opt_bwd = 0;           // matrix-core backward pass: 0 auto, 1 MFMA, 2 FP32
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
 };
 
@@ -730,20 +731,37 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     const size_t half = ctx->plane_lds / 2;
     std::vector<int> sm, lg;
     size_t fs = 0, bs = 0, fl = 0, bl = 0;
+    // the FP32 plane backward kernel as the backward pass: where every
+    // periodic in-plane axis has >= 18 points and its planes fit; chosen
+    // automatically for free-BC batches
+    size_t vs = 0, vl = 0;
+    bool vok = true, all_free = true;
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
       const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]);
       const size_t w = mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]);
+      const size_t v = plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)sizeof(float));
+      if (b.per) {
+        all_free = false;
+        if (b.N[0] < 18 || b.N[1] < 18) vok = false;
+      }
       if (std::max(f, w) <= half) {
         sm.push_back(i);
         fs = std::max(fs, f);
         bs = std::max(bs, w);
+        vs = std::max(vs, v);
       } else {
         lg.push_back(i);
         fl = std::max(fl, f);
         bl = std::max(bl, w);
+        vl = std::max(vl, v);
       }
     }
+    if (sm.empty() || lg.empty()) vs = vl = std::max(vs, vl);  // one launch
+    vok = vok && std::max(vs, vl) <= ctx->plane_lds;
+    ml.bvalu = vok && ctx->opt_bwd != 1 && (ctx->opt_bwd == 2 || all_free);
+    ml.vbwd = vs;
+    ml.vbwd2 = vl;
     if (!sm.empty() && !lg.empty()) {
       auto add = [&](const std::vector<int>& g, int& so, int& mo) {
         so = (int)starts.size();
@@ -1658,6 +1676,14 @@ extern "C" int bdft_ctx_set_option(bdft_ctx* ctx, int option, int value) {
         ctx->opt_zpot = value;
       }
       return BDFT_OK;
+    case BDFT_OPT_BWD:
+      if (value < 0 || value > 2)
+        return set_err(ctx, BDFT_ERR_ARG, "backward pass: 0, 1 or 2 (got %d)", value);
+      if (ctx->opt_bwd != value) {
+        drop_cached_plans(ctx);
+        ctx->opt_bwd = value;
+      }
+      return BDFT_OK;
     case BDFT_OPT_ZGROUP:
       if (value != 0 && value != 2 && value != 3)
         return set_err(ctx, BDFT_ERR_ARG, "z-line bundles per work-group: 0, 2 or 3 (got %d)",
@@ -1677,6 +1703,7 @@ extern "C" int bdft_ctx_get_option(const bdft_ctx* ctx, int option, int* value) 
     case BDFT_OPT_LINE_PASSES: *value = ctx->force_lines ? 1 : 0; return BDFT_OK;
     case BDFT_OPT_ZGROUP: *value = ctx->opt_zg; return BDFT_OK;
     case BDFT_OPT_ZPOT: *value = ctx->opt_zpot; return BDFT_OK;
+    case BDFT_OPT_BWD: *value = ctx->opt_bwd; return BDFT_OK;
   }
   return BDFT_ERR_ARG;
 }

@@ -270,6 +270,8 @@ struct MfmaLds {
   size_t fwd2, bwd2;                  // plane kernels of the large-box launches
   int zvalu;                          // 1: z potential pass by the FP32 line kernel
   int maxN;                           // largest coarse extent (line-kernel LDS)
+  int bvalu;                          // 1: backward pass by the FP32 plane kernel
+  size_t vbwd, vbwd2;                 // its LDS (box group of pb / pb2)
 };
 
 // density: orbitals sharing one box are split into ngroups groups of gsize;

@@ -1420,9 +1420,20 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     }
     mark(S_ZP, 1);
     mark(S_PB, 0);
-    if (PT.ntiles[P_PB] > 0)
-      k_mbwd<BM><<<PT.ntiles[P_PB], NTMP, L.bwd, s>>>(B, PT.t[P_PB]);
-    if (PT.ntiles_pb2 > 0) k_mbwd<BM><<<PT.ntiles_pb2, NTMP, L.bwd2, s>>>(B, PT.pb2);
+    if (L.bvalu) {
+      // G_y^T G_x^T + K and compress in FP32 on the vector units (the plane
+      // kernel of the FP32 sequence; T2 and K are unscaled FP32 in both)
+      hipError_t e = hipSuccess;
+      if (PT.ntiles[P_PB] > 0)
+        e = launch_plane_bwd<float>(B, PT.t[P_PB], PT.ntiles[P_PB], L.vbwd, s);
+      if (e == hipSuccess && PT.ntiles_pb2 > 0)
+        e = launch_plane_bwd<float>(B, PT.pb2, PT.ntiles_pb2, L.vbwd2, s);
+      if (e != hipSuccess) return e;
+    } else {
+      if (PT.ntiles[P_PB] > 0)
+        k_mbwd<BM><<<PT.ntiles[P_PB], NTMP, L.bwd, s>>>(B, PT.t[P_PB]);
+      if (PT.ntiles_pb2 > 0) k_mbwd<BM><<<PT.ntiles_pb2, NTMP, L.bwd2, s>>>(B, PT.pb2);
+    }
     mark(S_PB, 1);
     et.t[0] = PT.t[P_KZG];
     et.p[0] = PT.part[2];

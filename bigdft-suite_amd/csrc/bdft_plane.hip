@@ -204,9 +204,9 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTP / 64];
   const int tile = blockIdx.x;
-  const int b = find_box(tt, tile);
-  const BoxDesc bd = B.box[b];
-  const int j3 = tile - box_first_tile(tt, b);
+  const int lb = find_box(tt, tile);
+  const BoxDesc bd = B.box[table_box(tt, lb)];
+  const int j3 = tile - box_first_tile(tt, lb);
   const int N1 = bd.N[0], N2 = bd.N[1], L1 = bd.L[0], L2 = bd.L[1], per = bd.per;
   const int W1 = 2 * N1, W2 = 2 * N2;
   const int RA = W1 + 1;            // row stride of the wavelet plane
@@ -325,9 +325,9 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = blockIdx.x;
-  const int b = find_box(tt, tile);
-  const BoxDesc bd = B.box[b];
-  const int j3 = tile - box_first_tile(tt, b);
+  const int lb = find_box(tt, tile);   // tables over a subset carry a box map
+  const BoxDesc bd = B.box[table_box(tt, lb)];
+  const int j3 = tile - box_first_tile(tt, lb);
   const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], L1 = bd.L[0], L2 = bd.L[1],
             per = bd.per;
   const int W1 = 2 * N1, W2 = 2 * N2;

@@ -295,11 +295,18 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
  *                         at most 64 coarse points, else the matrix cores),
  *                         1 matrix cores (split precision), 2 FP32 vector
  *                         units where they apply
+ *   BDFT_OPT_BWD          backward plane pass (G_y^T G_x^T + K, compress)
+ *                         of the matrix-core precisions: 0 automatic (the
+ *                         FP32 vector-unit plane kernel for free-BC
+ *                         batches, the matrix cores otherwise), 1 matrix
+ *                         cores, 2 FP32 vector units where the plane kernel
+ *                         applies (periodic in-plane axes >= 18 points)
  * A change drops the context's cached plans; plans already created by
  * bdft_plan_create keep the options they were made with. */
 #define BDFT_OPT_LINE_PASSES 1
 #define BDFT_OPT_ZGROUP 2
 #define BDFT_OPT_ZPOT 3
+#define BDFT_OPT_BWD 4
 int bdft_ctx_set_option(bdft_ctx *ctx, int option, int value);
 int bdft_ctx_get_option(const bdft_ctx *ctx, int option, int *value);
 

@@ -11,7 +11,7 @@ from bigdft_hip.descriptors import dense_region
 
 class FakeCtx:
     def __init__(self):
-        self.opts = {"line_passes": 0, "zgroup": 0, "zpot": 0}
+        self.opts = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0}
         self.history = []
 
     def set_option(self, k, v):
@@ -37,27 +37,30 @@ def test_shape_keys():
 def test_candidates():
     assert len(autotune.candidates(bigdft_hip.PREC_FP64)) == 2
     c = autotune.candidates(bigdft_hip.PREC_F16X3)
-    assert len(c) == 7 and c[0] == {"line_passes": 0, "zgroup": 0, "zpot": 0}
+    assert len(c) == 14 and c[0] == {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0}
     assert {x["zgroup"] for x in c} == {0, 2, 3} and {x["zpot"] for x in c} == {0, 1}
+    assert {x["bwd"] for x in c} == {0, 1, 2}
     assert sum(x["line_passes"] for x in c) == 1
+    assert len({tuple(sorted(x.items())) for x in c}) == len(c)
 
 
 def test_tune_picks_the_fastest_and_caches(tmp_path):
     path = tmp_path / "t.json"
     reg = dense_region("P", (23, 23, 23), (0.45,) * 3)
     lrs = [FakeLr(reg)] * 3
-    cost = {(0, 0, 0): 5.0, (0, 2, 0): 4.0, (0, 3, 0): 4.5, (0, 0, 1): 6.0, (0, 2, 1): 5.5,
-            (0, 3, 1): 5.2, (1, 0, 0): 9.0}
-
     def timer(ctx, lrs_, pot, psi, prec, kw):
-        return cost[(ctx.opts["line_passes"], ctx.opts["zgroup"], ctx.opts["zpot"])]
+        o = ctx.opts
+        if o["line_passes"]:
+            return 9.0
+        return 5.0 - (o["zgroup"] == 2) + 0.5 * o["zpot"] - 0.25 * (o["bwd"] == 2)
     ctx = FakeCtx()
     key, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_BF16X4,
                                      cache=autotune.TuneCache(str(path)), timer=timer)
-    assert key == "P:23x23x23:bf16x4" and opts == {"line_passes": 0, "zgroup": 2, "zpot": 0}
-    assert ctx.opts == opts and len(times) == 7
+    assert key == "P:23x23x23:bf16x4"
+    assert opts == {"line_passes": 0, "zgroup": 2, "zpot": 0, "bwd": 2}
+    assert ctx.opts == opts and len(times) == 14
     table = json.loads(path.read_text())
-    assert table[key]["options"] == opts and min(table[key]["ms"].values()) == 4.0
+    assert table[key]["options"] == opts and min(table[key]["ms"].values()) == 3.75
     # a second tuner reads the table and times nothing
     ctx2 = FakeCtx()
 

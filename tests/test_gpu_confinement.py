@@ -52,11 +52,12 @@ CASES = [("P", dense_region("P", (15, 13, 17), (0.45, 0.4, 0.5)), -1),
                          ids=["fp64", "fp32", "bf16x4", "f16x3", "bf16x4-mzp", "f16x3-mzp"])
 @pytest.mark.parametrize("name,reg,off", CASES, ids=[c[0] for c in CASES])
 def test_confinement_vs_oracle(ctx, name, reg, off, precision, zpot):
-    """zpot 1: the matrix-core z potential pass (k_mzp), 0: the automatic
-    choice (the FP32 line kernel on these boxes)"""
+    """zpot 1: the matrix-core z potential and backward passes (k_mzp,
+    k_mbwd), 0: the automatic choice of units"""
     import bigdft_hip
     print(f"confinement {name} n={reg.n} precision={precision} zpot={zpot}", flush=True)
     ctx.set_option("zpot", zpot)
+    ctx.set_option("bwd", zpot)
     norb = 3
     h = reg.hgrid
     pot = random_pot(reg, 3, lo=-1.0, hi=0.0)
@@ -73,6 +74,7 @@ def test_confinement_vs_oracle(ctx, name, reg, off, precision, zpot):
     hg, ekg, epg = apply(ctx, [lr] * norb, dev(pot), dev(psi), precision, conf)
     lr.close()
     ctx.set_option("zpot", 0)
+    ctx.set_option("bwd", 0)
     err = np.abs(hg - ho).max() / np.abs(ho).max()
     ee = max(np.abs(ekg - eko).max() / np.abs(eko).max(), np.abs(epg - epo).max() / np.abs(epo).max())
     print(f"  hpsi {err:.2e} energies {ee:.2e}", flush=True)

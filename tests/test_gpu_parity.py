@@ -358,24 +358,33 @@ BF_E = 1e-6
 BF_H = 1e-4
 
 
-@pytest.mark.parametrize("seq", ["plane", "plane-mzp", "lines"])
+def set_passes(c, seq):
+    """units of the z potential and backward passes of the plane-fused
+    matrix-core sequence (launch options)"""
+    v = {"plane-mfma": 1, "plane-valu": 2}.get(seq, 0)
+    c.set_option("zpot", v)
+    c.set_option("bwd", v)
+
+
+@pytest.mark.parametrize("seq", ["plane", "plane-mfma", "plane-valu", "lines"])
 @pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
                                         ("F", (15, 13, 17), 3), ("P", (31, 19, 24), 4),
                                         ("F", (39, 39, 39), 5), ("P", (47, 20, 9), 6)])
 def test_bf16x4(ctx, ctx_lines, seq, geo, n, seed):
-    """plane: the plane-fused matrix-core sequence with the automatic z
-    potential pass (FP32 line kernel on these boxes); plane-mzp: with the
-    matrix-core z potential pass; lines: the per-pass sequence"""
+    """plane: the plane-fused matrix-core sequence with the automatic choice
+    of units for the z potential and backward passes; plane-mfma: both on
+    the matrix cores; plane-valu: both on the FP32 vector units (where they
+    apply); lines: the per-pass sequence"""
     reg = dense_region(geo, n, (0.45, 0.4, 0.5))
     print(f"bf16x4 case {seq} {geo} {n}", flush=True)
     psi = random_psi(reg, 2, 1000 + seed)
     pot = random_pot(reg, 1100 + seed)
     c = ctx_lines if seq == "lines" else ctx
-    c.set_option("zpot", 1 if seq == "plane-mzp" else 0)
+    set_passes(c, seq)
     try:
         hg, eg, pg = run_gpu(c, reg, pot, psi, precision=3)
     finally:
-        c.set_option("zpot", 0)
+        set_passes(c, "plane")
     ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
                                okeys(reg), pot, psi)
     eh = np.abs(hg - ho).max() / np.abs(ho).max()
@@ -431,23 +440,22 @@ def test_bf16x3_three_products(ctx):
 
 # split-FP16 forward transforms and kinetic (power-of-two scaling per box),
 # split-BF16 transposed transforms: same bars as the four-product BF16 form
-@pytest.mark.parametrize("seq", ["plane", "plane-mzp", "lines"])
+@pytest.mark.parametrize("seq", ["plane", "plane-mfma", "plane-valu", "lines"])
 @pytest.mark.parametrize("geo,n,seed", [("P", (23, 23, 23), 1), ("P", (7, 6, 8), 2),
                                         ("F", (15, 13, 17), 3), ("F", (39, 39, 39), 5),
                                         ("P", (47, 20, 9), 6)])
 def test_f16x3(ctx, ctx_lines, seq, geo, n, seed):
-    """plane: the fused sequence with the automatic z potential pass (FP32
-    line kernel on these boxes); plane-mzp: with the matrix-core one"""
+    """sequences as in test_bf16x4"""
     reg = dense_region(geo, n, (0.45, 0.4, 0.5))
     print(f"f16x3 case {seq} {geo} {n}", flush=True)
     psi = random_psi(reg, 2, 1500 + seed) * 1e-3
     pot = random_pot(reg, 1600 + seed)
     c = ctx_lines if seq == "lines" else ctx
-    c.set_option("zpot", 1 if seq == "plane-mzp" else 0)
+    set_passes(c, seq)
     try:
         hg, eg, pg = run_gpu(c, reg, pot, psi, precision=4)
     finally:
-        c.set_option("zpot", 0)
+        set_passes(c, "plane")
     ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid,
                                okeys(reg), pot, psi)
     eh = np.abs(hg - ho).max() / np.abs(ho).max()
