@@ -7,7 +7,14 @@ potential = sum of 8 Gaussians (centres uniform in the cell, depth U[-5,-1]
 Ha, width U[0.5,2] bohr, minimum-image distance).  One step = one full
 application hpsi += H_loc psi (+ ekin, epot per orbital) over the batch,
 replayed from a captured hipGraph.  Inputs are resident in HBM before the
-timed region.
+timed region.  Before each precision is timed, the launch options (kernel
+sequence, z bundling, units of the mixed sequences' z potential and
+backward passes) are chosen by timing the candidates on the same batch
+(bigdft_hip.autotune; --no-autotune keeps the automatic choice).
+
+Other workloads (--workload): precond (the conjugate-gradient
+preconditioner), density (rho += occ psir^2) and transfers (global <->
+localisation-region copies), each printing one JSON line of its own.
 
 Prints one JSON line.  value = grid-point*orbital applications per second
 (grid point = fine ISF point, n1i*n2i*n3i per orbital, the reference's unit,
