@@ -44,6 +44,16 @@ def test_null_arguments_are_errors():
     assert lib.bdft_plan_execute(None) == 1
     assert lib.bdft_lr_destroy(None) == 1
     assert lib.bdft_ctx_create(0, None) == 1
+    assert lib.bdft_ctx_set_option(None, 1, 0) == 1
+    v = ctypes.c_int()
+    assert lib.bdft_ctx_get_option(None, 1, ctypes.byref(v)) == 1
+
+
+def test_option_codes_match_the_header():
+    text = open(os.path.join(ROOT, "include", "bigdft_hip.h")).read()
+    codes = {m.group(1).lower(): int(m.group(2))
+             for m in re.finditer(r"#define BDFT_OPT_(\w+) (\d+)", text)}
+    assert codes == bigdft_hip.OPTIONS
 
 
 @pytest.mark.parametrize("n,nfl,nfu", [((14, 17, 12), (3, 5, 2), (10, 12, 9)),
