@@ -954,14 +954,20 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
                      epot, precision, true, out);
   if (rc) return rc;
   bdft_plan* plan = *out;
-  // capture the launch sequence once; replays reuse the instantiated graph
+  // capture the launch sequence once; replays reuse the instantiated graph.
+  // A second capture stream (default priority) lets independent passes
+  // become parallel branches of the graph.
   hipStream_t cs = nullptr;
+  Fork fk{nullptr, nullptr, nullptr};
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&fk.s2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&fk.a, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&fk.b, hipEventDisableTiming);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     hipError_t el = plan->precision >= BDFT_PREC_BF16X3
                          ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal,
-                                       plan->need_zero, plan->precision, cs, nullptr)
+                                       plan->need_zero, plan->precision, cs, nullptr, &fk)
                          : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
                                          plan->need_zero, plan->precision == BDFT_PREC_FP32,
                                          cs);
@@ -969,6 +975,9 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
     if (el != hipSuccess) e = el;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+  if (fk.a) hipEventDestroy(fk.a);
+  if (fk.b) hipEventDestroy(fk.b);
+  if (fk.s2) hipStreamDestroy(fk.s2);
   if (cs) hipStreamDestroy(cs);
   if (e != hipSuccess) {
     bdft_plan_destroy(plan);

@@ -360,9 +360,16 @@ size_t mfma_attributes(size_t lds);
 // on T2, epot partials per tile
 hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
                            double* part, hipStream_t s);
+// a second stream and two events: the z kinetic and z potential passes,
+// which read and write disjoint grids, run side by side between the forward
+// and the backward plane passes (fork after one, join before the other)
+struct Fork {
+  hipStream_t s2;
+  hipEvent_t a, b;
+};
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
                        long long wtotal, bool need_zero, int precision, hipStream_t s,
-                       hipEvent_t* ev);
+                       hipEvent_t* ev, const Fork* fk = nullptr);
 size_t plane_lds_fwd(int N1, int N2, int L1, int elem);
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem);
 size_t plane_attributes();

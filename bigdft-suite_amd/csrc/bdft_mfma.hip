@@ -1376,7 +1376,7 @@ size_t mfma_attributes(size_t lds) {
 template <int FM, int BM>
 static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const MfmaLds& L,
                                 long long wtotal, bool need_zero, hipStream_t s,
-                                hipEvent_t* ev) {
+                                hipEvent_t* ev, const Fork* fk) {
   auto mark = [&](int stage, int end) {
     if (ev) (void)hipEventRecord(ev[2 * stage + end], s);
   };
@@ -1397,6 +1397,30 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       k_mfwd<FM><<<PT.ntiles[P_PF], NTMP, L.fwd, s>>>(B, PT.t[P_PF]);
     if (PT.ntiles_pf2 > 0) k_mfwd<FM><<<PT.ntiles_pf2, NTMP, L.fwd2, s>>>(B, PT.pf2);
     mark(S_PF, 1);
+    // z kinetic (W, K) and z potential (T2) touch disjoint grids: with a
+    // fork they run side by side (not while stages are being timed)
+    const bool fork = fk && !ev;
+    hipStream_t sz = s;
+    if (fork) {
+      hipError_t e = hipEventRecord(fk->a, s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(fk->s2, fk->a, 0);
+      if (e != hipSuccess) return e;
+      sz = fk->s2;
+    }
+    mark(S_ZP, 0);
+    if (L.zvalu) {
+      // G_z, V, G_z^T in FP32 on the vector units (the pass where they beat
+      // the split-precision matrix-core form); T2 is unscaled FP32 in both
+      const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
+                                           PT.part[3], sz);
+      if (e != hipSuccess) return e;
+    } else if (PT.ntiles[P_ZPG] > 0) {
+      if (L.zg == 2)
+        k_mzp<FM, BM, 2><<<PT.ntiles[P_ZPG], NTM, L.zp, sz>>>(B, PT.t[P_ZPG], PT.part[3]);
+      else
+        k_mzp<FM, BM, 3><<<PT.ntiles[P_ZPG], NTM, L.zp, sz>>>(B, PT.t[P_ZPG], PT.part[3]);
+    }
+    mark(S_ZP, 1);
     mark(S_KZ, 0);
     if (PT.ntiles[P_KZG] > 0) {
       if (L.zg == 2)
@@ -1405,20 +1429,11 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
         k_mkz<FM, 3><<<PT.ntiles[P_KZG], NTM, L.kz, s>>>(B, PT.t[P_KZG], PT.part[2]);
     }
     mark(S_KZ, 1);
-    mark(S_ZP, 0);
-    if (L.zvalu) {
-      // G_z, V, G_z^T in FP32 on the vector units (the pass where they beat
-      // the split-precision matrix-core form); T2 is unscaled FP32 in both
-      const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
-                                           PT.part[3], s);
+    if (fork) {
+      hipError_t e = hipEventRecord(fk->b, fk->s2);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, fk->b, 0);
       if (e != hipSuccess) return e;
-    } else if (PT.ntiles[P_ZPG] > 0) {
-      if (L.zg == 2)
-        k_mzp<FM, BM, 2><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
-      else
-        k_mzp<FM, BM, 3><<<PT.ntiles[P_ZPG], NTM, L.zp, s>>>(B, PT.t[P_ZPG], PT.part[3]);
     }
-    mark(S_ZP, 1);
     mark(S_PB, 0);
     if (L.bvalu) {
       // G_y^T G_x^T + K and compress in FP32 on the vector units (the plane
@@ -1501,11 +1516,11 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
 
 hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
                        long long wtotal, bool need_zero, int precision, hipStream_t s,
-                       hipEvent_t* ev) {
+                       hipEvent_t* ev, const Fork* fk) {
   switch (precision) {
-    case BDFT_PREC_BF16X3: return launch_mfma_t<3, 3>(B, PT, L, wtotal, need_zero, s, ev);
-    case BDFT_PREC_BF16X4: return launch_mfma_t<4, 4>(B, PT, L, wtotal, need_zero, s, ev);
-    default: return launch_mfma_t<16, 4>(B, PT, L, wtotal, need_zero, s, ev);
+    case BDFT_PREC_BF16X3: return launch_mfma_t<3, 3>(B, PT, L, wtotal, need_zero, s, ev, fk);
+    case BDFT_PREC_BF16X4: return launch_mfma_t<4, 4>(B, PT, L, wtotal, need_zero, s, ev, fk);
+    default: return launch_mfma_t<16, 4>(B, PT, L, wtotal, need_zero, s, ev, fk);
   }
 }
 

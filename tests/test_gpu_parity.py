@@ -231,6 +231,38 @@ def test_graph_plan_replays(ctx):
     lr.close()
 
 
+@pytest.mark.parametrize("geo,n,precision", [("P", (23, 23, 23), 4), ("F", (19, 17, 21), 4),
+                                              ("P", (23, 23, 23), 3)])
+def test_graph_branches_match_the_direct_launches(ctx, geo, n, precision):
+    """matrix-core plans run the z kinetic and z potential passes as parallel
+    branches of their graph; every replay equals the direct (one-stream)
+    launch bit for bit, with eager work between the replays"""
+    reg = dense_region(geo, n, (0.4, 0.45, 0.5))
+    norb = 3
+    psi = random_psi(reg, norb, 120)
+    pot = random_pot(reg, 121)
+    lr = ctx.region(reg)
+    d_psi, d_pot = dev(psi), dev(pot)
+    h1 = torch.zeros_like(d_psi)
+    e1 = torch.zeros(norb, dtype=torch.float64, device="cuda")
+    p1 = torch.zeros_like(e1)
+    ctx.locham_batch([lr] * norb, d_pot, d_psi, h1, e1, p1, precision=precision)
+    ctx.sync()
+    h2 = torch.zeros_like(d_psi)
+    e2 = torch.zeros_like(e1)
+    p2 = torch.zeros_like(e1)
+    plan = ctx.plan([lr] * norb, d_pot, d_psi, h2, e2, p2, precision=precision)
+    for rep in range(3):
+        h2.zero_()
+        plan.execute()
+        ctx.sync()
+        assert torch.equal(h2, h1)
+        assert torch.equal(e2, e1) and torch.equal(p2, p1)
+        _ = (torch.randn(1000, device="cuda") * 2).sum().item()  # eager work
+    plan.close()
+    lr.close()
+
+
 def test_host_entry_point(ctx):
     reg = dense_region("F", (6, 7, 5), (0.45,) * 3)
     psi = random_psi(reg, 2, 110)
