@@ -803,6 +803,14 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     T.plane = plane ? 1 : 0;
     T.lds_pf = lpf;
     T.lds_pb = lpb;
+    // preconditioner plans (FP64): the x and y operator as one plane pass
+    // where the W and K planes of the largest box fit in LDS
+    size_t lk = 0;
+    for (int i = 0; i < norb; ++i)
+      lk = std::max(lk, plane_lds_kin(box[i].N[0], box[i].N[1], (int)sizeof(double)));
+    T.pc_plane = density && !grow_plane && !ctx->force_lines && !mfma && esz == 8 &&
+                 lk <= ctx->plane_lds;
+    T.lds_pc = lk;
   }
   bdft_plan* plan = new bdft_plan();
   plan->ctx = ctx;

@@ -250,6 +250,9 @@ struct PassTables {
   // launches of their own (t[P_PF], t[P_PB] then cover the other boxes)
   TileTable pf2, pb2;
   int ntiles_pf2, ntiles_pb2;
+  // preconditioner: x and y operator in one plane pass (FP64 planes that fit)
+  int pc_plane;
+  size_t lds_pc;
 };
 
 // Launch stages in order; profiling records one event pair per stage.
@@ -372,6 +375,10 @@ hipError_t launch_mfma(const Batch& B, const PassTables& PT, const MfmaLds& L,
                        hipEvent_t* ev, const Fork* fk = nullptr);
 size_t plane_lds_fwd(int N1, int N2, int L1, int elem);
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem);
+// preconditioner: K = c W + K_x W + K_y W per x-y plane (P_PF tiles)
+size_t plane_lds_kin(int N1, int N2, int elem);
+hipError_t launch_plane_kin_pc(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                               const double* pc, hipStream_t s);
 size_t plane_attributes();
 template <typename T>
 hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,

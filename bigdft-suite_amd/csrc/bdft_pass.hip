@@ -1276,9 +1276,14 @@ hipError_t launch_precond(const Batch& B, const PassTables& PT, int maxN, long l
   }
   auto apply = [&](const double* src, int update) {
     if (nu > 0) k_pc_scatter<<<nu, NT, 0, s>>>(B, tu, A, src, update);
-    if (PT.ntiles[P_XF] > 0) k_pc_kinx<<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], A);
-    if (PT.ntiles[P_KY] > 0)
-      k_kin<double, 1><<<PT.ntiles[P_KY], NT, lds, s>>>(B, PT.t[P_KY], PT.part[1]);
+    if (PT.pc_plane) {
+      // c W + K_x W + K_y W of each plane in one pass through LDS
+      (void)launch_plane_kin_pc(B, PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pc, A.pc, s);
+    } else {
+      if (PT.ntiles[P_XF] > 0) k_pc_kinx<<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], A);
+      if (PT.ntiles[P_KY] > 0)
+        k_kin<double, 1><<<PT.ntiles[P_KY], NT, lds, s>>>(B, PT.t[P_KY], PT.part[1]);
+    }
     if (PT.ntiles[P_KZ] > 0)
       k_kin<double, 2><<<PT.ntiles[P_KZ], NT, lds, s>>>(B, PT.t[P_KZ], PT.part[2]);
   };

@@ -318,6 +318,101 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
 }
 
 // ---------------------------------------------------------------------------
+// Preconditioner operator, x and y part, of one x-y plane of a box:
+//   K = c W + Ktilde_x W + Ktilde_y W      (c = cprecr of the box, pc[9 b])
+// the work of the x kinetic line pass (k_pc_kinx) and the y one (k_kin<1>)
+// in one read of the W plane and one write of K, in the same order of
+// operations (c W + K_x first, then + K_y)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTable tt,
+                                                                const double* pc) {
+  constexpr int NTP = PlaneNT<T>::v;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tile = blockIdx.x;
+  const int lb = find_box(tt, tile);
+  const int b = table_box(tt, lb);
+  const BoxDesc bd = B.box[b];
+  const int j3 = tile - box_first_tile(tt, lb);
+  const int N1 = bd.N[0], N2 = bd.N[1], per = bd.per;
+  const int W1 = 2 * N1, W2 = 2 * N2;
+  const int RA = W1 + 1;
+  T* A = (T*)smem;                  // W plane: A[j2*RA + j1]
+  T* C = A + W2 * RA;               // K plane, same layout
+  const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
+  const int tid = threadIdx.x;
+  const int tot = W1 * W2;
+  const float inv = 1.0f / (float)W1;
+  for (int base = tid; base < tot; base += NTP * 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * NTP;
+      v[u] = i < tot ? src[i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * NTP;
+      if (i < tot) {
+        int r = (int)((float)i * inv);
+        if ((r + 1) * W1 <= i) ++r;
+        if (r * W1 > i) --r;
+        A[r * RA + (i - r * W1)] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1], c = (T)pc[9 * b];
+  const int cx = (N1 + 3) >> 2;
+  for (int it = tid; it < W2 * cx; it += NTP) {
+    const int ch = it / W2, j2 = it - ch * W2;
+    const int i0 = 4 * ch;
+    T oc[4], od[4];
+    kin4_any(PLine<T>{A + j2 * RA, 1}, N1, i0, per, oc, od);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N1) {
+        const int a = j2 * RA + i0 + r, d = j2 * RA + N1 + i0 + r;
+        C[a] = fma(c, A[a], oc[r] * kx);
+        C[d] = fma(c, A[d], od[r] * kx);
+      }
+  }
+  __syncthreads();
+  const int cy = (N2 + 3) >> 2;
+  for (int it = tid; it < W1 * cy; it += NTP) {
+    const int ch = it / W1, j1 = it - ch * W1;
+    const int i0 = 4 * ch;
+    T oc[4], od[4];
+    kin4_any(PLine<T>{A + j1, RA}, N2, i0, per, oc, od);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (i0 + r < N2) {
+        const int a = (i0 + r) * RA + j1, d = (N2 + i0 + r) * RA + j1;
+        C[a] += oc[r] * ky;
+        C[d] += od[r] * ky;
+      }
+  }
+  __syncthreads();
+  T* kd = (T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
+  for (int i = tid; i < tot; i += NTP) {
+    int r = (int)((float)i * inv);
+    if ((r + 1) * W1 <= i) ++r;
+    if (r * W1 > i) --r;
+    kd[i] = C[r * RA + (i - r * W1)];
+  }
+}
+
+size_t plane_lds_kin(int N1, int N2, int elem) {
+  return (size_t)2 * (2 * N2) * (2 * N1 + 1) * elem;
+}
+
+hipError_t launch_plane_kin_pc(const Batch& B, const TileTable& t, int ntiles, size_t lds,
+                               const double* pc, hipStream_t s) {
+  if (ntiles > 0) k_plane_kin_pc<double><<<ntiles, PlaneNT<double>::v, lds, s>>>(B, t, pc);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // backward plane kernel with compress-accumulate
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -442,6 +537,9 @@ hipError_t plane_setup(int bytes) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_plane_fwd<T, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_plane_kin_pc<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_plane_bwd<T>,

@@ -21,6 +21,20 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(scope="module")
+def ctx_lines():
+    import os
+
+    import bigdft_hip
+    os.environ["BDFT_LINE_PASSES"] = "1"
+    try:
+        c = bigdft_hip.Context(0)
+    finally:
+        del os.environ["BDFT_LINE_PASSES"]
+    yield c
+    c.close()
+
+
 def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
@@ -29,10 +43,14 @@ CASES = [("P", dense_region("P", (19, 17, 21), (0.45, 0.4, 0.5))),
          ("F", mixed_sphere_region((24, 21, 26), 0.42, 13))]
 
 
+@pytest.mark.parametrize("seq", ["plane", "lines"])
 @pytest.mark.parametrize("ncong", [0, 1, 5, 12])
 @pytest.mark.parametrize("name,reg", CASES, ids=[c[0] for c in CASES])
-def test_precond_vs_oracle(ctx, name, reg, ncong):
-    print(f"precond {name} n={reg.n} ncong={ncong}", flush=True)
+def test_precond_vs_oracle(ctx, ctx_lines, name, reg, ncong, seq):
+    """plane: x and y operator in one plane pass (k_plane_kin_pc); lines:
+    the x and y line passes"""
+    print(f"precond {name} n={reg.n} ncong={ncong} {seq}", flush=True)
+    ctx = ctx if seq == "plane" else ctx_lines
     norb = 3
     rng = np.random.default_rng(ncong + 3)
     b = random_psi(reg, norb, 5)
