@@ -552,11 +552,15 @@ def main():
             ms, wall, avg, hm, ekm, epm = timed(bigdft_hip.PRECISIONS[name])
             err = errors(hm, ekm, epm)
             del hm
+        vpath = path_of(name, plane_seq(bigdft_hip.PRECISIONS[name]), avg)
+        vesz = 8 if name == "fp64" else 4
+        vpeak = (PEAK_BF16_TFLOPS if name in ("bf16x3", "bf16x4", "f16x3") else
+                 PEAK_FP64_TFLOPS if name == "fp64" else PEAK_FP32_TFLOPS)
         variants[name] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
                           "launch_options": chosen.get(bigdft_hip.PRECISIONS[name]),
-                          "wall_s": round(wall, 4), "path":
-                          path_of(name, plane_seq(bigdft_hip.PRECISIONS[name]), avg),
+                          "wall_s": round(wall, 4), "path": vpath,
                           "stages_ms": {k: round(v, 4) for k, v in avg.items()},
+                          "roofline": roofline(vpath, avg, units, vesz, vpeak),
                           "error_vs_fp64": err}
     head = variants[args.precision]
     ms_step, wall = head["ms_per_step"], head["wall_s"]
