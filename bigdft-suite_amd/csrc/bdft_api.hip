@@ -679,6 +679,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // plane kernels must fit in LDS
   MfmaLds ml{};
   ml.zg = zg;
+  bool mfma_bwd_ok = true, valu_bwd_ok = false;  // units that fit the backward pass
   if (mfma) {
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
@@ -699,9 +700,23 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       ml.kz = std::max(ml.kz, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp, zg));
       ml.zp = std::max(ml.zp, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp, zg));
     }
-    // plane-fused sequence when both plane kernels fit; the per-pass one
-    // otherwise (or when line passes are forced)
-    ml.fused = !ctx->force_lines && std::max(ml.fwd, ml.bwd) <= ctx->plane_lds;
+    // the FP32 plane backward kernel can stand in for the matrix-core one
+    // where every periodic in-plane axis has >= 18 points and its planes fit
+    valu_bwd_ok = ctx->opt_bwd != 1;
+    size_t vb_max = 0;
+    for (int i = 0; i < norb; ++i) {
+      const BoxDesc& b = box[i];
+      if (b.per && (b.N[0] < 18 || b.N[1] < 18)) valu_bwd_ok = false;
+      vb_max = std::max(vb_max, plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1],
+                                              (int)sizeof(float)));
+    }
+    valu_bwd_ok = valu_bwd_ok && vb_max <= ctx->plane_lds;
+    mfma_bwd_ok = ml.bwd <= ctx->plane_lds;
+    // plane-fused sequence when the forward plane kernel fits and the
+    // backward pass fits on either unit; the per-pass one otherwise (or
+    // when line passes are forced)
+    ml.fused = !ctx->force_lines && ml.fwd <= ctx->plane_lds &&
+               (mfma_bwd_ok || valu_bwd_ok);
     // z potential pass of the fused sequence: the FP32 line kernel where it
     // applies (every extent within the line kernels' range), unless the
     // matrix cores are asked for
@@ -712,7 +727,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     ml.maxN = maxN;
     size_t mx = std::max(ml.kin[2], ml.zpot);
     if (ml.fused)
-      mx = std::max(std::max(ml.kz, ml.zp), std::max(ml.fwd, ml.bwd));
+      mx = std::max(std::max(ml.kz, ml.zvalu ? (size_t)0 : ml.zp),
+                    std::max(ml.fwd, mfma_bwd_ok ? ml.bwd : (size_t)0));
     else
       mx = std::max(std::max(mx, std::max(ml.kin[0], ml.kin[1])),
                     std::max(ml.grow, ml.shrink));
@@ -731,21 +747,18 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     const size_t half = ctx->plane_lds / 2;
     std::vector<int> sm, lg;
     size_t fs = 0, bs = 0, fl = 0, bl = 0;
-    // the FP32 plane backward kernel as the backward pass: where every
-    // periodic in-plane axis has >= 18 points and its planes fit; chosen
-    // automatically for free-BC batches
     size_t vs = 0, vl = 0;
-    bool vok = true, all_free = true;
+    bool all_free = true;
+    for (int i = 0; i < norb; ++i) all_free = all_free && !box[i].per;
+    // FP32 backward pass: asked for, automatic on free-BC batches, or the
+    // only unit whose planes fit
+    const bool bv = valu_bwd_ok && (ctx->opt_bwd == 2 || all_free || !mfma_bwd_ok);
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
       const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]);
       const size_t w = mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]);
       const size_t v = plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)sizeof(float));
-      if (b.per) {
-        all_free = false;
-        if (b.N[0] < 18 || b.N[1] < 18) vok = false;
-      }
-      if (std::max(f, w) <= half) {
+      if (std::max(f, bv ? v : w) <= half) {
         sm.push_back(i);
         fs = std::max(fs, f);
         bs = std::max(bs, w);
@@ -758,8 +771,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       }
     }
     if (sm.empty() || lg.empty()) vs = vl = std::max(vs, vl);  // one launch
-    vok = vok && std::max(vs, vl) <= ctx->plane_lds;
-    ml.bvalu = vok && ctx->opt_bwd != 1 && (ctx->opt_bwd == 2 || all_free);
+    ml.bvalu = bv;
     ml.vbwd = vs;
     ml.vbwd2 = vl;
     if (!sm.empty() && !lg.empty()) {

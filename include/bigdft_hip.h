@@ -95,7 +95,8 @@ const char *bdft_last_error(const bdft_ctx *ctx);
  * validated and uploaded once.  Replaces allocate_data_OCL
  * (interface_ocl.f90:43-193).  geocode 'P' (periodic) or 'F' (free); n[d] is
  * the reference's n_d (coarse points n_d+1; the Hamiltonian
- * entry points take at most 64 per axis, the transfers at most 4096); nfl/nfu
+ * entry points take at most 64 per axis, 60 for free boxes in the split
+ * precisions, the transfers at most 4096); nfl/nfu
  * bound the fine region (ignored for 'P'); hgrid is the coarse spacing. */
 int bdft_lr_create(bdft_ctx *ctx, char geocode, const int n[3],
                    const int nfl[3], const int nfu[3], const double hgrid[3],

@@ -296,6 +296,52 @@ def test_descriptor_errors_are_reported(ctx):
     lr.close()
 
 
+def test_empty_batch_is_an_error(ctx):
+    import bigdft_hip
+    z = torch.zeros(1, dtype=torch.float64, device="cuda")
+    with pytest.raises(bigdft_hip.BigDFTError) as e:
+        ctx.locham_batch([], z, z, z, z, z)
+    assert e.value.code == 1
+
+
+@pytest.mark.parametrize("precision,n", [(0, 63), (4, 59)], ids=["fp64", "f16x3"])
+def test_largest_free_box(ctx, precision, n):
+    """the largest supported free box, one orbital: 64 coarse points per
+    axis (157 fine) in FP64, 60 in the split precisions, which run the
+    plane-fused sequence with the FP32 backward plane pass there (the
+    matrix-core backward kernel would not fit in LDS); FP64 within 1e-12,
+    F16X3 within 1e-4 of max|hpsi| and 1e-6 in the energies"""
+    reg = dense_region("F", (n, n, n), (0.45, 0.4, 0.5))
+    print(f"largest free box {reg.n} precision {precision}", flush=True)
+    psi = random_psi(reg, 1, 4242)
+    pot = random_pot(reg, 4243)
+    hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=precision)
+    ho, eo, po = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
+    eh = np.abs(hg - ho).max() / np.abs(ho).max()
+    ee = max(abs(eg[0] - eo[0]) / abs(eo[0]), abs(pg[0] - po[0]) / abs(po[0]))
+    print(f"  hpsi {eh:.2e} energies {ee:.2e}", flush=True)
+    if precision == 0:
+        assert rel_err(hg, ho) < TOL and ee < TOL
+    else:
+        assert eh < 1e-4 and ee < 1e-6
+
+
+@pytest.mark.parametrize("n,bwd", [((63, 63, 63), 0), ((59, 59, 59), 1)])
+def test_split_precision_refuses_free_boxes_beyond_its_lds(ctx, n, bwd):
+    """a free box whose backward pass fits neither unit's LDS (64 points per
+    axis), or only the FP32 one while the matrix cores are asked for: an
+    error with the LDS figure, not a launch"""
+    import bigdft_hip
+    reg = dense_region("F", n, (0.45, 0.4, 0.5))
+    ctx.set_option("bwd", bwd)
+    try:
+        with pytest.raises(bigdft_hip.BigDFTError, match="LDS") as e:
+            run_gpu(ctx, reg, random_pot(reg, 1), random_psi(reg, 1, 2), precision=4)
+        assert e.value.code == 5
+    finally:
+        ctx.set_option("bwd", 0)
+
+
 # ---------------------------------------------------------------------------
 # mixed precision (float working type; FP64 input, potential, energies and
 # hpsi accumulation).  Bars: ekin, epot within 1e-6 relative of the FP64
