@@ -23,7 +23,9 @@ $BDFT_TUNE_CACHE or ~/.cache/bigdft_hip/tune.json.
 import json
 import os
 
-from . import PRECISIONS
+from . import PRECISIONS, BigDFTError
+
+ERR_UNSUPPORTED = 5   # BDFT_ERR_UNSUPPORTED: the option set does not fit this batch
 
 PREC_NAMES = {v: k for k, v in PRECISIONS.items()}
 MATRIX_CORE = ("bf16x3", "bf16x4", "f16x3")
@@ -136,10 +138,18 @@ def tune(ctx, lrs, pot, psi, precision, cache=None, timer=time_plan, **batch_kw)
     best = None
     for opts in candidates(precision):
         apply(ctx, opts)
-        ms = timer(ctx, lrs, pot, psi, precision, batch_kw)
+        try:
+            ms = timer(ctx, lrs, pot, psi, precision, batch_kw)
+        except BigDFTError as e:
+            if e.code != ERR_UNSUPPORTED:
+                raise
+            times[cand_name(opts)] = None   # does not fit this batch
+            continue
         times[cand_name(opts)] = ms
         if best is None or ms < best[0]:
             best = (ms, opts)
+    if best is None:
+        raise BigDFTError(ERR_UNSUPPORTED, f"no launch option set fits the batch {key}")
     apply(ctx, best[1])
     cache.put(key, best[1], times)
     return key, best[1], times

@@ -76,3 +76,25 @@ def test_bad_cache_file(tmp_path):
     p.write_text("[1, 2]")
     with pytest.raises(ValueError):
         autotune.TuneCache(str(p))
+
+
+def test_tune_skips_option_sets_that_do_not_fit(tmp_path):
+    reg = dense_region("F", (59, 59, 59), (0.4,) * 3)
+    lrs = [FakeLr(reg)]
+
+    def timer(ctx, lrs_, pot, psi, prec, kw):
+        if ctx.opts["line_passes"] or ctx.opts["bwd"] == 1:
+            raise bigdft_hip.BigDFTError(5, "BDFT_ERR_UNSUPPORTED: LDS")
+        return 3.0 + ctx.opts["zgroup"]
+    ctx = FakeCtx()
+    _, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_F16X3,
+                                   cache=autotune.TuneCache(str(tmp_path / "t.json")),
+                                   timer=timer)
+    assert opts["bwd"] != 1 and opts["line_passes"] == 0 and opts["zgroup"] == 0
+    assert sum(v is None for v in times.values()) == 7
+
+    def other(ctx, *a):
+        raise bigdft_hip.BigDFTError(6, "BDFT_ERR_HIP")
+    with pytest.raises(bigdft_hip.BigDFTError):
+        autotune.tune(FakeCtx(), lrs, None, None, bigdft_hip.PREC_F16X3,
+                      cache=autotune.TuneCache(str(tmp_path / "u.json")), timer=other)

@@ -90,3 +90,27 @@ def test_tune_chooses_caches_and_keeps_results(ctx, tmp_path):
     with pytest.raises(bigdft_hip.BigDFTError):
         ctx.set_option("zgroup", 5)
     lr.close()
+
+
+def test_tune_on_a_box_that_only_fits_some_options(ctx, tmp_path):
+    """a 60^3 free box in F16X3: the matrix-core backward pass and the
+    per-pass sequence do not fit in LDS; the tuner skips them and its choice
+    gives the oracle's energies"""
+    import bigdft_hip
+    from bigdft_hip import autotune
+    reg = dense_region("F", (59, 59, 59), (0.45, 0.4, 0.5))
+    pot, psi = random_pot(reg, 31), random_psi(reg, 1, 32)
+    lr = ctx.region(reg)
+    dp, ds = dev(pot), dev(psi.ravel())
+    prec = bigdft_hip.PREC_F16X3
+    key, opts, times = autotune.tune(ctx, [lr], dp, ds, prec,
+                                     cache=autotune.TuneCache(str(tmp_path / "t.json")))
+    print(f"  {key}: {opts}; skipped {[k for k, v in times.items() if v is None]}", flush=True)
+    assert opts["bwd"] != 1 and opts["line_passes"] == 0
+    assert any(v is None for v in times.values())
+    h, ek, ep = run(ctx, lr, 1, dp, ds, prec)
+    ho, eko, epo = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
+    assert np.abs(h - ho.ravel()).max() < 1e-4 * np.abs(ho).max()
+    assert abs(ek[0] - eko[0]) < 1e-6 * abs(eko[0]) and abs(ep[0] - epo[0]) < 1e-6 * abs(epo[0])
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0})
+    lr.close()
