@@ -176,6 +176,44 @@ def test_uneven_batch_with_potential_offsets(ctx):
         l.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1, 4], ids=["fp64", "fp32", "f16x3"])
+def test_batch_mixing_periodic_and_free_boxes(ctx, precision):
+    """one batch of periodic and free boxes of different sizes (the kernel
+    sequence is chosen for the batch, the boundary conditions per box):
+    FP64 within 1e-12 per element and in the energies; FP32 and F16X3
+    within 1e-4 of max|hpsi| per orbital and 1e-6 in the energies"""
+    regs = [dense_region("P", (23, 19, 21), (0.45, 0.4, 0.5)),
+            dense_region("F", (11, 9, 13), (0.4, 0.45, 0.5)),
+            mixed_sphere_region((20, 18, 22), 0.42, 7),
+            dense_region("P", (17, 17, 17), (0.4,) * 3),
+            dense_region("F", (29, 21, 17), (0.45,) * 3)]
+    lrs = [ctx.region(r) for r in regs]
+    psis = [random_psi(r, 1, 90 + i)[0] for i, r in enumerate(regs)]
+    pots = [random_pot(r, 95 + i) for i, r in enumerate(regs)]
+    psi_off = np.cumsum([0] + [p.size for p in psis])[:-1]
+    pot_off = np.cumsum([0] + [p.size for p in pots])[:-1]
+    d_psi = dev(np.concatenate(psis))
+    d_h = torch.zeros_like(d_psi)
+    potdim = np.array([r.nfine for r in regs], dtype=np.int32).ravel()
+    ek = torch.zeros(len(regs), dtype=torch.float64, device="cuda")
+    ep = torch.zeros_like(ek)
+    ctx.locham_batch(lrs, dev(np.concatenate(pots)), d_psi, d_h, ek, ep, pot_off=pot_off,
+                     potdim=potdim, psi_off=psi_off, precision=precision)
+    ctx.sync()
+    h, ekh, eph = d_h.cpu().numpy(), ek.cpu().numpy(), ep.cpu().numpy()
+    for i, r in enumerate(regs):
+        ho, eo, po = oracle.locham(r.geocode, r.n, r.nfl, r.nfu, r.hgrid, okeys(r), pots[i],
+                                   psis[i])
+        hi = h[psi_off[i]:psi_off[i] + psis[i].size]
+        ee = max(abs(ekh[i] - eo[0]) / abs(eo[0]), abs(eph[i] - po[0]) / abs(po[0]))
+        if precision == 0:
+            assert rel_err(hi, ho[0]) < TOL and ee < TOL
+        else:
+            assert np.abs(hi - ho[0]).max() < 1e-4 * np.abs(ho[0]).max() and ee < 1e-6
+    for l in lrs:
+        l.close()
+
+
 def test_potential_box_shift(ctx):
     """global potential, region placed by ishift (apply_potential_lr_nobounds
     with ishift), compared with the oracle on the extracted box"""
