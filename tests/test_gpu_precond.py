@@ -108,3 +108,24 @@ def test_precond_host_entry(ctx):
     lr.close()
     for i in range(2):
         assert np.abs(x[i] - exp[i]).max() < 1e-10 * np.abs(exp[i]).max()
+
+
+def test_precond_batch_mixing_periodic_and_free(ctx):
+    """one preconditioner call over orbitals of a periodic and a free region
+    (per-orbital cprecr and offsets) gives each orbital the oracle's result"""
+    regs = [CASES[0][1], CASES[1][1], CASES[0][1]]
+    lrs_u = {id(r): ctx.region(r) for r in regs}
+    lrs = [lrs_u[id(r)] for r in regs]
+    bs = [random_psi(r, 1, 40 + i)[0] for i, r in enumerate(regs)]
+    xs = [0.05 * random_psi(r, 1, 50 + i)[0] for i, r in enumerate(regs)]
+    off = np.cumsum([0] + [b.size for b in bs])[:-1]
+    cpr = np.array([0.3, 0.8, 1.1])
+    x = dev(np.concatenate(xs))
+    ctx.precond_cg(lrs, cpr, 5, x, dev(np.concatenate(bs)), psi_off=off)
+    got = x.cpu().numpy()
+    for i, r in enumerate(regs):
+        exp = oracle.precond_cg(r.geocode, r.n, r.hgrid, cpr[i], 5, okeys(r), xs[i], bs[i])
+        g = got[off[i]:off[i] + bs[i].size]
+        assert np.abs(g - exp).max() < 1e-10 * np.abs(exp).max()
+    for l in lrs_u.values():
+        l.close()
