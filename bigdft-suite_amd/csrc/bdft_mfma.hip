@@ -168,40 +168,47 @@ __device__ __forceinline__ void mma(const Batch& B, const OpTab& t, int o0,
   mma_sweep<NTERM, TABB, 1>(B, t, o0, xh, xl, XS, 0, 1, &acc);
 }
 
+// D = T X with X read from a row-major plane: B[k][n] = X[k * XS + n0 + n]
+// (the K steps run down the rows), so an operator along the rows needs no
+// transposed copy; lane l gathers rows 8(l/32)..+8 of the step in column
+// n0 + l%32.  Same fragments, products and orientation as mma<NTERM, false>.
+template <int NTERM>
+__device__ __forceinline__ void mma_rows(const Batch& B, const OpTab& t, int o0,
+                                         const __bf16* __restrict__ xh,
+                                         const __bf16* __restrict__ xl, int XS, int n0,
+                                         f32x16& acc) {
+  const int lane = threadIdx.x & 63;
+  const uint2 mq = B.tmq[t.moff + (o0 >> 5)];
+  unsigned mask = mq.x;
+  const bf16x8* fp = (const bf16x8*)B.tfrag + (size_t)mq.y * 192 + lane;
+  const int c0 = 8 * (lane >> 5) * XS + n0 + (lane & 31);
+  while (mask) {
+    const int s = __builtin_ctz(mask);
+    mask &= mask - 1;
+    bf16x8 f[3];
+    load_frag<NTERM>(fp, f);
+    fp += 192;
+    const __bf16* ph = xh + c0 + 16 * s * XS;
+    const __bf16* pl = xl + c0 + 16 * s * XS;
+    bf16x8 h8, l8;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      h8[u] = ph[u * XS];
+      l8[u] = pl[u * XS];
+    }
+    acc = mf<NTERM>(f[0], h8, acc);
+    acc = mf<NTERM>(f[0], l8, acc);
+    acc = mf<NTERM>(f[1], h8, acc);
+    if (NTERM == 4) acc = mf<NTERM>(f[2], h8, acc);
+  }
+}
+
 template <int NTERM>
 __device__ __forceinline__ void mfma_block(const Batch& B, const OpTab& t, int m0,
                                            const __bf16* __restrict__ xh,
                                            const __bf16* __restrict__ xl, int XS,
                                            f32x16& acc) {
   mma<NTERM, false>(B, t, m0, xh, xl, XS, acc);
-}
-
-// Transpose of a 32 x 32 block of a line bundle through the matrix cores:
-// D = X I with X = the lines' columns n0..n0+31, so D[m = line][n = column]
-// holds hi + lo exactly (FP32), and lane l gets column n0 + l%32 of the 32
-// lines, four consecutive lines per register group.
-template <int M>
-__device__ __forceinline__ void mma_ident(const __bf16* __restrict__ xh,
-                                          const __bf16* __restrict__ xl, int XS, int n0,
-                                          f32x16& acc) {
-  const int lane = threadIdx.x & 63;
-  const int kh = 8 * (lane >> 5);
-  const int n = n0 + (lane & 31);
-  const __bf16* ah = xh + (lane & 31) * XS + kh;
-  const __bf16* al = xl + (lane & 31) * XS + kh;
-  const __bf16 one = M == 16 ? __builtin_bit_cast(__bf16, (_Float16)1.0f) : (__bf16)1.0f;
-  const __bf16 zero = (__bf16)0.0f;
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int k0 = n0 + 16 * q;
-    bf16x8 id;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) id[u] = k0 + kh + u == n ? one : zero;
-    const bf16x8 xh8 = *(const bf16x8*)(ah + k0);
-    const bf16x8 xl8 = *(const bf16x8*)(al + k0);
-    acc = mf<M>(xh8, id, acc);
-    acc = mf<M>(xl8, id, acc);
-  }
 }
 
 // The same product for up to NBMAX line bundles at once (bundle j at
@@ -683,13 +690,13 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
 //   K  = (Ktilde_x + Ktilde_y) W      (kinetic of the plane)
 //   T2 = G_y G_x W                    (real space in x, y; wavelet z)
 // LDS: R = the W plane as lines j2 [j2][j1] (bf16 hi, lo); C = G_x W as lines
-// p1 [p1][j2]; after G_y has consumed C the same space holds W^T [j1][j2]
-// (made by the matrix cores, mma_ident) for the y kinetic.
-// Work units: one wave per (operator, 32-row output block), sweeping all line
-// bundles of the plane, so each table fragment is read once per plane.
+// p1 [p1][j2].
+// Two phases: the kinetic tiles and the G_x units (both read R only), then
+// G_y (reads C).
 // Orientation: x kinetic with the table as B (rows j2, lanes j1 -> K stored
-// along x); y kinetic with the table as A (rows j2, lanes j1 -> added to K
-// along x); G_x as B (lanes p1 -> columns of C); G_y as A (lanes p1 -> T2).
+// along x); y kinetic with the table as A and R's rows as the K steps (rows
+// j2, lanes j1 -> added to K along x); G_x as B (lanes p1 -> columns of C);
+// G_y as A (lanes p1 -> T2).
 // ---------------------------------------------------------------------------
 template <int NT4>
 __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
@@ -708,7 +715,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const int nC = max(rup(L1, 32), KX);
   __bf16* rh = (__bf16*)smem;  // [j2 * XR + j1], j2 < nR, j1 < KX (zero padded)
   __bf16* rl = rh + nR * XR;
-  __bf16* ch = rl + nR * XR;   // [p1 * XC + j2] then [j1 * XC + j2]
+  __bf16* ch = rl + nR * XR;   // [p1 * XC + j2]
   __bf16* cl = ch + nC * XC;
   // load the plane, two x-neighbours per thread (W1 is even)
   {
@@ -750,22 +757,37 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const int nobG = tx.Dp / 32;      // output blocks p1 of G_x
   float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   const float ksx = (float)bd.kscale[0] * isc, ksy = (float)bd.kscale[1] * isc;
-  // G_x -> C
+  // The kinetic tiles and G_x -> C: both read only R, so they share one
+  // phase.  nK kinetic tiles (two operator sweeps each) come first, then the
+  // nG (output block, line block) units of G_x (one sweep); the waves take
+  // them in snake order (wave w gets w, 2 NW - 1 - w, 2 NW + w, ...), which
+  // evens out the sweeps per wave.
+  // Kinetic tile (j2 block lb, j1 block ob): x part from R with the table as
+  // B operand, y part down R's rows with the table as A operand; both land
+  // as rows j2, lanes j1, and K is stored once along x.
   {
-    const int ch2 = task_chunks(nobG, nlb, NW);
-    for (int q = wv; q < nobG * ch2; q += NW) {
-      const Task tk = task_of(q, ch2, nlb);
-      const int ob = tk.u;
-      for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBF) {
-        const int nb = min(MBF, tk.lb_end - lb0);
-        f32x16 acc[MBF] = {};
-        mma_multi<NT4, true, MBF>(B, tx, 32 * ob, rh + 32 * lb0 * XR, rl + 32 * lb0 * XR, XR,
-                                 32 * XR, nb, acc);
+    const int nK = nlb * nobK, nT = nK + nobG * nlb;
+    for (int r = 0; r * NW < nT; ++r) {
+      const int q = r * NW + ((r & 1) ? NW - 1 - wv : wv);
+      if (q >= nT) continue;
+      if (q < nK) {
+        const int lb = q / nobK, ob = q - lb * nobK;
+        f32x16 ax = {}, ay = {};
+        mma<NT4, true>(B, kx, 32 * ob, rh + 32 * lb * XR, rl + 32 * lb * XR, XR, ax);
+        mma_rows<NT4>(B, ky, 32 * lb, rh, rl, XR, 32 * ob, ay);
+        const int j1 = 32 * ob + (lane & 31);
+        if (j1 < W1) {
 #pragma unroll
-        for (int j = 0; j < MBF; ++j) {
-          if (j >= nb) break;
-          store_cols<NT4>(acc[j], ch, cl, XC, 32 * ob + (lane & 31), 32 * (lb0 + j));
+          for (int rr = 0; rr < 16; ++rr) {
+            const int j2 = 32 * lb + drow(rr, lane);
+            if (j2 < W2) K[j1 + W1 * j2] = ax[rr] * ksx + ay[rr] * ksy;
+          }
         }
+      } else {
+        const int g = q - nK, ob = g / nlb, lb = g - ob * nlb;
+        f32x16 acc = {};
+        mma<NT4, true>(B, tx, 32 * ob, rh + 32 * lb * XR, rl + 32 * lb * XR, XR, acc);
+        store_cols<NT4>(acc, ch, cl, XC, 32 * ob + (lane & 31), 32 * lb);
       }
     }
   }
@@ -795,32 +817,6 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
             }
           }
         }
-      }
-    }
-  }
-  __syncthreads();
-  // W^T into the C space
-  for (int q = wv; q < nlb * nobK; q += NW) {
-    const int lb = q / nobK, nb = q - lb * nobK;
-    f32x16 acc = {};
-    mma_ident<NT4>(rh + 32 * lb * XR, rl + 32 * lb * XR, XR, 32 * nb, acc);
-    store_cols<NT4>(acc, ch, cl, XC, 32 * nb + (lane & 31), 32 * lb);
-  }
-  __syncthreads();
-  // kinetic tiles (j2 block lb, j1 block ob): x part from R with the table as
-  // B operand, y part from W^T with the table as A operand; both land as
-  // rows j2, lanes j1, and K is stored once along x
-  for (int q = wv; q < nlb * nobK; q += NW) {
-    const int lb = q / nobK, ob = q - lb * nobK;
-    f32x16 ax = {}, ay = {};
-    mma<NT4, true>(B, kx, 32 * ob, rh + 32 * lb * XR, rl + 32 * lb * XR, XR, ax);
-    mma<NT4, false>(B, ky, 32 * lb, ch + 32 * ob * XC, cl + 32 * ob * XC, XC, ay);
-    const int j1 = 32 * ob + (lane & 31);
-    if (j1 < W1) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int j2 = 32 * lb + drow(r, lane);
-        if (j2 < W2) K[j1 + W1 * j2] = ax[r] * ksx + ay[r] * ksy;
       }
     }
   }
