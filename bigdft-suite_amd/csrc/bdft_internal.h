@@ -111,6 +111,72 @@ __device__ __forceinline__ int box_first_tile(const TileTable& tt, int b) {
   return tt.uniform > 0 ? b * tt.uniform : tt.start[b];
 }
 
+// hpsi += compress(O + K) over the x-y plane j3 of a box: the four
+// components of each coarse point (i1, i2) of the plane (s3 = j3 >= N3 fixed)
+// added at their compressed positions, compress_and_accumulate_mixed
+// (bigdft/src/wavelib/un-compress.f90:2174-2238).  O: the plane's operator
+// output [j2 * RO + j1]; Kp: its kinetic term [j1 + W1 * j2]; each sum is
+// formed in TS.  U points per thread are in flight at once: every load of a
+// step (indices, O, K, the old hpsi values) comes before its stores, so the
+// memory latencies of the U points overlap.  No two points share an hpsi
+// entry, and the two planes j3, j3 + N3 of a coarse plane write disjoint
+// components.
+template <typename TS, int U, int NT, typename TO, typename TK>
+__device__ __forceinline__ void compress_plane(const BoxDesc& bd, int j3, const TO* O, int RO,
+                                               const TK* Kp, double* hpsi) {
+  const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], W1 = 2 * N1;
+  const int s3 = j3 >= N3;
+  const int i3 = j3 - s3 * N3;
+  const int npl = N1 * N2;
+  double* hc = hpsi + bd.psi_off;
+  double* hf = hpsi + bd.psi_off + bd.nvctr_c + 3 * s3;  // components 1-3 or 4-7
+  const float in1 = 1.0f / (float)N1;
+  for (int i0 = threadIdx.x; i0 < npl; i0 += NT * U) {
+    int ci[U], fi[U];
+    double v[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NT;
+      ci[u] = -1;
+      fi[u] = -1;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[u][s] = 0.0;
+      if (i < npl) {
+        const int i2 = qdiv(i, N1, in1), i1 = i - i2 * N1;
+        const int lin = i + npl * i3;
+        ci[u] = s3 ? -1 : (bd.cinv ? bd.cinv[lin] : lin);
+        fi[u] = bd.finv ? bd.finv[lin] : lin;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int j1 = i1 + N1 * (s & 1), j2 = i2 + N2 * (s >> 1);
+          v[u][s] = (double)((TS)O[j2 * RO + j1] + (TS)Kp[j1 + W1 * j2]);
+        }
+      }
+    }
+    // s3 = 0: coarse value s = 0, fine components 1-3 from s = 1..3;
+    // s3 = 1: fine components 4-7 from s = 0..3
+    const int sf = 1 - s3;
+    double old[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double* f = hf + 7LL * max(fi[u], 0) - sf;
+      old[u][0] = ci[u] >= 0 ? hc[max(ci[u], 0)] : (s3 && fi[u] >= 0 ? f[0] : 0.0);
+#pragma unroll
+      for (int s = 1; s < 4; ++s) old[u][s] = fi[u] >= 0 ? f[s] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      double* f = hf + 7LL * max(fi[u], 0) - sf;
+      if (ci[u] >= 0) hc[ci[u]] = old[u][0] + v[u][0];
+      if (fi[u] >= 0) {
+        if (s3) f[0] = old[u][0] + v[u][0];
+#pragma unroll
+        for (int s = 1; s < 4; ++s) f[s] = old[u][s] + v[u][s];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Filter tables (values of the reference; host copies live in bdft_filters.cpp)
 // fil2(j,s), j=-14..16: bigdft/src/orbitals/v.inc:13-76

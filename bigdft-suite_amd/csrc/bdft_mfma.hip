@@ -927,42 +927,10 @@ __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   }
   __syncthreads();
   // + K, compress-accumulate (un-compress.f90:2174-2238 mixed, 2105-2170
-  // standard: the same scatter in this layout); one coarse point per thread,
-  // its four components of this plane (s3 fixed by j3) together
-  const int s3 = j3 >= N3;
-  const int i3 = j3 - s3 * N3;
-  const float* Kp = (const float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-  double* hc = B.hpsi + bd.psi_off;
-  double* hf = B.hpsi + bd.psi_off + bd.nvctr_c;
-  const int npl = N1 * N2;
-  const float in1 = 1.0f / (float)N1;
-  for (int i = threadIdx.x; i < npl; i += NTMP) {
-    const int i2 = qdiv(i, N1, in1), i1 = i - i2 * N1;
-    const int lin = i + npl * i3;
-    const int ci = bd.cinv ? bd.cinv[lin] : lin;
-    const int fi = bd.finv ? bd.finv[lin] : lin;
-    double v[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int jj1 = i1 + N1 * (s & 1), jj2 = i2 + N2 * (s >> 1);
-      v[s] = (double)O[jj2 * RO + jj1] + (double)Kp[jj1 + W1 * jj2];
-    }
-    if (s3 == 0) {
-      if (ci >= 0) hc[ci] += v[0];
-      if (fi >= 0) {
-        double* f = hf + 7LL * fi;
-        f[0] += v[1];
-        f[1] += v[2];
-        f[2] += v[3];
-      }
-    } else if (fi >= 0) {
-      double* f = hf + 7LL * fi + 3;
-      f[0] += v[0];
-      f[1] += v[1];
-      f[2] += v[2];
-      f[3] += v[3];
-    }
-  }
+  // standard: the same scatter in this layout)
+  compress_plane<double, 4, NTMP>(bd, j3, O, RO,
+                                  (const float*)B.K + bd.w_off + (long long)W1 * W2 * j3,
+                                  B.hpsi);
 }
 
 // ---------------------------------------------------------------------------

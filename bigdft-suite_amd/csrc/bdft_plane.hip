@@ -489,34 +489,8 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   // out = O + K, accumulated into hpsi at the compressed positions of the
   // plane's four components (s3 fixed by j3), compress_and_accumulate_mixed
   // (un-compress.f90:2174-2238)
-  const int s3 = j3 >= N3;
-  const int i3 = j3 - s3 * N3;
-  const T* Kp = (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-  const int npl = N1 * N2;
-  double* hc = B.hpsi + bd.psi_off;
-  double* hf = B.hpsi + bd.psi_off + bd.nvctr_c;
-  const float inv1 = 1.0f / (float)N1;
-  for (int i = tid; i < npl; i += NTP) {
-    int i2 = (int)((float)i * inv1);
-    if ((i2 + 1) * N1 <= i) ++i2;
-    if (i2 * N1 > i) --i2;
-    const int i1 = i - i2 * N1;
-    const int lin = i + npl * i3;
-    const int ci = bd.cinv ? bd.cinv[lin] : lin;
-    const int fi = bd.finv ? bd.finv[lin] : lin;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int s1 = s & 1, s2 = s >> 1;
-      const int j1 = i1 + N1 * s1, j2 = i2 + N2 * s2;
-      const int t = s1 + 2 * s2 + 4 * s3;
-      const double v = (double)(O[j2 * RO + j1] + Kp[j1 + W1 * j2]);
-      if (t == 0) {
-        if (ci >= 0) hc[ci] += v;
-      } else if (fi >= 0) {
-        hf[7LL * fi + t - 1] += v;
-      }
-    }
-  }
+  compress_plane<T, (NTP >= 768 ? 3 : 2), NTP>(
+      bd, j3, O, RO, (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3, B.hpsi);
 }
 
 // LDS bytes of the two plane kernels for the largest box extents
