@@ -211,11 +211,17 @@ __device__ __forceinline__ void mfma_block(const Batch& B, const OpTab& t, int m
   mma<NTERM, false>(B, t, m0, xh, xl, XS, acc);
 }
 
+// Bundles per sweep of the forward plane kernel's G_y phase: 2 (one
+// fragment load feeds two bundles) except with the fourth product term,
+// where the extra fragment registers would leave room for only one
+// work-group per CU.
+template <int NTERM>
+constexpr int mbf() { return NTERM == 4 ? 1 : 2; }
+constexpr int MBB = 2;  // bundles per sweep: backward plane kernel
+
 // The same product for up to NBMAX line bundles at once (bundle j at
 // xh + j*LS), so that each table fragment is fetched once per K step for all
 // of them: acc[j] += T[o0..o0+31, :] X_j (orientation as in mma).
-constexpr int MBF = 1;  // bundles per sweep: forward plane kernel
-constexpr int MBB = 2;  // bundles per sweep: backward plane kernel
 template <int NTERM, bool TABB, int NBMAX>
 __device__ __forceinline__ void mma_multi(const Batch& B, const OpTab& t, int o0,
                                           const __bf16* __restrict__ xh,
@@ -800,6 +806,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
     for (int q = wv; q < nob * ch3; q += NW) {
       const Task tk = task_of(q, ch3, nl1);
       const int ob = tk.u;
+      constexpr int MBF = mbf<NT4>();
       for (int lb0 = tk.lb_begin; lb0 < tk.lb_end; lb0 += MBF) {
         const int nb = min(MBF, tk.lb_end - lb0);
         f32x16 acc[MBF] = {};
