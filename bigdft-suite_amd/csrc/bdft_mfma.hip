@@ -764,8 +764,9 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   const float ksx = (float)bd.kscale[0] * isc, ksy = (float)bd.kscale[1] * isc;
   // The kinetic tiles and G_x -> C: both read only R, so they share one
-  // phase.  nK kinetic tiles (two operator sweeps each) come first, then the
-  // nG (output block, line block) units of G_x (one sweep); the waves take
+  // phase.  The nK kinetic tiles (two operator sweeps each) come first, then
+  // the nobG x nlb (output block, line block) units of G_x (one sweep); the
+  // waves take
   // them in snake order (wave w gets w, 2 NW - 1 - w, 2 NW + w, ...), which
   // evens out the sweeps per wave.
   // Kinetic tile (j2 block lb, j1 block ob): x part from R with the table as
