@@ -1,16 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the batched local-Hamiltonian apply on one MI355X.
 
-Workload (default): 256 orbitals on a periodic 48^3 coarse grid (96^3 fine),
-hgrid 0.35 bohr, coefficients N(0,1) normalised per orbital, shared
-potential = sum of 8 Gaussians (centres uniform in the cell, depth U[-5,-1]
-Ha, width U[0.5,2] bohr, minimum-image distance).  One step = one full
-application hpsi += H_loc psi (+ ekin, epot per orbital) over the batch,
-replayed from a captured hipGraph.  Inputs are resident in HBM before the
-timed region.  Before each precision is timed, the launch options (kernel
-sequence, z bundling, units of the mixed sequences' z potential and
-backward passes) are chosen by timing the candidates on the same batch
-(bigdft_hip.autotune; --no-autotune keeps the automatic choice).
+Headline workload (default, the largest single-GPU one): 2048 free-BC
+localisation boxes with N_d ~ U{16..40} coarse points per axis, hgrid 0.4
+bohr, coefficients N(0,1) times exp(-r^2/32), each box's potential sampled
+from one 8-Gaussian field (1.26e9 grid points per apply), in the split-FP16
+matrix-core precision (F16X3) next to FP64, FP32 and BF16X4.  The same JSON
+line carries the other workloads (--also): the periodic 48^3 x 256 and
+64^3 x 128 batches (8-Gaussian potential, minimum-image distance) and 1024
+compressed spheres in a 96^3 global box, each with its timings, errors
+against FP64, roofline entry and CPU baseline.
+
+One step = one full application hpsi += H_loc psi (+ ekin, epot per
+orbital) over the batch, replayed from a captured hipGraph.  Inputs are
+resident in HBM before the timed region.  Before each precision is timed,
+the launch options (kernel sequence, z bundling, units of the mixed
+sequences' z potential and backward passes) are chosen by timing the
+candidates on the same batch (bigdft_hip.autotune; --no-autotune keeps the
+automatic choice).  Errors are per orbital against the FP64 GPU run, with
+no floor; FP64 is checked against the oracle on a seeded sample, which is
+also the timed CPU baseline (oracle on the host cores, bounded by
+--cpu-seconds per workload).  Progress lines go to stderr.
 
 Other workloads (--workload): precond (the conjugate-gradient
 preconditioner), density (rho += occ psir^2) and transfers (global <->
@@ -36,30 +46,35 @@ PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 matrix
 PEAK_HBM_GBS = 8000.0
 
-# Per stage of each code path: (algorithmic flops per fine point, elements of
-# the working type read+written per fine point, extra FP64 bytes per point).
-# Flops follow the reference's count 524 = 192 (synthesis+magic) + 46*3
-# (kinetic) + 192 (magic^T+analysis) + 2 (bigdft/src/memguess.f90:2269):
-# one 1-D grow or shrink pass is 64, one kinetic pass 46, V*psi 2.
-# uncompress reads psi (8 B) and writes the working grid; compress reads and
-# writes hpsi (16 B).
-PATH_STAGES = {
-    "lines": {"uncompress": (0, 1, 8), "x_grow_kin": (110, 3, 0),
-              "y_grow": (64, 2, 0), "y_kin": (46, 3, 0), "z_kin": (46, 3, 0),
-              "z_pot": (130, 2, 0), "y_shrink": (64, 2, 0),
-              "x_shrink": (64, 3, 0), "compress": (0, 1, 16)},
-    "plane": {"uncompress": (0, 1, 8), "xy_fwd": (220, 3, 0),
-              "z_kin": (46, 3, 0), "z_pot": (130, 2, 0),
-              "xy_bwd_compress": (128, 2, 16)},
-    # matrix-core plane-fused sequence: the forward plane kernel writes K
-    # (x+y kinetic) and T2 from one read of W
-    "mfma": {"uncompress": (0, 1, 8), "xy_fwd": (220, 3, 0), "z_kin": (46, 3, 0),
-             "z_pot": (130, 2, 0), "xy_bwd_compress": (128, 2, 16)},
-    # matrix-core per-pass sequence (boxes whose planes do not fit in LDS)
-    "mfma_passes": {"uncompress": (0, 1, 8), "x_grow_kin": (46, 2, 0),
-                    "y_kin": (46, 3, 0), "z_kin": (46, 3, 0), "xy_fwd": (128, 2, 0),
-                    "z_pot": (130, 2, 0), "xy_bwd_compress": (128, 2, 16)},
-}
+# Algorithmic work of each timed stage.  Flops follow the reference's count
+# per fine grid point, 524 = 192 (synthesis+magic) + 46*3 (kinetic) + 192
+# (magic^T+analysis) + 2 (bigdft/src/memguess.f90:2269): one 1-D grow or
+# shrink pass is 64, one kinetic pass 46, V*psi 2.
+STAGE_FLOPS = {"uncompress": 0, "zero": 0, "xy_fwd": 220, "z_kin": 46, "z_pot": 130,
+               "xy_bwd_compress": 128, "x_grow_kin": 110, "y_grow": 64, "y_kin": 46,
+               "y_shrink": 64, "x_shrink": 64, "compress": 0}
+# per-pass matrix-core sequence: the x kinetic alone, then one plane grow
+STAGE_FLOPS_MFMA_PASSES = dict(STAGE_FLOPS, x_grow_kin=46, xy_fwd=128)
+
+
+def stage_bytes(batch, esz):
+    """compulsory HBM bytes per apply of each stage, from the batch's box
+    geometry: working grids of esz bytes -- W (wavelet volume, 8 N1 N2 N3
+    per box), K (same), T2 (L1 L2 2 N3), T1 (L1 2 N2 2 N3, line passes
+    only) -- FP64 psi read once (8 B per coefficient), hpsi read and written
+    once (16 B), each potential value read once (8 B)"""
+    regs = batch["regions"]
+    W = sum(8 * (r.n[0] + 1) * (r.n[1] + 1) * (r.n[2] + 1) for r in regs)
+    T2 = sum(r.nfine[0] * r.nfine[1] * 2 * (r.n[2] + 1) for r in regs)
+    T1 = sum(r.nfine[0] * 4 * (r.n[1] + 1) * (r.n[2] + 1) for r in regs)
+    nc = int(batch["ncoef"].sum())
+    V = 8 * min(int(batch["pot"].numel()), int(sum(r.n_isf for r in regs)))
+    return {"zero": esz * W, "uncompress": 8 * nc + esz * W,
+            "xy_fwd": esz * (2 * W + T2), "z_kin": esz * 3 * W, "z_pot": esz * 2 * T2 + V,
+            "xy_bwd_compress": esz * (T2 + W) + 16 * nc,
+            "x_grow_kin": esz * (2 * W + T1), "y_grow": esz * (T1 + T2),
+            "y_kin": esz * 3 * W, "y_shrink": esz * (T2 + T1),
+            "x_shrink": esz * (T1 + 2 * W), "compress": esz * W + 16 * nc}
 
 
 def path_of(prec, plane_enabled, stages):
@@ -70,32 +85,36 @@ def path_of(prec, plane_enabled, stages):
 
 
 def measured_traffic(precision, config, stage):
-    """HBM bytes per launch of a stage from the committed rocprofv3 counter
+    """HBM bytes per apply of a stage from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a
     device copy of known size), or None when there is no summary"""
-    if config != "c2":
-        return None
-    path = os.path.join(ROOT, "profiles", f"rocprof_{precision}_periodic48_256orb.md")
-    if not os.path.exists(path):
-        return None
-    text = open(path).read()
-    if "```json" not in text:
-        return None
-    blob = text.split("```json", 1)[1].split("```", 1)[0]
-    try:
-        return json.loads(blob)["bytes_per_launch"].get(stage)
-    except (ValueError, KeyError):
-        return None
+    pdir = os.path.join(ROOT, "profiles")
+    for f in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        if not f.startswith(f"rocprof_{precision}_") or not f.endswith(".md"):
+            continue
+        text = open(os.path.join(pdir, f)).read()
+        if "```json" not in text:
+            continue
+        blob = text.split("```json", 1)[1].split("```", 1)[0]
+        try:
+            data = json.loads(blob)
+        except ValueError:
+            continue
+        if data.get("config") == config and data.get("precision") == precision:
+            return data["bytes_per_launch"].get(stage), f"profiles/{f}"
+    return None, None
 
 
-def roofline(path, avg, units, esz, peak_c):
-    """roofline entry of the dominant (longest) kernel stage"""
-    tab = PATH_STAGES[path]
-    dom = max((k for k in avg if k in tab and avg[k] > 0), key=lambda k: avg[k])
-    fl, el, extra = tab[dom]
+def roofline(path, avg, units, bytes_of, peak_c):
+    """roofline entry of the dominant (longest) kernel stage: its
+    algorithmic bytes (bytes_of, from the box geometry) and flops (the
+    reference's per-point count) over its measured duration"""
+    fl_tab = STAGE_FLOPS_MFMA_PASSES if path == "mfma_passes" else STAGE_FLOPS
+    dom = max((k for k in avg if k in bytes_of and avg[k] > 0.02), key=lambda k: avg[k])
+    fl = fl_tab[dom]
     t_dom = avg[dom] * 1e-3
     ach_c = fl * units / t_dom / 1e12
-    ach_b = (el * esz + extra) * units / t_dom / 1e9
+    ach_b = bytes_of[dom] / t_dom / 1e9
     if ach_c / peak_c >= ach_b / PEAK_HBM_GBS:
         roof = {"bound": "mfma", "achieved": ach_c, "peak": peak_c,
                 "unit": "TFLOP/s", "frac": ach_c / peak_c,
@@ -106,7 +125,8 @@ def roofline(path, avg, units, esz, peak_c):
                 "flop_TFLOPs": ach_c, "flop_frac": ach_c / peak_c}
     roof.update({"traffic": None, "kernel": dom, "kernel_ms": avg[dom],
                  "path": path, "flops_per_pt": fl,
-                 "bytes_per_pt": el * esz + extra})
+                 "algorithmic_bytes": bytes_of[dom],
+                 "bytes_per_pt": bytes_of[dom] / units})
     return roof
 
 
@@ -128,29 +148,13 @@ CONFIGS = {
 
 def build_batch(cfg, seed, device):
     """regions, device arrays and offsets of a configuration"""
-    import numpy as np
-
-    from bigdft_hip import inputs, workloads
-    from bigdft_hip.descriptors import dense_region
+    from bigdft_hip import workloads
     if cfg["kind"] == "uneven":
         return workloads.uneven_free(cfg["norb"], seed, device=device)
     if cfg["kind"] == "spheres":
         return workloads.sphere_batch(cfg["norb"], seed, device=device)
-    n, h, norb = cfg["n"], cfg["hgrid"], cfg["norb"]
-    reg = dense_region(cfg["geocode"], (n, n, n), (h, h, h))
-    psi = inputs.coefficients(norb, reg.ncoef, seed, device=device,
-                              normalise=True).reshape(-1)
-    cell = [(n + 1) * h] * 3
-    gp = inputs.gaussian_params(seed + 1, cell)
-    pot = inputs.gaussian_field(gp, inputs.fine_coords("P", (n, n, n), (h, h, h),
-                                                       device=device),
-                                cell=cell).contiguous()
-    ncoef = np.full(norb, reg.ncoef, dtype=np.int64)
-    return dict(regions=[reg] * norb, psi=psi, ncoef=ncoef,
-                psi_off=np.arange(norb, dtype=np.int64) * reg.ncoef, pot=pot,
-                pot_off=np.zeros(norb, dtype=np.int64),
-                potdim=np.array(list(reg.nfine) * norb, dtype=np.int32), ishift=None,
-                units=reg.n_isf * norb)
+    return workloads.dense_periodic(cfg["n"], cfg["norb"], seed, hgrid=cfg["hgrid"],
+                                    device=device)
 
 
 # Algorithmic FP64 bytes per coefficient and CG step of the preconditioner
@@ -391,20 +395,39 @@ def bench_transfers(args):
     ctx.close()
 
 
+def cpu_model():
+    """model name of the host CPU (first /proc/cpuinfo entry)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS),
+                    help="workload of the headline value (default: the largest "
+                         "single-GPU workload, 2048 uneven free boxes)")
+    ap.add_argument("--also", default="c2,c3,c5",
+                    help="other workloads carried in the same JSON line (comma "
+                         "separated, '' for none)")
     ap.add_argument("--precision", default="f16x3",
                     choices=["fp64", "fp32", "bf16x3", "bf16x4", "f16x3"],
                     help="arithmetic of the headline run (fp32: FP32 working "
                          "grids; bf16x4: split-BF16 matrix-core path)")
     ap.add_argument("--variants", default="fp64,fp32,bf16x4,f16x3",
                     help="precisions timed beside the headline")
+    ap.add_argument("--also-variants", default="fp64,fp32,f16x3",
+                    help="precisions timed on the other workloads")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
-                    help="time budget of the CPU baseline sample")
+                    help="time budget of the CPU baseline sample of each workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="locham",
@@ -424,41 +447,26 @@ def parse():
     return ap.parse_args()
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the "
-              "batch shards by orbital across GPUs but no multi-GPU run is "
-              "part of this build.", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "transfers":
-        bench_transfers(args)
-        return
+DTYPES = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16", "bf16x4": "bf16", "f16x3": "f16"}
+MATRIX = ("bf16x3", "bf16x4", "f16x3")
+
+
+def run_config(args, name, names, headline, ctx, stream, tune_cache, progress):
+    """time the precisions `names` on workload `name` (FP64 first: the
+    reference of the error columns), then check FP64 against the oracle and
+    time the oracle on a bounded sample (the CPU baseline)"""
     import numpy as np
     import torch
 
     import bigdft_hip
-
-    cfg = CONFIGS[args.config]
-    prec = bigdft_hip.PRECISIONS[args.precision]
+    cfg = CONFIGS[name]
     dev = "cuda"
+    t_build = time.perf_counter()
     batch = build_batch(cfg, args.seed, dev)
     regions, psi = batch["regions"], batch["psi"]
     norb, units = len(regions), batch["units"]
-
-    if args.copy_mb > 0:
-        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
-    ctx = bigdft_hip.Context(0)
-    stream = torch.cuda.Stream()
-    ctx.set_stream(stream)
-    torch.cuda.synchronize()
-    torch.cuda.set_stream(stream)   # one stream for torch and the library
+    progress(f"{name}: {norb} orbitals, {units} grid points, built in "
+             f"{time.perf_counter() - t_build:.1f} s")
     lr_of = {}
     for r in regions:
         if id(r) not in lr_of:
@@ -466,20 +474,6 @@ def main():
     lrs = [lr_of[id(r)] for r in regions]
     bargs = dict(pot_off=batch["pot_off"], potdim=batch["potdim"],
                  ishift=batch["ishift"], psi_off=batch["psi_off"])
-    if args.workload in ("precond", "density"):
-        (bench_precond if args.workload == "precond" else bench_density)(
-            args, cfg, batch, ctx, lrs, stream)
-        for lr in lr_of.values():
-            lr.close()
-        ctx.close()
-        return
-
-    tune_cache = None
-    if args.autotune:
-        import tempfile
-
-        from bigdft_hip import autotune
-        tune_cache = autotune.TuneCache(os.path.join(tempfile.mkdtemp(), "tune.json"))
     chosen = {}
 
     def timed(prec):
@@ -487,6 +481,7 @@ def main():
         graph-replay timing, then per-stage HIP-event timing, then one
         clean application for the error analysis"""
         if tune_cache is not None:
+            from bigdft_hip import autotune
             _, opts, _ = autotune.tune(ctx, lrs, batch["pot"], psi, prec, cache=tune_cache,
                                        **bargs)
             chosen[prec] = opts
@@ -520,63 +515,58 @@ def main():
         return ms, wall, {k: v / nl for k, v in stages.items()}, h, ek, ep
 
     _, plane_enabled = ctx.info()
-
-    def plane_seq(prec):
-        """plane-fused sequence allowed for the run of this precision"""
-        o = chosen.get(prec)
-        return plane_enabled if o is None else o.get("line_passes", 0) == 0
-
+    if "fp64" not in names:
+        names = ["fp64"] + names
     ms64, wall64, st64, h64, ek64, ep64 = timed(bigdft_hip.PREC_FP64)
+    progress(f"{name}: fp64 {ms64:.3f} ms per apply")
     hmax = h64.abs().max()
 
     def errors(hm, ekm, epm):
         d = (hm - h64).abs()
         guard = torch.maximum(h64.abs(), 1e-3 * hmax)
+        rk = (ekm - ek64).abs() / ek64.abs()
+        rp = (epm - ep64).abs() / ep64.abs()
         return {"hpsi_max_abs_over_max": float(d.max() / hmax),
                 "hpsi_rms_over_rms": float(torch.sqrt((d * d).mean() /
                                                       (h64 * h64).mean())),
                 "hpsi_max_rel_guarded": float((d / guard).max()),
-                "ekin_max_rel": float(((ekm - ek64).abs() / ek64.abs()).max()),
-                "epot_max_rel": float(((epm - ep64).abs() / ep64.abs()).max()),
-                "reference": "FP64 GPU run on the same inputs"}
+                "ekin_max_rel": float(rk.max()), "epot_max_rel": float(rp.max()),
+                "ekin_rms_rel": float(torch.sqrt((rk * rk).mean())),
+                "epot_rms_rel": float(torch.sqrt((rp * rp).mean())),
+                "orbitals_above_1e-6": int(((rk > 1e-6) | (rp > 1e-6)).sum()),
+                "ekin_sum_rel": float(abs(ekm.sum() - ek64.sum()) / abs(ek64.sum())),
+                "epot_sum_rel": float(abs(epm.sum() - ep64.sum()) / abs(ep64.sum())),
+                "reference": "FP64 GPU run on the same inputs, per orbital, no floor"}
 
     variants = {}
-    names = [v for v in args.variants.split(",") if v]
-    if args.precision not in names:
-        names.append(args.precision)
-    for name in names:
-        if name == "fp64":
-            ms, wall, avg, hm, ekm, epm = ms64, wall64, st64, h64, ek64, ep64
-            err = None
+    for vname in names:
+        prec = bigdft_hip.PRECISIONS[vname]
+        if vname == "fp64":
+            ms, wall, avg, err = ms64, wall64, st64, None
         else:
-            ms, wall, avg, hm, ekm, epm = timed(bigdft_hip.PRECISIONS[name])
+            ms, wall, avg, hm, ekm, epm = timed(prec)
             err = errors(hm, ekm, epm)
             del hm
-        vpath = path_of(name, plane_seq(bigdft_hip.PRECISIONS[name]), avg)
-        vesz = 8 if name == "fp64" else 4
-        vpeak = (PEAK_BF16_TFLOPS if name in ("bf16x3", "bf16x4", "f16x3") else
-                 PEAK_FP64_TFLOPS if name == "fp64" else PEAK_FP32_TFLOPS)
-        variants[name] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
-                          "launch_options": chosen.get(bigdft_hip.PRECISIONS[name]),
-                          "wall_s": round(wall, 4), "path": vpath,
-                          "stages_ms": {k: round(v, 4) for k, v in avg.items()},
-                          "roofline": roofline(vpath, avg, units, vesz, vpeak),
-                          "error_vs_fp64": err}
-    head = variants[args.precision]
-    ms_step, wall = head["ms_per_step"], head["wall_s"]
-    avg = head["stages_ms"]
-    esz = 8 if prec == bigdft_hip.PREC_FP64 else 4
-    if args.precision in ("bf16x3", "bf16x4", "f16x3"):
-        peak_c = PEAK_BF16_TFLOPS
-    else:
-        peak_c = PEAK_FP64_TFLOPS if esz == 8 else PEAK_FP32_TFLOPS
-    roof = roofline(head["path"], avg, units, esz, peak_c)
-    roof["traffic"] = measured_traffic(args.precision, args.config, roof["kernel"])
-    if roof["traffic"] is not None:
-        roof["traffic_bytes_per_pt"] = roof["traffic"] / units
-        roof["traffic_source"] = (f"profiles/rocprof_{args.precision}_periodic48_256orb.md "
-                                  "(FETCH_SIZE+WRITE_SIZE per dispatch, copy-calibrated)")
-
+            progress(f"{name}: {vname} {ms:.3f} ms per apply, energies within "
+                     f"{max(err['ekin_max_rel'], err['epot_max_rel']):.2e}")
+        o = chosen.get(prec)
+        plane = plane_enabled if o is None else o.get("line_passes", 0) == 0
+        vpath = path_of(vname, plane, avg)
+        esz = 8 if vname == "fp64" else 4
+        vpeak = (PEAK_BF16_TFLOPS if vname in MATRIX else
+                 PEAK_FP64_TFLOPS if vname == "fp64" else PEAK_FP32_TFLOPS)
+        roof = roofline(vpath, avg, units, stage_bytes(batch, esz), vpeak)
+        traffic, src = measured_traffic(vname, name, roof["kernel"])
+        if traffic is not None:
+            roof["traffic"] = traffic
+            roof["traffic_bytes_per_pt"] = traffic / units
+            roof["traffic_source"] = src + " (FETCH_SIZE+WRITE_SIZE per apply, copy-calibrated)"
+        variants[vname] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
+                           "speedup_vs_fp64": ms64 / ms,
+                           "launch_options": chosen.get(prec), "wall_s": round(wall, 4),
+                           "path": vpath,
+                           "stages_ms": {k: round(v, 4) for k, v in avg.items() if v > 0.02},
+                           "roofline": roof, "error_vs_fp64": err}
     # ---- parity on a sample + CPU baseline (oracle, host cores) ----
     parity, cpu = None, None
     if not args.no_cpu:
@@ -614,34 +604,113 @@ def main():
                   "fp64_ekin_max_rel": float(max(eks)),
                   "fp64_epot_max_rel": float(max(eps)), "reference": "oracle"}
         cpu = {"value": pts / ts, "unit": "grid-pt*orbital/s",
-               "cores": oracle.num_threads(), "kind": "port",
-               "sample": f"{done} of the {norb} orbitals (seeded order), oracle "
-                         f"(C+OpenMP restatement of the reference Fortran), "
-                         f"{ts:.1f} s"}
+               "cores": oracle.num_threads(), "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"{done} of the {norb} orbitals (seeded order, {pts} grid "
+                         f"points), oracle (C+OpenMP restatement of the reference "
+                         f"Fortran), {ts:.1f} s"}
+        progress(f"{name}: oracle parity on {done} orbitals, CPU {pts / ts:.3g} pts/s")
+    head = variants[headline] if headline in variants else variants["fp64"]
+    out = {"workload": cfg["workload"], "orbitals": norb, "grid_pts": units,
+           "precision": headline, "value": head["value"], "ms_per_step": head["ms_per_step"],
+           "speedup_vs_fp64": head["speedup_vs_fp64"], "error_vs_fp64": head["error_vs_fp64"],
+           "roofline": head["roofline"], "launch_options": head["launch_options"],
+           "wall_s": head["wall_s"], "variants": variants, "cpu_baseline": cpu,
+           "parity": parity}
+    for lr in lr_of.values():
+        lr.close()
+    del batch, h64, psi
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
 
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the "
+              "batch shards by orbital across GPUs but no multi-GPU run is "
+              "part of this build.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "transfers":
+        bench_transfers(args)
+        return
+    import torch
+
+    import bigdft_hip
+
+    def progress(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    ctx = bigdft_hip.Context(0)
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
+    torch.cuda.synchronize()
+    torch.cuda.set_stream(stream)   # one stream for torch and the library
+    if args.workload in ("precond", "density"):
+        cfg = CONFIGS[args.config]
+        batch = build_batch(cfg, args.seed, "cuda")
+        lr_of = {}
+        for r in batch["regions"]:
+            if id(r) not in lr_of:
+                lr_of[id(r)] = ctx.region(r)
+        lrs = [lr_of[id(r)] for r in batch["regions"]]
+        (bench_precond if args.workload == "precond" else bench_density)(
+            args, cfg, batch, ctx, lrs, stream)
+        for lr in lr_of.values():
+            lr.close()
+        ctx.close()
+        return
+
+    tune_cache = None
+    if args.autotune:
+        import tempfile
+
+        from bigdft_hip import autotune
+        tune_cache = autotune.TuneCache(os.path.join(tempfile.mkdtemp(), "tune.json"))
+    names = [v for v in args.variants.split(",") if v]
+    if args.precision not in names:
+        names.append(args.precision)
+    main_cfg = run_config(args, args.config, names, args.precision, ctx, stream, tune_cache,
+                          progress)
+    others = {}
+    also_names = [v for v in args.also_variants.split(",") if v]
+    if args.precision not in also_names:
+        also_names.append(args.precision)
+    for name in [c for c in args.also.split(",") if c and c != args.config]:
+        others[name] = run_config(args, name, also_names, args.precision, ctx, stream,
+                                  tune_cache, progress)
+    units = main_cfg["grid_pts"]
+    ms_step = main_cfg["ms_per_step"]
     out = {
-        "metric": METRIC, "value": units / (ms_step * 1e-3),
+        "metric": METRIC, "value": main_cfg["value"],
         "unit": "grid-pt*orbital/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16",
-                  "bf16x4": "bf16", "f16x3": "f16"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (seeded N(0,1) coefficients, 8-Gaussian potential)",
-        "config": {"workload": cfg["workload"], "orbitals": norb,
-                   "grid_pts": units, "precision": args.precision,
-                   "graph": True, "launch_options": head["launch_options"],
-                   "wall_s": wall},
-        "roofline": roof,
-        "speedup_vs_fp64": ms64 / ms_step,
-        "error_vs_fp64": head["error_vs_fp64"],
-        "variants": variants,
+        "config": {"workload": main_cfg["workload"], "name": args.config,
+                   "orbitals": main_cfg["orbitals"], "grid_pts": units,
+                   "precision": args.precision, "graph": True,
+                   "launch_options": main_cfg["launch_options"],
+                   "wall_s": main_cfg["wall_s"]},
+        "roofline": main_cfg["roofline"],
+        "speedup_vs_fp64": main_cfg["speedup_vs_fp64"],
+        "error_vs_fp64": main_cfg["error_vs_fp64"],
+        "variants": main_cfg["variants"],
         "apply_equiv_tflops": 524.0 * units / (ms_step * 1e-3) / 1e12,
-        "cpu_baseline": cpu,
-        "parity": parity,
+        "cpu_baseline": main_cfg["cpu_baseline"],
+        "parity": main_cfg["parity"],
+        "other_workloads": others,
     }
     print(json.dumps(out))
-    for lr in lr_of.values():
-        lr.close()
     ctx.close()
 
 

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4, PREC_F16X3 = 0, 1, 2, 3, 4
 # launch options (BDFT_OPT_* of bigdft_hip.h)
-OPTIONS = {"line_passes": 1, "zgroup": 2, "zpot": 3, "bwd": 4}
+OPTIONS = {"line_passes": 1, "zgroup": 2, "zpot": 3, "bwd": 4, "chunk_mb": 5}
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
               "bf16x4": PREC_BF16X4, "f16x3": PREC_F16X3}
 
@@ -67,7 +67,7 @@ def load_library(path=LIB_PATH):
                            P, P, ctypes.POINTER(P)],
         "bdft_lr_destroy": [P],
         "bdft_lr_info": [P, I3, ctypes.POINTER(ctypes.c_int64)],
-        "bdft_locham_d": [P, P, I, P, P, P, P, P, I],
+        "bdft_locham_d": [P, P, I, I, P, P, P, P, P, P, I],
         "bdft_locham_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
         "bdft_locham_conf_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, P, I],
         "bdft_density_d": [P, P, I, P, P, P, I, P, I],
@@ -78,7 +78,7 @@ def load_library(path=LIB_PATH):
                              ctypes.POINTER(P)],
         "bdft_plan_execute": [P],
         "bdft_plan_destroy": [P],
-        "bdft_local_hamiltonian": [P, P, I, P, P, P, P, P, I],
+        "bdft_local_hamiltonian": [P, P, I, I, P, P, P, P, P, P, I],
         "bdft_uncompress_d": [P, P, I, P, P],
         "bdft_compress_acc_d": [P, P, I, P, P],
         "bdft_sync": [P],
@@ -123,6 +123,46 @@ def _ptr(t):
     if isinstance(t, np.ndarray):
         return t.ctypes.data_as(ctypes.c_void_p)
     return ctypes.c_void_p(int(t))
+
+
+def _dev(t, name, numel=0, dtype="float64"):
+    """A device array argument: a contiguous CUDA torch tensor of the given
+    dtype holding at least numel elements (ValueError otherwise), so that a
+    CPU, mistyped, strided or short tensor never reaches a kernel."""
+    import torch
+    want = getattr(torch, dtype)
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor on the GPU, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: tensor is on {t.device}, expected a CUDA device")
+    if t.dtype != want:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {want}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if t.numel() < numel:
+        raise ValueError(f"{name}: {t.numel()} elements, at least {int(numel)} needed")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _ncoef(lr):
+    r = lr.region
+    return r.nvctr_c + 7 * r.nvctr_f
+
+
+def _batch_sizes(lrs, pot_off, potdim, psi_off):
+    """elements the psi/hpsi and potential arrays of a batch must hold"""
+    nc = np.array([_ncoef(l) for l in lrs], dtype=np.int64)
+    # no psi_off: orbitals one after the other (as the library places them)
+    so = np.cumsum(nc) - nc if psi_off is None else np.asarray(psi_off, dtype=np.int64)
+    npsi = int((so + nc).max()) if len(lrs) else 0
+    if potdim is None:
+        pd = np.array([np.prod(l.region.nfine) for l in lrs], dtype=np.int64)
+    else:
+        pd = np.asarray(potdim, dtype=np.int64).reshape(-1, 3).prod(axis=1)
+    po = np.zeros(len(lrs), dtype=np.int64) if pot_off is None else \
+        np.asarray(pot_off, dtype=np.int64)
+    npot = int((po + pd).max()) if len(lrs) else 0
+    return npsi, npot
 
 
 class LocReg:
@@ -295,11 +335,22 @@ class Context:
 
     # ---- device-pointer entry points (torch tensors) ----
     @_stream_ordered
-    def locham(self, lr, pot, psi, hpsi, ekin, epot, precision=PREC_FP64):
+    def locham(self, lr, pot, psi, hpsi, ekin, epot, precision=PREC_FP64,
+               nspin=1, spinsgn=None):
+        """bdft_locham_d: norb = ekin.numel() consecutive orbitals of one
+        region; pot holds nspin potentials (pot(n1i,n2i,n3i,nspin)), orbital
+        i reads the spin-down one where spinsgn[i] <= 0 (host sequence)"""
         norb = ekin.numel()
-        self._check(lib().bdft_locham_d(self.handle, lr.handle, norb,
-                                        _ptr(pot), _ptr(psi), _ptr(hpsi),
-                                        _ptr(ekin), _ptr(epot), precision))
+        nc = _ncoef(lr) * norb
+        npot = int(np.prod(lr.region.nfine)) * nspin
+        sg = None if spinsgn is None else np.ascontiguousarray(spinsgn, dtype=np.float64)
+        if sg is not None and sg.size != norb:
+            raise ValueError("one spinsgn per orbital")
+        self._check(lib().bdft_locham_d(self.handle, lr.handle, norb, int(nspin),
+                                        _dev(pot, "pot", npot), _ptr(sg),
+                                        _dev(psi, "psi", nc), _dev(hpsi, "hpsi", nc),
+                                        _dev(ekin, "ekin", norb), _dev(epot, "epot", norb),
+                                        precision))
 
     @staticmethod
     def _batch_args(lrs, pot_off, potdim, ishift, psi_off):
@@ -317,19 +368,21 @@ class Context:
         """conf: None or one CONFPOT_DTYPE record per orbital (host)"""
         arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
                                                psi_off)
+        npsi, npot = _batch_sizes(lrs, po, pd, so)
+        n = len(lrs)
+        dp = (_dev(pot, "pot", npot), _dev(psi, "psi", npsi), _dev(hpsi, "hpsi", npsi),
+              _dev(ekin, "ekin", n), _dev(epot, "epot", n))
         if conf is None:
             self._check(lib().bdft_locham_batch_d(
-                self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
-                _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
-                _ptr(epot), precision))
+                self.handle, n, arr, dp[0], _ptr(po), _ptr(pd),
+                _ptr(sh), dp[1], _ptr(so), dp[2], dp[3], dp[4], precision))
             return
         c = np.ascontiguousarray(conf, dtype=CONFPOT_DTYPE).ravel()
-        if c.size != len(lrs):
+        if c.size != n:
             raise ValueError("one confinement record per orbital")
         self._check(lib().bdft_locham_conf_batch_d(
-            self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
-            _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
-            _ptr(epot), _ptr(c), precision))
+            self.handle, n, arr, dp[0], _ptr(po), _ptr(pd),
+            _ptr(sh), dp[1], _ptr(so), dp[2], dp[3], dp[4], _ptr(c), precision))
 
     @_stream_ordered
     def density(self, lr, psi, hfac, rho, spin=None, nspin=1, precision=PREC_FP64):
@@ -340,10 +393,10 @@ class Context:
         hf = np.ascontiguousarray(hfac, dtype=np.float64)
         sp = None if spin is None else np.ascontiguousarray(spin, dtype=np.int32)
         self._check(lib().bdft_density_d(
-            self.handle, lr.handle, norb, _ptr(psi),
+            self.handle, lr.handle, norb, _dev(psi, "psi", _ncoef(lr) * norb),
             hf.ctypes.data_as(ctypes.c_void_p),
             None if sp is None else sp.ctypes.data_as(ctypes.c_void_p),
-            nspin, _ptr(rho), precision))
+            nspin, _dev(rho, "rho", int(np.prod(lr.region.nfine)) * nspin), precision))
 
     @_stream_ordered
     def _xfer(self, fn, glr, llrs, ns, src, soff, dst, doff):
@@ -371,8 +424,10 @@ class Context:
         if c.size != len(lrs):
             raise ValueError("one cprecr per orbital")
         so = None if psi_off is None else np.ascontiguousarray(psi_off, dtype=np.int64)
+        npsi, _ = _batch_sizes(lrs, None, None, so)
         self._check(lib().bdft_precond_cg_d(self.handle, len(lrs), arr, _ptr(so), _ptr(c),
-                                            int(ncong), _ptr(x), _ptr(b)))
+                                            int(ncong), _dev(x, "x", npsi),
+                                            _dev(b, "b", npsi)))
 
     def precondition_host(self, lr, cprecr, ncong, x, b):
         """host arrays (norb, ncoef); x is overwritten"""
@@ -406,11 +461,13 @@ class Context:
              ishift=None, psi_off=None, precision=PREC_FP64):
         arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
                                                psi_off)
+        npsi, npot = _batch_sizes(lrs, po, pd, so)
+        n = len(lrs)
         h = ctypes.c_void_p()
         self._check(lib().bdft_plan_create(
-            self.handle, len(lrs), arr, _ptr(pot), _ptr(po), _ptr(pd),
-            _ptr(sh), _ptr(psi), _ptr(so), _ptr(hpsi), _ptr(ekin),
-            _ptr(epot), precision, ctypes.byref(h)))
+            self.handle, n, arr, _dev(pot, "pot", npot), _ptr(po), _ptr(pd),
+            _ptr(sh), _dev(psi, "psi", npsi), _ptr(so), _dev(hpsi, "hpsi", npsi),
+            _dev(ekin, "ekin", n), _dev(epot, "epot", n), precision, ctypes.byref(h)))
         return Plan(self, h, (lrs, pot, psi, hpsi, ekin, epot))
 
     @_stream_ordered
@@ -424,16 +481,25 @@ class Context:
                                               _ptr(psig), _ptr(hpsi)))
 
     # ---- host-array entry point (numpy) ----
-    def locham_host(self, lr, pot, psi, hpsi, precision=PREC_FP64):
-        """numpy in/out; hpsi is accumulated in place; returns (ekin, epot)."""
+    def locham_host(self, lr, pot, psi, hpsi, precision=PREC_FP64, nspin=1,
+                    spinsgn=None):
+        """numpy in/out (bdft_local_hamiltonian); pot holds nspin potentials;
+        hpsi is accumulated in place; returns (ekin, epot)."""
         psi = np.ascontiguousarray(psi, dtype=np.float64)
         norb = psi.shape[0] if psi.ndim > 1 else 1
-        assert hpsi.flags["C_CONTIGUOUS"] and hpsi.dtype == np.float64
+        if not (hpsi.flags["C_CONTIGUOUS"] and hpsi.dtype == np.float64 and
+                hpsi.size >= psi.size and psi.size >= norb * _ncoef(lr)):
+            raise ValueError("psi and hpsi: contiguous float64, norb x ncoef")
         ek, ep = np.zeros(norb), np.zeros(norb)
         pot = np.ascontiguousarray(pot, dtype=np.float64)
+        if pot.size < int(np.prod(lr.region.nfine)) * nspin:
+            raise ValueError("pot: n1i*n2i*n3i*nspin values needed")
+        sg = None if spinsgn is None else np.ascontiguousarray(spinsgn, dtype=np.float64)
+        if sg is not None and sg.size != norb:
+            raise ValueError("one spinsgn per orbital")
         self._check(lib().bdft_local_hamiltonian(
-            self.handle, lr.handle, norb, _ptr(pot), _ptr(psi), _ptr(hpsi),
-            _ptr(ek), _ptr(ep), precision))
+            self.handle, lr.handle, norb, int(nspin), _ptr(pot), _ptr(sg), _ptr(psi),
+            _ptr(hpsi), _ptr(ek), _ptr(ep), precision))
         return ek, ep
 
     def close(self):
@@ -449,17 +515,19 @@ class Context:
 
 
 def local_hamiltonian(ctx, lr, occup, pot, psi, hpsi, kwgt=1.0,
-                      precision=PREC_FP64):
+                      precision=PREC_FP64, nspin=1, spinsgn=None):
     """local_hamiltonian_OCL + finish_hamiltonian_OCL semantics
     (bigdft/src/interface_ocl.f90:391-607) on device tensors: hpsi += H_loc psi
-    for every orbital, and the returned energies are
+    for every orbital, with pot = pot(n1i,n2i,n3i,nspin) and the spin-down
+    potential for the orbitals whose spinsgn is not positive (:487-491), and
+    the returned energies are
     ekin_sum = sum_i kwgt*occup_i*ekin_i, epot_sum = sum_i kwgt*occup_i*epot_i
     (the per-orbital weighting of hpsiortho.f90:946-954)."""
     import torch
     norb = len(occup)
     ekin = torch.zeros(norb, dtype=torch.float64, device=psi.device)
     epot = torch.zeros(norb, dtype=torch.float64, device=psi.device)
-    ctx.locham(lr, pot, psi, hpsi, ekin, epot, precision)
+    ctx.locham(lr, pot, psi, hpsi, ekin, epot, precision, nspin=nspin, spinsgn=spinsgn)
     ctx.sync()
     w = np.asarray(occup, dtype=np.float64) * kwgt
     ek, ep = ekin.cpu().numpy(), epot.cpu().numpy()

@@ -7,23 +7,37 @@ bundling are options of the context (BDFT_OPT_LINE_PASSES, BDFT_OPT_ZGROUP);
 this module times the candidates on the caller's own batch and keeps the
 fastest per shape key in a JSON cache:
 
-    key = "<BC>:<n1>x<n2>x<n3>:<precision>"      every box of the batch alike
-          "<BC>:max<n1>x<n2>x<n3>:mixed:<precision>"   otherwise (the largest
-          extents, which set the LDS sizes the choices depend on)
+    key = "<device>|<BC>:<n1>x<n2>x<n3>:<precision>"   every box of the batch alike
+          "<device>|<BC>:max<n1>x<n2>x<n3>:mixed<digest>:<precision>"   otherwise
+          (the largest extents, which set the LDS sizes the choices depend on,
+          and a digest of the multiset of per-box (BC, extents), so batches
+          that differ in their boxes never share an entry)
+    <device> = device name, LDS bytes per work-group and the option-table
+    version (TUNE_VERSION), so a table never answers for another GPU or
+    for a library whose options mean something else.
 
 Candidates: the plane-fused and the line/per-pass sequences, and for the
 matrix-core precisions in the plane-fused sequence 2 or 3 z-line bundles per
 work-group, and the z potential pass and the backward plane pass each on the
-matrix cores or on the FP32 vector units.  Timing runs
+matrix cores or on the FP32 vector units; then, with the fastest of those,
+the orbitals in chunks of CHUNK_MB MiB of working grids
+(BDFT_OPT_CHUNK_MB).  Timing runs
 replay a captured plan into scratch outputs, so the caller's hpsi, ekin and
-epot are not touched.  The cache file is plain JSON ({key: {"options": {...},
+epot are not touched.  A cached choice is checked on the batch before it is
+applied (a plan is made and dropped); one the batch does not fit
+(BDFT_ERR_UNSUPPORTED) is removed from the table and the batch is tuned
+again.  The cache file is plain JSON ({key: {"options": {...},
 "ms": {candidate: ms}}}); its default location is
 $BDFT_TUNE_CACHE or ~/.cache/bigdft_hip/tune.json.
 """
+import hashlib
 import json
 import os
 
 from . import PRECISIONS, BigDFTError
+
+# version of the option table: bumped when an option's meaning changes
+TUNE_VERSION = 3
 
 ERR_UNSUPPORTED = 5   # BDFT_ERR_UNSUPPORTED: the option set does not fit this batch
 
@@ -36,28 +50,49 @@ def default_path():
         os.path.expanduser("~"), ".cache", "bigdft_hip", "tune.json")
 
 
-def shape_key(regions, precision):
+def device_tag(ctx):
+    """device name, LDS bytes per work-group and TUNE_VERSION of a context
+    ("" for stand-ins without a device)"""
+    if not hasattr(ctx, "info") or not hasattr(ctx, "device"):
+        return ""
+    import torch
+    lds, _ = ctx.info()
+    name = torch.cuda.get_device_name(ctx.device).replace("|", "/")
+    return f"{name}/lds{lds}/v{TUNE_VERSION}"
+
+
+def shape_key(regions, precision, device=""):
     """cache key of a batch (regions: descriptors with geocode and n)"""
     prec = PREC_NAMES.get(precision, str(precision))
     geos = {r.geocode for r in regions}
     geo = geos.pop() if len(geos) == 1 else "mixed"
-    shapes = {tuple(int(x) for x in r.n) for r in regions}
+    boxes = sorted((r.geocode,) + tuple(int(x) for x in r.n) for r in regions)
+    shapes = {b[1:] for b in boxes}
+    pre = f"{device}|" if device else ""
     if len(shapes) == 1:
         n = shapes.pop()
-        return f"{geo}:{n[0]}x{n[1]}x{n[2]}:{prec}"
+        return f"{pre}{geo}:{n[0]}x{n[1]}x{n[2]}:{prec}"
     m = [max(s[d] for s in shapes) for d in range(3)]
-    return f"{geo}:max{m[0]}x{m[1]}x{m[2]}:mixed:{prec}"
+    dig = hashlib.sha1(repr(boxes).encode()).hexdigest()[:12]
+    return f"{pre}{geo}:max{m[0]}x{m[1]}x{m[2]}:mixed{dig}:{prec}"
+
+
+# workspace per chunk of orbitals (MiB) tried after the kernel options
+# (BDFT_OPT_CHUNK_MB; 0, one chunk, is among the candidates)
+CHUNK_MB = (64, 128, 192)
 
 
 def candidates(precision):
     """option sets tried for a precision, the automatic choice first (the
     bundling and the z potential unit only matter in the plane-fused
     matrix-core sequence)"""
-    auto = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0}
+    auto = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0}
     lines = dict(auto, line_passes=1)
     if PREC_NAMES.get(precision) not in MATRIX_CORE:
         return [auto, lines]
-    fused = [dict(auto, zgroup=z, zpot=p, bwd=w) for p in (0, 1) for z in (0, 2, 3)
+    # F16X3 keeps its z potential pass in FP32 whatever zpot says
+    zpots = (0,) if PREC_NAMES.get(precision) == "f16x3" else (0, 1)
+    fused = [dict(auto, zgroup=z, zpot=p, bwd=w) for p in zpots for z in (0, 2, 3)
              for w in (1, 2)]
     return [auto] + fused + [lines]
 
@@ -85,6 +120,13 @@ class TuneCache:
 
     def put(self, key, options, times):
         self.table[key] = {"options": dict(options), "ms": dict(times)}
+        self._write()
+
+    def drop(self, key):
+        if self.table.pop(key, None) is not None:
+            self._write()
+
+    def _write(self):
         d = os.path.dirname(self.path)
         if d:
             os.makedirs(d, exist_ok=True)
@@ -124,19 +166,43 @@ def time_plan(ctx, lrs, pot, psi, precision, batch_kw, steps=3, warmup=1):
         plan.close()
 
 
-def tune(ctx, lrs, pot, psi, precision, cache=None, timer=time_plan, **batch_kw):
-    """set the context's options for this batch: from the cache, or by
-    timing every candidate (and caching the fastest).  Returns (key,
-    options, times or None when the cache answered)."""
+def check_plan(ctx, lrs, pot, psi, precision, batch_kw):
+    """make and drop a plan with the context's current options (raises
+    BDFT_ERR_UNSUPPORTED when they do not fit the batch); stand-ins without
+    plans pass"""
+    if not hasattr(ctx, "plan"):
+        return
+    import torch
+    norb = len(lrs)
+    h = torch.zeros_like(psi)
+    ek = torch.zeros(norb, dtype=torch.float64, device=psi.device)
+    ctx.plan(lrs, pot, psi, h, ek, ek.clone(), precision=precision, **batch_kw).close()
+
+
+def tune(ctx, lrs, pot, psi, precision, cache=None, timer=time_plan, check=check_plan,
+         **batch_kw):
+    """set the context's options for this batch: from the cache (after
+    checking that the cached choice fits the batch), or by timing every
+    candidate (and caching the fastest).  Returns (key, options, times or
+    None when the cache answered)."""
     cache = cache if cache is not None else TuneCache()
-    key = shape_key([l.region for l in lrs], precision)
+    key = shape_key([l.region for l in lrs], precision, device_tag(ctx))
     hit = cache.get(key)
     if hit is not None:
         apply(ctx, hit)
-        return key, hit, None
+        try:
+            if check is not None:
+                check(ctx, lrs, pot, psi, precision, batch_kw)
+            return key, hit, None
+        except BigDFTError as e:
+            if e.code != ERR_UNSUPPORTED:
+                raise
+            cache.drop(key)   # stale entry: tune again
     times = {}
     best = None
-    for opts in candidates(precision):
+
+    def trial(opts):
+        nonlocal best
         apply(ctx, opts)
         try:
             ms = timer(ctx, lrs, pot, psi, precision, batch_kw)
@@ -144,12 +210,18 @@ def tune(ctx, lrs, pot, psi, precision, cache=None, timer=time_plan, **batch_kw)
             if e.code != ERR_UNSUPPORTED:
                 raise
             times[cand_name(opts)] = None   # does not fit this batch
-            continue
+            return
         times[cand_name(opts)] = ms
         if best is None or ms < best[0]:
             best = (ms, opts)
+    for opts in candidates(precision):
+        trial(opts)
     if best is None:
         raise BigDFTError(ERR_UNSUPPORTED, f"no launch option set fits the batch {key}")
+    # then the orbital chunking, with the kernel options chosen above
+    base = best[1]
+    for mb in CHUNK_MB:
+        trial(dict(base, chunk_mb=mb))
     apply(ctx, best[1])
     cache.put(key, best[1], times)
     return key, best[1], times

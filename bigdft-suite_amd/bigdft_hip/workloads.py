@@ -15,7 +15,13 @@
   origin nl in coarse points), as apply_potential_lr places a region's
   potential (bigdft/src/orbitals/potential.f90:850-879).
 
-Both return a dict with the host regions and the device arrays the batched
+* ``dense_periodic``: norb orbitals sharing one periodic box of (n+1)^3
+  coarse points with dense keys, coefficients N(0,1) normalised per orbital,
+  and one potential, a sum of 8 Gaussians (centres uniform in the cell,
+  depth U[-5,-1] Ha, width U[0.5,2] bohr, minimum-image distance) on the
+  box's fine grid.
+
+Each returns a dict with the host regions and the device arrays the batched
 entry points take, plus ``units`` = sum of the boxes' fine grid sizes.
 """
 import numpy as np
@@ -118,3 +124,20 @@ def local_potential(batch, b):
         src.append(slice(a + q0, max(a, e) + q0))
     out[dst[2], dst[1], dst[0]] = pot[src[2], src[1], src[0]]
     return out.ravel()
+
+
+def dense_periodic(n, norb, seed, hgrid=0.35, device="cuda"):
+    h = (hgrid, hgrid, hgrid)
+    reg = dense_region("P", (n, n, n), h)
+    psi = inputs.coefficients(norb, reg.ncoef, seed, device=device,
+                              normalise=True).reshape(-1)
+    cell = [(n + 1) * hgrid] * 3
+    gp = inputs.gaussian_params(seed + 1, cell)
+    pot = inputs.gaussian_field(gp, inputs.fine_coords("P", (n, n, n), h, device=device),
+                                cell=cell).contiguous()
+    ncoef = np.full(norb, reg.ncoef, dtype=np.int64)
+    return dict(regions=[reg] * norb, psi=psi, ncoef=ncoef,
+                psi_off=np.arange(norb, dtype=np.int64) * reg.ncoef, pot=pot,
+                pot_off=np.zeros(norb, dtype=np.int64),
+                potdim=np.array(list(reg.nfine) * norb, dtype=np.int32), ishift=None,
+                units=reg.n_isf * norb)

@@ -16,9 +16,22 @@
 
 using namespace bdft;
 
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;   // false: a slice of a fixed device array (Arena)
+};
+
+// A plan made without allocations takes its buffers from one device array
+// of fixed size and uploads its descriptors with stream-ordered copies; the
+// host copies of those descriptors are kept in `keep` until the copies have
+// been read.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::vector<unsigned char>>* keep = nullptr;
 };
 
 struct bdft_ctx {
@@ -48,12 +61,15 @@ struct bdft_ctx {
   bool profile = false;
   std::vector<hipEvent_t> ev_free;                 // reusable events
   std::vector<std::vector<hipEvent_t>> ev_pending; // one set per launch
+  std::vector<char> ev_counts;  // per set: 1 when it starts an application
+                                // (the later parts of a chunked plan add 0)
   double prof_ms[S_COUNT] = {0};
   long long prof_n = 0;
   bool force_lines = false;  // disable the plane-fused sequence
   int opt_zg = 0;            // z-line bundles per work-group (0: automatic)
   int opt_zpot = 0;          // matrix-core z potential pass: 0 auto, 1 MFMA, 2 FP32
   int opt_bwd = 0;           // matrix-core backward pass: 0 auto, 1 MFMA, 2 FP32
+  int opt_chunk_mb = 0;      // workspace per chunk of orbitals (MiB), 0: one chunk
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
 };
 
@@ -80,6 +96,13 @@ struct bdft_plan {
   bool need_zero = true;
   bool owns_ws = false;
   DevBuf ws;           // own workspace (persistent plans)
+  // workspace not bound yet (a part of a chunked plan): bytes it needs and
+  // the offsets of W, K, T1, T2, T3 in it
+  size_t ws_need = 0;
+  size_t ws_at[5] = {0, 0, 0, 0, 0};
+  // chunked plan: parts over consecutive orbital ranges, run one after the
+  // other on one workspace (ws of this plan) sized for the largest part
+  std::vector<bdft_plan*> parts;
   DevBuf meta;         // box descriptors + tile starts + energy partials
   Batch B{};
   PassTables T{};
@@ -128,9 +151,33 @@ static int dev_alloc(bdft_ctx* ctx, DevBuf& b, size_t bytes) {
 }
 
 static void dev_free(DevBuf& b) {
-  if (b.p) hipFree(b.p);
+  if (b.p && b.owned) hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
+  b.owned = true;
+}
+
+// a buffer of the plan: from the arena when there is one, else allocated
+static int plan_buf(bdft_ctx* ctx, Arena* a, DevBuf& b, size_t bytes) {
+  if (!a) return dev_alloc(ctx, b, bytes);
+  const size_t at = (a->used + 255) & ~(size_t)255;
+  if (at + bytes > a->size)
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                   "the batch needs more than the %zu bytes of the fixed device array", a->size);
+  b.p = a->base + at;
+  b.bytes = bytes;
+  b.owned = false;
+  a->used = at + bytes;
+  return BDFT_OK;
+}
+
+// host -> device copy of plan data: synchronous, or stream-ordered from a
+// host copy kept alive in the arena's keep list
+static hipError_t plan_h2d(Arena* a, void* dst, const void* src, size_t n) {
+  if (!n) return hipSuccess;
+  if (!a) return hipMemcpy(dst, src, n, hipMemcpyHostToDevice);
+  a->keep->emplace_back((const unsigned char*)src, (const unsigned char*)src + n);
+  return hipMemcpyAsync(dst, a->keep->back().data(), n, hipMemcpyHostToDevice, a->stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +198,10 @@ extern "C" int bdft_ctx_create(int device, bdft_ctx** out) {
   }
   ctx->stream = ctx->own;
   ctx->plane_lds = plane_attributes();
-  if (ctx->plane_lds) mfma_attributes(ctx->plane_lds);
+  if (ctx->plane_lds) {
+    mfma_attributes(ctx->plane_lds);
+    line_attributes(ctx->plane_lds);
+  }
   const char* env = getenv("BDFT_LINE_PASSES");
   ctx->force_lines = env && env[0] == '1';
   *out = ctx;
@@ -179,9 +229,24 @@ extern "C" int bdft_ctx_destroy(bdft_ctx* ctx) {
   return BDFT_OK;
 }
 
+// The context's buffers (one-shot plan workspace, density, preconditioner
+// and transfer buffers) are shared by every call, so the new stream first
+// waits for everything queued on the old one: later calls cannot overwrite
+// a buffer that earlier work still reads, and synchronising the current
+// stream (plan and region destruction) also covers the old stream's work.
 extern "C" int bdft_ctx_set_stream(bdft_ctx* ctx, void* s) {
   if (!ctx) return BDFT_ERR_ARG;
-  ctx->stream = s ? (hipStream_t)s : ctx->own;
+  hipStream_t ns = s ? (hipStream_t)s : ctx->own;
+  if (ns == ctx->stream) return BDFT_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipEvent_t ev = nullptr;
+  HIPCHK(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ev, ctx->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ns, ev, 0);
+  hipEventDestroy(ev);
+  if (e != hipSuccess)
+    return set_err(ctx, BDFT_ERR_HIP, "stream switch: %s", hipGetErrorString(e));
+  ctx->stream = ns;
   return BDFT_OK;
 }
 
@@ -468,10 +533,25 @@ static void build_op(int op, int N, int per, std::vector<double>& M, int& rows, 
 // 16s + 8(l/32) .. +8), pieces of one step adjacent, so a wave reads each
 // piece as one contiguous 1 KB block.  Per row block: {bit mask of the K
 // steps stored, index of its first fragment}.
+//
+// fp16 tables (F16X3) keep every entry to 2^-22 only where its low piece
+// stays a normal fp16 number (|v| >= 2^-2), and fp16 stops at 2^-24: the
+// far tails of fil2 (down to 1.6e-9) would be lost, and they alone make
+// psir in the outer halo of a free box, where an orbital far from the
+// potential's wells gets all of its epot.  So the fp16 tables are scaled:
+//   G (op 0): row r by s_r = 2^(13 - ceil(log2 sum_c |G_rc|)), every row's
+//             entries as large as fp16 allows (row sum in (2^12, 2^13]);
+//             the kernels multiply a G output by 2^-13 before splitting it
+//             for the next pass and remove the rest, u_r = 1/s_r of each
+//             row used, in FP32 at the end (exact powers of two);
+//   K (op 2): all entries by KIN16 = 2^9 (largest 24.9 * 2^9 < 65504),
+//             removed with the kinetic scale.
+// rscale holds u_r for the rows of every scaled G table (OpTab::soff).
 struct TabSet {
   std::vector<OpTab> tabs;
   std::vector<uint16_t> frag;
   std::vector<uint32_t> mq;
+  std::vector<float> rscale;
   std::map<std::tuple<int, int, int, int>, int> index;
   // f16: fp16 pieces (hi, lo) instead of bf16 (hi, lo, lo2)
   int get(int op, int N, int per, bool f16) {
@@ -487,6 +567,24 @@ struct TabSet {
     t.Dp = (rows + 31) / 32 * 32;
     t.Kp = (cols + 15) / 16 * 16;
     t.off = 0;
+    t.soff = -1;
+    if (f16 && op == 0) {
+      t.soff = (int)rscale.size();
+      for (int r = 0; r < rows; ++r) {
+        double sum = 0.0;
+        for (int c = 0; c < cols; ++c) sum += std::fabs(M[(size_t)r * cols + c]);
+        int e = 0;
+        if (sum > 0.0) {
+          (void)std::frexp(sum, &e);       // sum in [2^(e-1), 2^e)
+          if (std::ldexp(1.0, e - 1) == sum) --e;
+        }
+        const double sr = std::ldexp(1.0, 13 - e);
+        for (int c = 0; c < cols; ++c) M[(size_t)r * cols + c] *= sr;
+        rscale.push_back((float)std::ldexp(1.0, e - 13));
+      }
+    } else if (f16 && op == 2) {
+      for (double& v : M) v *= KIN16;
+    }
     t.moff = (int)(mq.size() / 2);
     for (int q = 0; q < t.Dp / 32; ++q) {
       uint32_t m = 0;
@@ -552,12 +650,24 @@ static int box_tiles(const BoxDesc& b, int pass, int zg) {
   return 0;
 }
 
+extern "C" int bdft_plan_destroy(bdft_plan* plan);
+
+// point the plan's grids at their places in a workspace of ws_need bytes
+static void bind_ws(bdft_plan* plan, char* w) {
+  plan->B.W = w + plan->ws_at[0];
+  plan->B.K = w + plan->ws_at[1];
+  plan->B.T1 = w + plan->ws_at[2];
+  plan->B.T2 = w + plan->ws_at[3];
+  plan->B.T3 = w + plan->ws_at[4];
+}
+
 static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      const double* pot, const int64_t* pot_off, const int32_t* potdim,
                      const int32_t* ishift, const double* psi, const int64_t* psi_off,
                      double* hpsi, double* ekin, double* epot, int precision,
                      bool own_ws, bdft_plan** out, bool density = false,
-                     const bdft_confpot* conf = nullptr, bool grow_plane = false) {
+                     const bdft_confpot* conf = nullptr, bool grow_plane = false,
+                     Arena* arena = nullptr, bool defer_ws = false) {
   if (!ctx || !out) return BDFT_ERR_ARG;
   *out = nullptr;
   if (norb <= 0 || !lrs || !psi || (!density && (!pot || !hpsi || !ekin || !epot)))
@@ -565,13 +675,21 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_F16X3 ||
       (density && precision > BDFT_PREC_FP32))
     return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
+  // split precisions on a batch with a box beyond the matrix-core kernels
+  // (more than MFMA_MAXN coarse points on an axis) run the FP32 vector
+  // passes, which are at least as accurate
+  if (precision >= BDFT_PREC_BF16X3)
+    for (int i = 0; i < norb; ++i)
+      for (int d = 0; d < 3; ++d)
+        if (lrs[i] && lrs[i]->N[d] > MFMA_MAXN) precision = BDFT_PREC_FP32;
   const size_t esz = precision == BDFT_PREC_FP64 ? sizeof(double) : sizeof(float);
   const bool mfma = precision >= BDFT_PREC_BF16X3;
   const bool f16 = precision == BDFT_PREC_F16X3;
   TabSet tabs;
   std::vector<BoxDesc> box((size_t)norb);
-  long long wtot = 0, t1tot = 0, t2tot = 0;
+  long long wtot = 0, t1tot = 0, t2tot = 0, t3tot = 0, psi_run = 0;
   int maxN = 0;
+  bool xbig = false, zbig = false;
   bool need_zero = false;
   for (int i = 0; i < norb; ++i) {
     const bdft_lr* lr = lrs[i];
@@ -616,9 +734,12 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     b.w_off = wtot;
     b.t1_off = t1tot;
     b.t2_off = t2tot;
+    b.t3_off = t3tot;
     b.pot_off = pot_off ? pot_off[i] : 0;
     const long long ncoef = (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
-    b.psi_off = psi_off ? psi_off[i] : ncoef * i;
+    // no psi_off: orbitals stored one after the other
+    b.psi_off = psi_off ? psi_off[i] : psi_run;
+    psi_run += ncoef;
     if (mfma)
       for (int d = 0; d < 3; ++d) {
         b.opG[d] = tabs.get(0, b.N[d], b.per, f16);
@@ -628,6 +749,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     wtot += 8LL * b.N[0] * b.N[1] * b.N[2];
     t1tot += (long long)b.L[0] * 4 * b.N[1] * b.N[2];
     t2tot += (long long)b.L[0] * b.L[1] * 2 * b.N[2];
+    t3tot += (long long)b.L[0] * b.L[1] * b.L[2];
+    xbig = xbig || b.N[0] > REG_N || b.L[0] > REG_L;
+    zbig = zbig || b.L[2] > REG_L;
     if (!lr->dense) need_zero = true;
   }
   // bundles per z-line tile of the matrix-core sequence: three, unless a
@@ -722,8 +846,10 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     // matrix cores are asked for
     bool line_ok = true;
     for (int i = 0; i < norb; ++i)
-      for (int d = 0; d < 3; ++d) line_ok = line_ok && box[i].N[d] <= MAXN;
-    ml.zvalu = ml.fused && line_ok && ctx->opt_zpot != 1;
+      line_ok = line_ok && box[i].L[2] <= REG_L;
+    // (F16X3 always: its fp16 pieces are cut for the box scale, and the z
+    // lines of a free box's x-y halo lie far below it)
+    ml.zvalu = ml.fused && line_ok && (ctx->opt_zpot != 1 || f16);
     ml.maxN = maxN;
     size_t mx = std::max(ml.kin[2], ml.zpot);
     if (ml.fused)
@@ -732,11 +858,18 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     else
       mx = std::max(std::max(mx, std::max(ml.kin[0], ml.kin[1])),
                     std::max(ml.grow, ml.shrink));
-    if (mx > ctx->plane_lds)
+    if (mx > ctx->plane_lds) {
+      // automatic units: the FP32 vector passes instead; a matrix-core unit
+      // asked for by an option is refused
+      if (ctx->opt_bwd != 1 && ctx->opt_zpot != 1)
+        return make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi,
+                         ekin, epot, BDFT_PREC_FP32, own_ws, out, density, conf, grow_plane,
+                         arena, defer_ws);
       return set_err(ctx, BDFT_ERR_UNSUPPORTED,
                      "split-bf16 kernels need %zu bytes of LDS per work-group, the device "
                      "offers %zu",
                      mx, ctx->plane_lds);
+    }
   }
   // plane-fused matrix-core kernels: the launch reserves the largest box's
   // LDS for every work-group; boxes whose plane kernels fit in half a CU's
@@ -842,18 +975,20 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     const size_t nt = sizeof(OpTab) * tabs.tabs.size();
     const size_t nf = sizeof(uint16_t) * tabs.frag.size();
     const size_t nm = sizeof(uint32_t) * tabs.mq.size();
+    const size_t ns = sizeof(float) * tabs.rscale.size();
     auto al2 = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    int rc2 = dev_alloc(ctx, plan->tabbuf, al2(nt) + al2(nf) + al2(nm));
+    int rc2 = plan_buf(ctx, arena, plan->tabbuf, al2(nt) + al2(nf) + al2(nm) + al2(ns));
     if (rc2) {
       delete plan;
       return rc2;
     }
     char* q = (char*)plan->tabbuf.p;
-    hipError_t e = hipMemcpy(q, tabs.tabs.data(), nt, hipMemcpyHostToDevice);
-    if (e == hipSuccess && nf)
-      e = hipMemcpy(q + al2(nt), tabs.frag.data(), nf, hipMemcpyHostToDevice);
+    hipError_t e = plan_h2d(arena, q, tabs.tabs.data(), nt);
+    if (e == hipSuccess) e = plan_h2d(arena, q + al2(nt), tabs.frag.data(), nf);
+    if (e == hipSuccess) e = plan_h2d(arena, q + al2(nt) + al2(nf), tabs.mq.data(), nm);
     if (e == hipSuccess)
-      e = hipMemcpy(q + al2(nt) + al2(nf), tabs.mq.data(), nm, hipMemcpyHostToDevice);
+      e = plan_h2d(arena, q + al2(nt) + al2(nf) + al2(nm), tabs.rscale.data(), ns);
+    plan->B.trs = ns ? (const float*)(q + al2(nt) + al2(nf) + al2(nm)) : nullptr;
     if (e != hipSuccess) {
       dev_free(plan->tabbuf);
       delete plan;
@@ -867,8 +1002,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   for (int e = 0; e < 4; ++e) nb_part += sizeof(double) * T.ntiles[epass[e]];
   const size_t nb_max = f16 ? sizeof(unsigned) * norb + sizeof(float) * T.ntiles[P_UNC] + 256 : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  int rc = dev_alloc(ctx, plan->meta, al(nb_box) + al(nb_st) + al(nb_part) + al(nb_max));
+  int rc = plan_buf(ctx, arena, plan->meta, al(nb_box) + al(nb_st) + al(nb_part) + al(nb_max));
   if (rc) {
+    dev_free(plan->tabbuf);
     delete plan;
     return rc;
   }
@@ -880,11 +1016,11 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   plan->B.tmax = f16 ? (float*)(m + al(nb_box) + al(nb_st) + al(nb_part) +
                                 al(sizeof(unsigned) * norb))
                      : nullptr;
-  hipError_t e1 = hipMemcpy(dbox, box.data(), nb_box, hipMemcpyHostToDevice);
-  hipError_t e2 = starts.empty() ? hipSuccess
-                                 : hipMemcpy(dst, starts.data(), nb_st, hipMemcpyHostToDevice);
+  hipError_t e1 = plan_h2d(arena, dbox, box.data(), nb_box);
+  hipError_t e2 = plan_h2d(arena, dst, starts.data(), nb_st);
   if (e1 != hipSuccess || e2 != hipSuccess) {
     dev_free(plan->meta);
+    dev_free(plan->tabbuf);
     delete plan;
     return set_err(ctx, BDFT_ERR_HIP, "descriptor upload failed");
   }
@@ -904,32 +1040,129 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       q += T.ntiles[epass[e]];
     }
   }
-  // workspace: W | K | T1 | T2
+  // workspace: W | K | T1 | T2 | T3
   if (T.plane || mfma) t1tot = 0;
-  const size_t ws_bytes = esz * (2 * al(wtot) + al(t1tot) + al(t2tot));
-  DevBuf& ws = own_ws ? plan->ws : ctx->ws;
-  rc = dev_alloc(ctx, ws, ws_bytes);
-  if (rc) {
-    dev_free(plan->meta);
-    delete plan;
-    return rc;
-  }
-  char* w = (char*)ws.p;
+  if (!zbig) t3tot = 0;
+  T.xbig = xbig;
+  T.zbig = zbig;
+  const size_t ws_bytes = esz * (2 * al(wtot) + al(t1tot) + al(t2tot) + al(t3tot));
+  plan->ws_need = ws_bytes;
+  plan->ws_at[0] = 0;
+  plan->ws_at[1] = esz * al(wtot);
+  plan->ws_at[2] = esz * 2 * al(wtot);
+  plan->ws_at[3] = esz * (2 * al(wtot) + al(t1tot));
+  plan->ws_at[4] = esz * (2 * al(wtot) + al(t1tot) + al(t2tot));
   Batch& B = plan->B;
   B.box = dbox;
   B.nbox = norb;
   B.psi = psi;
   B.hpsi = hpsi;
   B.pot = pot;
-  B.W = w;
-  B.K = w + esz * al(wtot);
-  B.T1 = w + esz * 2 * al(wtot);
-  B.T2 = w + esz * (2 * al(wtot) + al(t1tot));
   B.ekin = ekin;
   B.epot = epot;
   plan->T = T;
+  if (!defer_ws) {
+    DevBuf& ws = own_ws ? plan->ws : ctx->ws;
+    rc = plan_buf(ctx, arena, ws, ws_bytes);
+    if (rc) {
+      dev_free(plan->meta);
+      dev_free(plan->tabbuf);
+      delete plan;
+      return rc;
+    }
+    bind_ws(plan, (char*)ws.p);
+  }
   *out = plan;
   return BDFT_OK;
+}
+
+// Chunked plan: orbitals split into consecutive ranges whose working grids
+// (W, K, T2, and T1 for line passes) take at most opt_chunk_mb MiB, each
+// range a plan of its own on one shared workspace, run in order.  A chunk's
+// grids are written and read again while they are still in the device's
+// last-level cache (256 MiB on MI355X) instead of making an HBM round trip
+// per pass; psi, hpsi and the potential stream through once.  Offsets and
+// defaults are resolved per orbital first, so every part sees explicit
+// psi/potential placements.
+static int make_plan_chunked(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                             const double* pot, const int64_t* pot_off,
+                             const int32_t* potdim, const int32_t* ishift,
+                             const double* psi, const int64_t* psi_off, double* hpsi,
+                             double* ekin, double* epot, int precision, bool own_ws,
+                             bdft_plan** out, const bdft_confpot* conf = nullptr) {
+  if (!ctx || !out) return BDFT_ERR_ARG;
+  const size_t budget = (size_t)ctx->opt_chunk_mb << 20;
+  std::vector<size_t> need;
+  size_t total = 0;
+  if (budget && norb > 1 && lrs && psi) {
+    const size_t esz = precision == BDFT_PREC_FP64 ? 8 : 4;
+    need.resize((size_t)norb);
+    for (int i = 0; i < norb; ++i) {
+      const bdft_lr* lr = lrs[i];
+      if (!lr) return set_err(ctx, BDFT_ERR_ARG, "orbital %d: region missing", i);
+      const size_t W = 8ull * lr->N[0] * lr->N[1] * lr->N[2];
+      const size_t T2 = (size_t)lr->L[0] * lr->L[1] * 2 * lr->N[2];
+      const size_t T1 = (size_t)lr->L[0] * 4 * lr->N[1] * lr->N[2];
+      const size_t T3 = lr->L[2] > REG_L ? (size_t)lr->L[0] * lr->L[1] * lr->L[2] : 0;
+      need[i] = esz * (2 * W + T2 + T1 + T3);
+      total += need[i];
+    }
+  }
+  if (!budget || total <= budget)
+    return make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
+                     epot, precision, own_ws, out, false, conf);
+  *out = nullptr;
+  std::vector<int64_t> so((size_t)norb), po((size_t)norb);
+  long long run = 0;
+  for (int i = 0; i < norb; ++i) {
+    so[i] = psi_off ? psi_off[i] : run;
+    run += (long long)lrs[i]->nvctr_c + 7LL * lrs[i]->nvctr_f;
+    po[i] = pot_off ? pot_off[i] : 0;
+  }
+  bdft_plan* parent = new bdft_plan();
+  parent->ctx = ctx;
+  parent->nbox = norb;
+  parent->precision = precision;
+  parent->owns_ws = true;
+  size_t wsmax = 0;
+  for (int o0 = 0; o0 < norb;) {
+    int o1 = o0 + 1;
+    size_t acc = need[o0];
+    while (o1 < norb && acc + need[o1] <= budget) acc += need[o1++];
+    bdft_plan* part = nullptr;
+    const int rc = make_plan(ctx, o1 - o0, lrs + o0, pot, po.data() + o0,
+                             potdim ? potdim + 3 * o0 : nullptr,
+                             ishift ? ishift + 3 * o0 : nullptr, psi, so.data() + o0, hpsi,
+                             ekin + o0, epot + o0, precision, true, &part, false,
+                             conf ? conf + o0 : nullptr, false, nullptr, true);
+    if (rc) {
+      for (bdft_plan* q : parent->parts) bdft_plan_destroy(q);
+      parent->parts.clear();
+      delete parent;
+      return rc;
+    }
+    parent->parts.push_back(part);
+    wsmax = std::max(wsmax, part->ws_need);
+    o0 = o1;
+  }
+  const int rc = dev_alloc(ctx, parent->ws, wsmax);
+  if (rc) {
+    bdft_plan_destroy(parent);
+    return rc;
+  }
+  for (bdft_plan* q : parent->parts) bind_ws(q, (char*)parent->ws.p);
+  *out = parent;
+  return BDFT_OK;
+}
+
+// the kernel sequence of one plan (or one part of a chunked plan) on s
+static hipError_t launch_plan_seq(const bdft_plan* plan, hipStream_t s, hipEvent_t* ev,
+                                  const Fork* fk) {
+  if (plan->precision >= BDFT_PREC_BF16X3)
+    return launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal, plan->need_zero,
+                       plan->precision, s, ev, fk);
+  return launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal, plan->need_zero,
+                       plan->precision == BDFT_PREC_FP32, s, ev);
 }
 
 static int run_plan(bdft_plan* plan) {
@@ -939,28 +1172,28 @@ static int run_plan(bdft_plan* plan) {
     HIPCHK(ctx, hipGraphLaunch(plan->exec, ctx->stream));
     return BDFT_OK;
   }
-  hipEvent_t* ev = nullptr;
-  if (ctx->profile) {
-    std::vector<hipEvent_t> set(2 * S_COUNT);
-    for (auto& x : set) {
-      if (!ctx->ev_free.empty()) {
-        x = ctx->ev_free.back();
-        ctx->ev_free.pop_back();
-      } else {
-        HIPCHK(ctx, hipEventCreate(&x));
+  const size_t nparts = plan->parts.empty() ? 1 : plan->parts.size();
+  for (size_t i = 0; i < nparts; ++i) {
+    const bdft_plan* part = plan->parts.empty() ? plan : plan->parts[i];
+    hipEvent_t* ev = nullptr;
+    if (ctx->profile) {
+      std::vector<hipEvent_t> set(2 * S_COUNT);
+      for (auto& x : set) {
+        if (!ctx->ev_free.empty()) {
+          x = ctx->ev_free.back();
+          ctx->ev_free.pop_back();
+        } else {
+          HIPCHK(ctx, hipEventCreate(&x));
+        }
       }
+      ctx->ev_pending.push_back(std::move(set));
+      ctx->ev_counts.push_back(i == 0 ? 1 : 0);
+      ev = ctx->ev_pending.back().data();
     }
-    ctx->ev_pending.push_back(std::move(set));
-    ev = ctx->ev_pending.back().data();
+    const hipError_t e = launch_plan_seq(part, ctx->stream, ev, nullptr);
+    if (e != hipSuccess)
+      return set_err(ctx, BDFT_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   }
-  const bool mf = plan->precision >= BDFT_PREC_BF16X3;
-  hipError_t e = mf ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal, plan->need_zero,
-                                  plan->precision, ctx->stream, ev)
-                     : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
-                                     plan->need_zero, plan->precision == BDFT_PREC_FP32,
-                                     ctx->stream, ev);
-  if (e != hipSuccess)
-    return set_err(ctx, BDFT_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return BDFT_OK;
 }
 
@@ -970,8 +1203,8 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
                                 const double* psi, const int64_t* psi_off, double* hpsi,
                                 double* ekin, double* epot, int precision,
                                 bdft_plan** out) {
-  int rc = make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
-                     epot, precision, true, out);
+  int rc = make_plan_chunked(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi,
+                             ekin, epot, precision, true, out);
   if (rc) return rc;
   bdft_plan* plan = *out;
   // capture the launch sequence once; replays reuse the instantiated graph.
@@ -985,12 +1218,10 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
   if (e == hipSuccess) e = hipEventCreateWithFlags(&fk.b, hipEventDisableTiming);
   if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t el = plan->precision >= BDFT_PREC_BF16X3
-                         ? launch_mfma(plan->B, plan->T, plan->mlds, plan->wtotal,
-                                       plan->need_zero, plan->precision, cs, nullptr, &fk)
-                         : launch_passes(plan->B, plan->T, plan->maxN, plan->wtotal,
-                                         plan->need_zero, plan->precision == BDFT_PREC_FP32,
-                                         cs);
+    hipError_t el = hipSuccess;
+    const size_t nparts = plan->parts.empty() ? 1 : plan->parts.size();
+    for (size_t i = 0; i < nparts && el == hipSuccess; ++i)
+      el = launch_plan_seq(plan->parts.empty() ? plan : plan->parts[i], cs, nullptr, &fk);
     e = hipStreamEndCapture(cs, &plan->graph);
     if (el != hipSuccess) e = el;
   }
@@ -1020,6 +1251,8 @@ extern "C" int bdft_plan_destroy(bdft_plan* plan) {
   }
   if (plan->exec) hipGraphExecDestroy(plan->exec);
   if (plan->graph) hipGraphDestroy(plan->graph);
+  for (bdft_plan* q : plan->parts) bdft_plan_destroy(q);
+  plan->parts.clear();
   dev_free(plan->meta);
   dev_free(plan->tabbuf);
   if (plan->owns_ws) dev_free(plan->ws);
@@ -1070,8 +1303,8 @@ static int locham_batch(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       ctx->cached_key.clear();
     }
     bdft_plan* plan = nullptr;
-    int rc = make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
-                       epot, precision, false, &plan, false, conf);
+    int rc = make_plan_chunked(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off,
+                               hpsi, ekin, epot, precision, false, &plan, conf);
     if (rc) return rc;
     ctx->cached = plan;
     ctx->cached_key.swap(key);
@@ -1332,13 +1565,14 @@ extern "C" int bdft_precond_cg_d(bdft_ctx* ctx, int norb, const bdft_lr* const* 
     return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
   if (ncong < 0 || ncong > 10000) return set_err(ctx, BDFT_ERR_ARG, "ncong %d", ncong);
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  long long ntot = 0;
+  long long ntot = 0, run = 0;
   std::vector<double> pc((size_t)norb * 9);
   for (int i = 0; i < norb; ++i) {
     const bdft_lr* lr = lrs[i];
     if (!lr) return set_err(ctx, BDFT_ERR_ARG, "orbital %d: null region", i);
     const long long nc = (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
-    const long long off = psi_off ? (long long)psi_off[i] : nc * i;  // as make_plan
+    const long long off = psi_off ? (long long)psi_off[i] : run;  // as make_plan
+    run += nc;
     if (off < 0) return set_err(ctx, BDFT_ERR_ARG, "negative offset");
     ntot = std::max(ntot, off + nc);
     const double c = cprecr[i];
@@ -1512,35 +1746,61 @@ extern "C" int bdft_locreg_bounds(bdft_ctx* ctx, const bdft_lr* lr, int32_t* bou
   return rc;
 }
 
-extern "C" int bdft_locham_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, const double* pot,
-                             const double* psi, double* hpsi, double* ekin, double* epot,
-                             int precision) {
+// potential component of each orbital: spin up (spinsgn > 0, or no spinsgn)
+// reads pot[0..npot), spin down pot[npot..2 npot), as local_hamiltonian_OCL
+// picks rhopot_up / rhopot_down from orbs%spinsgn (interface_ocl.f90:487-491)
+static int spin_offsets(bdft_ctx* ctx, int norb, int nspin, const double* spinsgn,
+                        long long npot, std::vector<int64_t>& off) {
+  if (nspin != 1 && nspin != 2)
+    return set_err(ctx, BDFT_ERR_ARG, "nspin must be 1 or 2 (got %d)", nspin);
+  off.assign((size_t)norb, 0);
+  if (!spinsgn) return BDFT_OK;
+  for (int i = 0; i < norb; ++i) {
+    if (spinsgn[i] > 0.0) continue;
+    if (nspin == 1)
+      return set_err(ctx, BDFT_ERR_ARG,
+                     "orbital %d is spin down (spinsgn %g) but nspin = 1", i, spinsgn[i]);
+    off[i] = npot;
+  }
+  return BDFT_OK;
+}
+
+extern "C" int bdft_locham_d(bdft_ctx* ctx, const bdft_lr* lr, int norb, int nspin,
+                             const double* pot, const double* spinsgn, const double* psi,
+                             double* hpsi, double* ekin, double* epot, int precision) {
   if (!ctx) return BDFT_ERR_ARG;
   if (!lr || norb <= 0) return set_err(ctx, BDFT_ERR_ARG, "null region or norb <= 0");
+  std::vector<int64_t> poff;
+  const long long npot = (long long)lr->L[0] * lr->L[1] * lr->L[2];
+  int rc = spin_offsets(ctx, norb, nspin, spinsgn, npot, poff);
+  if (rc) return rc;
   std::vector<const bdft_lr*> lrs((size_t)norb, lr);
-  return bdft_locham_batch_d(ctx, norb, lrs.data(), pot, nullptr, nullptr, nullptr, psi,
+  return bdft_locham_batch_d(ctx, norb, lrs.data(), pot, poff.data(), nullptr, nullptr, psi,
                              nullptr, hpsi, ekin, epot, precision);
 }
 
-extern "C" int bdft_local_hamiltonian(bdft_ctx* ctx, const bdft_lr* lr, int norb,
-                                      const double* pot, const double* psi, double* hpsi,
-                                      double* ekin, double* epot, int precision) {
+extern "C" int bdft_local_hamiltonian(bdft_ctx* ctx, const bdft_lr* lr, int norb, int nspin,
+                                      const double* pot, const double* spinsgn,
+                                      const double* psi, double* hpsi, double* ekin,
+                                      double* epot, int precision) {
   if (!ctx) return BDFT_ERR_ARG;
   if (!lr || norb <= 0 || !pot || !psi || !hpsi || !ekin || !epot)
     return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t ncoef = (size_t)lr->nvctr_c + 7 * (size_t)lr->nvctr_f;
   const size_t npot = (size_t)lr->L[0] * lr->L[1] * lr->L[2];
+  std::vector<int64_t> poff;
+  int rc = spin_offsets(ctx, norb, nspin, spinsgn, (long long)npot, poff);
+  if (rc) return rc;
   double *dpot = nullptr, *dpsi = nullptr, *dh = nullptr, *den = nullptr;
-  int rc = BDFT_OK;
-  hipError_t e = hipMalloc(&dpot, npot * sizeof(double));
+  hipError_t e = hipMalloc(&dpot, npot * nspin * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&dpsi, ncoef * norb * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&dh, ncoef * norb * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&den, 2 * (size_t)norb * sizeof(double));
   if (e != hipSuccess) rc = set_err(ctx, BDFT_ERR_NOMEM, "device buffers: %s", hipGetErrorString(e));
   hipStream_t s = ctx->stream;
   if (!rc) {
-    e = hipMemcpyAsync(dpot, pot, npot * sizeof(double), hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(dpot, pot, npot * nspin * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(dpsi, psi, ncoef * norb * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -1550,8 +1810,8 @@ extern "C" int bdft_local_hamiltonian(bdft_ctx* ctx, const bdft_lr* lr, int norb
   if (!rc) {
     std::vector<const bdft_lr*> lrs((size_t)norb, lr);
     bdft_plan* plan = nullptr;
-    rc = make_plan(ctx, norb, lrs.data(), dpot, nullptr, nullptr, nullptr, dpsi, nullptr, dh,
-                   den, den + norb, precision, true, &plan);
+    rc = make_plan_chunked(ctx, norb, lrs.data(), dpot, poff.data(), nullptr, nullptr, dpsi,
+                           nullptr, dh, den, den + norb, precision, true, &plan);
     if (!rc) {
       rc = run_plan(plan);
       if (!rc) {
@@ -1661,10 +1921,11 @@ extern "C" int bdft_profile_read(bdft_ctx* ctx, int nstage, double* ms, int64_t*
       HIPCHK(ctx, hipEventElapsedTime(&t, set[2 * s], set[2 * s + 1]));
       ctx->prof_ms[s] += t;
     }
-    ++ctx->prof_n;
     for (auto x : set) ctx->ev_free.push_back(x);
   }
+  for (char c : ctx->ev_counts) ctx->prof_n += c;
   ctx->ev_pending.clear();
+  ctx->ev_counts.clear();
   for (int s = 0; s < nstage && s < S_COUNT; ++s) ms[s] = ctx->prof_ms[s];
   if (nlaunch) *nlaunch = ctx->prof_n;
   if (reset) {
@@ -1713,6 +1974,14 @@ extern "C" int bdft_ctx_set_option(bdft_ctx* ctx, int option, int value) {
         ctx->opt_bwd = value;
       }
       return BDFT_OK;
+    case BDFT_OPT_CHUNK_MB:
+      if (value < 0 || value > (1 << 20))
+        return set_err(ctx, BDFT_ERR_ARG, "chunk size: 0 .. 2^20 MiB (got %d)", value);
+      if (ctx->opt_chunk_mb != value) {
+        drop_cached_plans(ctx);
+        ctx->opt_chunk_mb = value;
+      }
+      return BDFT_OK;
     case BDFT_OPT_ZGROUP:
       if (value != 0 && value != 2 && value != 3)
         return set_err(ctx, BDFT_ERR_ARG, "z-line bundles per work-group: 0, 2 or 3 (got %d)",
@@ -1733,6 +2002,7 @@ extern "C" int bdft_ctx_get_option(const bdft_ctx* ctx, int option, int* value) 
     case BDFT_OPT_ZGROUP: *value = ctx->opt_zg; return BDFT_OK;
     case BDFT_OPT_ZPOT: *value = ctx->opt_zpot; return BDFT_OK;
     case BDFT_OPT_BWD: *value = ctx->opt_bwd; return BDFT_OK;
+    case BDFT_OPT_CHUNK_MB: *value = ctx->opt_chunk_mb; return BDFT_OK;
   }
   return BDFT_ERR_ARG;
 }

@@ -22,7 +22,16 @@ constexpr int NT = TW * TY;     // threads per block
 constexpr int LDW = TW + 1;     // padded LDS row (doubles)
 constexpr int HL = 15;          // halo rows left of a coarse section
 constexpr int HR = 18;          // halo rows right of a coarse section
-constexpr int MAXN = 64;        // largest coarse extent handled per axis
+constexpr int MAXN = 256;       // largest coarse extent handled per axis
+// longest lines whose outputs the line kernels hold in registers: 64 coarse
+// points (x shrink, x kinetic), 192 fine points (x and z grow); longer
+// lines take the kernels' BIG forms (outputs stored as they are made) and
+// the two-kernel z potential pass
+constexpr int REG_N = 64;
+constexpr int REG_L = 192;
+// largest coarse extent of the matrix-core kernels (z-line loaders of at
+// most 128 columns); batches with a larger box run the FP32 vector passes
+constexpr int MFMA_MAXN = 64;
 
 // Per-box descriptor (device resident).
 struct BoxDesc {
@@ -37,6 +46,8 @@ struct BoxDesc {
   long long w_off;      // offset of the wavelet volume (8 N1 N2 N3)
   long long t1_off;     // offset of (L1, 2N2, 2N3)
   long long t2_off;     // offset of (L1, L2, 2N3)
+  long long t3_off;     // offset of the real volume (L1, L2, L3): z lines
+                        // longer than 192 points only (k_zgrowpot)
   long long pot_off;    // offset into the potential array
   long long psi_off;    // offset of this orbital's compressed coefficients
   const int* cmap;      // coarse points: dense linear index i1+N1*(i2+N2*i3)
@@ -83,8 +94,14 @@ struct OpTab {
   int Dp, Kp;       // padded to multiples of 32 and 16
   long long off;    // unused (kept for layout)
   int moff;         // offset of the row blocks' {mask, fragment} entries
-  int pad_;
+  int soff;         // fp16 G tables: offset of the rows' inverse scales u_r
+                    // in Batch::trs (row r holds s_r = 1/u_r times G's row);
+                    // -1: unscaled
 };
+// fp16 tables: scale of the kinetic table, and of every G output before it
+// is split into pieces again (G rows are scaled to sums in (2^12, 2^13])
+constexpr double KIN16 = 512.0;
+constexpr float G16_OUT = 0x1p-13f;
 
 // Tile bookkeeping for one pass: tiles of box b are [start[b], start[b+1]).
 struct TileTable {
@@ -288,6 +305,7 @@ struct Batch {
   void* K;              // kinetic accumulator volumes (wavelet layout)
   void* T1;             // (L1,2N2,2N3) volumes
   void* T2;             // (L1,L2,2N3) volumes
+  void* T3;             // (L1,L2,L3) volumes (long z lines only)
                         // (element type: double, or float for mixed precision)
   double* part;         // energy partial sums, one slot per (pass, tile)
   double* ekin;         // per-box kinetic energy (device)
@@ -297,6 +315,7 @@ struct Batch {
   const uint2* tmq;     // per 32-row block: {K-step mask, first fragment}
   unsigned* bmax;       // per box: largest |psi| (float bits), fp16 scaling
   float* tmax;          // per uncompress tile: largest |psi|
+  const float* trs;     // inverse row scales of the fp16 G tables (OpTab::soff)
 };
 
 // Passes of one application; each has its own tile table.
@@ -319,6 +338,10 @@ struct PassTables {
   // preconditioner: x and y operator in one plane pass (FP64 planes that fit)
   int pc_plane;
   size_t lds_pc;
+  // lines beyond the register-held outputs: x passes in their BIG forms
+  // (some N1 > REG_N or L1 > REG_L), z potential pass in two kernels through
+  // T3 (some L3 > REG_L)
+  int xbig, zbig;
 };
 
 // Launch stages in order; profiling records one event pair per stage.
@@ -410,6 +433,8 @@ struct BoundsArgs {
 hipError_t launch_bounds(const BoundsArgs& A, hipStream_t s);
 
 size_t lds_bytes(int maxN, int elem);
+// raise the dynamic-LDS limit of the line kernels to lds bytes
+void line_attributes(size_t lds);
 hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
                           long long wtotal, bool need_zero, int fp32, hipStream_t s);
 void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s);

@@ -78,6 +78,17 @@ __device__ __forceinline__ float box_scale(const Batch& B, int b) {
   return ldexpf(1.0f, 12 - e);
 }
 
+// factor that removes the fp16 kinetic table's scale (KIN16) from a product
+template <int M>
+__device__ __forceinline__ float kin_unscale() {
+  return M == 16 ? (float)(1.0 / KIN16) : 1.0f;
+}
+// inverse row scale u_r of a G table (fp16 tables scale each row, see
+// OpTab::soff; 1 for unscaled tables and padding rows)
+__device__ __forceinline__ float grow_u(const Batch& B, const OpTab& t, int r) {
+  return (t.soff >= 0 && r < t.rows) ? B.trs[t.soff + r] : 1.0f;
+}
+
 // D row held by lane in accumulator register r
 __device__ __forceinline__ int drow(int r, int lane) {
   return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -368,7 +379,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const float ks = (float)bd.kscale[AX];
+  const float ks = (float)bd.kscale[AX] * kin_unscale<NT4>();
   float* K = (float*)B.K;
   double ek = 0.0;
   for (int ob = wv; ob < t.Dp / 32; ob += NTM / 64) {
@@ -452,6 +463,7 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
       const int ob = it % nob, lb = it / nob;
       f32x16 acc = {};
       mfma_block<NT4>(B, tx, 32 * ob, rh + lb * 32 * XS1, rl + lb * 32 * XS1, XS1, acc);
+      if (NT4 == 16) acc *= G16_OUT;   // line p1 keeps its row scale s(p1)
       const int j2 = 32 * lb + (lane & 31);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -475,10 +487,13 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
       f32x16 acc = {};
       mfma_block<NT4>(B, ty, 32 * ob, ch + lb * 32 * XS2, cl + lb * 32 * XS2, XS2, acc);
       const int p1 = 32 * lb + (lane & 31);
+      // fp16 tables: remove the row scales of p1 (G_x) and p2 (G_y)
+      const float f = NT4 == 16 ? isc * grow_u(B, tx, p1) / G16_OUT : isc;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int p2 = 32 * ob + drow(r, lane);
-        if (p1 < L1 && p2 < L2) dst[p1 + (long long)L1 * p2] = acc[r] * isc;
+        const float u2 = NT4 == 16 ? grow_u(B, ty, p2) : 1.0f;
+        if (p1 < L1 && p2 < L2) dst[p1 + (long long)L1 * p2] = acc[r] * f * u2;
       }
     }
   }
@@ -556,7 +571,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
           v = pot[pz * q3];
           if (bd.conf_on) v += conf_term(bd, cxy + conf_axis2(bd, 2, p3));
         }
-        const double pr = (double)(acc[r] * isc);
+        const double pr = (double)(acc[r] * isc * grow_u(B, tg, p3));
         ep = fma(v * pr, pr, ep);
         __bf16 h, lo;
         split2<BM>((float)(v * pr), h, lo);
@@ -762,7 +777,8 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const int nobK = KX / 32;         // output blocks j1 of the kinetic
   const int nobG = tx.Dp / 32;      // output blocks p1 of G_x
   float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-  const float ksx = (float)bd.kscale[0] * isc, ksy = (float)bd.kscale[1] * isc;
+  const float ksx = (float)bd.kscale[0] * isc * kin_unscale<NT4>(),
+              ksy = (float)bd.kscale[1] * isc * kin_unscale<NT4>();
   // The kinetic tiles and G_x -> C: both read only R, so they share one
   // phase.  The nK kinetic tiles (two operator sweeps each) come first, then
   // the nobG x nlb (output block, line block) units of G_x (one sweep); the
@@ -794,6 +810,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
         const int g = q - nK, ob = g / nlb, lb = g - ob * nlb;
         f32x16 acc = {};
         mma<NT4, true>(B, tx, 32 * ob, rh + 32 * lb * XR, rl + 32 * lb * XR, XR, acc);
+        if (NT4 == 16) acc *= G16_OUT;   // line p1 keeps its row scale s(p1)
         store_cols<NT4>(acc, ch, cl, XC, 32 * ob + (lane & 31), 32 * lb);
       }
     }
@@ -818,10 +835,13 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
           if (j >= nb) break;
           const int p1 = 32 * (lb0 + j) + (lane & 31);
           if (p1 < L1) {
+            // fp16 tables: remove the row scales of p1 (G_x) and p2 (G_y)
+            const float f = NT4 == 16 ? isc * grow_u(B, tx, p1) / G16_OUT : isc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
               const int p2 = 32 * ob + drow(r, lane);
-              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r] * isc;
+              const float u2 = NT4 == 16 ? grow_u(B, ty, p2) : 1.0f;
+              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r] * f * u2;
             }
           }
         }
@@ -1055,7 +1075,7 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const float ks = (float)bd.kscale[2] * isc;
+  const float ks = (float)bd.kscale[2] * isc * kin_unscale<NT4>();
   float* K = (float*)B.K;
   double ek = 0.0;
   const int nobk = t.Dp / 32;
@@ -1199,7 +1219,7 @@ __global__ __launch_bounds__(NTM, 2) void k_mzp(Batch B, TileTable tt, double* p
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int r = 4 * g + u;
-        const double pr = (double)(a[r] * isc);
+        const double pr = (double)(a[r] * isc * grow_u(B, tg, 32 * ob + drow(r, lane)));
         ep = fma(vv[r] * pr, pr, ep);
         __bf16 x, y;
         split2<BM>((float)(vv[r] * pr), x, y);
@@ -1449,8 +1469,15 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       k_mgrow_plane<FM><<<PT.ntiles[P_PF], NTMP, L.grow, s>>>(B, PT.t[P_PF]);
     mark(S_PF, 1);
     mark(S_ZP, 0);
-    if (PT.ntiles[P_ZP] > 0)
+    if (FM == 16) {
+      // F16X3: G_z, V, G_z^T in FP32 (the z lines of a free box's x-y halo
+      // are far below the box scale the fp16 pieces are cut for)
+      const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
+                                           PT.part[3], s);
+      if (e != hipSuccess) return e;
+    } else if (PT.ntiles[P_ZP] > 0) {
       k_mzpot<FM, BM><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);
+    }
     mark(S_ZP, 1);
     mark(S_PB, 0);
     if (PT.ntiles[P_PB] > 0)

@@ -345,7 +345,10 @@ __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // x pass, forward: T1 = G_x W, K = K_x W, ekin partial
 // ---------------------------------------------------------------------------
-template <typename T>
+// BIG: lines longer than the register-held outputs cover (N1 > 64 or
+// L1 > 192): every thread stores its outputs straight to memory, 8 (G) or
+// 4 + 4 (kinetic) consecutive values of its line at a time
+template <typename T, bool BIG>
 __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
@@ -365,6 +368,40 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N1 + HL + HR;
   const T* col = lds + tx;
+  if (BIG) {
+    T* dst = (T*)B.T1 + bd.t1_off + (long long)(l0 + tx) * L1;
+    for (int p0 = 8 * ty; p0 < L1; p0 += 8 * TY) {
+      T g[8];
+      const int r0 = grow_row0(p0, per);
+      grow8(col + r0 * LDW, col + (NS + r0) * LDW, g);
+      if (tx < nl) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (p0 + r < L1) dst[p0 + r] = g[r];
+      }
+    }
+    T* kdst = (T*)B.K + bd.w_off + (long long)(l0 + tx) * W1;
+    const T ks = (T)bd.kscale[0];
+    double ek = 0.0;
+    for (int i0 = 4 * ty; i0 < N1; i0 += 4 * TY) {
+      T kc[4], kd[4];
+      kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, kc, kd);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        kc[r] *= ks;
+        kd[r] *= ks;
+        if (i0 + r < N1 && tx < nl) {
+          ek += (double)kc[r] * (double)col[(i0 + r + HL) * LDW] +
+                (double)kd[r] * (double)col[(NS + i0 + r + HL) * LDW];
+          kdst[i0 + r] = kc[r];
+          kdst[N1 + i0 + r] = kd[r];
+        }
+      }
+    }
+    const double eks = block_sum(ek, red);
+    if (threadIdx.x == 0) part[tile] = eks;
+    return;
+  }
   T g[3][8];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -612,6 +649,96 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
 }
 
 // ---------------------------------------------------------------------------
+// The same pass for z lines longer than its register-held outputs cover
+// (L3 > 192), in two kernels through the real volume R (L1, L2, L3) of the
+// box: k_zgrowpot R = V G_z T2 (+ epot partial), k_zshrink T2 = G_z^T R.
+// Same tiles, operators and order of operations as k_zpot.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void k_zgrowpot(Batch B, TileTable tt, double* part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
+  __shared__ double red[NT / 64];
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int L1 = bd.L[0], L2 = bd.L[1], N3 = bd.N[2], L3 = bd.L[2], per = bd.per;
+  const int nxt = (L1 + TW - 1) / TW;
+  const int j2 = lt / nxt;
+  const int j1 = (lt - j2 * nxt) * TW;
+  const int nl = min(TW, L1 - j1);
+  const long long pstride = (long long)L1 * L2;
+  const T* line = (const T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
+  fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
+  __syncthreads();
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
+  const int NS = N3 + HL + HR;
+  const T* col = lds + tx;
+  const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
+  const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
+                    q2 < bd.potdim[1];
+  const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
+  const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
+  T* R = (T*)B.T3 + bd.t3_off + j1 + tx + (long long)L1 * j2;
+  double ep = 0.0;
+  for (int p0 = 8 * ty; p0 < L3; p0 += 8 * TY) {
+    T g[8];
+    const int r0 = grow_row0(p0, per);
+    grow8(col + r0 * LDW, col + (NS + r0) * LDW, g);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int q3 = p0 + r - bd.ishift[2];
+      double v = 0.0;
+      if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) {
+        v = pot[pz * q3];
+        if (bd.conf_on) v += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
+      }
+      const double pr = (double)g[r];
+      ep = fma(v * pr, pr, ep);
+      if (tx < nl && p0 + r < L3) R[pstride * (p0 + r)] = (T)(v * pr);
+    }
+  }
+  const double eps = block_sum(ep, red);
+  if (threadIdx.x == 0) part[tile] = eps;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_zshrink(Batch B, TileTable tt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = (T*)smem;
+  const int tile = blockIdx.x;
+  const int b = find_box(tt, tile);
+  const BoxDesc bd = B.box[b];
+  const int lt = tile - box_first_tile(tt, b);
+  const int L1 = bd.L[0], L2 = bd.L[1], N3 = bd.N[2], L3 = bd.L[2], per = bd.per;
+  const int nxt = (L1 + TW - 1) / TW;
+  const int j2 = lt / nxt;
+  const int j1 = (lt - j2 * nxt) * TW;
+  const int nl = min(TW, L1 - j1);
+  const long long pstride = (long long)L1 * L2;
+  const T* R = (const T*)B.T3 + bd.t3_off + j1 + (long long)L1 * j2;
+  fill_real<false>(lds, R, 1, pstride, nl, N3, L3, per);
+  __syncthreads();
+  const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
+  const T* col = lds + tx;
+  T* line = (T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
+  for (int i0 = 4 * ty; i0 < N3; i0 += 4 * TY) {
+    T oc[4], od[4];
+    shrink4(col + 2 * i0 * LDW, oc, od);
+    if (tx < nl) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (i0 + r < N3) {
+          line[tx + pstride * (i0 + r)] = oc[r];
+          line[tx + pstride * (N3 + i0 + r)] = od[r];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // y pass, backward: T1 = G_y^T T2
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -651,7 +778,9 @@ __global__ __launch_bounds__(NT) void k_ybwd(Batch B, TileTable tt) {
 // ---------------------------------------------------------------------------
 // x pass, backward: W = G_x^T T1 + K
 // ---------------------------------------------------------------------------
-template <typename T>
+// BIG: N1 > 64 (more coarse outputs than the registers hold): outputs
+// plus K go straight to memory
+template <typename T, bool BIG>
 __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
@@ -669,6 +798,24 @@ __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
   __syncthreads();
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const T* col = lds + tx;
+  if (BIG) {
+    const long long o = bd.w_off + (long long)(l0 + tx) * W1;
+    T* W = (T*)B.W + o;
+    const T* K = (const T*)B.K + o;
+    for (int i0 = 4 * ty; i0 < N1; i0 += 4 * TY) {
+      T oc[4], od[4];
+      shrink4(col + 2 * i0 * LDW, oc, od);
+      if (tx < nl) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (i0 + r < N1) {
+            W[i0 + r] = oc[r] + K[i0 + r];
+            W[N1 + i0 + r] = od[r] + K[N1 + i0 + r];
+          }
+      }
+    }
+    return;
+  }
   T oc[2][4], od[2][4];
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
@@ -762,47 +909,52 @@ __global__ __launch_bounds__(NT) void k_zden(Batch B, DenArgs D) {
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
   const T* col = lds + tx;
-  double acc[NSPIN][3][8];
-#pragma unroll
-  for (int s = 0; s < NSPIN; ++s)
-#pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-      for (int r = 0; r < 8; ++r) acc[s][m][r] = 0.0;
   const int o0 = g * D.gsize, o1 = min(D.norb, o0 + D.gsize);
-  for (int o = o0; o < o1; ++o) {
-    const double hf = D.hfac[o];
-    const int sp = NSPIN == 2 ? D.spin[o] : 0;
-    const T* line = (const T*)B.T2 + B.box[o].t2_off + j1 + (long long)L1 * j2;
-    fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
-    __syncthreads();
+  // z lines longer than the 192 outputs the registers hold are done in
+  // rounds of 192 outputs, each reading the group's lines again
+  for (int pb = 0; pb < L3; pb += 24 * TY) {
+    double acc[NSPIN][3][8];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const int p0 = 8 * (ty + m * TY);
-      if (p0 < L3) {
-        T gv[8];
-        grow8(col + grow_row0(p0, per) * LDW, col + (NS + grow_row0(p0, per)) * LDW, gv);
+    for (int s = 0; s < NSPIN; ++s)
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          const double v = (double)gv[r];
-          if (NSPIN == 2 && sp == 1) acc[NSPIN - 1][m][r] += hf * (v * v);
-          else acc[0][m][r] += hf * (v * v);
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[s][m][r] = 0.0;
+    for (int o = o0; o < o1; ++o) {
+      const double hf = D.hfac[o];
+      const int sp = NSPIN == 2 ? D.spin[o] : 0;
+      const T* line = (const T*)B.T2 + B.box[o].t2_off + j1 + (long long)L1 * j2;
+      fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const int p0 = pb + 8 * (ty + m * TY);
+        if (p0 < L3) {
+          T gv[8];
+          grow8(col + grow_row0(p0, per) * LDW, col + (NS + grow_row0(p0, per)) * LDW, gv);
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            const double v = (double)gv[r];
+            if (NSPIN == 2 && sp == 1) acc[NSPIN - 1][m][r] += hf * (v * v);
+            else acc[0][m][r] += hf * (v * v);
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  if (tx >= nl) return;
+    if (tx < nl) {
 #pragma unroll
-  for (int s = 0; s < NSPIN; ++s) {
-    double* dst = D.part + ((long long)g * D.nspin + s) * D.nisf + j1 + tx +
-                  (long long)L1 * j2;
+      for (int s = 0; s < NSPIN; ++s) {
+        double* dst = D.part + ((long long)g * D.nspin + s) * D.nisf + j1 + tx +
+                      (long long)L1 * j2;
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const int p0 = 8 * (ty + m * TY);
+        for (int m = 0; m < 3; ++m) {
+          const int p0 = pb + 8 * (ty + m * TY);
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        if (p0 + r < L3) dst[pstride * (p0 + r)] = acc[s][m][r];
+          for (int r = 0; r < 8; ++r)
+            if (p0 + r < L3) dst[pstride * (p0 + r)] = acc[s][m][r];
+        }
+      }
     }
   }
 }
@@ -836,8 +988,10 @@ static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs
       if (e != hipSuccess) return e;
     }
   } else {
-    if (PT.ntiles[P_XF] > 0)
-      k_xfwd<T><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], PT.part[0]);
+    if (PT.ntiles[P_XF] > 0) {
+      if (PT.xbig) k_xfwd<T, true><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], PT.part[0]);
+      else k_xfwd<T, false><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], PT.part[0]);
+    }
     if (PT.ntiles[P_YF] > 0) k_yfwd<T><<<PT.ntiles[P_YF], NT, lds, s>>>(B, PT.t[P_YF]);
   }
   const int nblk = D.ngroups * D.nbund;
@@ -868,6 +1022,19 @@ size_t lds_bytes(int maxN, int elem) {
   return (size_t)2 * (maxN + HL + HR) * LDW * elem;
 }
 
+// z potential pass over the P_ZP tiles: one kernel, or two through T3 for
+// z lines longer than the registers hold
+template <typename T>
+static void zpot_pass(const Batch& B, const PassTables& PT, size_t lds, hipStream_t s) {
+  if (PT.ntiles[P_ZP] <= 0) return;
+  if (PT.zbig) {
+    k_zgrowpot<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3]);
+    k_zshrink<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP]);
+  } else {
+    k_zpot<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3]);
+  }
+}
+
 template <typename T>
 static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
                              long long wtotal, bool need_zero, hipStream_t s,
@@ -894,7 +1061,9 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
       launch_plane_fwd<T>(B, PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pf, PT.part[0], s);
     mark(S_PF, 1);
     BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
-    BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot<T>, B, PT.t[P_ZP], PT.part[3])
+    mark(S_ZP, 0);
+    zpot_pass<T>(B, PT, lds, s);
+    mark(S_ZP, 1);
     mark(S_PB, 0);
     if (PT.ntiles[P_PB] > 0)
       launch_plane_bwd<T>(B, PT.t[P_PB], PT.ntiles[P_PB], PT.lds_pb, s);
@@ -908,13 +1077,23 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
     et.nk = 2;
     et.n = 3;
   } else {
-    BDFT_STAGE(S_XF, P_XF, lds, k_xfwd<T>, B, PT.t[P_XF], PT.part[0])
+    if (PT.xbig) {
+      BDFT_STAGE(S_XF, P_XF, lds, (k_xfwd<T, true>), B, PT.t[P_XF], PT.part[0])
+    } else {
+      BDFT_STAGE(S_XF, P_XF, lds, (k_xfwd<T, false>), B, PT.t[P_XF], PT.part[0])
+    }
     BDFT_STAGE(S_YF, P_YF, lds, k_yfwd<T>, B, PT.t[P_YF])
     BDFT_STAGE(S_KY, P_KY, lds, (k_kin<T, 1>), B, PT.t[P_KY], PT.part[1])
     BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
-    BDFT_STAGE(S_ZP, P_ZP, lds, k_zpot<T>, B, PT.t[P_ZP], PT.part[3])
+    mark(S_ZP, 0);
+    zpot_pass<T>(B, PT, lds, s);
+    mark(S_ZP, 1);
     BDFT_STAGE(S_YB, P_YB, lds, k_ybwd<T>, B, PT.t[P_YB])
-    BDFT_STAGE(S_XB, P_XB, lds, k_xbwd<T>, B, PT.t[P_XB])
+    if (PT.xbig) {
+      BDFT_STAGE(S_XB, P_XB, lds, (k_xbwd<T, true>), B, PT.t[P_XB])
+    } else {
+      BDFT_STAGE(S_XB, P_XB, lds, (k_xbwd<T, false>), B, PT.t[P_XB])
+    }
     BDFT_STAGE(S_COMP, P_UNC, 0, (k_uncompress<T, true>), B, PT.t[P_UNC])
     et.t[0] = PT.t[P_XF];
     et.t[1] = PT.t[P_KY];
@@ -1112,7 +1291,9 @@ __global__ __launch_bounds__(NT) void k_pc_scatter(Batch B, TileTable tt, PcArgs
   }
 }
 
-// K = c W + K~_x W along x lines (the line loading of k_xfwd)
+// K = c W + K~_x W along x lines (the line loading of k_xfwd); BIG: more
+// than 64 coarse points per line, outputs stored as they are made
+template <bool BIG>
 __global__ __launch_bounds__(NT) void k_pc_kinx(Batch B, TileTable tt, PcArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* lds = (double*)smem;
@@ -1132,6 +1313,20 @@ __global__ __launch_bounds__(NT) void k_pc_kinx(Batch B, TileTable tt, PcArgs A)
   const int NS = N1 + HL + HR;
   const double* col = lds + tx;
   const double ks = bd.kscale[0], c = A.pc[9 * b];
+  if (BIG) {
+    double* kl = (double*)B.K + bd.w_off + (long long)(l0 + tx) * W1;
+    for (int i0 = 4 * ty; i0 < N1; i0 += 4 * TY) {
+      double oc[4], od[4];
+      kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, oc, od);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (i0 + r < N1 && tx < nl) {
+          kl[i0 + r] = fma(c, col[(i0 + r + HL) * LDW], oc[r] * ks);
+          kl[N1 + i0 + r] = fma(c, col[(NS + i0 + r + HL) * LDW], od[r] * ks);
+        }
+    }
+    return;
+  }
   double kc[2][4], kd[2][4];
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
@@ -1280,7 +1475,10 @@ hipError_t launch_precond(const Batch& B, const PassTables& PT, int maxN, long l
       // c W + K_x W + K_y W of each plane in one pass through LDS
       (void)launch_plane_kin_pc(B, PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pc, A.pc, s);
     } else {
-      if (PT.ntiles[P_XF] > 0) k_pc_kinx<<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], A);
+      if (PT.ntiles[P_XF] > 0) {
+        if (PT.xbig) k_pc_kinx<true><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], A);
+        else k_pc_kinx<false><<<PT.ntiles[P_XF], NT, lds, s>>>(B, PT.t[P_XF], A);
+      }
       if (PT.ntiles[P_KY] > 0)
         k_kin<double, 1><<<PT.ntiles[P_KY], NT, lds, s>>>(B, PT.t[P_KY], PT.part[1]);
     }
@@ -1301,6 +1499,29 @@ hipError_t launch_precond(const Batch& B, const PassTables& PT, int maxN, long l
   }
   k_pc_final<<<nu, NT, 0, s>>>(B, tu, A);
   return hipGetLastError();
+}
+
+// the line kernels may take up to lds_bytes(MAXN, 8) of dynamic LDS
+template <typename T>
+static void line_attr(int bytes) {
+  const void* k[] = {(const void*)k_xfwd<T, false>, (const void*)k_xfwd<T, true>,
+                     (const void*)k_yfwd<T>, (const void*)k_kin<T, 1>, (const void*)k_kin<T, 2>,
+                     (const void*)k_zpot<T>, (const void*)k_zgrowpot<T>,
+                     (const void*)k_zshrink<T>, (const void*)k_ybwd<T>,
+                     (const void*)k_xbwd<T, false>, (const void*)k_xbwd<T, true>,
+                     (const void*)k_zden<T, 1>, (const void*)k_zden<T, 2>};
+  for (const void* f : k)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+      (void)hipGetLastError();
+}
+void line_attributes(size_t lds) {
+  line_attr<double>((int)lds);
+  line_attr<float>((int)lds);
+  const void* k[] = {(const void*)k_pc_kinx<false>, (const void*)k_pc_kinx<true>};
+  for (const void* f : k)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess)
+      (void)hipGetLastError();
 }
 
 }  // namespace bdft

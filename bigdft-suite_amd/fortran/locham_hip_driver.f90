@@ -2,13 +2,14 @@
 !! segkeys does, bigdft/src/locregs/initlocregs.f90:1633-1686), deterministic
 !! orbitals and potential, one call of local_hamiltonian_hip, and the
 !! weighted energies printed for the test-suite to compare with the oracle.
-!! Usage: locham_hip_driver n norb precision
+!! Usage: locham_hip_driver n norb precision [line_passes [nspin]]
+!! nspin 2: a second potential, orbitals 2,4,... spin down
 program locham_hip_driver
   use iso_c_binding
   use module_locham_hip
   implicit none
   type(c_ptr) :: ctx, lr
-  integer :: n, norb, prec, ierr, nseg, nvctr, iseg, i2, i3, i, iorb, ncoef, npot
+  integer :: n, norb, prec, ierr, nseg, nvctr, iseg, i2, i3, i, iorb, ncoef, npot, nspin
   integer, dimension(:, :), allocatable :: keyg
   integer, dimension(:), allocatable :: keyv
   real(kind=8), dimension(:, :), allocatable :: psi, hpsi
@@ -36,17 +37,24 @@ program locham_hip_driver
   keyg(:, nseg + 1:) = keyg(:, 1:nseg)
   keyv(nseg + 1:) = keyv(1:nseg)
 
+  nspin = 1
+  if (command_argument_count() >= 5) then
+     call get_command_argument(5, arg); read(arg, *) nspin
+  end if
   ncoef = nvctr + 7 * nvctr
   npot = (2 * n + 2) ** 3
-  allocate(psi(ncoef, norb), hpsi(ncoef, norb), pot(npot), occup(norb))
+  allocate(psi(ncoef, norb), hpsi(ncoef, norb), pot(npot * 2), occup(norb), spinsgn(norb))
   do iorb = 1, norb
      do i = 1, ncoef
         psi(i, iorb) = sin(0.37d0 * i + 1.3d0 * iorb) * cos(0.011d0 * i)
      end do
      occup(iorb) = 2.0d0 / iorb
+     spinsgn(iorb) = 1.0d0
+     if (nspin == 2) spinsgn(iorb) = 1.0d0 - 2.0d0 * mod(iorb - 1, 2)
   end do
   do i = 1, npot
      pot(i) = -1.0d0 - cos(0.05d0 * i)
+     pot(npot + i) = -0.5d0 - sin(0.07d0 * i)
   end do
   hpsi = 0.0d0
 
@@ -64,8 +72,8 @@ program locham_hip_driver
      print '(a)', trim(bdft_error_message(ctx))
      stop 'region creation failed'
   end if
-  call local_hamiltonian_hip(ctx, lr, ncoef, norb, 1.0d0, occup, pot, psi, hpsi, &
-       ekin, epot, prec, ierr)
+  call local_hamiltonian_hip(ctx, lr, ncoef, norb, nspin, 1.0d0, occup, spinsgn, pot, &
+       psi, hpsi, ekin, epot, prec, ierr)
   if (ierr /= 0) then
      print '(a)', trim(bdft_error_message(ctx))
      stop 'local hamiltonian failed'
@@ -75,7 +83,7 @@ program locham_hip_driver
   print '(a,es25.17)', 'epot_sum ', epot
   print '(a,es25.17)', 'psi_hpsi ', hsum
   ! density, alternating spins, accumulated on a 1e-20 prefill
-  allocate(rho(npot, 2), spinsgn(norb))
+  allocate(rho(npot, 2))
   rho = 1.0d-20
   do iorb = 1, norb
      spinsgn(iorb) = 1.0d0 - 2.0d0 * mod(iorb - 1, 2)
@@ -86,7 +94,7 @@ program locham_hip_driver
      print '(a)', trim(bdft_error_message(ctx))
      stop 'density failed'
   end if
-  print '(a,es25.17)', 'rho_up_pot ', sum(rho(:, 1) * pot)
+  print '(a,es25.17)', 'rho_up_pot ', sum(rho(:, 1) * pot(1:npot))
   print '(a,es25.17)', 'rho_dn_sum ', sum(rho(:, 2))
   call free_gpu_hip(ctx, lr)
 end program locham_hip_driver

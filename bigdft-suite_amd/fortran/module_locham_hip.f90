@@ -15,7 +15,7 @@ module module_locham_hip
   !> launch options (BDFT_OPT_* of bigdft_hip.h), e.g. a choice cached by
   !! the autotuner
   integer, parameter, public :: BDFT_OPT_LINE_PASSES = 1, BDFT_OPT_ZGROUP = 2, &
-       BDFT_OPT_ZPOT = 3, BDFT_OPT_BWD = 4
+       BDFT_OPT_ZPOT = 3, BDFT_OPT_BWD = 4, BDFT_OPT_CHUNK_MB = 5
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
        local_partial_density_hip, free_gpu_hip, bdft_error_message, &
@@ -69,12 +69,12 @@ module module_locham_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: lr
      end function bdft_lr_destroy
-     integer(c_int) function bdft_local_hamiltonian(ctx, lr, norb, pot, psi, hpsi, &
-          ekin, epot, precision) bind(C, name='bdft_local_hamiltonian')
+     integer(c_int) function bdft_local_hamiltonian(ctx, lr, norb, nspin, pot, spinsgn, &
+          psi, hpsi, ekin, epot, precision) bind(C, name='bdft_local_hamiltonian')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx, lr
-       integer(c_int), value :: norb, precision
-       real(c_double), dimension(*), intent(in) :: pot, psi
+       integer(c_int), value :: norb, nspin, precision
+       real(c_double), dimension(*), intent(in) :: pot, spinsgn, psi
        real(c_double), dimension(*), intent(inout) :: hpsi
        real(c_double), dimension(*), intent(out) :: ekin, epot
      end function bdft_local_hamiltonian
@@ -184,15 +184,18 @@ contains
 
   !> counterpart of local_hamiltonian_OCL followed by finish_hamiltonian_OCL
   !! (interface_ocl.f90:391-607): hpsi(:,iorb) += H_loc psi(:,iorb) for the
-  !! norbp orbitals, and the weighted sums
+  !! norbp orbitals, with pot(n1i*n2i*n3i, nspin) and the spin-down
+  !! potential pot(:,2) for the orbitals whose spinsgn is not positive
+  !! (rhopot_up / rhopot_down, interface_ocl.f90:487-491), and the weighted
+  !! sums
   !!   ekin_sum = sum kwgt*occup(iorb)*ekin(iorb),
   !!   epot_sum = sum kwgt*occup(iorb)*epot(iorb)   (hpsiortho.f90:946-954)
-  subroutine local_hamiltonian_hip(ctx, lr, ncoef, norbp, kwgt, occup, pot, psi, &
-       hpsi, ekin_sum, epot_sum, precision, ierr)
+  subroutine local_hamiltonian_hip(ctx, lr, ncoef, norbp, nspin, kwgt, occup, spinsgn, &
+       pot, psi, hpsi, ekin_sum, epot_sum, precision, ierr)
     type(c_ptr), intent(in) :: ctx, lr
-    integer, intent(in) :: ncoef, norbp, precision
+    integer, intent(in) :: ncoef, norbp, nspin, precision
     real(kind=8), intent(in) :: kwgt
-    real(kind=8), dimension(norbp), intent(in) :: occup
+    real(kind=8), dimension(norbp), intent(in) :: occup, spinsgn
     real(kind=8), dimension(*), intent(in) :: pot
     real(kind=8), dimension(ncoef, norbp), intent(in) :: psi
     real(kind=8), dimension(ncoef, norbp), intent(inout) :: hpsi
@@ -200,8 +203,8 @@ contains
     integer, intent(out) :: ierr
     real(kind=8), dimension(norbp) :: ekin, epot
     integer :: iorb
-    ierr = bdft_local_hamiltonian(ctx, lr, int(norbp, c_int), pot, psi, hpsi, &
-         ekin, epot, int(precision, c_int))
+    ierr = bdft_local_hamiltonian(ctx, lr, int(norbp, c_int), int(nspin, c_int), pot, &
+         spinsgn, psi, hpsi, ekin, epot, int(precision, c_int))
     ekin_sum = 0.0d0
     epot_sum = 0.0d0
     if (ierr /= 0) return

@@ -67,15 +67,19 @@ enum {
 /* arithmetic of the convolution passes */
 enum {
   BDFT_PREC_FP64 = 0,    /* plain FP64 everywhere */
-  BDFT_PREC_FP32 = 1,    /* FP32 MFMA passes, FP64 energies and accumulate */
+  BDFT_PREC_FP32 = 1,    /* FP32 convolution passes on the vector units, FP64
+                            potential, energies and hpsi accumulation */
   BDFT_PREC_BF16X3 = 2,  /* split-BF16 MFMA, FP32 accumulation: filter and data
                             split hi+lo, products hi*hi + hi*lo + lo*hi */
   BDFT_PREC_BF16X4 = 3,  /* as BF16X3 plus the third filter piece times the data
                             hi part (removes the filter's fixed 2^-16 error) */
-  BDFT_PREC_F16X3 = 4    /* forward transforms and kinetic on fp16 pieces of the
-                            data scaled by a power of two per box (three
-                            products, 2^-22 representation), transposed
-                            transforms as BF16X4; FP32 accumulation */
+  BDFT_PREC_F16X3 = 4    /* x/y forward transforms and the kinetic on fp16
+                            pieces (three products, 2^-22 representation):
+                            data scaled by a power of two per box, each row
+                            of the G tables and each x line of the G_x output
+                            by its own power of two; z potential pass in
+                            FP32; transposed transforms as BF16X4; FP32
+                            accumulation */
 };
 
 /* Context on one device.  Replaces init_acceleration_OCL
@@ -85,7 +89,8 @@ int bdft_ctx_create(int device, bdft_ctx **ctx);
 /* Replaces free_gpu_OCL (interface_ocl.f90:196) for the context part. */
 int bdft_ctx_destroy(bdft_ctx *ctx);
 /* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the
- * context's own stream. */
+ * context's own stream.  Work queued later on the new stream waits for the
+ * work already queued on the old one. */
 int bdft_ctx_set_stream(bdft_ctx *ctx, void *hip_stream);
 /* Message of the last failure on this context ("" if none).  Replaces the
  * print-and-exit of oclErrorCheck (bigdft/src/OpenCL/OpenCL_wrappers.h:89). */
@@ -110,14 +115,18 @@ int bdft_lr_destroy(bdft_lr *lr);
 int bdft_lr_info(const bdft_lr *lr, int nfine[3], int64_t *ncoef);
 
 /* Local Hamiltonian of norb orbitals sharing one region, device pointers:
- * psi, hpsi: norb consecutive orbitals; pot: n1i*n2i*n3i shared;
- * ekin, epot: norb doubles each (device).  Replaces ocl_fulllocham_generic_k
+ * psi, hpsi: norb consecutive orbitals; pot: nspin consecutive potentials
+ * of n1i*n2i*n3i doubles (pot(n1i,n2i,n3i,nspin)); ekin, epot: norb doubles
+ * each (device).  spinsgn: host array of norb doubles (orbs%spinsgn of the
+ * batch) or NULL for all spin up; orbital i reads the first potential where
+ * spinsgn[i] > 0 and the second one otherwise (nspin = 2; a spin-down
+ * orbital with nspin = 1 is BDFT_ERR_ARG).  Replaces ocl_fulllocham_generic_k
  * (bigdft/src/OpenCL/Hamiltonian.c:105-172) called in the orbital loop of
- * local_hamiltonian_OCL (interface_ocl.f90:459-535), all orbitals in one
- * batch. */
-int bdft_locham_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
-                  const double *pot, const double *psi, double *hpsi,
-                  double *ekin, double *epot, int precision);
+ * local_hamiltonian_OCL (interface_ocl.f90:391-407 arguments, rhopot_up /
+ * rhopot_down chosen per orbital at :487-491), all orbitals in one batch. */
+int bdft_locham_d(bdft_ctx *ctx, const bdft_lr *lr, int norb, int nspin,
+                  const double *pot, const double *spinsgn, const double *psi,
+                  double *hpsi, double *ekin, double *epot, int precision);
 
 /* Batched application over orbitals living in different regions (the ket
  * loop over localisation regions, hpsiortho.f90:934-958 with
@@ -125,7 +134,10 @@ int bdft_locham_d(bdft_ctx *ctx, const bdft_lr *lr, int norb,
  * coefficients at psi + psi_off[i] (hpsi likewise), and the potential box
  * pot + pot_off[i] of extents potdim[3i..3i+2] placed at ishift[3i..3i+2] in
  * the region's fine grid (apply_potential_lr_nobounds,
- * bigdft/src/orbitals/potential.f90:850-879).  Device pointers. */
+ * bigdft/src/orbitals/potential.f90:850-879).  Device pointers.  Host
+ * arrays pot_off, potdim, ishift, psi_off may be NULL: potential offset 0,
+ * extents n1i,n2i,n3i of the region, no shift, orbitals stored one after
+ * the other. */
 int bdft_locham_batch_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
                         const double *pot, const int64_t *pot_off,
                         const int32_t *potdim, const int32_t *ishift,
@@ -246,11 +258,13 @@ int bdft_plan_execute(bdft_plan *plan);
 int bdft_plan_destroy(bdft_plan *plan);
 
 /* Host-array convenience with the argument meaning of local_hamiltonian_OCL
- * (interface_ocl.f90:391-407): copies pot and psi in, hpsi (accumulated) and
- * the per-orbital energies out, then synchronises. */
-int bdft_local_hamiltonian(bdft_ctx *ctx, const bdft_lr *lr, int norb,
-                           const double *pot, const double *psi, double *hpsi,
-                           double *ekin, double *epot, int precision);
+ * (interface_ocl.f90:391-407): pot(n1i,n2i,n3i,nspin), spinsgn(norb) (NULL:
+ * all spin up) as bdft_locham_d; copies pot and psi in, hpsi (accumulated)
+ * and the per-orbital energies out, then synchronises. */
+int bdft_local_hamiltonian(bdft_ctx *ctx, const bdft_lr *lr, int norb, int nspin,
+                           const double *pot, const double *spinsgn,
+                           const double *psi, double *hpsi, double *ekin,
+                           double *epot, int precision);
 
 /* Compressed <-> mixed-layout transfers of norb orbitals of one region,
  * device pointers; psig holds norb consecutive (0:n1,2,0:n2,2,0:n3,2)
@@ -294,20 +308,29 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
  *                         matrix-core precisions: 0 automatic (the FP32
  *                         vector-unit line kernel where every box extent is
  *                         at most 64 coarse points, else the matrix cores),
- *                         1 matrix cores (split precision), 2 FP32 vector
- *                         units where they apply
+ *                         1 matrix cores (split precision; F16X3 keeps the
+ *                         FP32 pass), 2 FP32 vector units where they apply
  *   BDFT_OPT_BWD          backward plane pass (G_y^T G_x^T + K, compress)
  *                         of the matrix-core precisions: 0 automatic (the
  *                         FP32 vector-unit plane kernel for free-BC
  *                         batches, the matrix cores otherwise), 1 matrix
  *                         cores, 2 FP32 vector units where the plane kernel
  *                         applies (periodic in-plane axes >= 18 points)
+ *   BDFT_OPT_CHUNK_MB     Hamiltonian batches and plans: 0 one launch per
+ *                         stage over the whole batch; n > 0 the orbitals in
+ *                         consecutive chunks whose working grids take at
+ *                         most n MiB, each chunk's stages run before the
+ *                         next chunk's on one workspace of that size, so
+ *                         the grids a stage writes are read back from the
+ *                         last-level cache (the density, preconditioner and
+ *                         allocation-free calls are not chunked)
  * A change drops the context's cached plans; plans already created by
  * bdft_plan_create keep the options they were made with. */
 #define BDFT_OPT_LINE_PASSES 1
 #define BDFT_OPT_ZGROUP 2
 #define BDFT_OPT_ZPOT 3
 #define BDFT_OPT_BWD 4
+#define BDFT_OPT_CHUNK_MB 5
 int bdft_ctx_set_option(bdft_ctx *ctx, int option, int value);
 int bdft_ctx_get_option(const bdft_ctx *ctx, int option, int *value);
 

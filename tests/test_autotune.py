@@ -11,7 +11,7 @@ from bigdft_hip.descriptors import dense_region
 
 class FakeCtx:
     def __init__(self):
-        self.opts = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0}
+        self.opts = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0}
         self.history = []
 
     def set_option(self, k, v):
@@ -30,14 +30,25 @@ def test_shape_keys():
     b = dense_region("F", (20, 31, 17), (0.4,) * 3)
     c = dense_region("F", (25, 18, 19), (0.4,) * 3)
     assert autotune.shape_key([a] * 4, bigdft_hip.PREC_F16X3) == "P:47x47x47:f16x3"
-    assert autotune.shape_key([b, c], bigdft_hip.PREC_FP64) == "F:max25x31x19:mixed:fp64"
+    k = autotune.shape_key([b, c], bigdft_hip.PREC_FP64)
+    assert k.startswith("F:max25x31x19:mixed") and k.endswith(":fp64")
     assert autotune.shape_key([a, b], bigdft_hip.PREC_FP32).startswith("mixed:max47x47x47")
+    # batches with the same largest extents but other boxes get other keys
+    d = dense_region("F", (25, 31, 19), (0.4,) * 3)
+    assert autotune.shape_key([b, c, d], bigdft_hip.PREC_FP64) != k
+    assert autotune.shape_key([c, b], bigdft_hip.PREC_FP64) == k
+    # the device part keeps tables of other GPUs or library versions apart
+    assert autotune.shape_key([a], bigdft_hip.PREC_FP64, "gfx950/lds163840/v3") == \
+        "gfx950/lds163840/v3|P:47x47x47:fp64"
 
 
 def test_candidates():
     assert len(autotune.candidates(bigdft_hip.PREC_FP64)) == 2
-    c = autotune.candidates(bigdft_hip.PREC_F16X3)
-    assert len(c) == 14 and c[0] == {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0}
+    c16 = autotune.candidates(bigdft_hip.PREC_F16X3)
+    assert len(c16) == 8 and {x["zpot"] for x in c16} == {0}
+    c = autotune.candidates(bigdft_hip.PREC_BF16X4)
+    assert len(c) == 14 and c[0] == {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0,
+                                     "chunk_mb": 0}
     assert {x["zgroup"] for x in c} == {0, 2, 3} and {x["zpot"] for x in c} == {0, 1}
     assert {x["bwd"] for x in c} == {0, 1, 2}
     assert sum(x["line_passes"] for x in c) == 1
@@ -57,8 +68,8 @@ def test_tune_picks_the_fastest_and_caches(tmp_path):
     key, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_BF16X4,
                                      cache=autotune.TuneCache(str(path)), timer=timer)
     assert key == "P:23x23x23:bf16x4"
-    assert opts == {"line_passes": 0, "zgroup": 2, "zpot": 0, "bwd": 2}
-    assert ctx.opts == opts and len(times) == 14
+    assert opts == {"line_passes": 0, "zgroup": 2, "zpot": 0, "bwd": 2, "chunk_mb": 0}
+    assert ctx.opts == opts and len(times) == 14 + len(autotune.CHUNK_MB)
     table = json.loads(path.read_text())
     assert table[key]["options"] == opts and min(table[key]["ms"].values()) == 3.75
     # a second tuner reads the table and times nothing
@@ -91,10 +102,34 @@ def test_tune_skips_option_sets_that_do_not_fit(tmp_path):
                                    cache=autotune.TuneCache(str(tmp_path / "t.json")),
                                    timer=timer)
     assert opts["bwd"] != 1 and opts["line_passes"] == 0 and opts["zgroup"] == 0
-    assert sum(v is None for v in times.values()) == 7
+    assert sum(v is None for v in times.values()) == 4
 
     def other(ctx, *a):
         raise bigdft_hip.BigDFTError(6, "BDFT_ERR_HIP")
     with pytest.raises(bigdft_hip.BigDFTError):
         autotune.tune(FakeCtx(), lrs, None, None, bigdft_hip.PREC_F16X3,
                       cache=autotune.TuneCache(str(tmp_path / "u.json")), timer=other)
+
+
+def test_stale_cache_entry_is_tuned_again(tmp_path):
+    """a cached choice the batch does not fit is dropped and re-tuned"""
+    path = tmp_path / "t.json"
+    reg = dense_region("F", (40, 40, 40), (0.4,) * 3)
+    lrs = [FakeLr(reg)]
+    key = autotune.shape_key([reg], bigdft_hip.PREC_F16X3)
+    cache = autotune.TuneCache(str(path))
+    cache.put(key, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 1}, {})
+
+    def check(ctx, *a):
+        if ctx.opts["bwd"] == 1:
+            raise bigdft_hip.BigDFTError(5, "BDFT_ERR_UNSUPPORTED: LDS")
+
+    def timer(ctx, *a):
+        if ctx.opts["bwd"] == 1:
+            raise bigdft_hip.BigDFTError(5, "BDFT_ERR_UNSUPPORTED: LDS")
+        return 1.0 + ctx.opts["zgroup"]
+    ctx = FakeCtx()
+    _, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_F16X3,
+                                   cache=cache, timer=timer, check=check)
+    assert times is not None and opts["bwd"] != 1
+    assert json.loads(path.read_text())[key]["options"] == opts

@@ -31,7 +31,10 @@ def dev(a):
 CASES = [("P", dense_region("P", (23, 19, 21), (0.45, 0.4, 0.5))),
          ("F", dense_region("F", (15, 13, 17), (0.4, 0.45, 0.5))),
          ("sphere", mixed_sphere_region((24, 22, 26), 0.4, 31)),
-         ("P47", dense_region("P", (47, 47, 47), (0.35,) * 3))]
+         ("P47", dense_region("P", (47, 47, 47), (0.35,) * 3)),
+         # z lines of 211 and 260 fine points: the z kernel works in rounds
+         ("F_large", dense_region("F", (30, 24, 90), (0.4, 0.45, 0.5))),
+         ("P_large", dense_region("P", (37, 45, 129), (0.4, 0.45, 0.5)))]
 
 
 @pytest.mark.parametrize("name,reg", CASES, ids=[c[0] for c in CASES])

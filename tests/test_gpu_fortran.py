@@ -16,10 +16,11 @@ DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no amdflang)")
-@pytest.mark.parametrize("n,norb,lines", [(11, 3, 0), (23, 2, 0), (23, 2, 1)])
-def test_fortran_driver_matches_oracle(n, norb, lines):
-    r = subprocess.run([DRIVER, str(n), str(norb), "0", str(lines)], capture_output=True,
-                       text=True, timeout=120)
+@pytest.mark.parametrize("n,norb,lines,nspin", [(11, 3, 0, 1), (23, 2, 0, 1), (23, 2, 1, 1),
+                                                (11, 4, 0, 2), (23, 3, 1, 2)])
+def test_fortran_driver_matches_oracle(n, norb, lines, nspin):
+    r = subprocess.run([DRIVER, str(n), str(norb), "0", str(lines), str(nspin)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     vals = {l.split()[0]: float(l.split()[1]) for l in r.stdout.splitlines() if l.strip()}
     reg = dense_region("P", (n, n, n), (0.45,) * 3)
@@ -27,8 +28,16 @@ def test_fortran_driver_matches_oracle(n, norb, lines):
     psi = np.stack([np.sin(0.37 * i + 1.3 * o) * np.cos(0.011 * i)
                     for o in range(1, norb + 1)])
     occup = 2.0 / np.arange(1, norb + 1)
-    pot = -1.0 - np.cos(0.05 * np.arange(1, reg.n_isf + 1, dtype=np.float64))
-    h, ek, ep = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
+    ip = np.arange(1, reg.n_isf + 1, dtype=np.float64)
+    pot = -1.0 - np.cos(0.05 * ip)
+    # nspin 2: orbitals 1,3,.. read pot(:,1), orbitals 2,4,.. pot(:,2)
+    # (rhopot_up / rhopot_down by spinsgn, interface_ocl.f90:487-491)
+    pot_dn = -0.5 - np.sin(0.07 * ip)
+    ek, ep = np.zeros(norb), np.zeros(norb)
+    for o in range(norb):
+        v = pot_dn if (nspin == 2 and o % 2 == 1) else pot
+        _, ek[o:o + 1], ep[o:o + 1] = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                                                    okeys(reg), v, psi[o:o + 1])
     assert abs(vals["ekin_sum"] - (occup * ek).sum()) < 1e-12 * abs((occup * ek).sum())
     assert abs(vals["epot_sum"] - (occup * ep).sum()) < 1e-12 * abs((occup * ep).sum())
     assert abs(vals["psi_hpsi"] - (ek + ep).sum()) < 1e-11 * abs((ek + ep).sum())

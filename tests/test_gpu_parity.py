@@ -82,6 +82,92 @@ def test_free_sphere(ctx, seed):
     check(mixed_sphere_region((14, 12, 16), 0.4, seed), 2, 30 + seed, ctx)
 
 
+@pytest.mark.parametrize("geo,host", [("P", False), ("F", False), ("P", True)])
+def test_two_spin_potentials(ctx, geo, host):
+    """pot(n1i,n2i,n3i,2): orbitals whose spinsgn is not positive read the
+    second potential (rhopot_down, interface_ocl.f90:487-491)"""
+    reg = dense_region(geo, (9, 10, 8), (0.45, 0.4, 0.5))
+    norb = 5
+    print(f"case nspin=2 {geo} host={host}", flush=True)
+    psi = random_psi(reg, norb, 61)
+    up, dn = random_pot(reg, 62), random_pot(reg, 63, -4.0, 1.0)
+    sg = np.array([1.0, -1.0, -1.0, 1.0, -0.5])
+    h0 = random_psi(reg, norb, 64)
+    lr = ctx.region(reg)
+    if host:
+        hg = h0.copy()
+        eg, pg = ctx.locham_host(lr, np.concatenate([up, dn]), psi, hg, nspin=2, spinsgn=sg)
+    else:
+        d_h = dev(h0)
+        ek = torch.zeros(norb, dtype=torch.float64, device="cuda")
+        ep = torch.zeros_like(ek)
+        ctx.locham(lr, dev(np.concatenate([up, dn])), dev(psi), d_h, ek, ep, nspin=2,
+                   spinsgn=sg)
+        ctx.sync()
+        hg, eg, pg = d_h.cpu().numpy(), ek.cpu().numpy(), ep.cpu().numpy()
+    for o in range(norb):
+        v = up if sg[o] > 0 else dn
+        ho, eo, po = oracle.locham(reg.geocode, reg.n, reg.nfl, reg.nfu, reg.hgrid,
+                                   okeys(reg), v, psi[o:o + 1], h0[o:o + 1])
+        assert rel_err(hg[o] - h0[o], ho[0] - h0[o]) < TOL
+        assert rel_err(eg[o:o + 1], eo) < TOL
+        assert rel_err(pg[o:o + 1], po) < TOL
+    # a spin-down orbital without a second potential is refused
+    import bigdft_hip
+    with pytest.raises(bigdft_hip.BigDFTError, match="nspin"):
+        ctx.locham_host(lr, up, psi, h0.copy(), nspin=1, spinsgn=sg)
+    lr.close()
+
+
+def test_stream_switch_orders_shared_buffers(ctx):
+    """bdft_ctx_set_stream: a call queued on the new stream waits for the
+    old stream's work, which shares the context's one-shot workspace; the
+    same batch queued on two streams through the raw C ABI (no ordering by
+    the Python layer) gives the same result twice"""
+    import ctypes
+
+    import bigdft_hip
+    L = bigdft_hip.lib()
+    reg = dense_region("P", (31, 31, 31), (0.45,) * 3)
+    norb = 8
+    psi, pot = random_psi(reg, norb, 71), random_pot(reg, 72)
+    c = bigdft_hip.Context(0)
+    lr = c.region(reg)
+    d_psi, d_pot = dev(psi), dev(pot)
+    outs = [(torch.zeros_like(d_psi), torch.zeros(norb, dtype=torch.float64, device="cuda"),
+             torch.zeros(norb, dtype=torch.float64, device="cuda")) for _ in range(2)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    P = ctypes.c_void_p
+    for (h, ek, ep), st in zip(outs, streams):
+        c._check(L.bdft_ctx_set_stream(c.handle, P(st.cuda_stream)))
+        c._check(L.bdft_locham_d(c.handle, lr.handle, norb, 1, P(d_pot.data_ptr()), None,
+                                 P(d_psi.data_ptr()), P(h.data_ptr()), P(ek.data_ptr()),
+                                 P(ep.data_ptr()), 0))
+    torch.cuda.synchronize()
+    for a, b in zip(outs[0], outs[1]):
+        assert torch.equal(a, b)
+    c._stream = streams[1]
+    lr.close()
+    c.close()
+
+
+def test_device_arrays_are_checked(ctx):
+    """CPU, mistyped, strided or short tensors are refused before any launch"""
+    reg = dense_region("P", (7, 7, 7), (0.45,) * 3)
+    lr = ctx.region(reg)
+    psi = torch.zeros((2, reg.ncoef), dtype=torch.float64, device="cuda")
+    pot = torch.zeros(reg.n_isf, dtype=torch.float64, device="cuda")
+    ek = torch.zeros(2, dtype=torch.float64, device="cuda")
+    bad = [(psi.cpu(), psi), (psi.float(), psi), (psi[:, ::2], psi), (psi[:1], psi)]
+    for p, h in bad:
+        with pytest.raises(ValueError):
+            ctx.locham(lr, pot, p, h, ek, ek.clone())
+    with pytest.raises(ValueError):
+        ctx.locham(lr, pot[:10], psi, psi.clone(), ek, ek.clone())
+    lr.close()
+
+
 def test_free_no_fine_points(ctx):
     lg = np.ones((9, 8, 7), dtype=bool)
     reg = region_from_grids("F", (8, 7, 6), (0.4,) * 3, lg, np.zeros_like(lg))
@@ -364,20 +450,47 @@ def test_largest_free_box(ctx, precision, n):
         assert eh < 1e-4 and ee < 1e-6
 
 
-@pytest.mark.parametrize("n,bwd", [((63, 63, 63), 0), ((59, 59, 59), 1)])
-def test_split_precision_refuses_free_boxes_beyond_its_lds(ctx, n, bwd):
-    """a free box whose backward pass fits neither unit's LDS (64 points per
-    axis), or only the FP32 one while the matrix cores are asked for: an
-    error with the LDS figure, not a launch"""
+def test_split_precision_refuses_a_forced_unit_beyond_its_lds(ctx):
+    """a free box whose matrix-core backward pass does not fit in LDS while
+    the option asks for the matrix cores: an error with the LDS figure, not
+    a launch (with the automatic units such a batch runs the FP32 passes,
+    test_large_boxes)"""
     import bigdft_hip
-    reg = dense_region("F", n, (0.45, 0.4, 0.5))
-    ctx.set_option("bwd", bwd)
+    reg = dense_region("F", (59, 59, 59), (0.45, 0.4, 0.5))
+    ctx.set_option("bwd", 1)
     try:
         with pytest.raises(bigdft_hip.BigDFTError, match="LDS") as e:
             run_gpu(ctx, reg, random_pot(reg, 1), random_psi(reg, 1, 2), precision=4)
         assert e.value.code == 5
     finally:
         ctx.set_option("bwd", 0)
+
+
+# Boxes beyond the line kernels' register-held outputs (N1 > 64, or more
+# than 192 fine points on x or z) and beyond the matrix-core kernels (more
+# than 64 coarse points: the split precisions run the FP32 passes there).
+# (37, 45, 64) is the 76 x 92 x 130 fine grid of the reference's kernel
+# check (bigdft/tests/libs/OpenCL/Makefile.am:28-31).
+LARGE = [("P", (37, 45, 64)), ("F", (63, 63, 63)), ("F", (70, 40, 90)),
+         ("P", (127, 127, 127))]
+
+
+@pytest.mark.parametrize("geo,n", LARGE, ids=[f"{g}{n[0]}x{n[1]}x{n[2]}" for g, n in LARGE])
+@pytest.mark.parametrize("precision", [0, 4], ids=["fp64", "f16x3"])
+def test_large_boxes(ctx, geo, n, precision):
+    reg = dense_region(geo, n, (0.45, 0.4, 0.5))
+    print(f"large box {geo} {n} fine {reg.nfine} precision {precision}", flush=True)
+    psi = random_psi(reg, 1, 4300 + sum(n))
+    pot = random_pot(reg, 4301 + sum(n))
+    hg, eg, pg = run_gpu(ctx, reg, pot, psi, precision=precision)
+    ho, eo, po = oracle.locham(geo, reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
+    eh = np.abs(hg - ho).max() / np.abs(ho).max()
+    ee = max(abs(eg[0] - eo[0]) / abs(eo[0]), abs(pg[0] - po[0]) / abs(po[0]))
+    print(f"  hpsi {eh:.2e} energies {ee:.2e}", flush=True)
+    if precision == 0:
+        assert rel_err(hg, ho) < TOL and ee < TOL
+    else:
+        assert eh < 1e-4 and ee < 1e-6
 
 
 # ---------------------------------------------------------------------------
@@ -581,3 +694,44 @@ def test_f16x3(ctx, ctx_lines, seq, geo, n, seed):
           flush=True)
     assert eh < BF_H
     assert ee < BF_E and ep_ < BF_E
+
+
+@pytest.mark.parametrize("prec", [0, 4], ids=["fp64", "f16x3"])
+def test_chunked_batches_match_one_chunk(ctx, prec):
+    """BDFT_OPT_CHUNK_MB: the batch in chunks of orbitals on one workspace
+    (one-shot calls and captured plans) gives the one-chunk result bit for
+    bit: every orbital runs the same kernels on the same data"""
+    import bigdft_hip
+    regs = [dense_region("F", (19, 23, 17), (0.4,) * 3), dense_region("P", (23, 19, 21),
+                                                                        (0.45, 0.4, 0.5)),
+            mixed_sphere_region((20, 18, 22), 0.4, 7)]
+    norb = 9
+    rs = [regs[i % 3] for i in range(norb)]
+    cache = {id(r): ctx.region(r) for r in regs}
+    lrs = [cache[id(r)] for r in rs]
+    psi = dev(np.concatenate([random_psi(r, 1, 300 + i)[0] for i, r in enumerate(rs)]))
+    pots = [random_pot(r, 400 + i) for i, r in enumerate(rs)]
+    pot = dev(np.concatenate(pots))
+    kw = dict(pot_off=np.concatenate([[0], np.cumsum([p.size for p in pots])[:-1]]),
+              potdim=np.array([r.nfine for r in rs]).ravel(), precision=prec)
+    outs = []
+    for mb, use_plan in ((0, False), (6, False), (6, True), (1, True)):
+        print(f"case chunk_mb={mb} plan={use_plan} precision={prec}", flush=True)
+        ctx.set_option("chunk_mb", mb)
+        h = torch.zeros_like(psi)
+        ek = torch.zeros(norb, dtype=torch.float64, device="cuda")
+        ep = torch.zeros_like(ek)
+        if use_plan:
+            plan = ctx.plan(lrs, pot, psi, h, ek, ep, **kw)
+            plan.execute()
+            plan.close()
+        else:
+            ctx.locham_batch(lrs, pot, psi, h, ek, ep, **kw)
+        ctx.sync()
+        outs.append((h, ek, ep))
+    ctx.set_option("chunk_mb", 0)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
+    for l in cache.values():
+        l.close()

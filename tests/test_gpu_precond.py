@@ -40,7 +40,9 @@ def dev(a):
 
 
 CASES = [("P", dense_region("P", (19, 17, 21), (0.45, 0.4, 0.5))),
-         ("F", mixed_sphere_region((24, 21, 26), 0.42, 13))]
+         ("F", mixed_sphere_region((24, 21, 26), 0.42, 13)),
+         # more than 64 coarse points on x: the x line pass in its BIG form
+         ("F_large", dense_region("F", (79, 20, 24), (0.4, 0.45, 0.5)))]
 
 
 @pytest.mark.parametrize("seq", ["plane", "lines"])

@@ -1,22 +1,26 @@
 """The full-size batched workloads through the C ABI.
 
-Uneven free boxes (2048 boxes, N_d ~ U{16..40}, own potential per box) and
+Uneven free boxes (2048 boxes, N_d ~ U{16..40}, own potential per box),
 compressed spheres (1024 orbitals of a 96^3 global box reading one global
-potential through ishift).  At these sizes the oracle checks a seeded sample
-of orbitals (FP64 within 1e-12; FP32 within 2e-6 max|hpsi| per element and
-1e-6 in the energies; split-BF16 within 1e-4 max|hpsi| per element); every
+potential through ishift), and the periodic 48^3 x 256 and 64^3 x 128
+batches.  At these sizes the oracle checks a seeded sample of orbitals
+(FP64 within 1e-12; FP32 within 2e-6 max|hpsi| per element and 1e-6 in the
+energies; split precisions within 1e-4 max|hpsi| per element); every
 orbital is checked through size-independent properties:
   * <psi | hpsi> = ekin + epot (hpsi starts at zero; FP64, 1e-10 relative),
-  * FP32 energies within 1e-6 relative of the FP64 ones,
-  * split-FP16 forward and split-BF16 (four products) energies within 1e-6
-    relative of max(|E|, 1e-12 |ekin|) per orbital.  On these workloads
-    some boxes lie far from every Gaussian of the potential: their epot
-    (1e-25..1e-41 Ha) comes from the far halo of the free box, where psir is
-    made of the smallest filter tails (fil2 down to 1.6e-9).  Two 11-bit
-    fp16 pieces cannot represent both ends of that range (5-bit exponent)
-    and two bf16 pieces carry 2^-17 per value, so the plain relative error
-    of such an epot is not meaningful for them; FP32 meets it everywhere.
-    The number of orbitals outside the plain relative bar is printed.
+  * FP32 and F16X3 energies within 1e-6 relative of the FP64 ones, per
+    orbital, with no floor.  On the free-BC workloads some boxes lie far from
+    every Gaussian of the potential: their epot (1e-25..1e-55 Ha) comes from
+    the outer halo of the box, where psir is made of the smallest filter
+    tails (fil2 down to 1.6e-9); F16X3 meets the bar there because its fp16
+    G tables carry a power-of-two scale per row and the x-halo lines keep
+    it into the y pass,
+  * split-BF16 (four products) energies within 1e-6 relative of
+    max(|E|, 1e-12 |ekin|) per orbital, epot 5e-5: two bf16 pieces hold a
+    value to 2^-17, which one halo point of such an epot shows directly.
+The number of orbitals outside the plain relative bar and the relative
+error of the weighted sums sum occup*E (hpsiortho.f90:946-954, occup 1 here)
+are printed.
 """
 import numpy as np
 import pytest
@@ -82,15 +86,19 @@ def run_workload(ctx, batch, sample, label):
 
     def n_plain(a, b):
         return int((((a - b).abs() / b.abs()) > 1e-6).sum())
+    def rel_sum(a, b):
+        return float(abs(a.sum() - b.sum()) / abs(b.sum()))
     f_k, f_p = rel(ek32, ek64), rel(ep32, ep64)
     print(f"{label}: identity {ident:.2e}, fp32 ekin {f_k:.2e} epot {f_p:.2e}", flush=True)
-    for name, ek, ep in (("bf16x4", ekbf, epbf), ("f16x3", ek16, ep16)):
-        print(f"{label}: {name} ekin {rel_scaled(ek, ek64):.2e} epot "
-              f"{rel_scaled(ep, ep64):.2e} (plain relative: max {rel(ep, ep64):.2e}, "
-              f"{n_plain(ep, ep64)} of {len(regs)} orbitals above 1e-6)", flush=True)
+    for name, ek, ep in (("fp32", ek32, ep32), ("bf16x4", ekbf, epbf), ("f16x3", ek16, ep16)):
+        print(f"{label}: {name} plain relative ekin {rel(ek, ek64):.2e} epot "
+              f"{rel(ep, ep64):.2e} ({n_plain(ep, ep64)} of {len(regs)} orbitals above "
+              f"1e-6), with the 1e-12 ekin floor ekin {rel_scaled(ek, ek64):.2e} epot "
+              f"{rel_scaled(ep, ep64):.2e}; weighted sums ekin {rel_sum(ek, ek64):.2e} "
+              f"epot {rel_sum(ep, ep64):.2e}", flush=True)
     assert ident < 1e-10
     assert f_k < 1e-6 and f_p < 1e-6
-    assert rel_scaled(ek16, ek64) < 1e-6 and rel_scaled(ep16, ep64) < 1e-6
+    assert rel(ek16, ek64) < 1e-6 and rel(ep16, ep64) < 1e-6
     assert rel_scaled(ekbf, ek64) < 1e-6 and rel_scaled(epbf, ep64) < 5e-5
     for b in sample:
         r = regs[b]
@@ -124,3 +132,83 @@ def test_compressed_spheres_1024(ctx):
     batch = workloads.sphere_batch(1024, 505, device="cuda")
     rng = np.random.default_rng(2)
     run_workload(ctx, batch, rng.choice(1024, 6, replace=False), "spheres")
+
+
+def run_dense(ctx, batch, sample, label, plan_replays=0):
+    """FP64 (optionally through a captured plan replayed several times)
+    against the oracle on a sample; every orbital's <psi|hpsi> = ekin + epot;
+    FP32, BF16X4 and F16X3 energies within 1e-6 relative of FP64 on every
+    orbital (no floor), max and RMS of their hpsi error printed"""
+    import bigdft_hip
+    reg = batch["regions"][0]
+    lr = ctx.region(reg)
+    norb = len(batch["regions"])
+    lrs = [lr] * norb
+    print(f"{label}: {norb} orbitals, {batch['units']} grid points", flush=True)
+    if plan_replays:
+        h64 = torch.zeros_like(batch["psi"])
+        ek64 = torch.zeros(norb, dtype=torch.float64, device="cuda")
+        ep64 = torch.zeros_like(ek64)
+        plan = ctx.plan(lrs, batch["pot"], batch["psi"], h64, ek64, ep64,
+                        precision=bigdft_hip.PREC_FP64)
+        plan.execute()
+        ctx.sync()
+        h1, e1, p1 = h64.clone(), ek64.clone(), ep64.clone()
+        for _ in range(plan_replays - 1):
+            plan.execute()
+        ctx.sync()
+        plan.close()
+        # every replay adds the same H psi and rewrites the same energies
+        assert torch.equal(ek64, e1) and torch.equal(ep64, p1)
+        if plan_replays == 2:   # x + x is exact
+            assert torch.equal(h64, 2 * h1)
+        h64 = h1
+    else:
+        h64, ek64, ep64 = apply(ctx, lrs, batch, bigdft_hip.PREC_FP64)
+    nc = reg.ncoef
+    dots = (batch["psi"].view(norb, nc) * h64.view(norb, nc)).sum(dim=1)
+    tot = ek64 + ep64
+    ident = float(((dots - tot).abs() / tot.abs()).max())
+    print(f"{label}: identity {ident:.2e}", flush=True)
+    assert ident < 1e-10
+    hmax = float(h64.abs().max())
+    hrms = float(torch.sqrt((h64 * h64).mean()))
+    for name in ("fp32", "bf16x4", "f16x3"):
+        h, ek, ep = apply(ctx, lrs, batch, bigdft_hip.PRECISIONS[name])
+        d = (h - h64).abs()
+        e_k = float(((ek - ek64).abs() / ek64.abs()).max())
+        e_p = float(((ep - ep64).abs() / ep64.abs()).max())
+        print(f"{label}: {name} hpsi max {float(d.max()) / hmax:.2e} rms "
+              f"{float(torch.sqrt((d * d).mean())) / hrms:.2e} of FP64, ekin {e_k:.2e} "
+              f"epot {e_p:.2e}", flush=True)
+        assert e_k < 1e-6 and e_p < 1e-6, (name, e_k, e_p)
+        assert float(d.max()) < 2e-5 * hmax
+        del h
+    pot = batch["pot"].cpu().numpy()
+    for b in sample:
+        psi = batch["psi"][b * nc:(b + 1) * nc].cpu().numpy()[None, :]
+        ho, eo, po = oracle.locham("P", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot,
+                                   psi)
+        g = h64[b * nc:(b + 1) * nc].cpu().numpy()
+        assert rel_err(g, ho[0]) < 1e-12
+        assert abs(float(ek64[b]) - eo[0]) < 1e-12 * abs(eo[0])
+        assert abs(float(ep64[b]) - po[0]) < 1e-12 * abs(po[0])
+    lr.close()
+
+
+def test_periodic48_256_orbitals(ctx):
+    """the batched periodic workload: 48^3 coarse (96^3 fine), 256 orbitals,
+    hgrid 0.35, 8-Gaussian potential; FP64 through a captured plan"""
+    from bigdft_hip import workloads
+    batch = workloads.dense_periodic(47, 256, 2024)
+    rng = np.random.default_rng(3)
+    run_dense(ctx, batch, rng.choice(256, 4, replace=False), "periodic48", plan_replays=2)
+
+
+def test_periodic64_128_orbitals(ctx):
+    """the precision workload: 64^3 coarse (128^3 fine), 128 orbitals, at the
+    largest periodic extent of the plane and z-line kernels"""
+    from bigdft_hip import workloads
+    batch = workloads.dense_periodic(63, 128, 2025)
+    rng = np.random.default_rng(4)
+    run_dense(ctx, batch, rng.choice(128, 4, replace=False), "periodic64")

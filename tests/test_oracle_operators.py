@@ -168,3 +168,33 @@ def test_free_matches_periodic_on_localised_orbital():
     np.testing.assert_allclose(hF, hP, rtol=0, atol=1e-12 * np.abs(hP).max())
     assert abs(eF[0] - eP[0]) < 1e-12 * abs(eP[0])
     assert abs(pF[0] - pP[0]) < 1e-12 * abs(pP[0])
+
+
+def test_free_fine_subbox_matches_full_fine_region():
+    """free BC with the fine region in a sub-box (nfl != 0): the oracle
+    places x_f(7, nfl1:nfu1, ...) by the region's fine bounds
+    (uncompress_forstandard, un-compress.f90:286-360; ConvolkineticT and
+    comb_grow/shrink bounds, convolut_ib_optim.f90:921-1063).  The same
+    orbital stored in a region whose fine grid is the whole box (nfl = 0)
+    is the same function, so H psi at the sphere's points is identical."""
+    n, h = (14, 12, 16), 0.4
+    sph = mixed_sphere_region(n, h, 9)
+    assert sph.nfl != (0, 0, 0) and sph.nfu != n
+    full = dense_region("F", n, (h, h, h))
+    rng = np.random.default_rng(10)
+    psi_s = rng.standard_normal(sph.ncoef)
+    # the same coefficients in the full region's compressed layout
+    psig = oracle.uncompress_psig(n, okeys(sph), psi_s)
+    psi_f = oracle.compress_acc_mixed(n, okeys(full), psig, np.zeros(full.ncoef))
+    rng2 = np.random.default_rng(11)
+    V = rng2.uniform(-2, 0, size=sph.n_isf)
+    hs, es, ps = oracle.locham("F", n, sph.nfl, sph.nfu, sph.hgrid, okeys(sph), V,
+                               psi_s[None, :])
+    hf, ef, pf = oracle.locham("F", n, full.nfl, full.nfu, full.hgrid, okeys(full), V,
+                               psi_f[None, :])
+    # hpsi of the full region at the sphere's points
+    hg = oracle.uncompress_psig(n, okeys(full), hf[0])
+    h_at = oracle.compress_acc_mixed(n, okeys(sph), hg, np.zeros(sph.ncoef))
+    np.testing.assert_allclose(hs[0], h_at, rtol=0, atol=1e-13 * np.abs(h_at).max())
+    assert abs(es[0] - ef[0]) < 1e-13 * abs(ef[0])
+    assert abs(ps[0] - pf[0]) < 1e-13 * abs(pf[0])
