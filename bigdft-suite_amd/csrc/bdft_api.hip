@@ -71,6 +71,11 @@ struct bdft_ctx {
   int opt_bwd = 0;           // matrix-core backward pass: 0 auto, 1 MFMA, 2 FP32
   int opt_chunk_mb = 0;      // workspace per chunk of orbitals (MiB), 0: one chunk
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
+  // regions and captured plans made on this context and not yet destroyed:
+  // destroying the context releases their device memory and detaches them
+  // (their own destroy calls then only free the host objects)
+  std::vector<bdft_lr*> regions;
+  std::vector<bdft_plan*> plans;
 };
 
 struct bdft_lr {
@@ -209,11 +214,23 @@ extern "C" int bdft_ctx_create(int device, bdft_ctx** out) {
 }
 
 extern "C" int bdft_plan_destroy(bdft_plan* plan);
+static void plan_release(bdft_plan* plan);
+static void lr_release(bdft_lr* lr);
 
 extern "C" int bdft_ctx_destroy(bdft_ctx* ctx) {
   if (!ctx) return BDFT_ERR_ARG;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  for (bdft_plan* p : ctx->plans) {
+    plan_release(p);
+    p->ctx = nullptr;
+  }
+  ctx->plans.clear();
+  for (bdft_lr* l : ctx->regions) {
+    lr_release(l);
+    l->ctx = nullptr;
+  }
+  ctx->regions.clear();
   for (auto& set : ctx->ev_pending)
     for (auto x : set) hipEventDestroy(x);
   for (auto x : ctx->ev_free) hipEventDestroy(x);
@@ -391,12 +408,24 @@ extern "C" int bdft_lr_create(bdft_ctx* ctx, char geocode, const int n[3],
     return rc;
   }
   *out = new bdft_lr(lr);
+  ctx->regions.push_back(*out);
   return BDFT_OK;
+}
+
+// device memory of a region (its coarse/fine maps and inverse maps)
+static void lr_release(bdft_lr* lr) {
+  if (lr->d_cmap) hipFree(lr->d_cmap);
+  if (lr->d_fmap) hipFree(lr->d_fmap);
+  if (lr->d_cinv) hipFree(lr->d_cinv);
+  if (lr->d_finv) hipFree(lr->d_finv);
+  lr->d_cmap = lr->d_fmap = lr->d_cinv = lr->d_finv = nullptr;
 }
 
 extern "C" int bdft_lr_destroy(bdft_lr* lr) {
   if (!lr) return BDFT_ERR_ARG;
   if (lr->ctx) {
+    auto& rs = lr->ctx->regions;
+    rs.erase(std::remove(rs.begin(), rs.end(), lr), rs.end());
     hipSetDevice(lr->ctx->device);
     hipStreamSynchronize(lr->ctx->stream);
     if (lr->ctx->cached) {  // a cached plan may reference this region's maps
@@ -414,11 +443,8 @@ extern "C" int bdft_lr_destroy(bdft_lr* lr) {
       lr->ctx->den_plan = nullptr;
       lr->ctx->den_key.clear();
     }
+    lr_release(lr);
   }
-  if (lr->d_cmap) hipFree(lr->d_cmap);
-  if (lr->d_fmap) hipFree(lr->d_fmap);
-  if (lr->d_cinv) hipFree(lr->d_cinv);
-  if (lr->d_finv) hipFree(lr->d_finv);
   delete lr;
   return BDFT_OK;
 }
@@ -1235,27 +1261,40 @@ extern "C" int bdft_plan_create(bdft_ctx* ctx, int norb, const bdft_lr* const* l
     *out = nullptr;
     return set_err(ctx, BDFT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   }
+  ctx->plans.push_back(plan);
   return BDFT_OK;
 }
 
 extern "C" int bdft_plan_execute(bdft_plan* plan) {
   if (!plan) return BDFT_ERR_ARG;
+  if (!plan->ctx) return BDFT_ERR_ARG;  // its context was destroyed
   return run_plan(plan);
 }
 
-extern "C" int bdft_plan_destroy(bdft_plan* plan) {
-  if (!plan) return BDFT_ERR_ARG;
+// device side of a plan: graph, parts, descriptors, tables, workspace
+static void plan_release(bdft_plan* plan) {
   if (plan->ctx) {
     hipSetDevice(plan->ctx->device);
     hipStreamSynchronize(plan->ctx->stream);
   }
   if (plan->exec) hipGraphExecDestroy(plan->exec);
   if (plan->graph) hipGraphDestroy(plan->graph);
+  plan->exec = nullptr;
+  plan->graph = nullptr;
   for (bdft_plan* q : plan->parts) bdft_plan_destroy(q);
   plan->parts.clear();
   dev_free(plan->meta);
   dev_free(plan->tabbuf);
   if (plan->owns_ws) dev_free(plan->ws);
+}
+
+extern "C" int bdft_plan_destroy(bdft_plan* plan) {
+  if (!plan) return BDFT_ERR_ARG;
+  if (plan->ctx) {
+    auto& ps = plan->ctx->plans;
+    ps.erase(std::remove(ps.begin(), ps.end(), plan), ps.end());
+  }
+  plan_release(plan);
   delete plan;
   return BDFT_OK;
 }

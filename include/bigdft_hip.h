@@ -86,7 +86,10 @@ enum {
  * (bigdft/src/interface_ocl.f90:20-41: ocl_create_context,
  * ocl_build_programs, ocl_create_command_queue_id). */
 int bdft_ctx_create(int device, bdft_ctx **ctx);
-/* Replaces free_gpu_OCL (interface_ocl.f90:196) for the context part. */
+/* Replaces free_gpu_OCL (interface_ocl.f90:196) for the context part.
+ * Regions and captured plans still alive are released on the device and
+ * detached: destroying them afterwards frees their host objects only, and
+ * executing a detached plan is BDFT_ERR_ARG. */
 int bdft_ctx_destroy(bdft_ctx *ctx);
 /* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the
  * context's own stream.  Work queued later on the new stream waits for the
@@ -100,8 +103,9 @@ const char *bdft_last_error(const bdft_ctx *ctx);
  * validated and uploaded once.  Replaces allocate_data_OCL
  * (interface_ocl.f90:43-193).  geocode 'P' (periodic) or 'F' (free); n[d] is
  * the reference's n_d (coarse points n_d+1; the Hamiltonian
- * entry points take at most 64 per axis, 60 for free boxes in the split
- * precisions, the transfers at most 4096); nfl/nfu
+ * entry points take at most 256 per axis, the split precisions running
+ * their FP32 vector passes on boxes beyond 64; the transfers at most
+ * 4096); nfl/nfu
  * bound the fine region (ignored for 'P'); hgrid is the coarse spacing. */
 int bdft_lr_create(bdft_ctx *ctx, char geocode, const int n[3],
                    const int nfl[3], const int nfu[3], const double hgrid[3],

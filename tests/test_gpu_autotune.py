@@ -60,7 +60,7 @@ def test_every_option_set_matches_the_oracle(ctx, name, reg):
             print(f"  {name} precision {prec} {opts}: hpsi {err:.2e} energies {ee:.2e}",
                   flush=True)
             assert err < bar and ee < ebar
-    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0})
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0})
     lr.close()
 
 
@@ -76,8 +76,9 @@ def test_tune_chooses_caches_and_keeps_results(ctx, tmp_path):
     cache = autotune.TuneCache(str(tmp_path / "tune.json"))
     key, opts, times = autotune.tune(ctx, [lr] * norb, pot, psi, prec, cache=cache)
     print(f"  {key}: {opts} {times}", flush=True)
-    assert key == "P:31x31x31:f16x3" and len(times) == 14
-    assert opts in autotune.candidates(prec)
+    assert key.endswith("|P:31x31x31:f16x3")
+    assert len(times) == len(autotune.candidates(prec)) + len(autotune.CHUNK_MB)
+    assert dict(opts, chunk_mb=0) in autotune.candidates(prec)
     assert (tmp_path / "tune.json").exists()
     h1, ek1, ep1 = run(ctx, lr, norb, pot, psi, prec)
     assert np.abs(h1 - h0).max() < 2e-6 * np.abs(h0).max()
@@ -86,7 +87,7 @@ def test_tune_chooses_caches_and_keeps_results(ctx, tmp_path):
     _, opts2, times2 = autotune.tune(ctx, [lr] * norb, pot, psi, prec,
                                      cache=autotune.TuneCache(str(tmp_path / "tune.json")))
     assert opts2 == opts and times2 is None
-    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0})
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0})
     with pytest.raises(bigdft_hip.BigDFTError):
         ctx.set_option("zgroup", 5)
     lr.close()
@@ -112,5 +113,5 @@ def test_tune_on_a_box_that_only_fits_some_options(ctx, tmp_path):
     ho, eko, epo = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
     assert np.abs(h - ho.ravel()).max() < 1e-4 * np.abs(ho).max()
     assert abs(ek[0] - eko[0]) < 1e-6 * abs(eko[0]) and abs(ep[0] - epo[0]) < 1e-6 * abs(epo[0])
-    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0})
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0})
     lr.close()

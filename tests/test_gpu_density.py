@@ -93,7 +93,10 @@ def test_density_argument_errors(ctx):
         ctx.density(lr, psi, [1.0, 1.0], rho, spin=[0, 2], nspin=2)
     with pytest.raises(bigdft_hip.BigDFTError, match="UNSUPPORTED"):
         ctx.density(lr, psi, [1.0, 1.0], rho, precision=bigdft_hip.PREC_BF16X4)
+    rho3 = torch.zeros((3, reg.n_isf), dtype=torch.float64, device="cuda")
     with pytest.raises(bigdft_hip.BigDFTError, match="ARG"):
+        ctx.density(lr, psi, [1.0, 1.0], rho3, nspin=3)
+    with pytest.raises(ValueError, match="rho"):   # too short for nspin = 3
         ctx.density(lr, psi, [1.0, 1.0], rho, nspin=3)
     lr.close()
 

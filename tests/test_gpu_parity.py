@@ -409,8 +409,10 @@ def test_descriptor_errors_are_reported(ctx):
     with pytest.raises(bigdft_hip.BigDFTError) as e:
         ctx.region(reg)
     assert e.value.code == 3
-    big = dense_region("P", (70, 3, 3), (0.4,) * 3)
-    lr = ctx.region(big)   # transfers accept it, the Hamiltonian does not
+    # 257 coarse points along x: transfers accept it, the Hamiltonian
+    # (at most 256 per axis) does not
+    big = dense_region("P", (256, 3, 3), (0.4,) * 3)
+    lr = ctx.region(big)
     with pytest.raises(bigdft_hip.BigDFTError) as e:
         run = torch.zeros(big.ncoef, dtype=torch.float64, device="cuda")
         ek = torch.zeros(1, dtype=torch.float64, device="cuda")
