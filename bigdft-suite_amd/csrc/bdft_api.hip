@@ -678,6 +678,18 @@ static int box_tiles(const BoxDesc& b, int pass, int zg) {
 
 extern "C" int bdft_plan_destroy(bdft_plan* plan);
 
+// Launch choices that depend on every box of a batch (the largest box's
+// needs): working precision, z-line bundling, kernel sequence and units.  A
+// chunked plan takes them from the whole batch, so each part runs the
+// kernels the one-chunk plan would run on its orbitals.
+struct PlanChoice {
+  int precision;
+  int zg;
+  int fused, zvalu, bvalu;  // matrix-core sequence
+  int plane;                // vector-unit sequence: plane-fused or line passes
+  int xbig, zbig;
+};
+
 // point the plan's grids at their places in a workspace of ws_need bytes
 static void bind_ws(bdft_plan* plan, char* w) {
   plan->B.W = w + plan->ws_at[0];
@@ -693,7 +705,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      double* hpsi, double* ekin, double* epot, int precision,
                      bool own_ws, bdft_plan** out, bool density = false,
                      const bdft_confpot* conf = nullptr, bool grow_plane = false,
-                     Arena* arena = nullptr, bool defer_ws = false) {
+                     Arena* arena = nullptr, bool defer_ws = false,
+                     const PlanChoice* force = nullptr, PlanChoice* got = nullptr) {
   if (!ctx || !out) return BDFT_ERR_ARG;
   *out = nullptr;
   if (norb <= 0 || !lrs || !psi || (!density && (!pot || !hpsi || !ekin || !epot)))
@@ -704,7 +717,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // split precisions on a batch with a box beyond the matrix-core kernels
   // (more than MFMA_MAXN coarse points on an axis) run the FP32 vector
   // passes, which are at least as accurate
-  if (precision >= BDFT_PREC_BF16X3)
+  if (force)
+    precision = force->precision;
+  else if (precision >= BDFT_PREC_BF16X3)
     for (int i = 0; i < norb; ++i)
       for (int d = 0; d < 3; ++d)
         if (lrs[i] && lrs[i]->N[d] > MFMA_MAXN) precision = BDFT_PREC_FP32;
@@ -795,6 +810,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     if (ctx->opt_zg == 2 || (ctx->opt_zg == 0 && mx > ctx->plane_lds / 2) ||
         mx > ctx->plane_lds)
       zg = 2;
+    if (force) zg = force->zg;
   }
   // tile tables
   std::vector<int> starts;
@@ -876,6 +892,10 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     // (F16X3 always: its fp16 pieces are cut for the box scale, and the z
     // lines of a free box's x-y halo lie far below it)
     ml.zvalu = ml.fused && line_ok && (ctx->opt_zpot != 1 || f16);
+    if (force) {
+      ml.fused = force->fused;
+      ml.zvalu = force->zvalu;
+    }
     ml.maxN = maxN;
     size_t mx = std::max(ml.kin[2], ml.zpot);
     if (ml.fused)
@@ -890,7 +910,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       if (ctx->opt_bwd != 1 && ctx->opt_zpot != 1)
         return make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi,
                          ekin, epot, BDFT_PREC_FP32, own_ws, out, density, conf, grow_plane,
-                         arena, defer_ws);
+                         arena, defer_ws, nullptr, got);
       return set_err(ctx, BDFT_ERR_UNSUPPORTED,
                      "split-bf16 kernels need %zu bytes of LDS per work-group, the device "
                      "offers %zu",
@@ -911,7 +931,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     for (int i = 0; i < norb; ++i) all_free = all_free && !box[i].per;
     // FP32 backward pass: asked for, automatic on free-BC batches, or the
     // only unit whose planes fit
-    const bool bv = valu_bwd_ok && (ctx->opt_bwd == 2 || all_free || !mfma_bwd_ok);
+    const bool bv = force ? force->bvalu != 0
+                          : valu_bwd_ok && (ctx->opt_bwd == 2 || all_free || !mfma_bwd_ok);
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
       const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]);
@@ -972,6 +993,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     }
     if (std::max(lpf, lpb) > ctx->plane_lds) plane = false;
     T.plane = plane ? 1 : 0;
+    if (force) T.plane = force->plane;
     T.lds_pf = lpf;
     T.lds_pb = lpb;
     // preconditioner plans (FP64): the x and y operator as one plane pass
@@ -982,6 +1004,21 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     T.pc_plane = density && !grow_plane && !ctx->force_lines && !mfma && esz == 8 &&
                  lk <= ctx->plane_lds;
     T.lds_pc = lk;
+  }
+  if (force) {
+    xbig = force->xbig;
+    zbig = force->zbig;
+  }
+  if (got) {
+    got->precision = precision;
+    got->zg = zg;
+    got->fused = ml.fused;
+    got->zvalu = ml.zvalu;
+    got->bvalu = ml.bvalu;
+    got->plane = T.plane;
+    got->xbig = xbig;
+    got->zbig = zbig;
+    return BDFT_OK;  // choices only: nothing allocated
   }
   bdft_plan* plan = new bdft_plan();
   plan->ctx = ctx;
@@ -1138,6 +1175,13 @@ static int make_plan_chunked(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     return make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
                      epot, precision, own_ws, out, false, conf);
   *out = nullptr;
+  // the whole batch's launch choices, imposed on every part
+  PlanChoice choice{};
+  bdft_plan* none = nullptr;
+  int rc0 = make_plan(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
+                      epot, precision, own_ws, &none, false, conf, false, nullptr, true,
+                      nullptr, &choice);
+  if (rc0) return rc0;
   std::vector<int64_t> so((size_t)norb), po((size_t)norb);
   long long run = 0;
   for (int i = 0; i < norb; ++i) {
@@ -1160,7 +1204,7 @@ static int make_plan_chunked(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                              potdim ? potdim + 3 * o0 : nullptr,
                              ishift ? ishift + 3 * o0 : nullptr, psi, so.data() + o0, hpsi,
                              ekin + o0, epot + o0, precision, true, &part, false,
-                             conf ? conf + o0 : nullptr, false, nullptr, true);
+                             conf ? conf + o0 : nullptr, false, nullptr, true, &choice);
     if (rc) {
       for (bdft_plan* q : parent->parts) bdft_plan_destroy(q);
       parent->parts.clear();
@@ -1369,6 +1413,7 @@ extern "C" int bdft_locham_conf_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* 
   return locham_batch(ctx, norb, lrs, pot, pot_off, potdim, ishift, psi, psi_off, hpsi, ekin,
                       epot, conf, precision);
 }
+
 
 // ---------------------------------------------------------------------------
 // density: local_partial_density_OCL (interface_ocl.f90:908-1040) /
