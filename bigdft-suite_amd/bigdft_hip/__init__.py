@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4, PREC_F16X3 = 0, 1, 2, 3, 4
 # launch options (BDFT_OPT_* of bigdft_hip.h)
 OPTIONS = {"line_passes": 1, "zgroup": 2, "zpot": 3, "bwd": 4, "chunk_mb": 5}
+# bytes of the fixed device array of bdft_locham_batch_fixed_d (BDFT_FIXED_BYTES)
+FIXED_BYTES = 128 << 20
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
               "bf16x4": PREC_BF16X4, "f16x3": PREC_F16X3}
 
@@ -41,7 +43,8 @@ EXPORTS = ("bdft_ctx_create", "bdft_ctx_destroy", "bdft_ctx_set_stream",
            "bdft_density", "bdft_psi_to_locreg_d", "bdft_locreg_to_psi_d",
            "bdft_bounds_layout", "bdft_locreg_bounds_d", "bdft_locreg_bounds",
            "bdft_locham_conf_batch_d", "bdft_precond_cg_d", "bdft_precondition",
-           "bdft_ctx_set_option", "bdft_ctx_get_option")
+           "bdft_ctx_set_option", "bdft_ctx_get_option",
+           "bdft_locham_batch_fixed_d")
 
 
 class BigDFTError(RuntimeError):
@@ -69,6 +72,7 @@ def load_library(path=LIB_PATH):
         "bdft_lr_info": [P, I3, ctypes.POINTER(ctypes.c_int64)],
         "bdft_locham_d": [P, P, I, I, P, P, P, P, P, P, I],
         "bdft_locham_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
+        "bdft_locham_batch_fixed_d": [P, I, P, P, P, P, P, P, P, P, P, P, I],
         "bdft_locham_conf_batch_d": [P, I, P, P, P, P, P, P, P, P, P, P, P, I],
         "bdft_density_d": [P, P, I, P, P, P, I, P, I],
         "bdft_density": [P, P, I, P, P, P, I, P, I],
@@ -383,6 +387,22 @@ class Context:
         self._check(lib().bdft_locham_conf_batch_d(
             self.handle, n, arr, dp[0], _ptr(po), _ptr(pd),
             _ptr(sh), dp[1], _ptr(so), dp[2], dp[3], dp[4], _ptr(c), precision))
+
+    @_stream_ordered
+    def locham_batch_fixed(self, lrs, pot, psi, hpsi, ekin, epot, pot_off=None,
+                           potdim=None, ishift=None, psi_off=None,
+                           precision=PREC_FP64):
+        """bdft_locham_batch_fixed_d: locham_batch without device
+        allocation (the library's fixed device array, FIXED_BYTES per
+        device); returns when the batch is complete"""
+        arr, po, pd, sh, so = self._batch_args(lrs, pot_off, potdim, ishift,
+                                               psi_off)
+        npsi, npot = _batch_sizes(lrs, po, pd, so)
+        n = len(lrs)
+        self._check(lib().bdft_locham_batch_fixed_d(
+            self.handle, n, arr, _dev(pot, "pot", npot), _ptr(po), _ptr(pd), _ptr(sh),
+            _dev(psi, "psi", npsi), _ptr(so), _dev(hpsi, "hpsi", npsi),
+            _dev(ekin, "ekin", n), _dev(epot, "epot", n), precision))
 
     @_stream_ordered
     def density(self, lr, psi, hfac, rho, spin=None, nspin=1, precision=PREC_FP64):

@@ -32,6 +32,7 @@ struct Arena {
   size_t size = 0, used = 0;
   hipStream_t stream = nullptr;
   std::vector<std::vector<unsigned char>>* keep = nullptr;
+  bool overflow = false;  // a request did not fit
 };
 
 struct bdft_ctx {
@@ -71,6 +72,10 @@ struct bdft_ctx {
   int opt_bwd = 0;           // matrix-core backward pass: 0 auto, 1 MFMA, 2 FP32
   int opt_chunk_mb = 0;      // workspace per chunk of orbitals (MiB), 0: one chunk
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
+  // device address of this device's fixed array (bdft_locham_batch_fixed_d),
+  // from hipGetSymbolAddress at creation; null with the reason in fixed_err
+  char* fixed = nullptr;
+  std::string fixed_err;
   // regions and captured plans made on this context and not yet destroyed:
   // destroying the context releases their device memory and detaches them
   // (their own destroy calls then only free the host objects)
@@ -166,9 +171,11 @@ static void dev_free(DevBuf& b) {
 static int plan_buf(bdft_ctx* ctx, Arena* a, DevBuf& b, size_t bytes) {
   if (!a) return dev_alloc(ctx, b, bytes);
   const size_t at = (a->used + 255) & ~(size_t)255;
-  if (at + bytes > a->size)
+  if (at + bytes > a->size) {
+    a->overflow = true;
     return set_err(ctx, BDFT_ERR_UNSUPPORTED,
                    "the batch needs more than the %zu bytes of the fixed device array", a->size);
+  }
   b.p = a->base + at;
   b.bytes = bytes;
   b.owned = false;
@@ -184,6 +191,18 @@ static hipError_t plan_h2d(Arena* a, void* dst, const void* src, size_t n) {
   a->keep->emplace_back((const unsigned char*)src, (const unsigned char*)src + n);
   return hipMemcpyAsync(dst, a->keep->back().data(), n, hipMemcpyHostToDevice, a->stream);
 }
+
+// ---------------------------------------------------------------------------
+// Fixed device array of the allocation-free entry point
+// (bdft_locham_batch_fixed_d): for each chunk of orbitals in turn it holds
+// the plan's descriptors (box table, tile starts, energy partials), the
+// matrix-core operator tables and the working grids W, K, T2.  Every device
+// the library runs on holds its own copy until the process ends.  The name
+// is a host-side stand-in: host code uses only the device address that
+// hipGetSymbolAddress returns (bdft_ctx::fixed, looked up per context with
+// its device current).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[BDFT_FIXED_BYTES];
 
 // ---------------------------------------------------------------------------
 // context
@@ -209,6 +228,15 @@ extern "C" int bdft_ctx_create(int device, bdft_ctx** out) {
   }
   const char* env = getenv("BDFT_LINE_PASSES");
   ctx->force_lines = env && env[0] == '1';
+  // this device's copy of the fixed array (its device is current here)
+  void* fx = nullptr;
+  const hipError_t fe = hipGetSymbolAddress(&fx, HIP_SYMBOL(scratch));
+  if (fe == hipSuccess && fx) {
+    ctx->fixed = (char*)fx;
+  } else {
+    ctx->fixed_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(fe);
+    (void)hipGetLastError();
+  }
   *out = ctx;
   return BDFT_OK;
 }
@@ -1414,6 +1442,107 @@ extern "C" int bdft_locham_conf_batch_d(bdft_ctx* ctx, int norb, const bdft_lr* 
                       epot, conf, precision);
 }
 
+// The batch of bdft_locham_batch_d with no device allocation: the orbitals
+// run in consecutive chunks, each chunk's plan (descriptors uploaded with
+// stream-ordered copies, operator tables, working grids) laid out in the
+// device's fixed array, one chunk after the other on the context's stream.
+// The host waits for a chunk's kernels before the next chunk's descriptors
+// are written over theirs.  Every buffer the kernels need zeroed is zeroed
+// by a kernel of the sequence itself.  Returns after the stream has
+// finished the batch (the host copies of the descriptors live until then).
+extern "C" int bdft_locham_batch_fixed_d(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
+                                         const double* pot, const int64_t* pot_off,
+                                         const int32_t* potdim, const int32_t* ishift,
+                                         const double* psi, const int64_t* psi_off,
+                                         double* hpsi, double* ekin, double* epot,
+                                         int precision) {
+  if (!ctx) return BDFT_ERR_ARG;
+  if (norb <= 0 || !lrs || !pot || !psi || !hpsi || !ekin || !epot)
+    return set_err(ctx, BDFT_ERR_ARG, "null pointer or norb <= 0");
+  if (!ctx->fixed)
+    return set_err(ctx, BDFT_ERR_HIP, "fixed device array unavailable (%s)",
+                   ctx->fixed_err.c_str());
+  if (precision < BDFT_PREC_FP64 || precision > BDFT_PREC_F16X3)
+    return set_err(ctx, BDFT_ERR_UNSUPPORTED, "precision %d not available", precision);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t cap = BDFT_FIXED_BYTES;
+  // working grids per orbital (W, K, T2, and T1 for line passes, T3 for
+  // long z lines), the bound a chunk is first cut to
+  const size_t esz = precision == BDFT_PREC_FP64 ? 8 : 4;
+  std::vector<size_t> need((size_t)norb);
+  std::vector<int64_t> so((size_t)norb), po((size_t)norb);
+  long long run = 0;
+  for (int i = 0; i < norb; ++i) {
+    const bdft_lr* lr = lrs[i];
+    if (!lr || lr->ctx != ctx)
+      return set_err(ctx, BDFT_ERR_ARG, "orbital %d: region missing or from another context", i);
+    const size_t W = 8ull * lr->N[0] * lr->N[1] * lr->N[2];
+    const size_t T2 = (size_t)lr->L[0] * lr->L[1] * 2 * lr->N[2];
+    const size_t T1 = (size_t)lr->L[0] * 4 * lr->N[1] * lr->N[2];
+    const size_t T3 = lr->L[2] > REG_L ? (size_t)lr->L[0] * lr->L[1] * lr->L[2] : 0;
+    need[i] = esz * (2 * W + T2 + T1 + T3);
+    if (need[i] > cap)
+      return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                     "orbital %d needs %zu bytes of working grids, the fixed device array "
+                     "holds %zu",
+                     i, need[i], cap);
+    so[i] = psi_off ? psi_off[i] : run;
+    run += (long long)lr->nvctr_c + 7LL * lr->nvctr_f;
+    po[i] = pot_off ? pot_off[i] : 0;
+  }
+  // the whole batch's launch choices, imposed on every chunk (the kernels
+  // bdft_locham_batch_d runs)
+  PlanChoice choice{};
+  bdft_plan* none = nullptr;
+  int rc = make_plan(ctx, norb, lrs, pot, po.data(), potdim, ishift, psi, so.data(), hpsi, ekin,
+                     epot, precision, true, &none, false, nullptr, false, nullptr, true, nullptr,
+                     &choice);
+  if (rc) return rc;
+  std::vector<std::vector<unsigned char>> keep;
+  for (int o0 = 0; o0 < norb && rc == BDFT_OK;) {
+    int o1 = o0 + 1;
+    size_t acc = need[o0];
+    while (o1 < norb && acc + need[o1] <= cap) acc += need[o1++];
+    if (o0 > 0) {
+      const hipError_t ew = hipStreamSynchronize(ctx->stream);
+      if (ew != hipSuccess) {
+        rc = set_err(ctx, BDFT_ERR_HIP, "hipStreamSynchronize failed: %s",
+                     hipGetErrorString(ew));
+        break;
+      }
+      keep.clear();
+    }
+    for (;;) {
+      Arena a;
+      a.base = ctx->fixed;
+      a.size = cap;
+      a.stream = ctx->stream;
+      a.keep = &keep;
+      bdft_plan* part = nullptr;
+      rc = make_plan(ctx, o1 - o0, lrs + o0, pot, po.data() + o0,
+                     potdim ? potdim + 3 * o0 : nullptr, ishift ? ishift + 3 * o0 : nullptr,
+                     psi, so.data() + o0, hpsi, ekin + o0, epot + o0, precision, true, &part,
+                     false, nullptr, false, &a, false, &choice);
+      if (rc == BDFT_ERR_UNSUPPORTED && a.overflow && o1 - o0 > 1) {
+        o1 = o0 + (o1 - o0) / 2;  // descriptors and tables did not fit beside the grids
+        rc = BDFT_OK;
+        continue;
+      }
+      if (rc) break;
+      const hipError_t e = launch_plan_seq(part, ctx->stream, nullptr, nullptr);
+      delete part;  // its buffers are slices of the fixed array
+      if (e != hipSuccess)
+        rc = set_err(ctx, BDFT_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+      break;
+    }
+    o0 = o1;
+  }
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (rc) return rc;
+  if (es != hipSuccess)
+    return set_err(ctx, BDFT_ERR_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(es));
+  return BDFT_OK;
+}
 
 // ---------------------------------------------------------------------------
 // density: local_partial_density_OCL (interface_ocl.f90:908-1040) /

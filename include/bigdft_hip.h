@@ -148,6 +148,26 @@ int bdft_locham_batch_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
                         const double *psi, const int64_t *psi_off,
                         double *hpsi, double *ekin, double *epot,
                         int precision);
+/* Bytes of the fixed device array of bdft_locham_batch_fixed_d (one copy
+ * per device the library runs on, held until the process ends). */
+#define BDFT_FIXED_BYTES (128ull << 20)
+/* bdft_locham_batch_d without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * kernels as bdft_locham_batch_d; the plan's descriptors, operator tables
+ * and working grids live in a fixed device array of BDFT_FIXED_BYTES per
+ * device, and the orbitals run in consecutive chunks that fit in it.  An
+ * orbital whose working grids alone exceed it (8*(2*W + T2) bytes in FP64,
+ * half in the other precisions, W = 8*N1*N2*N3 and T2 = n1i*n2i*2*N3) is
+ * BDFT_ERR_UNSUPPORTED.  The call returns when the batch is complete.  All
+ * calls on one device share the array: calls that overlap on one device
+ * (several host threads, or contexts on different streams) race. */
+int bdft_locham_batch_fixed_d(bdft_ctx *ctx, int norb, const bdft_lr *const *lrs,
+                              const double *pot, const int64_t *pot_off,
+                              const int32_t *potdim, const int32_t *ishift,
+                              const double *psi, const int64_t *psi_off,
+                              double *hpsi, double *ekin, double *epot,
+                              int precision);
 /* The same with a confining potential per orbital (conf[norb], host
  * array; the ket loop passes psi%confdata to psir_to_vpsi,
  * modules/orbital_basis.f90:830-831; the linear-scaling Hamiltonian of
