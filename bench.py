@@ -441,6 +441,9 @@ def parse():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the automatic launch options instead of timing the "
                          "candidates on the batch first (bigdft_hip.autotune)")
+    ap.add_argument("--options", default="",
+                    help="launch options set on the context instead of tuning, "
+                         "e.g. 'chunk_mb=128,zgroup=2' (names of bigdft_hip.OPTIONS)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (the "
                          "known-size kernel that scales profiler byte counters)")
@@ -670,7 +673,11 @@ def main():
         return
 
     tune_cache = None
-    if args.autotune:
+    if args.options:
+        for kv in args.options.split(","):
+            k, v = kv.split("=")
+            ctx.set_option(k.strip(), int(v))
+    elif args.autotune:
         import tempfile
 
         from bigdft_hip import autotune

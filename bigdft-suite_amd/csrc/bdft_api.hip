@@ -1080,6 +1080,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     if (e == hipSuccess)
       e = plan_h2d(arena, q + al2(nt) + al2(nf) + al2(nm), tabs.rscale.data(), ns);
     plan->B.trs = ns ? (const float*)(q + al2(nt) + al2(nf) + al2(nm)) : nullptr;
+    // F16X3: T2 lines carry their fp16 table scales into the FP32 z pass
+    plan->B.t2_scaled = f16 ? 1 : 0;
     if (e != hipSuccess) {
       dev_free(plan->tabbuf);
       delete plan;

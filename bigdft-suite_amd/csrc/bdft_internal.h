@@ -316,7 +316,21 @@ struct Batch {
   unsigned* bmax;       // per box: largest |psi| (float bits), fp16 scaling
   float* tmax;          // per uncompress tile: largest |psi|
   const float* trs;     // inverse row scales of the fp16 G tables (OpTab::soff)
+  // 1 (F16X3 plans): the forward plane pass leaves in T2 the z line at
+  // (p1, p2) times G16_OUT / (u_x(p1) u_y(p2)), u the row scales of the x
+  // and y G tables; the z potential pass removes that factor as it loads
+  // the line
+  int t2_scaled;
 };
+
+// the factor a z line (p1, p2) of box bd carries in T2 when t2_scaled
+__device__ __forceinline__ float t2_line_unscale(const Batch& B, const BoxDesc& bd, int p1,
+                                                 int p2) {
+  const OpTab tx = B.tabs[bd.opG[0]], ty = B.tabs[bd.opG[1]];
+  const float ux = (tx.soff >= 0 && p1 < tx.rows) ? B.trs[tx.soff + p1] : 1.0f;
+  const float uy = (ty.soff >= 0 && p2 < ty.rows) ? B.trs[ty.soff + p2] : 1.0f;
+  return ux * uy * (1.0f / G16_OUT);
+}
 
 // Passes of one application; each has its own tile table.
 // P_KZG, P_ZPG: z-line passes of the matrix-core sequence, zg (2 or 3)

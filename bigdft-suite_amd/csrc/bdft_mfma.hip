@@ -487,13 +487,12 @@ __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
       f32x16 acc = {};
       mfma_block<NT4>(B, ty, 32 * ob, ch + lb * 32 * XS2, cl + lb * 32 * XS2, XS2, acc);
       const int p1 = 32 * lb + (lane & 31);
-      // fp16 tables: remove the row scales of p1 (G_x) and p2 (G_y)
-      const float f = NT4 == 16 ? isc * grow_u(B, tx, p1) / G16_OUT : isc;
+      // fp16 tables: the line keeps the row scales of p1 (G_x) and p2 (G_y)
+      // and G16_OUT; the z potential pass removes them (Batch::t2_scaled)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int p2 = 32 * ob + drow(r, lane);
-        const float u2 = NT4 == 16 ? grow_u(B, ty, p2) : 1.0f;
-        if (p1 < L1 && p2 < L2) dst[p1 + (long long)L1 * p2] = acc[r] * f * u2;
+        if (p1 < L1 && p2 < L2) dst[p1 + (long long)L1 * p2] = acc[r] * isc;
       }
     }
   }
@@ -834,14 +833,14 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
         for (int j = 0; j < MBF; ++j) {
           if (j >= nb) break;
           const int p1 = 32 * (lb0 + j) + (lane & 31);
+          // fp16 tables: the line keeps the row scales of p1 (G_x) and p2
+          // (G_y) and G16_OUT; the z potential pass removes them
+          // (Batch::t2_scaled)
           if (p1 < L1) {
-            // fp16 tables: remove the row scales of p1 (G_x) and p2 (G_y)
-            const float f = NT4 == 16 ? isc * grow_u(B, tx, p1) / G16_OUT : isc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
               const int p2 = 32 * ob + drow(r, lane);
-              const float u2 = NT4 == 16 ? grow_u(B, ty, p2) : 1.0f;
-              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r] * f * u2;
+              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r] * isc;
             }
           }
         }
@@ -1400,7 +1399,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       sz = fk->s2;
     }
     mark(S_ZP, 0);
-    if (L.zvalu) {
+    if (L.zvalu || FM == 16) {
       // G_z, V, G_z^T in FP32 on the vector units (the pass where they beat
       // the split-precision matrix-core form); T2 is unscaled FP32 in both
       const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,

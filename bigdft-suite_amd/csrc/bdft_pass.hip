@@ -47,12 +47,15 @@ __device__ __forceinline__ int wrap(int i, int n) {
 // CONTIG: lines are contiguous in memory (x pass), else positions are.
 // Loads are issued in batches of UNR per thread before their LDS stores, so
 // that many global loads are in flight at once.
+// SCALE (positions along lines only): every value is multiplied by mul, a
+// factor of this thread's line (a thread always loads the same line l =
+// threadIdx.x mod TW there).
 constexpr int UNR = 8;
 
-template <bool CONTIG, typename T, typename S>
+template <bool CONTIG, typename T, typename S, bool SCALE = false>
 __device__ __forceinline__ void fill_wav(T* lds, const S* __restrict__ src,
                                          long long lstride, long long pstride,
-                                         int nl, int N, int per) {
+                                         int nl, int N, int per, T mul = (T)1) {
   const int NS = N + HL + HR;
   const int rows = 2 * NS;
   if (CONTIG) {
@@ -107,6 +110,7 @@ __device__ __forceinline__ void fill_wav(T* lds, const S* __restrict__ src,
         else ok = ok && k >= 0 && k < N;
         v[u] = ok ? (T)src[(long long)l * lstride + (long long)(sec * N + k) * pstride]
                   : (T)0;
+        if (SCALE) v[u] *= mul;
       }
 #pragma unroll
       for (int u = 0; u < UNR; ++u) {
@@ -575,7 +579,13 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   const int nl = min(TW, L1 - j1);
   const long long pstride = (long long)L1 * L2;
   T* line = (T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
-  fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
+  if (sizeof(T) == 4 && B.t2_scaled) {
+    // the fp16 forward pass left the line's table scales on it
+    const T m = (T)t2_line_unscale(B, bd, j1 + (threadIdx.x & (TW - 1)), j2);
+    fill_wav<false, T, T, true>(lds, line, 1, pstride, nl, N3, per, m);
+  } else {
+    fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
+  }
   __syncthreads();
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
@@ -585,8 +595,10 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
   const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
                     q2 < bd.potdim[1];
-  const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
-  const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  // lanes off the potential box read its first value (a valid address) and
+  // discard it, so every potential load below is unconditional
+  const double* pot = B.pot + bd.pot_off + (in12 ? q1 + (long long)bd.potdim[0] * q2 : 0);
+  const long long pz = in12 ? (long long)bd.potdim[0] * bd.potdim[1] : 0;
   const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
   T g[3][8];
   double ep = 0.0;
@@ -594,19 +606,26 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   for (int m = 0; m < 3; ++m) {
     const int p0 = 8 * (ty + m * TY);
     if (p0 < L3) {
+      // the chunk's potential values are loaded first, with no use between
+      // the loads, so they are in flight together while the grow runs
+      double v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int q3 = p0 + r - bd.ishift[2];
+        const int qc = min(max(q3, 0), bd.potdim[2] - 1);
+        v[r] = pot[pz * qc];
+      }
       const int r0 = grow_row0(p0, per);
       grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int q3 = p0 + r - bd.ishift[2];
-        double v = 0.0;
-        if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) {
-          v = pot[pz * q3];
-          if (bd.conf_on) v += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
-        }
+        const bool ok = in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2];
+        double vr = ok ? v[r] : 0.0;
+        if (bd.conf_on && ok) vr += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
         const double pr = (double)g[m][r];
-        ep = fma(v * pr, pr, ep);
-        g[m][r] = (T)(v * pr);
+        ep = fma(vr * pr, pr, ep);
+        g[m][r] = (T)(vr * pr);
       }
     }
   }
@@ -678,23 +697,30 @@ __global__ __launch_bounds__(NT) void k_zgrowpot(Batch B, TileTable tt, double* 
   const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
   const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
                     q2 < bd.potdim[1];
-  const double* pot = B.pot + bd.pot_off + q1 + (long long)bd.potdim[0] * q2;
-  const long long pz = (long long)bd.potdim[0] * bd.potdim[1];
+  // lanes off the potential box read its first value (a valid address) and
+  // discard it, so every potential load below is unconditional
+  const double* pot = B.pot + bd.pot_off + (in12 ? q1 + (long long)bd.potdim[0] * q2 : 0);
+  const long long pz = in12 ? (long long)bd.potdim[0] * bd.potdim[1] : 0;
   const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
   T* R = (T*)B.T3 + bd.t3_off + j1 + tx + (long long)L1 * j2;
   double ep = 0.0;
   for (int p0 = 8 * ty; p0 < L3; p0 += 8 * TY) {
+    // the chunk's potential values first, in flight while the grow runs
+    double pv[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int q3 = p0 + r - bd.ishift[2];
+      pv[r] = pot[pz * min(max(q3, 0), bd.potdim[2] - 1)];
+    }
     T g[8];
     const int r0 = grow_row0(p0, per);
     grow8(col + r0 * LDW, col + (NS + r0) * LDW, g);
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int q3 = p0 + r - bd.ishift[2];
-      double v = 0.0;
-      if (in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2]) {
-        v = pot[pz * q3];
-        if (bd.conf_on) v += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
-      }
+      const bool ok = in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2];
+      double v = ok ? pv[r] : 0.0;
+      if (bd.conf_on && ok) v += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
       const double pr = (double)g[r];
       ep = fma(v * pr, pr, ep);
       if (tx < nl && p0 + r < L3) R[pstride * (p0 + r)] = (T)(v * pr);
