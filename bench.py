@@ -444,6 +444,9 @@ def parse():
     ap.add_argument("--options", default="",
                     help="launch options set on the context instead of tuning, "
                          "e.g. 'chunk_mb=128,zgroup=2' (names of bigdft_hip.OPTIONS)")
+    ap.add_argument("--host-build", action="store_true",
+                    help="generate the synthetic batch on the host and copy it to the "
+                         "device (no generator kernels on the GPU; profiler counter runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (the "
                          "known-size kernel that scales profiler byte counters)")
@@ -465,7 +468,12 @@ def run_config(args, name, names, headline, ctx, stream, tune_cache, progress):
     cfg = CONFIGS[name]
     dev = "cuda"
     t_build = time.perf_counter()
-    batch = build_batch(cfg, args.seed, dev)
+    if args.host_build:
+        batch = build_batch(cfg, args.seed, "cpu")
+        for k in ("psi", "pot"):
+            batch[k] = batch[k].to(dev)
+    else:
+        batch = build_batch(cfg, args.seed, dev)
     regions, psi = batch["regions"], batch["psi"]
     norb, units = len(regions), batch["units"]
     progress(f"{name}: {norb} orbitals, {units} grid points, built in "

@@ -774,6 +774,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       b.ishift[d] = ishift ? ishift[3 * i + d] : 0;
       if (b.potdim[d] <= 0)
         return set_err(ctx, BDFT_ERR_ARG, "orbital %d: potential extent <= 0", i);
+      if (d == 2 && (long long)b.potdim[0] * b.potdim[1] * b.potdim[2] >= (1LL << 31))
+        return set_err(ctx, BDFT_ERR_UNSUPPORTED,
+                       "orbital %d: potential box of 2^31 or more points", i);
       maxN = std::max(maxN, lr->N[d]);
       if (lr->N[d] > MAXN)
         return set_err(ctx, BDFT_ERR_UNSUPPORTED,

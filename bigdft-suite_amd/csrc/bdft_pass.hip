@@ -577,28 +577,62 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   const int j2 = lt / nxt;
   const int j1 = (lt - j2 * nxt) * TW;
   const int nl = min(TW, L1 - j1);
-  const long long pstride = (long long)L1 * L2;
-  T* line = (T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
-  if (sizeof(T) == 4 && B.t2_scaled) {
-    // the fp16 forward pass left the line's table scales on it
-    const T m = (T)t2_line_unscale(B, bd, j1 + (threadIdx.x & (TW - 1)), j2);
-    fill_wav<false, T, T, true>(lds, line, 1, pstride, nl, N3, per, m);
-  } else {
-    fill_wav<false>(lds, line, 1, pstride, nl, N3, per);
-  }
-  __syncthreads();
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
+  // offsets inside one box's T2 (L1 L2 2N3 < 2^31 elements) and potential
+  // box (checked on the host) are 32-bit; the bases are uniform
+  const int pstride = L1 * L2;
+  T* line = (T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
+  {
+    // this thread's line tx, wavelet rows j = ty, ty + TY, ... (c: j < N3,
+    // d: j >= N3) into LDS rows HL + j and NS + HL + j - N3; the F16X3
+    // forward pass leaves its table scales on the line (Batch::t2_scaled)
+    const T mul = (sizeof(T) == 4 && B.t2_scaled)
+                      ? (T)t2_line_unscale(B, bd, j1 + tx, j2) : (T)1;
+    const int nz = 2 * N3;
+    for (int j0 = ty; j0 < nz; j0 += TY * UNR) {
+      T v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int j = j0 + u * TY;
+        v[u] = (j < nz && tx < nl) ? line[tx + pstride * j] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int j = j0 + u * TY;
+        if (j < nz) lds[(j < N3 ? HL + j : NS + HL + j - N3) * LDW + tx] = v[u] * mul;
+      }
+    }
+    // halo rows (HL before and HR after each section): zero for free
+    // boxes, the wrapped rows of the line for periodic ones
+    if (!per) {
+      for (int h = ty; h < 2 * (HL + HR); h += TY) {
+        const int sec = h >= HL + HR, r = h - sec * (HL + HR);
+        const int row = r < HL ? r : HL + N3 + (r - HL);
+        lds[(sec * NS + row) * LDW + tx] = (T)0;
+      }
+    } else {
+      __syncthreads();
+      for (int h = ty; h < 2 * (HL + HR); h += TY) {
+        const int sec = h >= HL + HR, r = h - sec * (HL + HR);
+        const int k = r < HL ? r - HL : N3 + (r - HL);   // coarse index of the row
+        lds[(sec * NS + k + HL) * LDW + tx] = lds[(sec * NS + wrap(k, N3) + HL) * LDW + tx];
+      }
+    }
+  }
+  __syncthreads();
   const T* col = lds + tx;
   // potential index of this line; outside the potential box V = 0, which
-  // reproduces the zeroing of psir there (potential.f90:874-947)
+  // reproduces the zeroing of psir there (potential.f90:874-947).  Lanes off
+  // the potential box read its first value (a valid address) and discard it,
+  // so every potential load is unconditional.
   const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
   const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
                     q2 < bd.potdim[1];
-  // lanes off the potential box read its first value (a valid address) and
-  // discard it, so every potential load below is unconditional
-  const double* pot = B.pot + bd.pot_off + (in12 ? q1 + (long long)bd.potdim[0] * q2 : 0);
-  const long long pz = in12 ? (long long)bd.potdim[0] * bd.potdim[1] : 0;
+  const double* potb = B.pot + bd.pot_off;
+  const int off12 = in12 ? q1 + bd.potdim[0] * q2 : 0;
+  const int pz = bd.potdim[0] * bd.potdim[1];
+  const int pd3 = bd.potdim[2];
   const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
   T g[3][8];
   double ep = 0.0;
@@ -608,19 +642,19 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
     if (p0 < L3) {
       // the chunk's potential values are loaded first, with no use between
       // the loads, so they are in flight together while the grow runs
+      const int q30 = p0 - bd.ishift[2];
       double v[8];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int q3 = p0 + r - bd.ishift[2];
-        const int qc = min(max(q3, 0), bd.potdim[2] - 1);
-        v[r] = pot[pz * qc];
-      }
+      for (int r = 0; r < 8; ++r)
+        v[r] = potb[off12 + pz * min(max(q30 + r, 0), pd3 - 1)];
+      // points r of the chunk inside the box and the potential box
+      const int rlo = max(0, -q30);
+      const int rhi = in12 ? min(L3 - p0, pd3 - q30) : 0;
       const int r0 = grow_row0(p0, per);
       grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        const int q3 = p0 + r - bd.ishift[2];
-        const bool ok = in12 && p0 + r < L3 && q3 >= 0 && q3 < bd.potdim[2];
+        const bool ok = r >= rlo && r < rhi;
         double vr = ok ? v[r] : 0.0;
         if (bd.conf_on && ok) vr += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
         const double pr = (double)g[m][r];
