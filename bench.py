@@ -86,22 +86,23 @@ def path_of(prec, plane_enabled, stages):
 
 def measured_traffic(precision, config, stage):
     """HBM bytes per apply of a stage from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated on a
-    device copy of known size), or None when there is no summary"""
+    summaries (the ```json blocks of profiles/*.md: FETCH_SIZE + WRITE_SIZE,
+    calibrated on a device copy of known size), or None when none covers
+    this precision and workload"""
     pdir = os.path.join(ROOT, "profiles")
     for f in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
-        if not f.startswith(f"rocprof_{precision}_") or not f.endswith(".md"):
+        if not f.endswith(".md"):
             continue
         text = open(os.path.join(pdir, f)).read()
-        if "```json" not in text:
-            continue
-        blob = text.split("```json", 1)[1].split("```", 1)[0]
-        try:
-            data = json.loads(blob)
-        except ValueError:
-            continue
-        if data.get("config") == config and data.get("precision") == precision:
-            return data["bytes_per_launch"].get(stage), f"profiles/{f}"
+        for blob in text.split("```json")[1:]:
+            try:
+                data = json.loads(blob.split("```", 1)[0])
+            except ValueError:
+                continue
+            if data.get("config") == config and data.get("precision") == precision:
+                v = data["bytes_per_launch"].get(stage)
+                if v is not None:
+                    return v, f"profiles/{f}"
     return None, None
 
 

@@ -71,6 +71,7 @@ struct bdft_ctx {
   int opt_zpot = 0;          // matrix-core z potential pass: 0 auto, 1 MFMA, 2 FP32
   int opt_bwd = 0;           // matrix-core backward pass: 0 auto, 1 MFMA, 2 FP32
   int opt_chunk_mb = 0;      // workspace per chunk of orbitals (MiB), 0: one chunk
+  int opt_order = 0;         // plane and (un)compress tiles: 0 dispatch order, 1 per XCD
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
   // device address of this device's fixed array (bdft_locham_batch_fixed_d),
   // from hipGetSymbolAddress at creation; null with the reason in fixed_err
@@ -1035,6 +1036,15 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     T.pc_plane = density && !grow_plane && !ctx->force_lines && !mfma && esz == 8 &&
                  lk <= ctx->plane_lds;
     T.lds_pc = lk;
+  }
+  // tile order: the z-line and line passes deal their tiles in contiguous
+  // ranges per XCD (neighbouring bundles share cache lines); the plane and
+  // (un)compress passes as BDFT_OPT_TILE_ORDER asks
+  for (int p = 0; p < P_COUNT; ++p) T.t[p].xcd = 1;
+  {
+    const int po = ctx->opt_order == 1 ? 1 : 0;
+    T.t[P_PF].xcd = T.t[P_PB].xcd = T.t[P_UNC].xcd = po;
+    T.pf2.xcd = T.pb2.xcd = po;
   }
   if (force) {
     xbig = force->xbig;
@@ -2200,6 +2210,14 @@ extern "C" int bdft_ctx_set_option(bdft_ctx* ctx, int option, int value) {
         ctx->opt_chunk_mb = value;
       }
       return BDFT_OK;
+    case BDFT_OPT_TILE_ORDER:
+      if (value != 0 && value != 1)
+        return set_err(ctx, BDFT_ERR_ARG, "tile order: 0 or 1 (got %d)", value);
+      if (ctx->opt_order != value) {
+        drop_cached_plans(ctx);
+        ctx->opt_order = value;
+      }
+      return BDFT_OK;
     case BDFT_OPT_ZGROUP:
       if (value != 0 && value != 2 && value != 3)
         return set_err(ctx, BDFT_ERR_ARG, "z-line bundles per work-group: 0, 2 or 3 (got %d)",
@@ -2221,6 +2239,7 @@ extern "C" int bdft_ctx_get_option(const bdft_ctx* ctx, int option, int* value) 
     case BDFT_OPT_ZPOT: *value = ctx->opt_zpot; return BDFT_OK;
     case BDFT_OPT_BWD: *value = ctx->opt_bwd; return BDFT_OK;
     case BDFT_OPT_CHUNK_MB: *value = ctx->opt_chunk_mb; return BDFT_OK;
+    case BDFT_OPT_TILE_ORDER: *value = ctx->opt_order; return BDFT_OK;
   }
   return BDFT_ERR_ARG;
 }

@@ -109,7 +109,24 @@ struct TileTable {
   int nbox;
   int uniform;          // >0: every box has exactly this many tiles
   const int* bmap;      // null, or the batch index of each of the nbox boxes
+  int xcd;              // 1: tiles dealt in contiguous ranges per XCD (xcd_tile)
 };
+
+// Tile of work-group b in a launch of n work-groups.  Work-groups are
+// dispatched to the 8 XCDs in turn (b mod 8), each XCD with its own L2; the
+// tiles are dealt so that every XCD gets one contiguous range of them, and
+// tiles that share cache lines (neighbouring line bundles of a plane, the
+// two planes that write one fine record of hpsi, the operator tables of a
+// box) meet in the same L2.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
+// tile of this work-group in a launch over table tt
+__device__ __forceinline__ int tile_of(const TileTable& tt) {
+  return tt.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+}
 
 __device__ __forceinline__ int find_box(const TileTable& tt, int tile) {
   if (tt.uniform > 0) return tile / tt.uniform;

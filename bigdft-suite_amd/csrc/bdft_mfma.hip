@@ -315,7 +315,7 @@ template <int AX, int NT4, bool TOTAL>
 __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
 template <int NT4>
 __global__ __launch_bounds__(NTMP) void k_mgrow_plane(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int j3 = tile - box_first_tile(tt, b);
@@ -505,7 +505,7 @@ template <int NT4, int BM>
 __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(NTM) void k_mzpot(Batch B, TileTable tt, double* pa
 template <int BM>
 __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int j3 = tile - box_first_tile(tt, b);
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
 template <int NT4>
 __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);
   const int b = table_box(tt, lb);
   const BoxDesc bd = B.box[b];
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
 template <int BM>
 __global__ __launch_bounds__(NTMP) void k_mbwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);
   const int b = table_box(tt, lb);
   const BoxDesc bd = B.box[b];
@@ -1053,7 +1053,7 @@ template <int NT4, int ZG>
 __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -1152,7 +1152,7 @@ template <int NT4, int BM, int ZG>
 __global__ __launch_bounds__(NTM, 2) void k_mzp(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTM / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);

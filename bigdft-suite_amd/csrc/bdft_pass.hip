@@ -317,7 +317,7 @@ __global__ __launch_bounds__(NT) void k_zero(T* __restrict__ p, long long n) {
 // uncompress (psi -> W, converted to T) or compress-accumulate (hpsi += W)
 template <typename T, bool COMPRESS>
 __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_xfwd(Batch B, TileTable tt, double* part
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
   __shared__ double red[NT / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -483,7 +483,7 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_yfwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
   __shared__ double red[NT / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
   __shared__ double red[NT / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(NT) void k_zgrowpot(Batch B, TileTable tt, double* 
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
   __shared__ double red[NT / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -768,7 +768,7 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_zshrink(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -805,7 +805,7 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_ybwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -844,7 +844,7 @@ template <typename T, bool BIG>
 __global__ __launch_bounds__(NT) void k_xbwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* lds = (T*)smem;
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -1203,7 +1203,7 @@ void extern_zero_float(float* p, long long n, int nblocks, hipStream_t s) {
 // tmax[tile]; k_box_max then reduces the tiles of each box into bmax[box]
 __global__ __launch_bounds__(NT) void k_uncompress_max(Batch B, TileTable tt, float* tmax) {
   __shared__ float red[NT / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -1319,7 +1319,7 @@ __device__ __forceinline__ PcRange pc_range(const BoxDesc& bd, int lt) {
 __global__ __launch_bounds__(NT) void k_pc_scatter(Batch B, TileTable tt, PcArgs A,
                                                    const double* src, int update) {
   __shared__ double st[7 * CPT];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
@@ -1357,7 +1357,7 @@ template <bool BIG>
 __global__ __launch_bounds__(NT) void k_pc_kinx(Batch B, TileTable tt, PcArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* lds = (double*)smem;
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -1426,7 +1426,7 @@ __global__ __launch_bounds__(NT) void k_pc_kinx(Batch B, TileTable tt, PcArgs A)
 __global__ __launch_bounds__(NT) void k_pc_gather(Batch B, TileTable tt, PcArgs A, int init) {
   __shared__ double red[NT / 64];
   __shared__ double st[7 * CPT];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
@@ -1465,7 +1465,7 @@ __global__ __launch_bounds__(NT) void k_pc_gather(Batch B, TileTable tt, PcArgs 
 // the tile's contiguous range)
 __global__ __launch_bounds__(NT) void k_pc_update(Batch B, TileTable tt, PcArgs A, int last) {
   __shared__ double red[NT / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));
@@ -1488,7 +1488,7 @@ __global__ __launch_bounds__(NT) void k_pc_update(Batch B, TileTable tt, PcArgs 
 
 // x <- S x (scale_psi, OpenCL/Uncompress.c:377-470)
 __global__ __launch_bounds__(NT) void k_pc_final(Batch B, TileTable tt, PcArgs A) {
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const PcRange R = pc_range(bd, tile - box_first_tile(tt, b));

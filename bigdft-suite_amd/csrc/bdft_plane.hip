@@ -203,7 +203,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTP / 64];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);
   const BoxDesc bd = B.box[table_box(tt, lb)];
   const int j3 = tile - box_first_tile(tt, lb);
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
                                                                 const double* pc) {
   constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);
   const int b = table_box(tt, lb);
   const BoxDesc bd = B.box[b];
@@ -419,7 +419,7 @@ template <typename T>
 __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable tt) {
   constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int tile = blockIdx.x;
+  const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);   // tables over a subset carry a box map
   const BoxDesc bd = B.box[table_box(tt, lb)];
   const int j3 = tile - box_first_tile(tt, lb);

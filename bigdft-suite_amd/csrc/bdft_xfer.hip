@@ -28,7 +28,7 @@ __device__ __forceinline__ int xfer_find(const XferArgs& A, int tile) {
 }
 
 __global__ __launch_bounds__(XT) void k_xfer(XferArgs A) {
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const XferDesc D = A.d[xfer_find(A, tile)];
   const int lt = tile - D.tile0;
   const int ctiles = (D.nvctr_c + XC - 1) / XC;

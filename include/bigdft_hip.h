@@ -348,6 +348,11 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
  *                         the grids a stage writes are read back from the
  *                         last-level cache (the density, preconditioner and
  *                         allocation-free calls are not chunked)
+ *   BDFT_OPT_TILE_ORDER   0: the plane and (un)compress passes take their
+ *                         tiles in dispatch order (default); 1: in one
+ *                         contiguous range per XCD, as the z-line and line
+ *                         passes always do (work-groups are dealt to the 8
+ *                         XCDs in turn, each with its own L2)
  * A change drops the context's cached plans; plans already created by
  * bdft_plan_create keep the options they were made with. */
 #define BDFT_OPT_LINE_PASSES 1
@@ -355,6 +360,7 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
 #define BDFT_OPT_ZPOT 3
 #define BDFT_OPT_BWD 4
 #define BDFT_OPT_CHUNK_MB 5
+#define BDFT_OPT_TILE_ORDER 6
 int bdft_ctx_set_option(bdft_ctx *ctx, int option, int value);
 int bdft_ctx_get_option(const bdft_ctx *ctx, int option, int *value);
 
