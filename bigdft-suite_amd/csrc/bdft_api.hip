@@ -1071,6 +1071,26 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   plan->owns_ws = own_ws;
   // metadata: boxes | starts | partials
   const size_t nb_box = sizeof(BoxDesc) * norb;
+  // tile -> box tables of the non-uniform passes (one load per work-group
+  // instead of a binary search over the box starts), up to 4M tiles each
+  int tb_off[P_COUNT];
+  int tb_g[2] = {-1, -1};  // the two plane-kernel box groups
+  {
+    constexpr int TB_MAX = 1 << 22;
+    auto tiles_of = [&](int so, int nb) {
+      const int off = (int)starts.size();
+      for (int i = 0; i < nb; ++i)
+        for (int t = starts[so + i]; t < starts[so + i + 1]; ++t) starts.push_back(i);
+      return off;
+    };
+    for (int p = 0; p < P_COUNT; ++p)
+      tb_off[p] = (start_off[p] >= 0 && T.ntiles[p] <= TB_MAX)
+                      ? tiles_of(start_off[p], norb) : -1;
+    if (g_off[0] >= 0) {
+      if (T.ntiles[P_PF] <= TB_MAX) tb_g[0] = tiles_of(g_off[0], T.t[P_PF].nbox);
+      if (T.ntiles_pf2 <= TB_MAX) tb_g[1] = tiles_of(g_off[2], T.pf2.nbox);
+    }
+  }
   const size_t nb_st = sizeof(int) * starts.size();
   int epass[4] = {P_XF, P_KY, P_KZ, P_ZP};
   if (T.plane) epass[0] = P_PF;
@@ -1130,14 +1150,19 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     delete plan;
     return set_err(ctx, BDFT_ERR_HIP, "descriptor upload failed");
   }
-  for (int p = 0; p < P_COUNT; ++p) T.t[p].start = start_off[p] >= 0 ? dst + start_off[p] : nullptr;
+  for (int p = 0; p < P_COUNT; ++p) {
+    T.t[p].start = start_off[p] >= 0 ? dst + start_off[p] : nullptr;
+    T.t[p].tbox = tb_off[p] >= 0 ? dst + tb_off[p] : nullptr;
+  }
   if (g_off[0] >= 0) {
     for (int p : {P_PF, P_PB}) {
       T.t[p].start = dst + g_off[0];
       T.t[p].bmap = dst + g_off[1];
+      T.t[p].tbox = tb_g[0] >= 0 ? dst + tb_g[0] : nullptr;
     }
     T.pf2.start = T.pb2.start = dst + g_off[2];
     T.pf2.bmap = T.pb2.bmap = dst + g_off[3];
+    T.pf2.tbox = T.pb2.tbox = tb_g[1] >= 0 ? dst + tb_g[1] : nullptr;
   }
   {
     double* q = dpart;

@@ -110,6 +110,7 @@ struct TileTable {
   int uniform;          // >0: every box has exactly this many tiles
   const int* bmap;      // null, or the batch index of each of the nbox boxes
   int xcd;              // 1: tiles dealt in contiguous ranges per XCD (xcd_tile)
+  const int* tbox;      // null, or the box (index into start) of every tile
 };
 
 // Tile of work-group b in a launch of n work-groups.  Work-groups are
@@ -130,6 +131,7 @@ __device__ __forceinline__ int tile_of(const TileTable& tt) {
 
 __device__ __forceinline__ int find_box(const TileTable& tt, int tile) {
   if (tt.uniform > 0) return tile / tt.uniform;
+  if (tt.tbox) return tt.tbox[tile];
   int lo = 0, hi = tt.nbox;  // invariant start[lo] <= tile < start[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
