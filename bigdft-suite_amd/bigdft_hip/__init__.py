@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libbigdft_hip.so")
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X4, PREC_F16X3 = 0, 1, 2, 3, 4
 # launch options (BDFT_OPT_* of bigdft_hip.h)
 OPTIONS = {"line_passes": 1, "zgroup": 2, "zpot": 3, "bwd": 4, "chunk_mb": 5,
-           "tile_order": 6}
+           "tile_order": 6, "persist": 7}
 # bytes of the fixed device array of bdft_locham_batch_fixed_d (BDFT_FIXED_BYTES)
 FIXED_BYTES = 128 << 20
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3,
@@ -312,7 +312,7 @@ class Context:
 
     def set_option(self, name, value):
         """launch option (OPTIONS: "line_passes" 0/1, "zgroup" 0/2/3, "zpot"
-        0/1/2, "bwd" 0/1/2, "chunk_mb" MiB, "tile_order" 0/1); cached plans of the context are dropped when a
+        0/1/2, "bwd" 0/1/2, "chunk_mb" MiB, "tile_order" 0/1, "persist" 0/1); cached plans of the context are dropped when a
         value changes"""
         self._check(lib().bdft_ctx_set_option(self.handle, OPTIONS[name], int(value)))
 

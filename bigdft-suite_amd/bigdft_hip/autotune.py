@@ -22,7 +22,8 @@ work-group, and the z potential pass and the backward plane pass each on the
 matrix cores or on the FP32 vector units; then, with the fastest of those,
 the orbitals in chunks of CHUNK_MB MiB of working grids
 (BDFT_OPT_CHUNK_MB), then the plane and (un)compress tiles in one contiguous
-range per XCD (BDFT_OPT_TILE_ORDER).  Timing runs
+range per XCD (BDFT_OPT_TILE_ORDER), then the z potential pass as a
+persistent launch over a work queue (BDFT_OPT_PERSIST).  Timing runs
 replay a captured plan into scratch outputs, so the caller's hpsi, ekin and
 epot are not touched.  A cached choice is checked on the batch before it is
 applied (a plan is made and dropped); one the batch does not fit
@@ -38,7 +39,7 @@ import os
 from . import PRECISIONS, BigDFTError
 
 # version of the option table: bumped when an option's meaning changes
-TUNE_VERSION = 4
+TUNE_VERSION = 5
 
 ERR_UNSUPPORTED = 5   # BDFT_ERR_UNSUPPORTED: the option set does not fit this batch
 
@@ -88,7 +89,7 @@ def candidates(precision):
     bundling and the z potential unit only matter in the plane-fused
     matrix-core sequence)"""
     auto = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0,
-            "tile_order": 0}
+            "tile_order": 0, "persist": 0}
     lines = dict(auto, line_passes=1)
     if PREC_NAMES.get(precision) not in MATRIX_CORE:
         return [auto, lines]
@@ -224,8 +225,10 @@ def tune(ctx, lrs, pot, psi, precision, cache=None, timer=time_plan, check=check
     base = best[1]
     for mb in CHUNK_MB:
         trial(dict(base, chunk_mb=mb))
-    # then the tile order of the plane and (un)compress passes
+    # then the tile order of the plane and (un)compress passes, then the
+    # persistent z potential pass
     trial(dict(best[1], tile_order=1))
+    trial(dict(best[1], persist=1))
     apply(ctx, best[1])
     cache.put(key, best[1], times)
     return key, best[1], times

@@ -72,6 +72,7 @@ struct bdft_ctx {
   int opt_bwd = 0;           // matrix-core backward pass: 0 auto, 1 MFMA, 2 FP32
   int opt_chunk_mb = 0;      // workspace per chunk of orbitals (MiB), 0: one chunk
   int opt_order = 0;         // plane and (un)compress tiles: 0 dispatch order, 1 per XCD
+  int opt_persist = 0;       // z potential pass: 0 one work-group per tile, 1 work queue
   size_t plane_lds = 0;      // LDS bytes the plane kernels may use
   // device address of this device's fixed array (bdft_locham_batch_fixed_d),
   // from hipGetSymbolAddress at creation; null with the reason in fixed_err
@@ -775,9 +776,9 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       b.ishift[d] = ishift ? ishift[3 * i + d] : 0;
       if (b.potdim[d] <= 0)
         return set_err(ctx, BDFT_ERR_ARG, "orbital %d: potential extent <= 0", i);
-      if (d == 2 && (long long)b.potdim[0] * b.potdim[1] * b.potdim[2] >= (1LL << 31))
+      if (d == 2 && (long long)b.potdim[0] * b.potdim[1] * b.potdim[2] >= (1LL << 29))
         return set_err(ctx, BDFT_ERR_UNSUPPORTED,
-                       "orbital %d: potential box of 2^31 or more points", i);
+                       "orbital %d: potential box of 2^29 or more points (4 GiB)", i);
       maxN = std::max(maxN, lr->N[d]);
       if (lr->N[d] > MAXN)
         return set_err(ctx, BDFT_ERR_UNSUPPORTED,
@@ -1091,6 +1092,22 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       if (T.ntiles_pf2 <= TB_MAX) tb_g[1] = tiles_of(g_off[2], T.pf2.nbox);
     }
   }
+  // work queue of the persistent z potential pass: the boxes in order of
+  // decreasing work (fine points), each box's tiles together
+  int zq_off = -1;
+  if (ctx->opt_persist == 1 && T.ntiles[P_ZP] > 0) {
+    std::vector<int> order((size_t)norb);
+    for (int i = 0; i < norb; ++i) order[i] = i;
+    auto work = [&](int i) { return (long long)box[i].L[0] * box[i].L[1] * box[i].L[2]; };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b2) { return work(a) > work(b2); });
+    zq_off = (int)starts.size();
+    for (int i : order) {
+      const int cnt = box_tiles(box[i], P_ZP, zg);
+      const int first = start_off[P_ZP] >= 0 ? starts[start_off[P_ZP] + i] : i * cnt;
+      for (int t = 0; t < cnt; ++t) starts.push_back(first + t);
+    }
+  }
   const size_t nb_st = sizeof(int) * starts.size();
   int epass[4] = {P_XF, P_KY, P_KZ, P_ZP};
   if (T.plane) epass[0] = P_PF;
@@ -1154,6 +1171,10 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     T.t[p].start = start_off[p] >= 0 ? dst + start_off[p] : nullptr;
     T.t[p].tbox = tb_off[p] >= 0 ? dst + tb_off[p] : nullptr;
   }
+  T.zq = zq_off >= 0 ? dst + zq_off : nullptr;
+  T.zq_grid = 0;
+  if (zq_off >= 0 && hipSetDevice(ctx->device) == hipSuccess)
+    T.zq_grid = std::min(zpot_queue_grid(esz == sizeof(float) ? 1 : 0, maxN), T.ntiles[P_ZP]);
   if (g_off[0] >= 0) {
     for (int p : {P_PF, P_PB}) {
       T.t[p].start = dst + g_off[0];
@@ -2243,6 +2264,14 @@ extern "C" int bdft_ctx_set_option(bdft_ctx* ctx, int option, int value) {
         ctx->opt_order = value;
       }
       return BDFT_OK;
+    case BDFT_OPT_PERSIST:
+      if (value != 0 && value != 1)
+        return set_err(ctx, BDFT_ERR_ARG, "persistent z pass: 0 or 1 (got %d)", value);
+      if (ctx->opt_persist != value) {
+        drop_cached_plans(ctx);
+        ctx->opt_persist = value;
+      }
+      return BDFT_OK;
     case BDFT_OPT_ZGROUP:
       if (value != 0 && value != 2 && value != 3)
         return set_err(ctx, BDFT_ERR_ARG, "z-line bundles per work-group: 0, 2 or 3 (got %d)",
@@ -2265,6 +2294,7 @@ extern "C" int bdft_ctx_get_option(const bdft_ctx* ctx, int option, int* value) 
     case BDFT_OPT_BWD: *value = ctx->opt_bwd; return BDFT_OK;
     case BDFT_OPT_CHUNK_MB: *value = ctx->opt_chunk_mb; return BDFT_OK;
     case BDFT_OPT_TILE_ORDER: *value = ctx->opt_order; return BDFT_OK;
+    case BDFT_OPT_PERSIST: *value = ctx->opt_persist; return BDFT_OK;
   }
   return BDFT_ERR_ARG;
 }

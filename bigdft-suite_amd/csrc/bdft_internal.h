@@ -375,6 +375,11 @@ struct PassTables {
   // (some N1 > REG_N or L1 > REG_L), z potential pass in two kernels through
   // T3 (some L3 > REG_L)
   int xbig, zbig;
+  // z potential pass as a persistent launch (BDFT_OPT_PERSIST): the P_ZP
+  // tiles in work-queue order and the grid of resident work-groups (0: one
+  // work-group per tile)
+  const int* zq;
+  int zq_grid;
 };
 
 // Launch stages in order; profiling records one event pair per stage.
@@ -486,7 +491,10 @@ size_t mfma_attributes(size_t lds);
 // the FP32 z-line potential pass (k_zpot<float>) over P_ZP tiles, in place
 // on T2, epot partials per tile
 hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
-                           double* part, hipStream_t s);
+                           double* part, const int* zq, int zq_grid, hipStream_t s);
+// resident work-groups of the persistent z potential pass on the current
+// device (a multiple of the 8 XCDs), for working type FP32 or FP64
+int zpot_queue_grid(int fp32, int maxN);
 // a second stream and two events: the z kinetic and z potential passes,
 // which read and write disjoint grids, run side by side between the forward
 // and the backward plane passes (fork after one, join before the other)

@@ -563,12 +563,10 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
 // ---------------------------------------------------------------------------
 // z pass with the potential: T2 <- G_z^T ( V . G_z T2 ), epot partial
 // ---------------------------------------------------------------------------
+// one tile (32 z lines of T2 at one y) of the z potential pass
 template <typename T>
-__global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* lds = (T*)smem;
-  __shared__ double red[NT / 64];
-  const int tile = tile_of(tt);
+__device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, double* part,
+                                          int tile, T* lds, double* red) {
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
@@ -606,10 +604,10 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
     // halo rows (HL before and HR after each section): zero for free
     // boxes, the wrapped rows of the line for periodic ones
     if (!per) {
-      for (int h = ty; h < 2 * (HL + HR); h += TY) {
-        const int sec = h >= HL + HR, r = h - sec * (HL + HR);
-        const int row = r < HL ? r : HL + N3 + (r - HL);
-        lds[(sec * NS + row) * LDW + tx] = (T)0;
+      for (int h = ty; h < HL + HR; h += TY) {
+        const int row = h < HL ? h : N3 + h;   // coarse index h - HL or N3 + h - HL
+        lds[row * LDW + tx] = (T)0;
+        lds[(NS + row) * LDW + tx] = (T)0;
       }
     } else {
       __syncthreads();
@@ -629,9 +627,11 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
   const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
   const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
                     q2 < bd.potdim[1];
-  const double* potb = B.pot + bd.pot_off;
-  const int off12 = in12 ? q1 + bd.potdim[0] * q2 : 0;
-  const int pz = bd.potdim[0] * bd.potdim[1];
+  // byte offsets from the box's potential base (uniform): the host keeps a
+  // potential box under 2^29 points, so they fit in 32 bits
+  const char* potb = (const char*)(B.pot + bd.pot_off);
+  const unsigned off12 = 8u * (unsigned)(in12 ? q1 + bd.potdim[0] * q2 : 0);
+  const unsigned pz = 8u * (unsigned)(bd.potdim[0] * bd.potdim[1]);
   const int pd3 = bd.potdim[2];
   const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
   T g[3][8];
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
       double v[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r)
-        v[r] = potb[off12 + pz * min(max(q30 + r, 0), pd3 - 1)];
+        v[r] = *(const double*)(potb + (off12 + pz * (unsigned)min(max(q30 + r, 0), pd3 - 1)));
       // points r of the chunk inside the box and the potential box
       const int rlo = max(0, -q30);
       const int rhi = in12 ? min(L3 - p0, pd3 - q30) : 0;
@@ -698,6 +698,34 @@ __global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part
           line[tx + pstride * (N3 + i0 + r)] = od[r];
         }
     }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NT / 64];
+  zpot_tile<T>(B, tt, part, tile_of(tt), (T*)smem, red);
+}
+
+// The same pass as a persistent launch over a work queue: a grid of
+// resident work-groups walks the queue of tiles (the boxes in order of
+// decreasing work, each box's tiles together), every XCD one contiguous
+// stretch of it, work-group i of an XCD its entries i, i + g, i + 2g, ...
+// (g work-groups on that XCD).  Each work-group stops at the end of its
+// stretch.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_zpot_q(Batch B, TileTable tt, double* part,
+                                                const int* __restrict__ queue, int n) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NT / 64];
+  const int G = gridDim.x, x = blockIdx.x & 7, i = blockIdx.x >> 3;
+  const int g = (G >> 3) + (x < (G & 7) ? 1 : 0);
+  const int q = n >> 3, r = n & 7;
+  const int lo = x * q + min(x, r), hi = lo + q + (x < r ? 1 : 0);
+  for (int k = lo + i; k < hi; k += g) {
+    zpot_tile<T>(B, tt, part, queue[k], (T*)smem, red);
+    __syncthreads();
   }
 }
 
@@ -1064,9 +1092,28 @@ static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs
 }
 
 hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
-                           double* part, hipStream_t s) {
-  if (ntiles > 0) k_zpot<float><<<ntiles, NT, lds_bytes(maxN, sizeof(float)), s>>>(B, t, part);
+                           double* part, const int* zq, int zq_grid, hipStream_t s) {
+  const size_t lds = lds_bytes(maxN, sizeof(float));
+  if (ntiles > 0 && zq && zq_grid > 0)
+    k_zpot_q<float><<<zq_grid, NT, lds, s>>>(B, t, part, zq, ntiles);
+  else if (ntiles > 0)
+    k_zpot<float><<<ntiles, NT, lds, s>>>(B, t, part);
   return hipGetLastError();
+}
+
+int zpot_queue_grid(int fp32, int maxN) {
+  const size_t lds = lds_bytes(maxN, fp32 ? sizeof(float) : sizeof(double));
+  int per_cu = 0, dev = 0, ncu = 0;
+  const hipError_t e =
+      fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zpot_q<float>, NT, lds)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zpot_q<double>, NT, lds);
+  if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      per_cu <= 0 || ncu <= 0) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return ((per_cu * ncu) >> 3) << 3;
 }
 
 hipError_t launch_density(const Batch& B, const PassTables& T, const DenArgs& D, int maxN,
@@ -1090,6 +1137,9 @@ static void zpot_pass(const Batch& B, const PassTables& PT, size_t lds, hipStrea
   if (PT.zbig) {
     k_zgrowpot<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3]);
     k_zshrink<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP]);
+  } else if (PT.zq && PT.zq_grid > 0) {
+    k_zpot_q<T><<<PT.zq_grid, NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3], PT.zq,
+                                            PT.ntiles[P_ZP]);
   } else {
     k_zpot<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3]);
   }

@@ -16,7 +16,7 @@ module module_locham_hip
   !! the autotuner
   integer, parameter, public :: BDFT_OPT_LINE_PASSES = 1, BDFT_OPT_ZGROUP = 2, &
        BDFT_OPT_ZPOT = 3, BDFT_OPT_BWD = 4, BDFT_OPT_CHUNK_MB = 5, &
-       BDFT_OPT_TILE_ORDER = 6
+       BDFT_OPT_TILE_ORDER = 6, BDFT_OPT_PERSIST = 7
 
   public :: init_acceleration_hip, allocate_data_hip, local_hamiltonian_hip, &
        local_partial_density_hip, free_gpu_hip, bdft_error_message, &

@@ -353,6 +353,11 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
  *                         contiguous range per XCD, as the z-line and line
  *                         passes always do (work-groups are dealt to the 8
  *                         XCDs in turn, each with its own L2)
+ *   BDFT_OPT_PERSIST      0: the z potential pass launches one work-group
+ *                         per tile (default); 1: a persistent launch of
+ *                         resident work-groups over a work queue of the
+ *                         tiles, boxes in order of decreasing size (each
+ *                         work-group ends at the end of its stretch)
  * A change drops the context's cached plans; plans already created by
  * bdft_plan_create keep the options they were made with. */
 #define BDFT_OPT_LINE_PASSES 1
@@ -361,6 +366,7 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
 #define BDFT_OPT_BWD 4
 #define BDFT_OPT_CHUNK_MB 5
 #define BDFT_OPT_TILE_ORDER 6
+#define BDFT_OPT_PERSIST 7
 int bdft_ctx_set_option(bdft_ctx *ctx, int option, int value);
 int bdft_ctx_get_option(const bdft_ctx *ctx, int option, int *value);
 

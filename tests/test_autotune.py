@@ -12,7 +12,7 @@ from bigdft_hip.descriptors import dense_region
 class FakeCtx:
     def __init__(self):
         self.opts = {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0,
-                     "tile_order": 0}
+                     "tile_order": 0, "persist": 0}
         self.history = []
 
     def set_option(self, k, v):
@@ -49,7 +49,7 @@ def test_candidates():
     assert len(c16) == 8 and {x["zpot"] for x in c16} == {0}
     c = autotune.candidates(bigdft_hip.PREC_BF16X4)
     assert len(c) == 14 and c[0] == {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0,
-                                     "chunk_mb": 0, "tile_order": 0}
+                                     "chunk_mb": 0, "tile_order": 0, "persist": 0}
     assert {x["zgroup"] for x in c} == {0, 2, 3} and {x["zpot"] for x in c} == {0, 1}
     assert {x["bwd"] for x in c} == {0, 1, 2}
     assert sum(x["line_passes"] for x in c) == 1
@@ -65,14 +65,14 @@ def test_tune_picks_the_fastest_and_caches(tmp_path):
         if o["line_passes"]:
             return 9.0
         return (5.0 - (o["zgroup"] == 2) + 0.5 * o["zpot"] - 0.25 * (o["bwd"] == 2)
-                - 0.125 * o["tile_order"])
+                - 0.125 * o["tile_order"] + 0.5 * o["persist"])
     ctx = FakeCtx()
     key, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_BF16X4,
                                      cache=autotune.TuneCache(str(path)), timer=timer)
     assert key == "P:23x23x23:bf16x4"
     assert opts == {"line_passes": 0, "zgroup": 2, "zpot": 0, "bwd": 2, "chunk_mb": 0,
-                    "tile_order": 1}
-    assert ctx.opts == opts and len(times) == 14 + len(autotune.CHUNK_MB) + 1
+                    "tile_order": 1, "persist": 0}
+    assert ctx.opts == opts and len(times) == 14 + len(autotune.CHUNK_MB) + 2
     table = json.loads(path.read_text())
     assert table[key]["options"] == opts and min(table[key]["ms"].values()) == 3.625
     # a second tuner reads the table and times nothing

@@ -60,7 +60,7 @@ def test_every_option_set_matches_the_oracle(ctx, name, reg):
             print(f"  {name} precision {prec} {opts}: hpsi {err:.2e} energies {ee:.2e}",
                   flush=True)
             assert err < bar and ee < ebar
-    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0, "tile_order": 0})
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0, "tile_order": 0, "persist": 0})
     lr.close()
 
 
@@ -77,8 +77,8 @@ def test_tune_chooses_caches_and_keeps_results(ctx, tmp_path):
     key, opts, times = autotune.tune(ctx, [lr] * norb, pot, psi, prec, cache=cache)
     print(f"  {key}: {opts} {times}", flush=True)
     assert key.endswith("|P:31x31x31:f16x3")
-    assert len(times) == len(autotune.candidates(prec)) + len(autotune.CHUNK_MB) + 1
-    assert dict(opts, chunk_mb=0, tile_order=0) in autotune.candidates(prec)
+    assert len(times) == len(autotune.candidates(prec)) + len(autotune.CHUNK_MB) + 2
+    assert dict(opts, chunk_mb=0, tile_order=0, persist=0) in autotune.candidates(prec)
     assert (tmp_path / "tune.json").exists()
     h1, ek1, ep1 = run(ctx, lr, norb, pot, psi, prec)
     assert np.abs(h1 - h0).max() < 2e-6 * np.abs(h0).max()
@@ -87,7 +87,7 @@ def test_tune_chooses_caches_and_keeps_results(ctx, tmp_path):
     _, opts2, times2 = autotune.tune(ctx, [lr] * norb, pot, psi, prec,
                                      cache=autotune.TuneCache(str(tmp_path / "tune.json")))
     assert opts2 == opts and times2 is None
-    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0, "tile_order": 0})
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0, "tile_order": 0, "persist": 0})
     with pytest.raises(bigdft_hip.BigDFTError):
         ctx.set_option("zgroup", 5)
     lr.close()
@@ -113,15 +113,16 @@ def test_tune_on_a_box_that_only_fits_some_options(ctx, tmp_path):
     ho, eko, epo = oracle.locham("F", reg.n, reg.nfl, reg.nfu, reg.hgrid, okeys(reg), pot, psi)
     assert np.abs(h - ho.ravel()).max() < 1e-4 * np.abs(ho).max()
     assert abs(ek[0] - eko[0]) < 1e-6 * abs(eko[0]) and abs(ep[0] - epo[0]) < 1e-6 * abs(epo[0])
-    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0, "tile_order": 0})
+    autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0, "tile_order": 0, "persist": 0})
     lr.close()
 
 
 @pytest.mark.parametrize("prec", [0, 1, 4], ids=["fp64", "fp32", "f16x3"])
 def test_tile_order_leaves_results_bit_equal(ctx, prec):
-    """BDFT_OPT_TILE_ORDER only changes which work-group takes which tile:
-    hpsi and the energies are bit-equal in both orders, on a batch mixing
-    periodic, free and sphere boxes (plane and line sequences)"""
+    """BDFT_OPT_TILE_ORDER and BDFT_OPT_PERSIST only change which
+    work-group takes which tile: hpsi and the energies are bit-equal in every
+    order, on a batch mixing periodic, free and sphere boxes (plane and line
+    sequences, the z potential pass per tile and as a persistent work queue)"""
     from bigdft_hip import autotune
     regs = [CASES[0][1], CASES[1][1], dense_region("F", (17, 21, 15), (0.4,) * 3)]
     rs = [regs[i % 3] for i in range(7)]
@@ -134,18 +135,20 @@ def test_tile_order_leaves_results_bit_equal(ctx, prec):
               potdim=np.array([r.nfine for r in rs]).ravel(), precision=prec)
     for lines in (0, 1):
         outs = []
-        for order in (0, 1):
-            print(f"  precision {prec} line_passes {lines} tile_order {order}", flush=True)
+        for order, persist in ((0, 0), (1, 0), (0, 1), (1, 1)):
+            print(f"  precision {prec} line_passes {lines} tile_order {order} persist {persist}",
+                  flush=True)
             autotune.apply(ctx, {"line_passes": lines, "zgroup": 0, "zpot": 0, "bwd": 0,
-                                 "chunk_mb": 0, "tile_order": order})
+                                 "chunk_mb": 0, "tile_order": order, "persist": persist})
             h = torch.zeros_like(psi)
             ek = torch.zeros(len(rs), dtype=torch.float64, device="cuda")
             ep = torch.zeros_like(ek)
             ctx.locham_batch(lrs, pot, psi, h, ek, ep, **kw)
             ctx.sync()
             outs.append((h, ek, ep))
-        for a, b in zip(*outs):
-            assert torch.equal(a, b)
+        for o in outs[1:]:
+            for a, b in zip(outs[0], o):
+                assert torch.equal(a, b)
     autotune.apply(ctx, {"line_passes": 0, "zgroup": 0, "zpot": 0, "bwd": 0, "chunk_mb": 0,
                          "tile_order": 0})
     for l in lrs_of.values():
