@@ -487,6 +487,7 @@ def run_config(args, name, names, headline, ctx, stream, tune_cache, progress):
     bargs = dict(pot_off=batch["pot_off"], potdim=batch["potdim"],
                  ishift=batch["ishift"], psi_off=batch["psi_off"])
     chosen = {}
+    tuned_ms = {}
 
     def timed(prec):
         """launch options (autotuned on this batch unless --no-autotune),
@@ -494,9 +495,12 @@ def run_config(args, name, names, headline, ctx, stream, tune_cache, progress):
         clean application for the error analysis"""
         if tune_cache is not None:
             from bigdft_hip import autotune
-            _, opts, _ = autotune.tune(ctx, lrs, batch["pot"], psi, prec, cache=tune_cache,
-                                       **bargs)
+            _, opts, ttimes = autotune.tune(ctx, lrs, batch["pot"], psi, prec,
+                                            cache=tune_cache, **bargs)
             chosen[prec] = opts
+            if ttimes:
+                tuned_ms[prec] = {k: (None if v is None else round(v, 4))
+                                  for k, v in ttimes.items()}
         h = torch.zeros_like(psi)
         ek = torch.zeros(norb, dtype=torch.float64, device=dev)
         ep = torch.zeros_like(ek)
@@ -575,7 +579,8 @@ def run_config(args, name, names, headline, ctx, stream, tune_cache, progress):
             roof["traffic_source"] = src + " (FETCH_SIZE+WRITE_SIZE per apply, copy-calibrated)"
         variants[vname] = {"value": units / (ms * 1e-3), "ms_per_step": ms,
                            "speedup_vs_fp64": ms64 / ms,
-                           "launch_options": chosen.get(prec), "wall_s": round(wall, 4),
+                           "launch_options": chosen.get(prec),
+                           "tuning_ms": tuned_ms.get(prec), "wall_s": round(wall, 4),
                            "path": vpath,
                            "stages_ms": {k: round(v, 4) for k, v in avg.items() if v > 0.02},
                            "roofline": roof, "error_vs_fp64": err}

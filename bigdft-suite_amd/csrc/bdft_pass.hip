@@ -634,6 +634,11 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const unsigned pz = 8u * (unsigned)(bd.potdim[0] * bd.potdim[1]);
   const int pd3 = bd.potdim[2];
   const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
+  // potential box = the box, no confinement, and (periodic) no chunk past
+  // the end of the line: no per-point selection (uniform over the box)
+  const bool vexact = bd.ishift[0] == 0 && bd.ishift[1] == 0 && bd.ishift[2] == 0 &&
+                      bd.potdim[0] == L1 && bd.potdim[1] == L2 && pd3 == L3 &&
+                      !bd.conf_on && (!per || (L3 & 7) == 0);
   T g[3][8];
   double ep = 0.0;
 #pragma unroll
@@ -647,19 +652,32 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         v[r] = *(const double*)(potb + (off12 + pz * (unsigned)min(max(q30 + r, 0), pd3 - 1)));
-      // points r of the chunk inside the box and the potential box
-      const int rlo = max(0, -q30);
-      const int rhi = in12 ? min(L3 - p0, pd3 - q30) : 0;
       const int r0 = grow_row0(p0, per);
       grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
+      if (vexact) {
+        // the potential box is the box: every point of a line in the box
+        // reads its own value, and the points without one (lines past the
+        // box edge, free-BC points past L3) have psir = 0, so the values
+        // they read (valid addresses) are multiplied by zero
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const bool ok = r >= rlo && r < rhi;
-        double vr = ok ? v[r] : 0.0;
-        if (bd.conf_on && ok) vr += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
-        const double pr = (double)g[m][r];
-        ep = fma(vr * pr, pr, ep);
-        g[m][r] = (T)(vr * pr);
+        for (int r = 0; r < 8; ++r) {
+          const double pr = (double)g[m][r];
+          ep = fma(v[r] * pr, pr, ep);
+          g[m][r] = (T)(v[r] * pr);
+        }
+      } else {
+        // points r of the chunk inside the box and the potential box
+        const int rlo = max(0, -q30);
+        const int rhi = in12 ? min(L3 - p0, pd3 - q30) : 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const bool ok = r >= rlo && r < rhi;
+          double vr = ok ? v[r] : 0.0;
+          if (bd.conf_on && ok) vr += conf_term(bd, cxy + conf_axis2(bd, 2, p0 + r));
+          const double pr = (double)g[m][r];
+          ep = fma(vr * pr, pr, ep);
+          g[m][r] = (T)(vr * pr);
+        }
       }
     }
   }
