@@ -955,6 +955,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // LDS get launches of their own (two work-groups per CU), the others a
   // second pair of launches
   int g_off[4] = {-1, -1, -1, -1};  // small starts, small map, large starts, large map
+  int vb_off[4][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // FP32 backward groups
   if (mfma && ml.fused) {
     const size_t half = ctx->plane_lds / 2;
     std::vector<int> sm, lg;
@@ -971,7 +972,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]);
       const size_t w = mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]);
       const size_t v = plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)sizeof(float));
-      if (std::max(f, bv ? v : w) <= half) {
+      if ((bv ? f : std::max(f, w)) <= half) {
         sm.push_back(i);
         fs = std::max(fs, f);
         bs = std::max(bs, w);
@@ -987,19 +988,44 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     ml.bvalu = bv;
     ml.vbwd = vs;
     ml.vbwd2 = vl;
-    if (!sm.empty() && !lg.empty()) {
-      auto add = [&](const std::vector<int>& g, int& so, int& mo) {
-        so = (int)starts.size();
-        int acc = 0;
-        for (int i : g) {
-          starts.push_back(acc);
-          acc += box_tiles(box[i], P_PF, zg);
-        }
+    auto add = [&](const std::vector<int>& g, int& so, int& mo) {
+      so = (int)starts.size();
+      int acc = 0;
+      for (int i : g) {
         starts.push_back(acc);
-        mo = (int)starts.size();
-        for (int i : g) starts.push_back(i);
-        return acc;
-      };
+        acc += box_tiles(box[i], P_PF, zg);
+      }
+      starts.push_back(acc);
+      mo = (int)starts.size();
+      for (int i : g) starts.push_back(i);
+      return acc;
+    };
+    // FP32 backward plane pass: the boxes grouped by how many of their
+    // work-groups fit a CU's LDS (1-4), one launch per group with the
+    // group's largest LDS, so the largest box does not set the LDS of every
+    // work-group
+    T.n_vbg = 0;
+    if (bv) {
+      std::vector<int> grp[4];
+      size_t gl[4] = {0, 0, 0, 0};
+      for (int i = 0; i < norb; ++i) {
+        const BoxDesc& b = box[i];
+        const size_t v = plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)sizeof(float));
+        const size_t fit = ctx->plane_lds / std::max(v, (size_t)1);
+        const int c = (int)std::min((size_t)4, std::max((size_t)1, fit));
+        grp[c - 1].push_back(i);
+        gl[c - 1] = std::max(gl[c - 1], v);
+      }
+      for (int c = 4; c >= 1; --c)
+        if (!grp[c - 1].empty()) {
+          const int k = T.n_vbg++;
+          T.nt_vbg[k] = add(grp[c - 1], vb_off[k][0], vb_off[k][1]);
+          T.vbg[k].nbox = (int)grp[c - 1].size();
+          T.vbg[k].uniform = 0;
+          ml.lds_vbg[k] = gl[c - 1];
+        }
+    }
+    if (!sm.empty() && !lg.empty()) {
       T.ntiles[P_PF] = T.ntiles[P_PB] = add(sm, g_off[0], g_off[1]);
       T.ntiles_pf2 = T.ntiles_pb2 = add(lg, g_off[2], g_off[3]);
       for (int p : {P_PF, P_PB}) {
@@ -1046,6 +1072,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     const int po = ctx->opt_order == 1 ? 1 : 0;
     T.t[P_PF].xcd = T.t[P_PB].xcd = T.t[P_UNC].xcd = po;
     T.pf2.xcd = T.pb2.xcd = po;
+    for (int k = 0; k < 4; ++k) T.vbg[k].xcd = po;
   }
   if (force) {
     xbig = force->xbig;
@@ -1076,6 +1103,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // instead of a binary search over the box starts), up to 4M tiles each
   int tb_off[P_COUNT];
   int tb_g[2] = {-1, -1};  // the two plane-kernel box groups
+  int tb_v[4] = {-1, -1, -1, -1};  // the FP32 backward groups
   {
     constexpr int TB_MAX = 1 << 22;
     auto tiles_of = [&](int so, int nb) {
@@ -1091,6 +1119,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       if (T.ntiles[P_PF] <= TB_MAX) tb_g[0] = tiles_of(g_off[0], T.t[P_PF].nbox);
       if (T.ntiles_pf2 <= TB_MAX) tb_g[1] = tiles_of(g_off[2], T.pf2.nbox);
     }
+    for (int k = 0; k < T.n_vbg; ++k)
+      tb_v[k] = T.nt_vbg[k] <= TB_MAX ? tiles_of(vb_off[k][0], T.vbg[k].nbox) : -1;
   }
   // work queue of the persistent z potential pass: the boxes in order of
   // decreasing work (fine points), each box's tiles together
@@ -1170,6 +1200,11 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   for (int p = 0; p < P_COUNT; ++p) {
     T.t[p].start = start_off[p] >= 0 ? dst + start_off[p] : nullptr;
     T.t[p].tbox = tb_off[p] >= 0 ? dst + tb_off[p] : nullptr;
+  }
+  for (int k = 0; k < T.n_vbg; ++k) {
+    T.vbg[k].start = dst + vb_off[k][0];
+    T.vbg[k].bmap = dst + vb_off[k][1];
+    T.vbg[k].tbox = tb_v[k] >= 0 ? dst + tb_v[k] : nullptr;
   }
   T.zq = zq_off >= 0 ? dst + zq_off : nullptr;
   T.zq_grid = 0;

@@ -368,6 +368,12 @@ struct PassTables {
   // launches of their own (t[P_PF], t[P_PB] then cover the other boxes)
   TileTable pf2, pb2;
   int ntiles_pf2, ntiles_pb2;
+  // FP32 backward plane pass of the matrix-core sequence: the boxes in up
+  // to four groups by how many of their work-groups fit a CU's LDS, one
+  // launch per group
+  TileTable vbg[4];
+  int nt_vbg[4];
+  int n_vbg;
   // preconditioner: x and y operator in one plane pass (FP64 planes that fit)
   int pc_plane;
   size_t lds_pc;
@@ -402,6 +408,7 @@ struct MfmaLds {
   int maxN;                           // largest coarse extent (line-kernel LDS)
   int bvalu;                          // 1: backward pass by the FP32 plane kernel
   size_t vbwd, vbwd2;                 // its LDS (box group of pb / pb2)
+  size_t lds_vbg[4];                  // its LDS per occupancy group (vbg)
 };
 
 // density: orbitals sharing one box are split into ngroups groups of gsize;

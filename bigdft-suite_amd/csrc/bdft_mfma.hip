@@ -1430,10 +1430,9 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       // G_y^T G_x^T + K and compress in FP32 on the vector units (the plane
       // kernel of the FP32 sequence; T2 and K are unscaled FP32 in both)
       hipError_t e = hipSuccess;
-      if (PT.ntiles[P_PB] > 0)
-        e = launch_plane_bwd<float>(B, PT.t[P_PB], PT.ntiles[P_PB], L.vbwd, s);
-      if (e == hipSuccess && PT.ntiles_pb2 > 0)
-        e = launch_plane_bwd<float>(B, PT.pb2, PT.ntiles_pb2, L.vbwd2, s);
+      for (int k = 0; k < PT.n_vbg && e == hipSuccess; ++k)
+        if (PT.nt_vbg[k] > 0)
+          e = launch_plane_bwd<float>(B, PT.vbg[k], PT.nt_vbg[k], L.lds_vbg[k], s);
       if (e != hipSuccess) return e;
     } else {
       if (PT.ntiles[P_PB] > 0)
