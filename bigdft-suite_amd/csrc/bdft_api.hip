@@ -814,12 +814,18 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     // no psi_off: orbitals stored one after the other
     b.psi_off = psi_off ? psi_off[i] : psi_run;
     psi_run += ncoef;
-    if (mfma)
+    if (mfma) {
       for (int d = 0; d < 3; ++d) {
         b.opG[d] = tabs.get(0, b.N[d], b.per, f16);
         b.opS[d] = tabs.get(1, b.N[d], b.per, false);
         b.opK[d] = tabs.get(2, b.N[d], b.per, f16);
       }
+      // forward plane kernel in two parts where that brings its LDS within
+      // half a CU's (two work-groups per CU)
+      const size_t half = ctx->plane_lds / 2;
+      b.mfp = (mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0], 1) > half &&
+               mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0], 2) <= half) ? 2 : 1;
+    }
     wtot += 8LL * b.N[0] * b.N[1] * b.N[2];
     t1tot += (long long)b.L[0] * 4 * b.N[1] * b.N[2];
     t2tot += (long long)b.L[0] * b.L[1] * 2 * b.N[2];
@@ -894,7 +900,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
       const OpTab* tb = tabs.tabs.data();
-      ml.fwd = std::max(ml.fwd, mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]));
+      ml.fwd = std::max(ml.fwd, mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0], b.mfp));
       ml.bwd = std::max(ml.bwd, mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]));
       ml.kz = std::max(ml.kz, mfma_lds_kz(2 * b.N[2], tb[b.opK[2]].Kp, zg));
       ml.zp = std::max(ml.zp, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp, zg));
@@ -969,7 +975,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                           : valu_bwd_ok && (ctx->opt_bwd == 2 || all_free || !mfma_bwd_ok);
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
-      const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0]);
+      const size_t f = mfma_lds_fwd(2 * b.N[0], 2 * b.N[1], b.L[0], b.mfp);
       const size_t w = mfma_lds_bwd(2 * b.N[0], b.L[0], b.L[1]);
       const size_t v = plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)sizeof(float));
       if ((bv ? f : std::max(f, w)) <= half) {

@@ -56,6 +56,9 @@ struct BoxDesc {
   const int* finv;      // grid point -> fine point index or -1
                         // (null: identity, every point present in key order)
   int opG[3], opS[3], opK[3];  // operator tables per axis (matrix-core path)
+  int mfp;              // matrix-core forward plane kernel: parts of the G_x
+                        // output held in LDS at a time (1 or 2)
+  int pad2_;
   // confining potential prefac * r2^conf_half added to V (conf_on = 0: none);
   // r2 = sum_d (conf_hh_d (p_d + 1 + conf_ioff_d) - conf_r_d)^2 at the fine
   // point p of the box (0-based)
@@ -490,7 +493,7 @@ size_t mfma_lds_kin(int Wd, int Kp);
 size_t mfma_lds_grow(int W2, int L1, int Kp1, int Kp2);
 size_t mfma_lds_zpot(int KpG, int KpS);
 size_t mfma_lds_shrink(int L1, int W1, int W2, int KpU, int KpB);
-size_t mfma_lds_fwd(int W1, int W2, int L1);
+size_t mfma_lds_fwd(int W1, int W2, int L1, int parts);
 size_t mfma_lds_kz(int W3, int Kp, int zg);
 size_t mfma_lds_zp(int KpG, int DpG, int zg);
 size_t mfma_lds_bwd(int W1, int L1, int L2);

@@ -719,7 +719,7 @@ __global__ __launch_bounds__(NTMP) void k_mshrink_plane(Batch B, TileTable tt) {
 // G_y as A (lanes p1 -> T2).
 // ---------------------------------------------------------------------------
 template <int NT4>
-__global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
+__global__ __launch_bounds__(NTMP, 4) void k_mfwd(Batch B, TileTable tt) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);
@@ -732,7 +732,13 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   const OpTab kx = B.tabs[bd.opK[0]], ky = B.tabs[bd.opK[1]];
   const int KX = rup(W1, 32), nR = rup(W2, 32);
   const int XR = KX + 8, XC = nR + 8;
-  const int nC = max(rup(L1, 32), KX);
+  // G_x output blocks of p1, produced and consumed in bd.mfp parts (two for
+  // large planes: the C buffer then holds half of them, which keeps two
+  // work-groups per CU)
+  const int nobG = tx.Dp / 32;
+  const int np = bd.mfp > 1 ? bd.mfp : 1;
+  const int hb = (nobG + np - 1) / np;
+  const int nC = 32 * hb;
   __bf16* rh = (__bf16*)smem;  // [j2 * XR + j1], j2 < nR, j1 < KX (zero padded)
   __bf16* rl = rh + nR * XR;
   __bf16* ch = rl + nR * XR;   // [p1 * XC + j2]
@@ -774,7 +780,6 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   constexpr int NW = NTMP / 64;
   const int nlb = nR / 32;          // line blocks j2
   const int nobK = KX / 32;         // output blocks j1 of the kinetic
-  const int nobG = tx.Dp / 32;      // output blocks p1 of G_x
   float* K = (float*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   const float ksx = (float)bd.kscale[0] * isc * kin_unscale<NT4>(),
               ksy = (float)bd.kscale[1] * isc * kin_unscale<NT4>();
@@ -787,8 +792,10 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   // Kinetic tile (j2 block lb, j1 block ob): x part from R with the table as
   // B operand, y part down R's rows with the table as A operand; both land
   // as rows j2, lanes j1, and K is stored once along x.
+  for (int h = 0; h < np; ++h) {
+  const int ob0 = h * hb, ob1 = min(nobG, ob0 + hb);
   {
-    const int nK = nlb * nobK, nT = nK + nobG * nlb;
+    const int nK = h == 0 ? nlb * nobK : 0, nT = nK + (ob1 - ob0) * nlb;
     for (int r = 0; r * NW < nT; ++r) {
       const int q = r * NW + ((r & 1) ? NW - 1 - wv : wv);
       if (q >= nT) continue;
@@ -806,11 +813,11 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
           }
         }
       } else {
-        const int g = q - nK, ob = g / nlb, lb = g - ob * nlb;
+        const int g = q - nK, obl = g / nlb, lb = g - obl * nlb, ob = ob0 + obl;
         f32x16 acc = {};
         mma<NT4, true>(B, tx, 32 * ob, rh + 32 * lb * XR, rl + 32 * lb * XR, XR, acc);
         if (NT4 == 16) acc *= G16_OUT;   // line p1 keeps its row scale s(p1)
-        store_cols<NT4>(acc, ch, cl, XC, 32 * ob + (lane & 31), 32 * lb);
+        store_cols<NT4>(acc, ch, cl, XC, 32 * obl + (lane & 31), 32 * lb);
       }
     }
   }
@@ -818,7 +825,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
   // G_y: lines p1, outputs p2 -> T2 plane, lanes along p1
   {
     float* dst = (float*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
-    const int nl1 = rup(L1, 32) / 32, nob = ty.Dp / 32;
+    const int nl1 = ob1 - ob0, nob = ty.Dp / 32;   // line blocks p1 of this part
     const int ch3 = task_chunks(nob, nl1, NW);
     for (int q = wv; q < nob * ch3; q += NW) {
       const Task tk = task_of(q, ch3, nl1);
@@ -832,7 +839,7 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
 #pragma unroll
         for (int j = 0; j < MBF; ++j) {
           if (j >= nb) break;
-          const int p1 = 32 * (lb0 + j) + (lane & 31);
+          const int p1 = 32 * (ob0 + lb0 + j) + (lane & 31);
           // fp16 tables: the line keeps the row scales of p1 (G_x) and p2
           // (G_y) and G16_OUT; the z potential pass removes them
           // (Batch::t2_scaled)
@@ -846,6 +853,8 @@ __global__ __launch_bounds__(NTMP) void k_mfwd(Batch B, TileTable tt) {
         }
       }
     }
+  }
+  if (h + 1 < np) __syncthreads();   // the next part overwrites C
   }
 }
 
@@ -1324,9 +1333,10 @@ size_t mfma_lds_shrink(int L1, int W1, int W2, int KpU, int KpB) {
   return (u > o ? u : o) + (size_t)2 * 2 * rup_h(W2, 32) * (KpB + 8);
 }
 
-size_t mfma_lds_fwd(int W1, int W2, int L1) {
+size_t mfma_lds_fwd(int W1, int W2, int L1, int parts) {
   const int nR = rup_h(W2, 32), KX = rup_h(W1, 32);
-  const int nC = rup_h(L1, 32) > KX ? rup_h(L1, 32) : KX;
+  const int nob = rup_h(L1, 32) / 32;
+  const int nC = 32 * ((nob + parts - 1) / parts);
   return (size_t)4 * nR * (KX + 8) + (size_t)4 * nC * (nR + 8);
 }
 size_t mfma_lds_kz(int W3, int Kp, int zg) {
