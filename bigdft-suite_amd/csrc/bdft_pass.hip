@@ -577,28 +577,36 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const int nl = min(TW, L1 - j1);
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
-  // offsets inside one box's T2 (L1 L2 2N3 < 2^31 elements) and potential
-  // box (checked on the host) are 32-bit; the bases are uniform
-  const int pstride = L1 * L2;
-  T* line = (T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2;
+  // byte offsets from the tile's first line (a uniform base): one box's T2
+  // holds L1 L2 2 N3 values, below 2^32 bytes for boxes of up to 256 coarse
+  // points per axis; lanes past the box (tx >= nl) address line 0
+  char* lineb = (char*)((T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2);
+  const unsigned ls = (unsigned)sizeof(T) * (unsigned)(L1 * L2);   // bytes per z row
+  const unsigned lo = (unsigned)sizeof(T) * (unsigned)(tx < nl ? tx : 0);
   {
-    // this thread's line tx, wavelet rows j = ty, ty + TY, ... (c: j < N3,
-    // d: j >= N3) into LDS rows HL + j and NS + HL + j - N3; the F16X3
-    // forward pass leaves its table scales on the line (Batch::t2_scaled)
+    // this thread's line tx, wavelet rows j = ty, ty + TY, ... of each
+    // section (c rows j, d rows N3 + j) into LDS rows HL + j and NS + HL + j;
+    // lanes past the box store zeros; the F16X3 forward pass leaves its
+    // table scales on the line (Batch::t2_scaled)
     const T mul = (sizeof(T) == 4 && B.t2_scaled)
                       ? (T)t2_line_unscale(B, bd, j1 + tx, j2) : (T)1;
-    const int nz = 2 * N3;
-    for (int j0 = ty; j0 < nz; j0 += TY * UNR) {
-      T v[UNR];
+    const T keep = tx < nl ? mul : (T)0;
 #pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        const int j = j0 + u * TY;
-        v[u] = (j < nz && tx < nl) ? line[tx + pstride * j] : (T)0;
-      }
+    for (int sec = 0; sec < 2; ++sec) {
+      const unsigned g0 = lo + ls * (unsigned)(sec * N3 + ty);
+      T* l0 = lds + (sec * NS + HL + ty) * LDW + tx;
+      for (int k0 = 0; ty + k0 * TY < N3; k0 += UNR) {
+        T v[UNR];
 #pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        const int j = j0 + u * TY;
-        if (j < nz) lds[(j < N3 ? HL + j : NS + HL + j - N3) * LDW + tx] = v[u] * mul;
+        for (int u = 0; u < UNR; ++u) {
+          const int k = k0 + u;
+          v[u] = ty + k * TY < N3 ? *(const T*)(lineb + (g0 + ls * (unsigned)(k * TY))) : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+          const int k = k0 + u;
+          if (ty + k * TY < N3) l0[k * TY * LDW] = v[u] * keep;
+        }
       }
     }
     // halo rows (HL before and HR after each section): zero for free
@@ -649,9 +657,15 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
       // the loads, so they are in flight together while the grow runs
       const int q30 = p0 - bd.ishift[2];
       double v[8];
+      if (vexact) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        v[r] = *(const double*)(potb + (off12 + pz * (unsigned)min(max(q30 + r, 0), pd3 - 1)));
+        for (int r = 0; r < 8; ++r)
+          v[r] = *(const double*)(potb + (off12 + pz * (unsigned)min(q30 + r, pd3 - 1)));
+      } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          v[r] = *(const double*)(potb + (off12 + pz * (unsigned)min(max(q30 + r, 0), pd3 - 1)));
+      }
       const int r0 = grow_row0(p0, per);
       grow8(col + r0 * LDW, col + (NS + r0) * LDW, g[m]);
       if (vexact) {
@@ -684,36 +698,43 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const double eps = block_sum(ep, red);
   if (threadIdx.x == 0) part[tile] = eps;
   __syncthreads();
-  // V psir as a real line bundle: row q holds fine index q-14
+  // V psir as a real line bundle: row q holds fine index q (free) or q - 14
+  // wrapped (periodic), rows 0 .. Q - 1.  Every chunk stores its eight rows:
+  // free-BC points past L3 hold V psir = 0 (psir = 0 there, and V = 0 off
+  // the exact case), and for periodic boxes the rows past the line are halo
+  // rows, written again below.
   const int Q = 2 * N3 + 36;
+  const int rofs = per ? 14 : 0;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const int p0 = 8 * (ty + m * TY);
+    if (p0 < L3) {
+      T* dst = lds + (p0 + rofs) * LDW + tx;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int p = p0 + r;
-      if (p0 < L3 && p < L3) {
-        if (per) {
-          for (int q = wrap(p + 14, 2 * N3); q < Q; q += 2 * N3)
-            lds[q * LDW + tx] = g[m][r];
-        } else {
-          lds[p * LDW + tx] = g[m][r];
-        }
-      }
+      for (int r = 0; r < 8; ++r) dst[r * LDW] = g[m][r];
     }
   }
-  if (!per)
-    for (int q = L3 + ty; q < Q; q += TY) lds[q * LDW + tx] = (T)0;
+  if (!per) {
+    for (int q = 8 * ((L3 + 7) >> 3) + ty; q < Q; q += TY) lds[q * LDW + tx] = (T)0;
+  } else {
+    __syncthreads();
+    // rows 0..13 and 2 N3 + 14 .. Q - 1 from the wrapped rows of the line
+    for (int h = ty; h < 36; h += TY) {
+      const int q = h < 14 ? h : 2 * N3 + h;
+      lds[q * LDW + tx] = lds[(wrap(q - 14, 2 * N3) + 14) * LDW + tx];
+    }
+  }
   __syncthreads();
   for (int i0 = 4 * ty; i0 < N3; i0 += 4 * TY) {
     T oc[4], od[4];
     shrink4(col + 2 * i0 * LDW, oc, od);
     if (tx < nl) {
+      char* dc = lineb + (lo + ls * (unsigned)i0);
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         if (i0 + r < N3) {
-          line[tx + pstride * (i0 + r)] = oc[r];
-          line[tx + pstride * (N3 + i0 + r)] = od[r];
+          *(T*)(dc + ls * (unsigned)r) = oc[r];
+          *(T*)(dc + ls * (unsigned)(N3 + r)) = od[r];
         }
     }
   }
