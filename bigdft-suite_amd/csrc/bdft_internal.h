@@ -155,64 +155,101 @@ __device__ __forceinline__ int box_first_tile(const TileTable& tt, int b) {
 // added at their compressed positions, compress_and_accumulate_mixed
 // (bigdft/src/wavelib/un-compress.f90:2174-2238).  O: the plane's operator
 // output [j2 * RO + j1]; Kp: its kinetic term [j1 + W1 * j2]; each sum is
-// formed in TS.  U points per thread are in flight at once: every load of a
-// step (indices, O, K, the old hpsi values) comes before its stores, so the
-// memory latencies of the U points overlap.  No two points share an hpsi
-// entry, and the two planes j3, j3 + N3 of a coarse plane write disjoint
-// components.
-template <typename TS, int U, int NT, typename TO, typename TK>
-__device__ __forceinline__ void compress_plane(const BoxDesc& bd, int j3, const TO* O, int RO,
-                                               const TK* Kp, double* hpsi) {
+// formed in TS.  The work goes in steps of U points per thread (points
+// i0 + u NT): compress_load issues every load of a step that does not need
+// O (the compressed indices, K, the old hpsi values), compress_store adds O
+// and stores, so the memory latencies of the U points overlap, and a kernel
+// may load its first step before it computes O.  No two points share an
+// hpsi entry, and the two planes j3, j3 + N3 of a coarse plane write
+// disjoint components.
+template <typename TK, int U>
+struct CompressStep {
+  int ci[U], fi[U];   // coarse / fine record of each point, -1: none
+  TK k[U][4];         // K of the four components
+  double old[U][4];   // hpsi before the step
+};
+
+template <int U, int NT, typename TK>
+__device__ __forceinline__ void compress_load(const BoxDesc& bd, int j3, const TK* Kp,
+                                              const double* hpsi, int i0,
+                                              CompressStep<TK, U>& st) {
   const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], W1 = 2 * N1;
   const int s3 = j3 >= N3;
   const int i3 = j3 - s3 * N3;
   const int npl = N1 * N2;
-  double* hc = hpsi + bd.psi_off;
-  double* hf = hpsi + bd.psi_off + bd.nvctr_c + 3 * s3;  // components 1-3 or 4-7
+  const double* hc = hpsi + bd.psi_off;
+  const double* hf = hpsi + bd.psi_off + bd.nvctr_c + 3 * s3;  // components 1-3 or 4-7
   const float in1 = 1.0f / (float)N1;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = i0 + u * NT;
+    st.ci[u] = -1;
+    st.fi[u] = -1;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) st.k[u][s] = (TK)0;
+    if (i < npl) {
+      const int i2 = qdiv(i, N1, in1), i1 = i - i2 * N1;
+      const int lin = i + npl * i3;
+      st.ci[u] = s3 ? -1 : (bd.cinv ? bd.cinv[lin] : lin);
+      st.fi[u] = bd.finv ? bd.finv[lin] : lin;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        st.k[u][s] = Kp[i1 + N1 * (s & 1) + W1 * (i2 + N2 * (s >> 1))];
+    }
+  }
+  // s3 = 0: coarse value s = 0, fine components 1-3 from s = 1..3;
+  // s3 = 1: fine components 4-7 from s = 0..3
+  const int sf = 1 - s3;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const double* f = hf + 7LL * max(st.fi[u], 0) - sf;
+    st.old[u][0] = st.ci[u] >= 0 ? hc[max(st.ci[u], 0)]
+                                 : (s3 && st.fi[u] >= 0 ? f[0] : 0.0);
+#pragma unroll
+    for (int s = 1; s < 4; ++s) st.old[u][s] = st.fi[u] >= 0 ? f[s] : 0.0;
+  }
+}
+
+template <typename TS, int U, int NT, typename TO, typename TK>
+__device__ __forceinline__ void compress_store(const BoxDesc& bd, int j3, const TO* O, int RO,
+                                               double* hpsi, int i0,
+                                               const CompressStep<TK, U>& st) {
+  const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2];
+  const int s3 = j3 >= N3;
+  const int npl = N1 * N2;
+  double* hc = hpsi + bd.psi_off;
+  double* hf = hpsi + bd.psi_off + bd.nvctr_c + 3 * s3;
+  const float in1 = 1.0f / (float)N1;
+  const int sf = 1 - s3;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = i0 + u * NT;
+    if (i < npl) {
+      const int i2 = qdiv(i, N1, in1), i1 = i - i2 * N1;
+      double v[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        v[s] = (double)((TS)O[(i2 + N2 * (s >> 1)) * RO + i1 + N1 * (s & 1)] +
+                        (TS)st.k[u][s]);
+      double* f = hf + 7LL * max(st.fi[u], 0) - sf;
+      if (st.ci[u] >= 0) hc[st.ci[u]] = st.old[u][0] + v[0];
+      if (st.fi[u] >= 0) {
+        if (s3) f[0] = st.old[u][0] + v[0];
+#pragma unroll
+        for (int s = 1; s < 4; ++s) f[s] = st.old[u][s] + v[s];
+      }
+    }
+  }
+}
+
+template <typename TS, int U, int NT, typename TO, typename TK>
+__device__ __forceinline__ void compress_plane(const BoxDesc& bd, int j3, const TO* O, int RO,
+                                               const TK* Kp, double* hpsi) {
+  const int npl = bd.N[0] * bd.N[1];
   for (int i0 = threadIdx.x; i0 < npl; i0 += NT * U) {
-    int ci[U], fi[U];
-    double v[U][4];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = i0 + u * NT;
-      ci[u] = -1;
-      fi[u] = -1;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) v[u][s] = 0.0;
-      if (i < npl) {
-        const int i2 = qdiv(i, N1, in1), i1 = i - i2 * N1;
-        const int lin = i + npl * i3;
-        ci[u] = s3 ? -1 : (bd.cinv ? bd.cinv[lin] : lin);
-        fi[u] = bd.finv ? bd.finv[lin] : lin;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int j1 = i1 + N1 * (s & 1), j2 = i2 + N2 * (s >> 1);
-          v[u][s] = (double)((TS)O[j2 * RO + j1] + (TS)Kp[j1 + W1 * j2]);
-        }
-      }
-    }
-    // s3 = 0: coarse value s = 0, fine components 1-3 from s = 1..3;
-    // s3 = 1: fine components 4-7 from s = 0..3
-    const int sf = 1 - s3;
-    double old[U][4];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const double* f = hf + 7LL * max(fi[u], 0) - sf;
-      old[u][0] = ci[u] >= 0 ? hc[max(ci[u], 0)] : (s3 && fi[u] >= 0 ? f[0] : 0.0);
-#pragma unroll
-      for (int s = 1; s < 4; ++s) old[u][s] = fi[u] >= 0 ? f[s] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      double* f = hf + 7LL * max(fi[u], 0) - sf;
-      if (ci[u] >= 0) hc[ci[u]] = old[u][0] + v[u][0];
-      if (fi[u] >= 0) {
-        if (s3) f[0] = old[u][0] + v[u][0];
-#pragma unroll
-        for (int s = 1; s < 4; ++s) f[s] = old[u][s] + v[u][s];
-      }
-    }
+    CompressStep<TK, U> st;
+    compress_load<U, NT>(bd, j3, Kp, hpsi, i0, st);
+    compress_store<TS, U, NT>(bd, j3, O, RO, hpsi, i0, st);
   }
 }
 

@@ -432,6 +432,14 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   T* Bq = U + max(L2 * RU, W2 * RO);  // G_y^T output: Bq[j2*RU + p1]
   const int tid = threadIdx.x;
   const T* src = (const T*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
+  // the compress's first point per thread (indices, K, old hpsi) is loaded
+  // before the plane, so that its latency overlaps the plane load and the
+  // two passes (one point: more registers held through the passes cost
+  // occupancy); later steps take UC points each
+  constexpr int UC = NTP >= 768 ? 3 : 2, UP = 1;
+  const T* Kp = (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
+  CompressStep<T, UP> st0;
+  compress_load<UP, NTP>(bd, j3, Kp, B.hpsi, tid, st0);
   {
     const int tot = L1 * L2;
     const float inv = 1.0f / (float)L1;
@@ -489,8 +497,12 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   // out = O + K, accumulated into hpsi at the compressed positions of the
   // plane's four components (s3 fixed by j3), compress_and_accumulate_mixed
   // (un-compress.f90:2174-2238)
-  compress_plane<T, (NTP >= 768 ? 3 : 2), NTP>(
-      bd, j3, O, RO, (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3, B.hpsi);
+  compress_store<T, UP, NTP>(bd, j3, O, RO, B.hpsi, tid, st0);
+  for (int i0 = tid + NTP * UP; i0 < N1 * N2; i0 += NTP * UC) {
+    CompressStep<T, UC> st;
+    compress_load<UC, NTP>(bd, j3, Kp, B.hpsi, i0, st);
+    compress_store<T, UC, NTP>(bd, j3, O, RO, B.hpsi, i0, st);
+  }
 }
 
 // LDS bytes of the two plane kernels for the largest box extents
