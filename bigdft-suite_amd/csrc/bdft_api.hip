@@ -761,6 +761,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   long long wtot = 0, t1tot = 0, t2tot = 0, t3tot = 0, psi_run = 0;
   int maxN = 0;
   bool xbig = false, zbig = false;
+  int maxL3 = 0;
   bool need_zero = false;
   for (int i = 0; i < norb; ++i) {
     const bdft_lr* lr = lrs[i];
@@ -832,6 +833,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     t3tot += (long long)b.L[0] * b.L[1] * b.L[2];
     xbig = xbig || b.N[0] > REG_N || b.L[0] > REG_L;
     zbig = zbig || b.L[2] > REG_L;
+    maxL3 = std::max(maxL3, b.L[2]);
     if (!lr->dense) need_zero = true;
   }
   // bundles per z-line tile of the matrix-core sequence: three, unless a
@@ -1238,6 +1240,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   if (!zbig) t3tot = 0;
   T.xbig = xbig;
   T.zbig = zbig;
+  T.zshort = maxL3 <= ZSHORT_L;
   const size_t ws_bytes = esz * (2 * al(wtot) + al(t1tot) + al(t2tot) + al(t3tot));
   plan->ws_need = ws_bytes;
   plan->ws_at[0] = 0;

@@ -29,6 +29,7 @@ constexpr int MAXN = 256;       // largest coarse extent handled per axis
 // the two-kernel z potential pass
 constexpr int REG_N = 64;
 constexpr int REG_L = 192;
+constexpr int ZSHORT_L = 128;
 // largest coarse extent of the matrix-core kernels (z-line loaders of at
 // most 128 columns); batches with a larger box run the FP32 vector passes
 constexpr int MFMA_MAXN = 64;
@@ -426,6 +427,9 @@ struct PassTables {
   // work-group per tile)
   const int* zq;
   int zq_grid;
+  // every z line of the batch within ZSHORT_L points: the z potential pass
+  // holds fewer fine chunks per thread (fewer registers, same arithmetic)
+  int zshort;
 };
 
 // Launch stages in order; profiling records one event pair per stage.
@@ -538,7 +542,8 @@ size_t mfma_attributes(size_t lds);
 // the FP32 z-line potential pass (k_zpot<float>) over P_ZP tiles, in place
 // on T2, epot partials per tile
 hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
-                           double* part, const int* zq, int zq_grid, hipStream_t s);
+                           double* part, const int* zq, int zq_grid, int zshort,
+                           hipStream_t s);
 // resident work-groups of the persistent z potential pass on the current
 // device (a multiple of the 8 XCDs), for working type FP32 or FP64
 int zpot_queue_grid(int fp32, int maxN);

@@ -1413,7 +1413,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       // G_z, V, G_z^T in FP32 on the vector units (the pass where they beat
       // the split-precision matrix-core form); T2 is unscaled FP32 in both
       const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
-                                           PT.part[3], PT.zq, PT.zq_grid, sz);
+                                           PT.part[3], PT.zq, PT.zq_grid, PT.zshort, sz);
       if (e != hipSuccess) return e;
     } else if (PT.ntiles[P_ZPG] > 0) {
       if (L.zg == 2)
@@ -1481,7 +1481,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
       // F16X3: G_z, V, G_z^T in FP32 (the z lines of a free box's x-y halo
       // are far below the box scale the fp16 pieces are cut for)
       const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
-                                           PT.part[3], PT.zq, PT.zq_grid, s);
+                                           PT.part[3], PT.zq, PT.zq_grid, PT.zshort, s);
       if (e != hipSuccess) return e;
     } else if (PT.ntiles[P_ZP] > 0) {
       k_mzpot<FM, BM><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);

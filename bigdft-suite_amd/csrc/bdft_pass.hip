@@ -269,7 +269,9 @@ __device__ __forceinline__ void kin4(const T* __restrict__ cc, const T* __restri
   }
 }
 
-// Block reduction of one double per thread; result valid in thread 0.
+// Block reduction of one double per thread (work-group of NTB threads, red
+// holds NTB / 64 entries); result valid in thread 0.
+template <int NTB = NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -279,7 +281,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < NT / 64; ++i) s += red[i];
+    for (int i = 0; i < NTB / 64; ++i) s += red[i];
   return s;
 }
 
@@ -563,8 +565,25 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
 // ---------------------------------------------------------------------------
 // z pass with the potential: T2 <- G_z^T ( V . G_z T2 ), epot partial
 // ---------------------------------------------------------------------------
-// one tile (32 z lines of T2 at one y) of the z potential pass
+// work-group size of the z potential pass (k_zpot) per working type: the
+// FP32 pass runs fastest as 128 threads (two waves, more chunks per thread),
+// the FP64 one as 256 (measured on 2048 uneven free boxes)
 template <typename T>
+struct ZNT;
+template <>
+struct ZNT<float> {
+  static constexpr int v = 128;
+};
+template <>
+struct ZNT<double> {
+  static constexpr int v = NT;
+};
+
+
+// one tile (32 z lines of T2 at one y) of the z potential pass, by a
+// work-group of NTZ threads: TY = NTZ / 32 threads per line, each holding
+// its fine chunks of 8 points (at most MZ for lines of up to LMAX points)
+template <typename T, int NTZ, int LMAX>
 __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, double* part,
                                           int tile, T* lds, double* red) {
   const int b = find_box(tt, tile);
@@ -575,6 +594,7 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const int j2 = lt / nxt;
   const int j1 = (lt - j2 * nxt) * TW;
   const int nl = min(TW, L1 - j1);
+  constexpr int TY = NTZ / TW, MZ = (LMAX + 8 * TY - 1) / (8 * TY);
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
   // byte offsets from the tile's first line (a uniform base): one box's T2
@@ -647,10 +667,10 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const bool vexact = bd.ishift[0] == 0 && bd.ishift[1] == 0 && bd.ishift[2] == 0 &&
                       bd.potdim[0] == L1 && bd.potdim[1] == L2 && pd3 == L3 &&
                       !bd.conf_on && (!per || (L3 & 7) == 0);
-  T g[3][8];
+  T g[MZ][8];
   double ep = 0.0;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
+  for (int m = 0; m < MZ; ++m) {
     const int p0 = 8 * (ty + m * TY);
     if (p0 < L3) {
       // the chunk's potential values are loaded first, with no use between
@@ -695,7 +715,7 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
       }
     }
   }
-  const double eps = block_sum(ep, red);
+  const double eps = block_sum<NTZ>(ep, red);
   if (threadIdx.x == 0) part[tile] = eps;
   __syncthreads();
   // V psir as a real line bundle: row q holds fine index q (free) or q - 14
@@ -706,7 +726,7 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const int Q = 2 * N3 + 36;
   const int rofs = per ? 14 : 0;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
+  for (int m = 0; m < MZ; ++m) {
     const int p0 = 8 * (ty + m * TY);
     if (p0 < L3) {
       T* dst = lds + (p0 + rofs) * LDW + tx;
@@ -740,11 +760,11 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_zpot(Batch B, TileTable tt, double* part) {
+template <typename T, int NTZ, int LMAX>
+__global__ __launch_bounds__(NTZ) void k_zpot(Batch B, TileTable tt, double* part) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[NT / 64];
-  zpot_tile<T>(B, tt, part, tile_of(tt), (T*)smem, red);
+  __shared__ double red[NTZ / 64];
+  zpot_tile<T, NTZ, LMAX>(B, tt, part, tile_of(tt), (T*)smem, red);
 }
 
 // The same pass as a persistent launch over a work queue: a grid of
@@ -763,9 +783,19 @@ __global__ __launch_bounds__(NT) void k_zpot_q(Batch B, TileTable tt, double* pa
   const int q = n >> 3, r = n & 7;
   const int lo = x * q + min(x, r), hi = lo + q + (x < r ? 1 : 0);
   for (int k = lo + i; k < hi; k += g) {
-    zpot_tile<T>(B, tt, part, queue[k], (T*)smem, red);
+    zpot_tile<T, NT, REG_L>(B, tt, part, queue[k], (T*)smem, red);
     __syncthreads();
   }
+}
+
+// the z potential pass over its tiles, in the short-line form when every
+// line of the batch has at most ZSHORT_L points
+template <typename T>
+static void launch_zpot(const Batch& B, const TileTable& t, int ntiles, int zshort,
+                        size_t lds, double* part, hipStream_t s) {
+  constexpr int n = ZNT<T>::v;
+  if (zshort) k_zpot<T, n, ZSHORT_L><<<ntiles, n, lds, s>>>(B, t, part);
+  else k_zpot<T, n, REG_L><<<ntiles, n, lds, s>>>(B, t, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -1131,12 +1161,13 @@ static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs
 }
 
 hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
-                           double* part, const int* zq, int zq_grid, hipStream_t s) {
+                           double* part, const int* zq, int zq_grid, int zshort,
+                           hipStream_t s) {
   const size_t lds = lds_bytes(maxN, sizeof(float));
   if (ntiles > 0 && zq && zq_grid > 0)
     k_zpot_q<float><<<zq_grid, NT, lds, s>>>(B, t, part, zq, ntiles);
   else if (ntiles > 0)
-    k_zpot<float><<<ntiles, NT, lds, s>>>(B, t, part);
+    launch_zpot<float>(B, t, ntiles, zshort, lds, part, s);
   return hipGetLastError();
 }
 
@@ -1180,7 +1211,7 @@ static void zpot_pass(const Batch& B, const PassTables& PT, size_t lds, hipStrea
     k_zpot_q<T><<<PT.zq_grid, NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3], PT.zq,
                                             PT.ntiles[P_ZP]);
   } else {
-    k_zpot<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3]);
+    launch_zpot<T>(B, PT.t[P_ZP], PT.ntiles[P_ZP], PT.zshort, lds, PT.part[3], s);
   }
 }
 
@@ -1655,7 +1686,8 @@ template <typename T>
 static void line_attr(int bytes) {
   const void* k[] = {(const void*)k_xfwd<T, false>, (const void*)k_xfwd<T, true>,
                      (const void*)k_yfwd<T>, (const void*)k_kin<T, 1>, (const void*)k_kin<T, 2>,
-                     (const void*)k_zpot<T>, (const void*)k_zgrowpot<T>,
+                     (const void*)k_zpot<T, ZNT<T>::v, ZSHORT_L>,
+                     (const void*)k_zpot<T, ZNT<T>::v, REG_L>, (const void*)k_zgrowpot<T>,
                      (const void*)k_zshrink<T>, (const void*)k_ybwd<T>,
                      (const void*)k_xbwd<T, false>, (const void*)k_xbwd<T, true>,
                      (const void*)k_zden<T, 1>, (const void*)k_zden<T, 2>};
