@@ -32,6 +32,61 @@ struct PLine {
   int es;      // element stride
 };
 
+// Work item it = tid + k step (k = 0, 1, ...) of a loop over ch * n + j
+// (0 <= j < n): one division per thread, then a carry per step.
+struct ItDiv {
+  int ch, j, dch, dj;
+  __device__ __forceinline__ ItDiv(int tid, int n, int step) {
+    ch = tid / n;
+    j = tid - ch * n;
+    dch = step / n;
+    dj = step - dch * n;
+  }
+  __device__ __forceinline__ void next(int n) {
+    ch += dch;
+    j += dj;
+    if (j >= n) {
+      j -= n;
+      ++ch;
+    }
+  }
+};
+
+// A contiguous n1 x n2 plane (x fastest) from memory into LDS rows of
+// stride n1 + 1: element i = tid + k NTP of the plane, its row carried from
+// step to step (no division per element), UR loads in flight per thread.
+template <typename T, int NTP>
+__device__ __forceinline__ void plane_load(T* dst, const T* __restrict__ src, int n1, int n2) {
+  constexpr int UR = 8;
+  const int tot = n1 * n2;
+  ItDiv d(threadIdx.x, n1, NTP);
+  for (int base = threadIdx.x; base < tot; base += NTP * UR) {
+    T v[UR];
+    int a[UR];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const int i = base + u * NTP;
+      v[u] = i < tot ? src[i] : (T)0;
+      a[u] = i + d.ch;   // row r, column i - r n1 at r (n1 + 1) + i - r n1
+      d.next(n1);
+    }
+#pragma unroll
+    for (int u = 0; u < UR; ++u)
+      if (base + u * NTP < tot) dst[a[u]] = v[u];
+  }
+}
+
+// the reverse: LDS rows of stride n1 + 1 into a contiguous n1 x n2 plane
+template <typename T, int NTP>
+__device__ __forceinline__ void plane_store(T* __restrict__ dst, const T* src, int n1, int n2) {
+  const int tot = n1 * n2;
+  ItDiv d(threadIdx.x, n1, NTP);
+  for (int i = threadIdx.x; i < tot; i += NTP) {
+    dst[i] = src[i + d.ch];
+    d.next(n1);
+  }
+}
+
 // G on a wavelet line [c(0..N-1); d(0..N-1)]: 8 fine outputs from
 // k = k0 .. k0+18 (k0 = first coarse index of the chunk).
 template <typename T, bool INTERIOR>
@@ -216,35 +271,14 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tid = threadIdx.x;
   // load the plane (contiguous), UNR loads in flight per thread
-  {
-    const int tot = W1 * W2;
-    const float inv = 1.0f / (float)W1;
-    for (int base = tid; base < tot; base += NTP * 8) {
-      T v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = base + u * NTP;
-        v[u] = i < tot ? src[i] : (T)0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = base + u * NTP;
-        if (i < tot) {
-          int r = (int)((float)i * inv);
-          if ((r + 1) * W1 <= i) ++r;
-          if (r * W1 > i) --r;
-          A[r * RA + (i - r * W1)] = v[u];
-        }
-      }
-    }
-  }
+  plane_load<T, NTP>(A, src, W1, W2);
   __syncthreads();
   if constexpr (KIN) {
   // K_x: rows j2, chunks of 4 coarse x positions -> C
   const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1];
   const int cx = (N1 + 3) >> 2;
-  for (int it = tid; it < W2 * cx; it += NTP) {
-    const int ch = it / W2, j2 = it - ch * W2;
+  for (ItDiv d(tid, W2, NTP); d.ch < cx; d.next(W2)) {
+    const int ch = d.ch, j2 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
     kin4_any(PLine<T>{A + j2 * RA, 1}, N1, i0, per, oc, od);
@@ -259,8 +293,8 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   // K_y: columns j1, chunks of 4 coarse y positions, accumulated into C
   double ek = 0.0;
   const int cy = (N2 + 3) >> 2;
-  for (int it = tid; it < W1 * cy; it += NTP) {
-    const int ch = it / W1, j1 = it - ch * W1;
+  for (ItDiv d(tid, W1, NTP); d.ch < cy; d.next(W1)) {
+    const int ch = d.ch, j1 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
     kin4_any(PLine<T>{A + j1, RA}, N2, i0, per, oc, od);
@@ -280,21 +314,14 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   // K plane -> memory (contiguous)
   {
     T* kd = (T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-    const int tot = W1 * W2;
-    const float inv = 1.0f / (float)W1;
-    for (int i = tid; i < tot; i += NTP) {
-      int r = (int)((float)i * inv);
-      if ((r + 1) * W1 <= i) ++r;
-      if (r * W1 > i) --r;
-      kd[i] = C[r * RA + (i - r * W1)];
-    }
+    plane_store<T, NTP>(kd, C, W1, W2);
   }
   __syncthreads();
   }
   // G_x: rows j2 -> C as (W2 rows) x (L1 cols), row stride RB
   const int gx = (L1 + 7) >> 3;
-  for (int it = tid; it < W2 * gx; it += NTP) {
-    const int ch = it / W2, j2 = it - ch * W2;
+  for (ItDiv d(tid, W2, NTP); d.ch < gx; d.next(W2)) {
+    const int ch = d.ch, j2 = d.j;
     const int p0 = 8 * ch;
     T o[8];
     grow8_any(PLine<T>{A + j2 * RA, 1}, N1, grow_k0(p0, per), per, o);
@@ -306,8 +333,8 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   // G_y: columns p1 of C -> T2 plane (real x,y at wavelet z j3)
   T* dst = (T*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
   const int gy = (L2 + 7) >> 3;
-  for (int it = tid; it < L1 * gy; it += NTP) {
-    const int ch = it / L1, p1 = it - ch * L1;
+  for (ItDiv d(tid, L1, NTP); d.ch < gy; d.next(L1)) {
+    const int ch = d.ch, p1 = d.j;
     const int p0 = 8 * ch;
     T o[8];
     grow8_any(PLine<T>{C + p1, RB}, N2, grow_k0(p0, per), per, o);
@@ -341,31 +368,12 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
   T* C = A + W2 * RA;               // K plane, same layout
   const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tid = threadIdx.x;
-  const int tot = W1 * W2;
-  const float inv = 1.0f / (float)W1;
-  for (int base = tid; base < tot; base += NTP * 8) {
-    T v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + u * NTP;
-      v[u] = i < tot ? src[i] : (T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + u * NTP;
-      if (i < tot) {
-        int r = (int)((float)i * inv);
-        if ((r + 1) * W1 <= i) ++r;
-        if (r * W1 > i) --r;
-        A[r * RA + (i - r * W1)] = v[u];
-      }
-    }
-  }
+  plane_load<T, NTP>(A, src, W1, W2);
   __syncthreads();
   const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1], c = (T)pc[9 * b];
   const int cx = (N1 + 3) >> 2;
-  for (int it = tid; it < W2 * cx; it += NTP) {
-    const int ch = it / W2, j2 = it - ch * W2;
+  for (ItDiv d(tid, W2, NTP); d.ch < cx; d.next(W2)) {
+    const int ch = d.ch, j2 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
     kin4_any(PLine<T>{A + j2 * RA, 1}, N1, i0, per, oc, od);
@@ -379,8 +387,8 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
   }
   __syncthreads();
   const int cy = (N2 + 3) >> 2;
-  for (int it = tid; it < W1 * cy; it += NTP) {
-    const int ch = it / W1, j1 = it - ch * W1;
+  for (ItDiv d(tid, W1, NTP); d.ch < cy; d.next(W1)) {
+    const int ch = d.ch, j1 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
     kin4_any(PLine<T>{A + j1, RA}, N2, i0, per, oc, od);
@@ -394,12 +402,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
   }
   __syncthreads();
   T* kd = (T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-  for (int i = tid; i < tot; i += NTP) {
-    int r = (int)((float)i * inv);
-    if ((r + 1) * W1 <= i) ++r;
-    if (r * W1 > i) --r;
-    kd[i] = C[r * RA + (i - r * W1)];
-  }
+  plane_store<T, NTP>(kd, C, W1, W2);
 }
 
 size_t plane_lds_kin(int N1, int N2, int elem) {
@@ -440,33 +443,12 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   const T* Kp = (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   CompressStep<T, UP> st0;
   compress_load<UP, NTP>(bd, j3, Kp, B.hpsi, tid, st0);
-  {
-    const int tot = L1 * L2;
-    const float inv = 1.0f / (float)L1;
-    for (int base = tid; base < tot; base += NTP * 8) {
-      T v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = base + u * NTP;
-        v[u] = i < tot ? src[i] : (T)0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = base + u * NTP;
-        if (i < tot) {
-          int r = (int)((float)i * inv);
-          if ((r + 1) * L1 <= i) ++r;
-          if (r * L1 > i) --r;
-          U[r * RU + (i - r * L1)] = v[u];
-        }
-      }
-    }
-  }
+  plane_load<T, NTP>(U, src, L1, L2);
   __syncthreads();
   // G_y^T: columns p1 -> Bq rows (c: i2, d: N2+i2)
   const int cy = (N2 + 3) >> 2;
-  for (int it = tid; it < L1 * cy; it += NTP) {
-    const int ch = it / L1, p1 = it - ch * L1;
+  for (ItDiv d(tid, L1, NTP); d.ch < cy; d.next(L1)) {
+    const int ch = d.ch, p1 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
     shrink4_any(PLine<T>{U + p1, RU}, N2, L2, i0, per, oc, od);
@@ -481,8 +463,8 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   // G_x^T: rows j2 of Bq -> output plane O (reuses U), row stride RO
   T* O = U;
   const int cx = (N1 + 3) >> 2;
-  for (int it = tid; it < W2 * cx; it += NTP) {
-    const int ch = it / W2, j2 = it - ch * W2;
+  for (ItDiv d(tid, W2, NTP); d.ch < cx; d.next(W2)) {
+    const int ch = d.ch, j2 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
     shrink4_any(PLine<T>{Bq + j2 * RU, 1}, N1, L1, i0, per, oc, od);
