@@ -52,11 +52,16 @@ struct ItDiv {
   }
 };
 
+// LDS row stride for rows of n values: odd (n + 1 or n + 2), so that the
+// lanes of a wave that each walk their own row hit distinct banks
+__host__ __device__ constexpr int odd_stride(int n) { return (n + 1) | 1; }
+
 // A contiguous n1 x n2 plane (x fastest) from memory into LDS rows of
-// stride n1 + 1: element i = tid + k NTP of the plane, its row carried from
+// stride rs > n1: element i = tid + k NTP of the plane, its row carried from
 // step to step (no division per element), UR loads in flight per thread.
 template <typename T, int NTP>
-__device__ __forceinline__ void plane_load(T* dst, const T* __restrict__ src, int n1, int n2) {
+__device__ __forceinline__ void plane_load(T* dst, int rs, const T* __restrict__ src, int n1,
+                                           int n2) {
   constexpr int UR = 8;
   const int tot = n1 * n2;
   ItDiv d(threadIdx.x, n1, NTP);
@@ -67,7 +72,7 @@ __device__ __forceinline__ void plane_load(T* dst, const T* __restrict__ src, in
     for (int u = 0; u < UR; ++u) {
       const int i = base + u * NTP;
       v[u] = i < tot ? src[i] : (T)0;
-      a[u] = i + d.ch;   // row r, column i - r n1 at r (n1 + 1) + i - r n1
+      a[u] = i + d.ch * (rs - n1);   // row r, column i - r n1 at r rs + i - r n1
       d.next(n1);
     }
 #pragma unroll
@@ -76,13 +81,14 @@ __device__ __forceinline__ void plane_load(T* dst, const T* __restrict__ src, in
   }
 }
 
-// the reverse: LDS rows of stride n1 + 1 into a contiguous n1 x n2 plane
+// the reverse: LDS rows of stride rs into a contiguous n1 x n2 plane
 template <typename T, int NTP>
-__device__ __forceinline__ void plane_store(T* __restrict__ dst, const T* src, int n1, int n2) {
+__device__ __forceinline__ void plane_store(T* __restrict__ dst, const T* src, int rs, int n1,
+                                            int n2) {
   const int tot = n1 * n2;
   ItDiv d(threadIdx.x, n1, NTP);
   for (int i = threadIdx.x; i < tot; i += NTP) {
-    dst[i] = src[i + d.ch];
+    dst[i] = src[i + d.ch * (rs - n1)];
     d.next(n1);
   }
 }
@@ -265,13 +271,13 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   const int N1 = bd.N[0], N2 = bd.N[1], L1 = bd.L[0], L2 = bd.L[1], per = bd.per;
   const int W1 = 2 * N1, W2 = 2 * N2;
   const int RA = W1 + 1;            // row stride of the wavelet plane
-  const int RB = L1 + 1;            // row stride of the G_x output
+  const int RB = odd_stride(L1);    // row stride of the G_x output
   T* A = (T*)smem;                  // W plane: A[j2*RA + j1]
   T* C = A + W2 * RA;               // kinetic accumulator, later G_x output
   const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tid = threadIdx.x;
   // load the plane (contiguous), UNR loads in flight per thread
-  plane_load<T, NTP>(A, src, W1, W2);
+  plane_load<T, NTP>(A, RA, src, W1, W2);
   __syncthreads();
   if constexpr (KIN) {
   // K_x: rows j2, chunks of 4 coarse x positions -> C
@@ -314,7 +320,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   // K plane -> memory (contiguous)
   {
     T* kd = (T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-    plane_store<T, NTP>(kd, C, W1, W2);
+    plane_store<T, NTP>(kd, C, RA, W1, W2);
   }
   __syncthreads();
   }
@@ -368,7 +374,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
   T* C = A + W2 * RA;               // K plane, same layout
   const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tid = threadIdx.x;
-  plane_load<T, NTP>(A, src, W1, W2);
+  plane_load<T, NTP>(A, RA, src, W1, W2);
   __syncthreads();
   const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1], c = (T)pc[9 * b];
   const int cx = (N1 + 3) >> 2;
@@ -402,7 +408,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
   }
   __syncthreads();
   T* kd = (T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
-  plane_store<T, NTP>(kd, C, W1, W2);
+  plane_store<T, NTP>(kd, C, RA, W1, W2);
 }
 
 size_t plane_lds_kin(int N1, int N2, int elem) {
@@ -429,10 +435,11 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], L1 = bd.L[0], L2 = bd.L[1],
             per = bd.per;
   const int W1 = 2 * N1, W2 = 2 * N2;
-  const int RU = L1 + 1;
+  const int RU = L1 + 1;            // T2 plane rows (read by columns)
+  const int RQ = odd_stride(L1);    // G_y^T output rows (read by rows)
   const int RO = W1 + 1;
   T* U = (T*)smem;                // T2 plane: U[p2*RU + p1], later output O
-  T* Bq = U + max(L2 * RU, W2 * RO);  // G_y^T output: Bq[j2*RU + p1]
+  T* Bq = U + max(L2 * RU, W2 * RO);  // G_y^T output: Bq[j2*RQ + p1]
   const int tid = threadIdx.x;
   const T* src = (const T*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
   // the compress's first point per thread (indices, K, old hpsi) is loaded
@@ -443,7 +450,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   const T* Kp = (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   CompressStep<T, UP> st0;
   compress_load<UP, NTP>(bd, j3, Kp, B.hpsi, tid, st0);
-  plane_load<T, NTP>(U, src, L1, L2);
+  plane_load<T, NTP>(U, RU, src, L1, L2);
   __syncthreads();
   // G_y^T: columns p1 -> Bq rows (c: i2, d: N2+i2)
   const int cy = (N2 + 3) >> 2;
@@ -455,8 +462,8 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (i0 + r < N2) {
-        Bq[(i0 + r) * RU + p1] = oc[r];
-        Bq[(N2 + i0 + r) * RU + p1] = od[r];
+        Bq[(i0 + r) * RQ + p1] = oc[r];
+        Bq[(N2 + i0 + r) * RQ + p1] = od[r];
       }
   }
   __syncthreads();
@@ -467,7 +474,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
     const int ch = d.ch, j2 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
-    shrink4_any(PLine<T>{Bq + j2 * RU, 1}, N1, L1, i0, per, oc, od);
+    shrink4_any(PLine<T>{Bq + j2 * RQ, 1}, N1, L1, i0, per, oc, od);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (i0 + r < N1) {
@@ -489,14 +496,14 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
 
 // LDS bytes of the two plane kernels for the largest box extents
 size_t plane_lds_fwd(int N1, int N2, int L1, int elem) {
-  const size_t RA = 2 * N1 + 1, RB = L1 + 1;
+  const size_t RA = 2 * N1 + 1, RB = odd_stride(L1);
   const size_t a = 2 * N2 * RA, c = 2 * N2 * (RA > RB ? RA : RB);
   return (a + c) * elem;
 }
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem) {
-  const size_t RU = L1 + 1, RO = 2 * N1 + 1;
+  const size_t RU = L1 + 1, RQ = odd_stride(L1), RO = 2 * N1 + 1;
   const size_t u = (size_t)L2 * RU > 2 * N2 * RO ? (size_t)L2 * RU : 2 * N2 * RO;
-  return (u + 2 * N2 * RU) * elem;
+  return (u + 2 * N2 * RQ) * elem;
 }
 
 template <typename T>
