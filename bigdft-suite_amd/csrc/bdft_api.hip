@@ -702,6 +702,7 @@ static int box_tiles(const BoxDesc& b, int pass, int zg) {
     case P_PB: return 2 * N3;
     case P_KZG: return cdiv((long long)cdiv(2 * N1, TW) * 2 * N2, zg);
     case P_ZPG: return cdiv((long long)cdiv(L1, TW) * L2, zg);
+    case P_ZPF: return cdiv((long long)L1 * L2, TW);
   }
   return 0;
 }
@@ -1133,7 +1134,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // work queue of the persistent z potential pass: the boxes in order of
   // decreasing work (fine points), each box's tiles together
   int zq_off = -1;
-  if (ctx->opt_persist == 1 && T.ntiles[P_ZP] > 0) {
+  if (ctx->opt_persist == 1 && T.ntiles[P_ZPF] > 0) {
     std::vector<int> order((size_t)norb);
     for (int i = 0; i < norb; ++i) order[i] = i;
     auto work = [&](int i) { return (long long)box[i].L[0] * box[i].L[1] * box[i].L[2]; };
@@ -1141,8 +1142,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
                      [&](int a, int b2) { return work(a) > work(b2); });
     zq_off = (int)starts.size();
     for (int i : order) {
-      const int cnt = box_tiles(box[i], P_ZP, zg);
-      const int first = start_off[P_ZP] >= 0 ? starts[start_off[P_ZP] + i] : i * cnt;
+      const int cnt = box_tiles(box[i], P_ZPF, zg);
+      const int first = start_off[P_ZPF] >= 0 ? starts[start_off[P_ZPF] + i] : i * cnt;
       for (int t = 0; t < cnt; ++t) starts.push_back(first + t);
     }
   }
@@ -1217,7 +1218,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   T.zq = zq_off >= 0 ? dst + zq_off : nullptr;
   T.zq_grid = 0;
   if (zq_off >= 0 && hipSetDevice(ctx->device) == hipSuccess)
-    T.zq_grid = std::min(zpot_queue_grid(esz == sizeof(float) ? 1 : 0, maxN), T.ntiles[P_ZP]);
+    T.zq_grid = std::min(zpot_queue_grid(esz == sizeof(float) ? 1 : 0, maxN), T.ntiles[P_ZPF]);
   if (g_off[0] >= 0) {
     for (int p : {P_PF, P_PB}) {
       T.t[p].start = dst + g_off[0];

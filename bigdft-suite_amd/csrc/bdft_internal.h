@@ -394,9 +394,12 @@ __device__ __forceinline__ float t2_line_unscale(const Batch& B, const BoxDesc& 
 
 // Passes of one application; each has its own tile table.
 // P_KZG, P_ZPG: z-line passes of the matrix-core sequence, zg (2 or 3)
-// bundles of 32 lines per tile, chosen per plan (MfmaLds::zg)
+// bundles of 32 lines per tile, chosen per plan (MfmaLds::zg).
+// P_ZP: z lines by rows of x (32 lines of one y per tile); P_ZPF: the same
+// lines as one run x + L1 y, 32 consecutive lines per tile (a tile may span
+// two rows), so that only a box's last tile is partly empty (k_zpot).
 enum Pass { P_UNC = 0, P_XF, P_YF, P_KY, P_KZ, P_ZP, P_YB, P_XB, P_PF, P_PB, P_KZG, P_ZPG,
-            P_COUNT };
+            P_ZPF, P_COUNT };
 constexpr int ZGROUP = 3;   // bundles per z-line tile unless LDS asks for 2
 
 struct PassTables {
@@ -539,11 +542,15 @@ size_t mfma_lds_kz(int W3, int Kp, int zg);
 size_t mfma_lds_zp(int KpG, int DpG, int zg);
 size_t mfma_lds_bwd(int W1, int L1, int L2);
 size_t mfma_attributes(size_t lds);
-// the FP32 z-line potential pass (k_zpot<float>) over P_ZP tiles, in place
-// on T2, epot partials per tile
-hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
-                           double* part, const int* zq, int zq_grid, int zshort,
-                           hipStream_t s);
+// the FP32 z-line potential pass over its tiles (k_zpot<float> on P_ZPF, or
+// the two kernels through T3 for lines beyond REG_L), in place on T2, epot
+// partials per tile
+hipError_t launch_zpot_f32(const Batch& B, const PassTables& PT, int maxN, hipStream_t s);
+// tile table of the z potential pass's epot partials (P_ZPF for k_zpot,
+// P_ZP for the two-kernel form of long lines)
+inline const TileTable& zpot_table(const PassTables& PT) {
+  return PT.zbig ? PT.t[P_ZP] : PT.t[P_ZPF];
+}
 // resident work-groups of the persistent z potential pass on the current
 // device (a multiple of the 8 XCDs), for working type FP32 or FP64
 int zpot_queue_grid(int fp32, int maxN);

@@ -1412,8 +1412,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     if (L.zvalu || FM == 16) {
       // G_z, V, G_z^T in FP32 on the vector units (the pass where they beat
       // the split-precision matrix-core form); T2 is unscaled FP32 in both
-      const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
-                                           PT.part[3], PT.zq, PT.zq_grid, PT.zshort, sz);
+      const hipError_t e = launch_zpot_f32(B, PT, L.maxN, sz);
       if (e != hipSuccess) return e;
     } else if (PT.ntiles[P_ZPG] > 0) {
       if (L.zg == 2)
@@ -1452,7 +1451,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     mark(S_PB, 1);
     et.t[0] = PT.t[P_KZG];
     et.p[0] = PT.part[2];
-    et.t[1] = L.zvalu ? PT.t[P_ZP] : PT.t[P_ZPG];
+    et.t[1] = (L.zvalu || FM == 16) ? zpot_table(PT) : PT.t[P_ZPG];
     et.p[1] = PT.part[3];
     et.nk = 1;
     et.n = 2;
@@ -1480,8 +1479,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     if (FM == 16) {
       // F16X3: G_z, V, G_z^T in FP32 (the z lines of a free box's x-y halo
       // are far below the box scale the fp16 pieces are cut for)
-      const hipError_t e = launch_zpot_f32(B, PT.t[P_ZP], PT.ntiles[P_ZP], L.maxN,
-                                           PT.part[3], PT.zq, PT.zq_grid, PT.zshort, s);
+      const hipError_t e = launch_zpot_f32(B, PT, L.maxN, s);
       if (e != hipSuccess) return e;
     } else if (PT.ntiles[P_ZP] > 0) {
       k_mzpot<FM, BM><<<PT.ntiles[P_ZP], NTM, L.zpot, s>>>(B, PT.t[P_ZP], PT.part[3]);
@@ -1497,7 +1495,7 @@ static hipError_t launch_mfma_t(const Batch& B, const PassTables& PT, const Mfma
     et.p[1] = PT.part[1];
     et.t[2] = PT.t[P_KZ];
     et.p[2] = PT.part[2];
-    et.t[3] = PT.t[P_ZP];
+    et.t[3] = FM == 16 ? zpot_table(PT) : PT.t[P_ZP];
     et.p[3] = PT.part[3];
     et.nk = 3;
     et.n = 4;

@@ -580,7 +580,7 @@ struct ZNT<double> {
 };
 
 
-// one tile (32 z lines of T2 at one y) of the z potential pass, by a
+// one tile (32 consecutive z lines of T2, P_ZPF) of the z potential pass, by a
 // work-group of NTZ threads: TY = NTZ / 32 threads per line, each holding
 // its fine chunks of 8 points (at most MZ for lines of up to LMAX points)
 template <typename T, int NTZ, int LMAX>
@@ -590,17 +590,19 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const BoxDesc bd = B.box[b];
   const int lt = tile - box_first_tile(tt, b);
   const int L1 = bd.L[0], L2 = bd.L[1], N3 = bd.N[2], L3 = bd.L[2], per = bd.per;
-  const int nxt = (L1 + TW - 1) / TW;
-  const int j2 = lt / nxt;
-  const int j1 = (lt - j2 * nxt) * TW;
-  const int nl = min(TW, L1 - j1);
   constexpr int TY = NTZ / TW, MZ = (LMAX + 8 * TY - 1) / (8 * TY);
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
   const int NS = N3 + HL + HR;
+  // the tile's lines: x + L1 y = q0 .. q0 + nl - 1 (P_ZPF); this thread's
+  // line (x, y) (lanes past the box take the first line)
+  const int q0 = lt * TW;
+  const int nl = min(TW, L1 * L2 - q0);
+  const int ql = q0 + (tx < nl ? tx : 0);
+  const int ly = qdiv(ql, L1, 1.0f / (float)L1), lx = ql - ly * L1;
   // byte offsets from the tile's first line (a uniform base): one box's T2
   // holds L1 L2 2 N3 values, below 2^32 bytes for boxes of up to 256 coarse
   // points per axis; lanes past the box (tx >= nl) address line 0
-  char* lineb = (char*)((T*)B.T2 + bd.t2_off + j1 + (long long)L1 * j2);
+  char* lineb = (char*)((T*)B.T2 + bd.t2_off + q0);
   const unsigned ls = (unsigned)sizeof(T) * (unsigned)(L1 * L2);   // bytes per z row
   const unsigned lo = (unsigned)sizeof(T) * (unsigned)(tx < nl ? tx : 0);
   {
@@ -609,7 +611,7 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
     // lanes past the box store zeros; the F16X3 forward pass leaves its
     // table scales on the line (Batch::t2_scaled)
     const T mul = (sizeof(T) == 4 && B.t2_scaled)
-                      ? (T)t2_line_unscale(B, bd, j1 + tx, j2) : (T)1;
+                      ? (T)t2_line_unscale(B, bd, lx, ly) : (T)1;
     const T keep = tx < nl ? mul : (T)0;
 #pragma unroll
     for (int sec = 0; sec < 2; ++sec) {
@@ -652,7 +654,7 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   // reproduces the zeroing of psir there (potential.f90:874-947).  Lanes off
   // the potential box read its first value (a valid address) and discard it,
   // so every potential load is unconditional.
-  const int q1 = j1 + tx - bd.ishift[0], q2 = j2 - bd.ishift[1];
+  const int q1 = lx - bd.ishift[0], q2 = ly - bd.ishift[1];
   const bool in12 = tx < nl && q1 >= 0 && q1 < bd.potdim[0] && q2 >= 0 &&
                     q2 < bd.potdim[1];
   // byte offsets from the box's potential base (uniform): the host keeps a
@@ -661,7 +663,7 @@ __device__ __forceinline__ void zpot_tile(const Batch& B, const TileTable& tt, d
   const unsigned off12 = 8u * (unsigned)(in12 ? q1 + bd.potdim[0] * q2 : 0);
   const unsigned pz = 8u * (unsigned)(bd.potdim[0] * bd.potdim[1]);
   const int pd3 = bd.potdim[2];
-  const double cxy = bd.conf_on ? conf_axis2(bd, 0, j1 + tx) + conf_axis2(bd, 1, j2) : 0.0;
+  const double cxy = bd.conf_on ? conf_axis2(bd, 0, lx) + conf_axis2(bd, 1, ly) : 0.0;
   // potential box = the box, no confinement, and (periodic) no chunk past
   // the end of the line: no per-point selection (uniform over the box)
   const bool vexact = bd.ishift[0] == 0 && bd.ishift[1] == 0 && bd.ishift[2] == 0 &&
@@ -772,18 +774,19 @@ __global__ __launch_bounds__(NTZ) void k_zpot(Batch B, TileTable tt, double* par
 // decreasing work, each box's tiles together), every XCD one contiguous
 // stretch of it, work-group i of an XCD its entries i, i + g, i + 2g, ...
 // (g work-groups on that XCD).  Each work-group stops at the end of its
-// stretch.
-template <typename T>
-__global__ __launch_bounds__(NT) void k_zpot_q(Batch B, TileTable tt, double* part,
-                                                const int* __restrict__ queue, int n) {
+// stretch.  Same work-group size as k_zpot, so the epot partials are
+// summed in the same order.
+template <typename T, int NTZ>
+__global__ __launch_bounds__(NTZ) void k_zpot_q(Batch B, TileTable tt, double* part,
+                                                 const int* __restrict__ queue, int n) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[NT / 64];
+  __shared__ double red[NTZ / 64];
   const int G = gridDim.x, x = blockIdx.x & 7, i = blockIdx.x >> 3;
   const int g = (G >> 3) + (x < (G & 7) ? 1 : 0);
   const int q = n >> 3, r = n & 7;
   const int lo = x * q + min(x, r), hi = lo + q + (x < r ? 1 : 0);
   for (int k = lo + i; k < hi; k += g) {
-    zpot_tile<T, NT, REG_L>(B, tt, part, queue[k], (T*)smem, red);
+    zpot_tile<T, NTZ, REG_L>(B, tt, part, queue[k], (T*)smem, red);
     __syncthreads();
   }
 }
@@ -1160,23 +1163,14 @@ static hipError_t launch_den(const Batch& B, const PassTables& PT, const DenArgs
   return hipGetLastError();
 }
 
-hipError_t launch_zpot_f32(const Batch& B, const TileTable& t, int ntiles, int maxN,
-                           double* part, const int* zq, int zq_grid, int zshort,
-                           hipStream_t s) {
-  const size_t lds = lds_bytes(maxN, sizeof(float));
-  if (ntiles > 0 && zq && zq_grid > 0)
-    k_zpot_q<float><<<zq_grid, NT, lds, s>>>(B, t, part, zq, ntiles);
-  else if (ntiles > 0)
-    launch_zpot<float>(B, t, ntiles, zshort, lds, part, s);
-  return hipGetLastError();
-}
-
 int zpot_queue_grid(int fp32, int maxN) {
   const size_t lds = lds_bytes(maxN, fp32 ? sizeof(float) : sizeof(double));
   int per_cu = 0, dev = 0, ncu = 0;
   const hipError_t e =
-      fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zpot_q<float>, NT, lds)
-           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zpot_q<double>, NT, lds);
+      fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &per_cu, k_zpot_q<float, ZNT<float>::v>, ZNT<float>::v, lds)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &per_cu, k_zpot_q<double, ZNT<double>::v>, ZNT<double>::v, lds);
   if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       per_cu <= 0 || ncu <= 0) {
@@ -1208,11 +1202,16 @@ static void zpot_pass(const Batch& B, const PassTables& PT, size_t lds, hipStrea
     k_zgrowpot<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3]);
     k_zshrink<T><<<PT.ntiles[P_ZP], NT, lds, s>>>(B, PT.t[P_ZP]);
   } else if (PT.zq && PT.zq_grid > 0) {
-    k_zpot_q<T><<<PT.zq_grid, NT, lds, s>>>(B, PT.t[P_ZP], PT.part[3], PT.zq,
-                                            PT.ntiles[P_ZP]);
+    k_zpot_q<T, ZNT<T>::v><<<PT.zq_grid, ZNT<T>::v, lds, s>>>(B, PT.t[P_ZPF], PT.part[3],
+                                                             PT.zq, PT.ntiles[P_ZPF]);
   } else {
-    launch_zpot<T>(B, PT.t[P_ZP], PT.ntiles[P_ZP], PT.zshort, lds, PT.part[3], s);
+    launch_zpot<T>(B, PT.t[P_ZPF], PT.ntiles[P_ZPF], PT.zshort, lds, PT.part[3], s);
   }
+}
+
+hipError_t launch_zpot_f32(const Batch& B, const PassTables& PT, int maxN, hipStream_t s) {
+  zpot_pass<float>(B, PT, lds_bytes(maxN, sizeof(float)), s);
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -1252,7 +1251,7 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
     et.p[0] = PT.part[0];
     et.t[1] = PT.t[P_KZ];
     et.p[1] = PT.part[2];
-    et.t[2] = PT.t[P_ZP];
+    et.t[2] = zpot_table(PT);
     et.p[2] = PT.part[3];
     et.nk = 2;
     et.n = 3;
@@ -1278,7 +1277,7 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
     et.t[0] = PT.t[P_XF];
     et.t[1] = PT.t[P_KY];
     et.t[2] = PT.t[P_KZ];
-    et.t[3] = PT.t[P_ZP];
+    et.t[3] = zpot_table(PT);
     for (int i = 0; i < 4; ++i) et.p[i] = PT.part[i];
     et.nk = 3;
     et.n = 4;
