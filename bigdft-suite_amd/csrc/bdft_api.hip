@@ -700,7 +700,7 @@ static int box_tiles(const BoxDesc& b, int pass, int zg) {
     case P_ZP: return cdiv(L1, TW) * L2;
     case P_PF:
     case P_PB: return 2 * N3;
-    case P_KZG: return cdiv((long long)cdiv(2 * N1, TW) * 2 * N2, zg);
+    case P_KZG: return cdiv((long long)cdiv(4LL * N1 * N2, TW), zg);
     case P_ZPG: return cdiv((long long)cdiv(L1, TW) * L2, zg);
     case P_ZPF: return cdiv((long long)L1 * L2, TW);
   }

@@ -986,7 +986,10 @@ constexpr int ZU = 24;   // loads in flight per thread in the z-line loaders
 // bundle and row follow from the step index with no division, and both rows
 // go to LDS in one 32-bit store per piece.  Every load of the thread is in
 // flight before the first store.  Requires Kp <= 128 (n <= 64).
-template <int NT4, bool PAIRS, bool KEEPF, int ZG>
+// FLAT: bundle bi is the 32 consecutive lines 32 bi .. of the plane (a
+// bundle may span two rows; nxt and rowstride unused), valid for line index
+// < nvalid.
+template <int NT4, bool PAIRS, bool KEEPF, int ZG, bool FLAT = false>
 __device__ __forceinline__ void stage_zlines(const float* __restrict__ src, int rowstride,
                                              long long pstride, int nxt, int lt, int nb,
                                              int nvalid, int nrows, int Kp, float sc,
@@ -1005,9 +1008,14 @@ __device__ __forceinline__ void stage_zlines(const float* __restrict__ src, int 
 #pragma unroll
   for (int j = 0; j < ZG; ++j) {
     const int bi = ZG * lt + j;
-    const int y = bi / nxt;
-    x0[j] = (bi - y * nxt) * 32;
-    base[j] = x0[j] + (long long)rowstride * y;
+    if (FLAT) {
+      x0[j] = 32 * bi;
+      base[j] = x0[j];
+    } else {
+      const int y = bi / nxt;
+      x0[j] = (bi - y * nxt) * 32;
+      base[j] = x0[j] + (long long)rowstride * y;
+    }
   }
   float v[NS][2][LPT];
 #pragma unroll
@@ -1070,16 +1078,16 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1], W3 = 2 * bd.N[2];
   const OpTab t = B.tabs[bd.opK[2]];
   const int XS = t.Kp + 8, XF = W3 + 1, KpW = t.Kp;
-  const int nxt = (W1 + 31) / 32;
-  const int nbund = nxt * W2;
+  const int nlin = W1 * W2;                  // z lines x + W1 y of the box
+  const int nbund = (nlin + 31) / 32;
   const int nb = min(ZG, nbund - ZG * lt);
   const long long pstride = (long long)W1 * W2;
   __bf16* xh = (__bf16*)smem;          // [j][line][k]
   __bf16* xl = xh + ZG * 32 * XS;
   float* xf = (float*)(xl + ZG * 32 * XS);  // [j][line][k], k < W3
-  // z lines of 32 consecutive x (W1 even: pairs of x-neighbours per load)
-  stage_zlines<NT4, true, true, ZG>((const float*)B.W + bd.w_off, W1, pstride, nxt, lt, nb, W1,
-                                    W3, KpW, sc, xh, xl, XS, xf, XF);
+  // bundles of 32 consecutive lines (W1 even: pairs of neighbours per load)
+  stage_zlines<NT4, true, true, ZG, true>((const float*)B.W + bd.w_off, W1, pstride, 1, lt, nb,
+                                          nlin, W3, KpW, sc, xh, xl, XS, xf, XF);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1093,10 +1101,9 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
     // own bundle; with three blocks for four waves sharing them is faster)
     for (int it = wv; it < nobk * nb; it += NTM / 64) {
       const int ob = it / nb, j = it - ob * nb;
-      const int bi = ZG * lt + j;
-      const int jo = bi / nxt, j1 = (bi - jo * nxt) * 32 + (lane & 31);
-      const bool okj = j1 < W1;
-      const long long basej = bd.w_off + j1 + (long long)W1 * jo;
+      const int li = 32 * (ZG * lt + j) + (lane & 31);
+      const bool okj = li < nlin;
+      const long long basej = bd.w_off + li;
       float kold[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -1126,10 +1133,9 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
     bool ok[ZG];
 #pragma unroll
     for (int j = 0; j < ZG; ++j) {
-      const int bi = ZG * lt + j;
-      const int jo = bi / nxt, j1 = (bi - jo * nxt) * 32 + (lane & 31);
-      ok[j] = j < nb && j1 < W1;
-      base[j] = bd.w_off + j1 + (long long)W1 * jo;
+      const int li = 32 * (ZG * lt + j) + (lane & 31);
+      ok[j] = j < nb && li < nlin;
+      base[j] = bd.w_off + li;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int o = 32 * ob + drow(r, lane);
