@@ -696,7 +696,7 @@ static int box_tiles(const BoxDesc& b, int pass, int zg) {
     case P_YF:
     case P_YB: return cdiv(L1, TW) * 2 * N3;
     case P_KY: return cdiv(2 * N1, TW) * 2 * N3;
-    case P_KZ: return cdiv(2 * N1, TW) * 2 * N2;
+    case P_KZ: return cdiv(4LL * N1 * N2, TW);
     case P_ZP: return cdiv(L1, TW) * L2;
     case P_PF:
     case P_PB: return 2 * N3;

@@ -338,14 +338,21 @@ __global__ __launch_bounds__(NTM) void k_mkin(Batch B, TileTable tt, double* par
     base = bd.w_off + (long long)l0 * W1;
     lstride = W1;
     pstride = 1;
-  } else {
+  } else if (AX == 1) {
     const int nxt = (W1 + 31) / 32;
     const int jo = lt / nxt;
     const int j1 = (lt - jo * nxt) * 32;
     nl = min(32, W1 - j1);
-    base = bd.w_off + j1 + (AX == 1 ? (long long)W1 * W2 * jo : (long long)W1 * jo);
+    base = bd.w_off + j1 + (long long)W1 * W2 * jo;
     lstride = 1;
-    pstride = AX == 1 ? (long long)W1 : (long long)W1 * W2;
+    pstride = W1;
+  } else {
+    // z lines x + W1 y of the box, 32 consecutive ones per tile (P_KZ)
+    const int l0 = lt * 32;
+    nl = min(32, W1 * W2 - l0);
+    base = bd.w_off + l0;
+    lstride = 1;
+    pstride = (long long)W1 * W2;
   }
   // the K steps read columns k < t.Kp: columns Wd..Kp-1 and lines nl..31
   // are zero

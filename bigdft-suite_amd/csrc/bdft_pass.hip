@@ -528,13 +528,14 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
   const int lt = tile - box_first_tile(tt, b);
   const int W1 = 2 * bd.N[0], W2 = 2 * bd.N[1];
   const int Nd = bd.N[AX];
+  // AX = 1: 32 lines of one z (x = j1..); AX = 2: 32 consecutive z lines
+  // x + W1 y of the box (P_KZ)
   const int nxt = (W1 + TW - 1) / TW;
-  const int jo = lt / nxt;  // the other (slow) index: j3 for AX=1, j2 for AX=2
-  const int j1 = (lt - jo * nxt) * TW;
-  const int nl = min(TW, W1 - j1);
+  const int jo = AX == 1 ? lt / nxt : 0;   // z for AX = 1
+  const int j1 = AX == 1 ? (lt - jo * nxt) * TW : lt * TW;
+  const int nl = min(TW, (AX == 1 ? W1 : W1 * W2) - j1);
   const long long pstride = AX == 1 ? (long long)W1 : (long long)W1 * W2;
-  const long long base = bd.w_off + j1 +
-                         (AX == 1 ? (long long)W1 * W2 * jo : (long long)W1 * jo);
+  const long long base = bd.w_off + j1 + (AX == 1 ? (long long)W1 * W2 * jo : 0LL);
   fill_wav<false>(lds, (const T*)B.W + base, 1, pstride, nl, Nd, bd.per);
   __syncthreads();
   const int tx = threadIdx.x & (TW - 1), ty = threadIdx.x >> 5;
