@@ -136,3 +136,33 @@ def test_stale_cache_entry_is_tuned_again(tmp_path):
                                    cache=cache, timer=timer, check=check)
     assert times is not None and opts["bwd"] != 1
     assert json.loads(path.read_text())[key]["options"] == opts
+
+
+def test_close_candidates_are_timed_again(tmp_path):
+    """a candidate that was fast once by chance loses to one that is fast
+    every time: the candidates within REFINE of the fastest are timed again
+    and the mean decides"""
+    reg = dense_region("F", (20, 21, 22), (0.4,) * 3)
+    lrs = [FakeLr(reg)] * 2
+    calls = {}
+
+    def timer(ctx, lrs_, pot, psi, prec, kw):
+        o = ctx.opts
+        name = autotune.cand_name(o)
+        calls[name] = calls.get(name, 0) + 1
+        if o["line_passes"] or o["chunk_mb"] or o["tile_order"] or o["persist"]:
+            return 9.0
+        if o["zgroup"] == 3 and o["bwd"] == 2:   # lucky first timing
+            return 3.50 if calls[name] == 1 else 4.20
+        if o["zgroup"] == 2 and o["bwd"] == 2:
+            return 3.60
+        return 5.0
+    ctx = FakeCtx()
+    _, opts, times = autotune.tune(ctx, lrs, None, None, bigdft_hip.PREC_F16X3,
+                                   cache=autotune.TuneCache(str(tmp_path / "t.json")),
+                                   timer=timer)
+    assert opts["zgroup"] == 2 and opts["bwd"] == 2 and ctx.opts == opts
+    lucky = autotune.cand_name(dict(opts, zgroup=3))
+    assert calls[lucky] == 1 + autotune.REFINE_ROUNDS
+    assert times[lucky] == pytest.approx((3.5 + autotune.REFINE_ROUNDS * 4.2) /
+                                         (1 + autotune.REFINE_ROUNDS))
