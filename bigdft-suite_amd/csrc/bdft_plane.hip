@@ -180,17 +180,18 @@ __device__ __forceinline__ void kin4p(PLine<T> ln, int N, int i0, int per, T oc[
 }
 
 // G^T on a real line of L points (periodic: fine index p, L = 2N; free: fine
-// index p-14): 4 coarse outputs i0..i0+3 (c and d) from fine I = 2i0-14+qq.
-template <typename T, bool INTERIOR>
+// index p-14): NO coarse outputs i0..i0+NO-1 (c and d) from fine
+// I = 2i0-14+qq, qq < 2 NO + 29.
+template <typename T, bool INTERIOR, int NO = 4>
 __device__ __forceinline__ void shrink4p(PLine<T> ln, int N, int L, int i0, int per,
-                                         T oc[4], T od[4]) {
+                                         T oc[NO], T od[NO]) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < NO; ++r) {
     oc[r] = (T)0;
     od[r] = (T)0;
   }
 #pragma unroll
-  for (int qq = 0; qq < 37; ++qq) {
+  for (int qq = 0; qq < 2 * NO + 29; ++qq) {
     T x;
     if (INTERIOR) {
       x = ln.p[(2 * i0 + qq - (per ? 14 : 0)) * ln.es];
@@ -205,7 +206,7 @@ __device__ __forceinline__ void shrink4p(PLine<T> ln, int N, int L, int i0, int 
       x = v ? x : (T)0;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < NO; ++r) {
       const int j = qq - 2 * r - 14;
       if (j >= -14 && j <= 16) {
         oc[r] = fma((T)Fil::F2S[j + 14], x, oc[r]);
@@ -246,12 +247,13 @@ __device__ __forceinline__ void kin4_any(PLine<T> ln, int N, int i0, int per, T 
   else if (per && N >= 18) kin4p<T, false, true>(ln, N, i0, per, oc, od);
   else kin4p<T, false>(ln, N, i0, per, oc, od);
 }
-template <typename T>
+template <typename T, int NO = 4>
 __device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, int per,
-                                            T oc[4], T od[4]) {
-  const bool in = per ? (2 * i0 - 14 >= 0 && 2 * i0 + 22 < L) : (2 * i0 + 36 < L);
-  if (in) shrink4p<T, true>(ln, N, L, i0, per, oc, od);
-  else shrink4p<T, false>(ln, N, L, i0, per, oc, od);
+                                            T oc[NO], T od[NO]) {
+  const bool in = per ? (2 * i0 - 14 >= 0 && 2 * i0 + 2 * NO + 14 < L)
+                      : (2 * i0 + 2 * NO + 28 < L);
+  if (in) shrink4p<T, true, NO>(ln, N, L, i0, per, oc, od);
+  else shrink4p<T, false, NO>(ln, N, L, i0, per, oc, od);
 }
 
 // ---------------------------------------------------------------------------
@@ -452,15 +454,20 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   compress_load<UP, NTP>(bd, j3, Kp, B.hpsi, tid, st0);
   plane_load<T, NTP>(U, RU, src, L1, L2);
   __syncthreads();
-  // G_y^T: columns p1 -> Bq rows (c: i2, d: N2+i2)
-  const int cy = (N2 + 3) >> 2;
+  // G_y^T: columns p1 -> Bq rows (c: i2, d: N2+i2), NOY coarse outputs
+  // per work item (NOX for G_x^T): two in FP32, so that the last round of
+  // items leaves fewer lanes idle between the barriers (C4 FP32 backward
+  // pass 3.41 -> 3.22 ms); four in FP64 (two was slower on the periodic
+  // 48^3 planes, 2.27 -> 2.47 ms)
+  constexpr int NOY = sizeof(T) == 4 ? 2 : 4, NOX = NOY;
+  const int cy = (N2 + NOY - 1) / NOY;
   for (ItDiv d(tid, L1, NTP); d.ch < cy; d.next(L1)) {
     const int ch = d.ch, p1 = d.j;
-    const int i0 = 4 * ch;
-    T oc[4], od[4];
-    shrink4_any(PLine<T>{U + p1, RU}, N2, L2, i0, per, oc, od);
+    const int i0 = NOY * ch;
+    T oc[NOY], od[NOY];
+    shrink4_any<T, NOY>(PLine<T>{U + p1, RU}, N2, L2, i0, per, oc, od);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < NOY; ++r)
       if (i0 + r < N2) {
         Bq[(i0 + r) * RQ + p1] = oc[r];
         Bq[(N2 + i0 + r) * RQ + p1] = od[r];
@@ -469,14 +476,14 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   __syncthreads();
   // G_x^T: rows j2 of Bq -> output plane O (reuses U), row stride RO
   T* O = U;
-  const int cx = (N1 + 3) >> 2;
+  const int cx = (N1 + NOX - 1) / NOX;
   for (ItDiv d(tid, W2, NTP); d.ch < cx; d.next(W2)) {
     const int ch = d.ch, j2 = d.j;
-    const int i0 = 4 * ch;
-    T oc[4], od[4];
-    shrink4_any(PLine<T>{Bq + j2 * RQ, 1}, N1, L1, i0, per, oc, od);
+    const int i0 = NOX * ch;
+    T oc[NOX], od[NOX];
+    shrink4_any<T, NOX>(PLine<T>{Bq + j2 * RQ, 1}, N1, L1, i0, per, oc, od);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < NOX; ++r)
       if (i0 + r < N1) {
         O[j2 * RO + i0 + r] = oc[r];
         O[j2 * RO + N1 + i0 + r] = od[r];
