@@ -813,10 +813,17 @@ __global__ __launch_bounds__(NTMP, 4) void k_mfwd(Batch B, TileTable tt) {
         mma_rows<NT4>(B, ky, 32 * lb, rh, rl, XR, 32 * ob, ay);
         const int j1 = 32 * ob + (lane & 31);
         if (j1 < W1) {
+          // byte offsets from the plane's K (uniform base): row j2 of
+          // register rr is drow(0, lane) + 8 (rr / 4) + rr % 4
+          char* kb = (char*)K;
+          const unsigned rs = 4u * (unsigned)W1;
+          const unsigned o0 = 4u * (unsigned)(j1 + W1 * (32 * lb + drow(0, lane)));
+          const bool full = 32 * lb + 32 <= W2;
 #pragma unroll
           for (int rr = 0; rr < 16; ++rr) {
-            const int j2 = 32 * lb + drow(rr, lane);
-            if (j2 < W2) K[j1 + W1 * j2] = ax[rr] * ksx + ay[rr] * ksy;
+            const int dr = 8 * (rr >> 2) + (rr & 3);
+            if (full || 32 * lb + drow(rr, lane) < W2)
+              *(float*)(kb + (o0 + rs * (unsigned)dr)) = ax[rr] * ksx + ay[rr] * ksy;
           }
         }
       } else {
@@ -851,10 +858,18 @@ __global__ __launch_bounds__(NTMP, 4) void k_mfwd(Batch B, TileTable tt) {
           // (G_y) and G16_OUT; the z potential pass removes them
           // (Batch::t2_scaled)
           if (p1 < L1) {
+            // byte offsets from the plane's T2 (uniform base; a plane holds
+            // L1 L2 < 2^30 values): row p2 of register r is
+            // drow(0, lane) + 8 (r / 4) + r % 4
+            char* db = (char*)dst;
+            const unsigned rs = 4u * (unsigned)L1;
+            const unsigned o0 = 4u * (unsigned)(p1 + L1 * (32 * ob + drow(0, lane)));
+            const bool full = 32 * ob + 32 <= L2;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-              const int p2 = 32 * ob + drow(r, lane);
-              if (p2 < L2) dst[p1 + (long long)L1 * p2] = acc[j][r] * isc;
+              const int dr = 8 * (r >> 2) + (r & 3);
+              if (full || 32 * ob + drow(r, lane) < L2)
+                *(float*)(db + (o0 + rs * (unsigned)dr)) = acc[j][r] * isc;
             }
           }
         }
