@@ -1114,7 +1114,10 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const float ks = (float)bd.kscale[2] * isc * kin_unscale<NT4>();
-  float* K = (float*)B.K;
+  // byte offsets from the box's K (uniform base; a box holds W1 W2 W3 <
+  // 2^30 values): line li, row o at 4 (li + o W1 W2)
+  char* kb = (char*)((float*)B.K + bd.w_off);
+  const unsigned rs = 4u * (unsigned)(W1 * W2);
   double ek = 0.0;
   const int nobk = t.Dp / 32;
   if (2 * nobk <= NTM / 64 && nb > 1) {
@@ -1125,12 +1128,13 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
       const int ob = it / nb, j = it - ob * nb;
       const int li = 32 * (ZG * lt + j) + (lane & 31);
       const bool okj = li < nlin;
-      const long long basej = bd.w_off + li;
+      const unsigned o0 = 4u * (unsigned)(okj ? li : 0) + rs * (unsigned)(32 * ob + drow(0, lane));
       float kold[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int o = 32 * ob + drow(r, lane);
-        kold[r] = (okj && o < W3) ? K[basej + o * pstride] : 0.0f;
+        const unsigned dr = (unsigned)(8 * (r >> 2) + (r & 3));
+        kold[r] = (okj && o < W3) ? *(const float*)(kb + (o0 + rs * dr)) : 0.0f;
       }
       f32x16 acc[1] = {};
       mma_multi<NT4, false, 1>(B, t, 32 * ob, xh + j * 32 * XS, xl + j * 32 * XS, XS,
@@ -1142,7 +1146,7 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
           if (o < W3) {
             const float kn = kold[r] + acc[0][r] * ks;
             ek += (double)kn * (double)xf[(j * 32 + (lane & 31)) * XF + o];
-            K[basej + o * pstride] = kn;
+            *(float*)(kb + (o0 + rs * (unsigned)(8 * (r >> 2) + (r & 3)))) = kn;
           }
         }
       }
@@ -1151,17 +1155,18 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
   for (int ob = wv; ob < t.Dp / 32; ob += NTM / 64) {
     // the K values this wave updates, read while the matrix cores run
     float kold[ZG][16];
-    long long base[ZG];
+    unsigned base[ZG];
     bool ok[ZG];
 #pragma unroll
     for (int j = 0; j < ZG; ++j) {
       const int li = 32 * (ZG * lt + j) + (lane & 31);
       ok[j] = j < nb && li < nlin;
-      base[j] = bd.w_off + li;
+      base[j] = 4u * (unsigned)(ok[j] ? li : 0) + rs * (unsigned)(32 * ob + drow(0, lane));
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int o = 32 * ob + drow(r, lane);
-        kold[j][r] = (ok[j] && o < W3) ? K[base[j] + o * pstride] : 0.0f;
+        const unsigned dr = (unsigned)(8 * (r >> 2) + (r & 3));
+        kold[j][r] = (ok[j] && o < W3) ? *(const float*)(kb + (base[j] + rs * dr)) : 0.0f;
       }
     }
     f32x16 acc[ZG] = {};
@@ -1175,7 +1180,7 @@ __global__ __launch_bounds__(NTM) void k_mkz(Batch B, TileTable tt, double* part
         if (o < W3) {
           const float kn = kold[j][r] + acc[j][r] * ks;
           ek += (double)kn * (double)xf[(j * 32 + (lane & 31)) * XF + o];
-          K[base[j] + o * pstride] = kn;
+          *(float*)(kb + (base[j] + rs * (unsigned)(8 * (r >> 2) + (r & 3)))) = kn;
         }
       }
     }
