@@ -316,9 +316,39 @@ __global__ __launch_bounds__(NT) void k_zero(T* __restrict__ p, long long n) {
     p[i] = (T)0;
 }
 
+// W offset of component t = 1..7 of fine point lin: (s1,s2,s3) = bits of t
+// (un-compress.f90:930-936), each bit a fixed step from the (0,0,0) value
+__device__ __forceinline__ long long fine_step(const BoxDesc& bd, int t) {
+  const long long N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2];
+  return (t & 1) * N1 + ((t >> 1) & 1) * 2 * N1 * N2 + ((t >> 2) & 1) * 4 * N1 * N2 * N3;
+}
+
+// A fine tile's records (7 consecutive coefficients per point, CPT points)
+// move between memory and LDS in one coalesced sweep (7 accesses per thread
+// in flight) instead of a 56-byte stride per lane; each thread then works
+// on its own record in LDS (stride 7: no bank conflicts).  FP64 uncompress
+// 1.55 -> 1.19 ms on 2048 uneven boxes, 0.85 -> 0.65 on the periodic 48^3
+// batch.
+__device__ __forceinline__ void records_in(double* st, const double* __restrict__ src,
+                                           int n) {
+  double v[7];
+#pragma unroll
+  for (int u = 0; u < 7; ++u) {
+    const int i = threadIdx.x + u * NT;
+    v[u] = i < n ? src[i] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 7; ++u) {
+    const int i = threadIdx.x + u * NT;
+    if (i < n) st[i] = v[u];
+  }
+}
+
 // uncompress (psi -> W, converted to T) or compress-accumulate (hpsi += W)
 template <typename T, bool COMPRESS>
 __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
+  static_assert(CPT == NT, "one fine record per thread");
+  __shared__ double st[7 * CPT];
   const int tile = tile_of(tt);
   const int b = find_box(tt, tile);
   const BoxDesc bd = B.box[b];
@@ -332,18 +362,35 @@ __global__ __launch_bounds__(NT) void k_uncompress(Batch B, TileTable tt) {
       if (COMPRESS) B.hpsi[bd.psi_off + k] += (double)W[w];
       else W[w] = (T)B.psi[bd.psi_off + k];
     }
-  } else {
+  } else if constexpr (!COMPRESS && sizeof(T) == 4) {
+    // FP32 grids: each thread reads its own record (the staged form below
+    // measured slower here, 0.86 -> 0.90 ms on 2048 uneven boxes)
     const int k = (lt - ctiles) * CPT + threadIdx.x;
     if (k < bd.nvctr_f) {
-      const int lin = bd.fmap[k];
+      const long long w0 = psig_index(bd, bd.fmap[k], 0, 0, 0);
       const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
 #pragma unroll
+      for (int t = 1; t <= 7; ++t) W[w0 + fine_step(bd, t)] = (T)B.psi[base + t - 1];
+    }
+  } else {
+    const int k0 = (lt - ctiles) * CPT;
+    const int nk = min(CPT, bd.nvctr_f - k0);
+    double* rec = (COMPRESS ? B.hpsi : (double*)B.psi) + bd.psi_off + bd.nvctr_c + 7LL * k0;
+    records_in(st, rec, 7 * nk);
+    __syncthreads();
+    if ((int)threadIdx.x < nk) {
+      const long long w0 = psig_index(bd, bd.fmap[k0 + threadIdx.x], 0, 0, 0);
+      double* r = st + 7 * threadIdx.x;
+#pragma unroll
       for (int t = 1; t <= 7; ++t) {
-        // component t = 1..7 <-> (s1,s2,s3) = bits of t, un-compress.f90:930-936
-        const long long w = psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1);
-        if (COMPRESS) B.hpsi[base + t - 1] += (double)W[w];
-        else W[w] = (T)B.psi[base + t - 1];
+        const long long w = w0 + fine_step(bd, t);
+        if (COMPRESS) r[t - 1] += (double)W[w];
+        else W[w] = (T)r[t - 1];
       }
+    }
+    if (COMPRESS) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < 7 * nk; i += NT) rec[i] = st[i];
     }
   }
 }
@@ -1340,12 +1387,12 @@ __global__ __launch_bounds__(NT) void k_uncompress_max(Batch B, TileTable tt, fl
   } else {
     const int k = (lt - ctiles) * CPT + threadIdx.x;
     if (k < bd.nvctr_f) {
-      const int lin = bd.fmap[k];
+      const long long w0 = psig_index(bd, bd.fmap[k], 0, 0, 0);
       const long long base = bd.psi_off + bd.nvctr_c + 7LL * k;
 #pragma unroll
       for (int t = 1; t <= 7; ++t) {
         const float v = (float)B.psi[base + t - 1];
-        W[psig_index(bd, lin, t & 1, (t >> 1) & 1, (t >> 2) & 1)] = v;
+        W[w0 + fine_step(bd, t)] = v;
         mx = fmaxf(mx, fabsf(v));
       }
     }
