@@ -11,9 +11,10 @@
 // x/y line passes and the compress pass: the (L1,2N2,2N3) intermediate and
 // the separate wavelet output volume never reach memory.  Operators and
 // filters are those of the line kernels (growshrink.f90 comb_grow/comb_shrink,
-// convolut_ib_optim.f90 ConvolkineticT); periodic wrap and free zero padding
-// are applied by index arithmetic, with a branch-free path for chunks whose
-// stencil lies inside the line.
+// convolut_ib_optim.f90 ConvolkineticT).  Chunks whose stencil lies inside
+// the line read it directly; the forward pass's other chunks read through a
+// per-box line map (periodic wrap, or a zero slot for free padding), the
+// backward pass's apply the wrap or padding by index arithmetic.
 #include "bdft_internal.h"
 
 namespace bdft {
@@ -93,88 +94,113 @@ __device__ __forceinline__ void plane_store(T* __restrict__ dst, const T* src, i
   }
 }
 
-// G on a wavelet line [c(0..N-1); d(0..N-1)]: 8 fine outputs from
-// k = k0 .. k0+18 (k0 = first coarse index of the chunk).
-template <typename T, bool INTERIOR>
-__device__ __forceinline__ void grow8p(PLine<T> ln, int N, int k0, int per, T o[8]) {
+// G on a wavelet line [c(0..N-1); d(0..N-1)]: the taps of coarse input
+// k = k0 + kk on the 8 fine outputs of a chunk (k0 = first coarse index)
+template <typename T>
+__device__ __forceinline__ void grow_tap(int kk, T c, T d, T o[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int j = r + 16 - 2 * kk;
+    if (j >= -14 && j <= 16) {
+      o[r] = fma((T)Fil::F2S[j + 14], c, o[r]);
+      o[r] = fma((T)Fil::F2W[j + 14], d, o[r]);
+    }
+  }
+}
+
+// wavelet-basis kinetic (unscaled): the taps of coarse input k = i0-14+kk
+// (kk < 32) on the 4 coarse outputs i0..i0+3 (c and d)
+template <typename T>
+__device__ __forceinline__ void kin_tap(int kk, T c, T d, T oc[4], T od[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int l = kk - 14 - r;
+    if (l >= -14 && l <= 14) {
+      const int al = l < 0 ? -l : l;
+      oc[r] = fma((T)Fil::KA[al], c, oc[r]);
+      oc[r] = fma((T)Fil::KC[-l + 14], d, oc[r]);
+      od[r] = fma((T)Fil::KC[l + 14], c, od[r]);
+      od[r] = fma((T)Fil::KE[al], d, od[r]);
+    }
+  }
+}
+
+// Line map of one axis of a box's plane in LDS: entry k + MOFF, for coarse
+// index k = -MOFF .. N + 17, holds the byte offsets from the line's element
+// 0 of c(k) and d(k): k taken modulo N on a periodic line; on a free line a
+// k outside 0..N-1 gives, for both, the offset of a slot that holds zero.
+// The stencil chunks that reach past a line end read through it, one map
+// entry per tap in place of wrap or bounds arithmetic.  Every tap of the
+// grow (k0 >= -15, k0 + 18 <= N + 17 for L <= 2N + 29) and kinetic chunks
+// (i0 - 14 >= -14, i0 + 17 <= N + 16) lies in that range.
+constexpr int MOFF = 15;
+__host__ __device__ constexpr int map_len(int N) { return N + MOFF + 18; }
+
+// es: byte stride of the line's elements; zero: element index of the zero slot
+__device__ __forceinline__ void build_map(int2* m, int N, int per, int es, int zero, int tid,
+                                          int nt) {
+  for (int e = tid; e < map_len(N); e += nt) {
+    const int k = e - MOFF;
+    int c, d;
+    if (per) {
+      int kw = k % N;
+      kw += kw < 0 ? N : 0;
+      c = kw;
+      d = N + kw;
+    } else if ((unsigned)k < (unsigned)N) {
+      c = k;
+      d = N + k;
+    } else {
+      c = d = zero;
+    }
+    m[e] = make_int2(c * es, d * es);
+  }
+}
+
+// G on a wavelet line: 8 fine outputs from k = k0 .. k0+18, read directly
+// where the chunk lies inside the line, else through the line map m
+template <typename T>
+__device__ __forceinline__ void grow8_any(PLine<T> ln, const int2* m, int N, int k0, T o[8]) {
 #pragma unroll
   for (int r = 0; r < 8; ++r) o[r] = (T)0;
+  if (k0 >= 0 && k0 + 18 < N) {
 #pragma unroll
-  for (int kk = 0; kk < 19; ++kk) {
-    const int k = k0 + kk;
-    T c, d;
-    if (INTERIOR) {
-      c = ln.p[k * ln.es];
-      d = ln.p[(N + k) * ln.es];
-    } else if (per) {
-      const int kw = k < 0 ? k + N : (k >= N ? k - N : k);
-      c = ln.p[kw * ln.es];
-      d = ln.p[(N + kw) * ln.es];
-    } else {
-      const bool v = (unsigned)k < (unsigned)N;
-      const int kc = v ? k : 0;
-      c = ln.p[kc * ln.es];
-      d = ln.p[(N + kc) * ln.es];
-      c = v ? c : (T)0;
-      d = v ? d : (T)0;
-    }
+    for (int kk = 0; kk < 19; ++kk)
+      grow_tap(kk, ln.p[(k0 + kk) * ln.es], ln.p[(N + k0 + kk) * ln.es], o);
+  } else {
+    const int2* mm = m + (k0 + MOFF);
+    const char* p = (const char*)ln.p;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int j = r + 16 - 2 * kk;
-      if (j >= -14 && j <= 16) {
-        o[r] = fma((T)Fil::F2S[j + 14], c, o[r]);
-        o[r] = fma((T)Fil::F2W[j + 14], d, o[r]);
-      }
+    for (int kk = 0; kk < 19; ++kk) {
+      const int2 e = mm[kk];
+      grow_tap(kk, *(const T*)(p + e.x), *(const T*)(p + e.y), o);
     }
   }
 }
 
 // wavelet-basis kinetic (unscaled) on a wavelet line: 4 coarse outputs
-// i0..i0+3 (c and d) from k = i0-14 .. i0+17.  WRAP1: periodic line with
-// N >= 18, where one conditional shift wraps every k of the stencil.
-template <typename T, bool INTERIOR, bool WRAP1 = false>
-__device__ __forceinline__ void kin4p(PLine<T> ln, int N, int i0, int per, T oc[4],
-                                      T od[4]) {
+// i0..i0+3 (c and d) from k = i0-14 .. i0+17, direct or through the map
+template <typename T>
+__device__ __forceinline__ void kin4_any(PLine<T> ln, const int2* m, int N, int i0, T oc[4],
+                                         T od[4]) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     oc[r] = (T)0;
     od[r] = (T)0;
   }
+  if (i0 - 14 >= 0 && i0 + 17 < N) {
 #pragma unroll
-  for (int kk = 0; kk < 32; ++kk) {
-    const int k = i0 - 14 + kk;
-    T c, d;
-    if (INTERIOR) {
-      c = ln.p[k * ln.es];
-      d = ln.p[(N + k) * ln.es];
-    } else if (WRAP1) {
-      const int kw = k < 0 ? k + N : (k >= N ? k - N : k);
-      c = ln.p[kw * ln.es];
-      d = ln.p[(N + kw) * ln.es];
-    } else if (per) {
-      int kw = k;
-      while (kw < 0) kw += N;
-      while (kw >= N) kw -= N;
-      c = ln.p[kw * ln.es];
-      d = ln.p[(N + kw) * ln.es];
-    } else {
-      const bool v = (unsigned)k < (unsigned)N;
-      const int kc = v ? k : 0;
-      c = ln.p[kc * ln.es];
-      d = ln.p[(N + kc) * ln.es];
-      c = v ? c : (T)0;
-      d = v ? d : (T)0;
+    for (int kk = 0; kk < 32; ++kk) {
+      const int k = i0 - 14 + kk;
+      kin_tap(kk, ln.p[k * ln.es], ln.p[(N + k) * ln.es], oc, od);
     }
+  } else {
+    const int2* mm = m + (i0 - 14 + MOFF);
+    const char* p = (const char*)ln.p;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int l = kk - 14 - r;
-      if (l >= -14 && l <= 14) {
-        const int al = l < 0 ? -l : l;
-        oc[r] = fma((T)Fil::KA[al], c, oc[r]);
-        oc[r] = fma((T)Fil::KC[-l + 14], d, oc[r]);
-        od[r] = fma((T)Fil::KC[l + 14], c, od[r]);
-        od[r] = fma((T)Fil::KE[al], d, od[r]);
-      }
+    for (int kk = 0; kk < 32; ++kk) {
+      const int2 e = mm[kk];
+      kin_tap(kk, *(const T*)(p + e.x), *(const T*)(p + e.y), oc, od);
     }
   }
 }
@@ -235,18 +261,6 @@ __device__ __forceinline__ int grow_k0(int p0, int per) {
   return per ? (p0 >> 1) - 8 : (p0 >> 1) - 15;
 }
 
-template <typename T>
-__device__ __forceinline__ void grow8_any(PLine<T> ln, int N, int k0, int per, T o[8]) {
-  if (k0 >= 0 && k0 + 18 < N) grow8p<T, true>(ln, N, k0, per, o);
-  else grow8p<T, false>(ln, N, k0, per, o);
-}
-template <typename T>
-__device__ __forceinline__ void kin4_any(PLine<T> ln, int N, int i0, int per, T oc[4],
-                                         T od[4]) {
-  if (i0 - 14 >= 0 && i0 + 17 < N) kin4p<T, true>(ln, N, i0, per, oc, od);
-  else if (per && N >= 18) kin4p<T, false, true>(ln, N, i0, per, oc, od);
-  else kin4p<T, false>(ln, N, i0, per, oc, od);
-}
 template <typename T, int NO = 4>
 __device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, int per,
                                             T oc[NO], T od[NO]) {
@@ -254,6 +268,24 @@ __device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, i
                       : (2 * i0 + 2 * NO + 28 < L);
   if (in) shrink4p<T, true, NO>(ln, N, L, i0, per, oc, od);
   else shrink4p<T, false, NO>(ln, N, L, i0, per, oc, od);
+}
+
+// LDS layout of the forward and preconditioner plane kernels: the W plane A
+// (W2 rows of stride RA = W1 + 1, then a zero row; column W1 of every row is
+// zero: the free lines' zero slots), the second plane C (W2 + 1 rows of
+// stride RC), then the line maps (x on the rows of A, y on the columns of A,
+// y on the columns of C).  Byte offset of the maps:
+__host__ __device__ inline size_t plane_maps_off(int N1, int N2, int RC, int elem) {
+  const size_t rows = 2 * N2 + 1, RA = 2 * N1 + 1;
+  return ((rows * RA + rows * RC) * elem + 7) & ~(size_t)7;
+}
+
+// zero slots: column W1 and row W2 of A, row W2 of C (stride RC)
+template <typename T, int NTP>
+__device__ __forceinline__ void plane_zero_slots(T* A, int RA, int W1, int W2, T* C, int RC) {
+  for (int i = threadIdx.x; i < W2; i += NTP) A[i * RA + W1] = (T)0;
+  for (int i = threadIdx.x; i < RA; i += NTP) A[W2 * RA + i] = (T)0;
+  for (int i = threadIdx.x; i < RC; i += NTP) C[W2 * RC + i] = (T)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -274,12 +306,20 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
   const int W1 = 2 * N1, W2 = 2 * N2;
   const int RA = W1 + 1;            // row stride of the wavelet plane
   const int RB = odd_stride(L1);    // row stride of the G_x output
+  const int RC = max(RA, RB);
   T* A = (T*)smem;                  // W plane: A[j2*RA + j1]
-  T* C = A + W2 * RA;               // kinetic accumulator, later G_x output
+  T* C = A + (W2 + 1) * RA;         // kinetic accumulator, later G_x output
+  int2* mx = (int2*)(smem + plane_maps_off(N1, N2, RC, sizeof(T)));
+  int2* mya = mx + map_len(N1);
+  int2* myc = mya + map_len(N2);
   const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tid = threadIdx.x;
   // load the plane (contiguous), UNR loads in flight per thread
   plane_load<T, NTP>(A, RA, src, W1, W2);
+  build_map(mx, N1, per, sizeof(T), W1, tid, NTP);
+  build_map(mya, N2, per, RA * sizeof(T), W2, tid, NTP);
+  build_map(myc, N2, per, RB * sizeof(T), W2, tid, NTP);
+  plane_zero_slots<T, NTP>(A, RA, W1, W2, C, RB);
   __syncthreads();
   if constexpr (KIN) {
   // K_x: rows j2, chunks of 4 coarse x positions -> C
@@ -289,7 +329,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
     const int ch = d.ch, j2 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
-    kin4_any(PLine<T>{A + j2 * RA, 1}, N1, i0, per, oc, od);
+    kin4_any(PLine<T>{A + j2 * RA, 1}, mx, N1, i0, oc, od);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (i0 + r < N1) {
@@ -305,7 +345,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
     const int ch = d.ch, j1 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
-    kin4_any(PLine<T>{A + j1, RA}, N2, i0, per, oc, od);
+    kin4_any(PLine<T>{A + j1, RA}, mya, N2, i0, oc, od);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (i0 + r < N2) {
@@ -332,7 +372,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
     const int ch = d.ch, j2 = d.j;
     const int p0 = 8 * ch;
     T o[8];
-    grow8_any(PLine<T>{A + j2 * RA, 1}, N1, grow_k0(p0, per), per, o);
+    grow8_any(PLine<T>{A + j2 * RA, 1}, mx, N1, grow_k0(p0, per), o);
 #pragma unroll
     for (int r = 0; r < 8; ++r)
       if (p0 + r < L1) C[j2 * RB + p0 + r] = o[r];
@@ -345,7 +385,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
     const int ch = d.ch, p1 = d.j;
     const int p0 = 8 * ch;
     T o[8];
-    grow8_any(PLine<T>{C + p1, RB}, N2, grow_k0(p0, per), per, o);
+    grow8_any(PLine<T>{C + p1, RB}, myc, N2, grow_k0(p0, per), o);
 #pragma unroll
     for (int r = 0; r < 8; ++r)
       if (p0 + r < L2) dst[p1 + (long long)L1 * (p0 + r)] = o[r];
@@ -373,10 +413,15 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
   const int W1 = 2 * N1, W2 = 2 * N2;
   const int RA = W1 + 1;
   T* A = (T*)smem;                  // W plane: A[j2*RA + j1]
-  T* C = A + W2 * RA;               // K plane, same layout
+  T* C = A + (W2 + 1) * RA;         // K plane, same layout
+  int2* mx = (int2*)(smem + plane_maps_off(N1, N2, RA, sizeof(T)));
+  int2* mya = mx + map_len(N1);
   const T* src = (const T*)B.W + bd.w_off + (long long)W1 * W2 * j3;
   const int tid = threadIdx.x;
   plane_load<T, NTP>(A, RA, src, W1, W2);
+  build_map(mx, N1, per, sizeof(T), W1, tid, NTP);
+  build_map(mya, N2, per, RA * sizeof(T), W2, tid, NTP);
+  plane_zero_slots<T, NTP>(A, RA, W1, W2, C, RA);
   __syncthreads();
   const T kx = (T)bd.kscale[0], ky = (T)bd.kscale[1], c = (T)pc[9 * b];
   const int cx = (N1 + 3) >> 2;
@@ -384,7 +429,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
     const int ch = d.ch, j2 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
-    kin4_any(PLine<T>{A + j2 * RA, 1}, N1, i0, per, oc, od);
+    kin4_any(PLine<T>{A + j2 * RA, 1}, mx, N1, i0, oc, od);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (i0 + r < N1) {
@@ -399,7 +444,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
     const int ch = d.ch, j1 = d.j;
     const int i0 = 4 * ch;
     T oc[4], od[4];
-    kin4_any(PLine<T>{A + j1, RA}, N2, i0, per, oc, od);
+    kin4_any(PLine<T>{A + j1, RA}, mya, N2, i0, oc, od);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (i0 + r < N2) {
@@ -414,7 +459,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_kin_pc(Batch B, TileTab
 }
 
 size_t plane_lds_kin(int N1, int N2, int elem) {
-  return (size_t)2 * (2 * N2) * (2 * N1 + 1) * elem;
+  return plane_maps_off(N1, N2, 2 * N1 + 1, elem) + 8 * (size_t)(map_len(N1) + map_len(N2));
 }
 
 hipError_t launch_plane_kin_pc(const Batch& B, const TileTable& t, int ntiles, size_t lds,
@@ -503,9 +548,9 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
 
 // LDS bytes of the two plane kernels for the largest box extents
 size_t plane_lds_fwd(int N1, int N2, int L1, int elem) {
-  const size_t RA = 2 * N1 + 1, RB = odd_stride(L1);
-  const size_t a = 2 * N2 * RA, c = 2 * N2 * (RA > RB ? RA : RB);
-  return (a + c) * elem;
+  const int RA = 2 * N1 + 1, RB = odd_stride(L1);
+  return plane_maps_off(N1, N2, RA > RB ? RA : RB, elem) +
+         8 * (size_t)(map_len(N1) + 2 * map_len(N2));
 }
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem) {
   const size_t RU = L1 + 1, RQ = odd_stride(L1), RO = 2 * N1 + 1;
