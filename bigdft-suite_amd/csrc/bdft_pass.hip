@@ -592,6 +592,18 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
   double ek = 0.0;
   T* kp = (T*)B.K + base + tx;
   for (int i0 = 4 * ty; i0 < Nd; i0 += 4 * TY) {
+    // the chunk's 8 K values are read before the stencil and written after
+    // it: read one at a time between the writes (the compiler cannot tell
+    // the rows apart), each read waited for the previous write
+    T k0[4], k1[4];
+    if (tx < nl) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (i0 + r < Nd) {
+          k0[r] = kp[pstride * (i0 + r)];
+          k1[r] = kp[pstride * (Nd + i0 + r)];
+        }
+    }
     T oc[4], od[4];
     kin4(col + (i0 + 1) * LDW, col + (NS + i0 + 1) * LDW, oc, od);
     if (tx < nl) {
@@ -601,8 +613,8 @@ __global__ __launch_bounds__(NT) void k_kin(Batch B, TileTable tt, double* part)
           const T vc = oc[r] * ks, vd = od[r] * ks;
           ek += (double)vc * (double)col[(i0 + r + HL) * LDW] +
                 (double)vd * (double)col[(NS + i0 + r + HL) * LDW];
-          kp[pstride * (i0 + r)] += vc;
-          kp[pstride * (Nd + i0 + r)] += vd;
+          kp[pstride * (i0 + r)] = k0[r] + vc;
+          kp[pstride * (Nd + i0 + r)] = k1[r] + vd;
         }
     }
   }
