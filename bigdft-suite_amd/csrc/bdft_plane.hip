@@ -60,7 +60,8 @@ __host__ __device__ constexpr int odd_stride(int n) { return (n + 1) | 1; }
 // A contiguous n1 x n2 plane (x fastest) from memory into LDS rows of
 // stride rs > n1: element i = tid + k NTP of the plane, its row carried from
 // step to step (no division per element), UR loads in flight per thread.
-template <typename T, int NTP>
+// TRANS: stored transposed, x = j of row ch at j rs + ch (rs >= n2, odd).
+template <typename T, int NTP, bool TRANS = false>
 __device__ __forceinline__ void plane_load(T* dst, int rs, const T* __restrict__ src, int n1,
                                            int n2) {
   constexpr int UR = 8;
@@ -73,7 +74,8 @@ __device__ __forceinline__ void plane_load(T* dst, int rs, const T* __restrict__
     for (int u = 0; u < UR; ++u) {
       const int i = base + u * NTP;
       v[u] = i < tot ? src[i] : (T)0;
-      a[u] = i + d.ch * (rs - n1);   // row r, column i - r n1 at r rs + i - r n1
+      // row r, column i - r n1 at r rs + i - r n1 (transposed: (i - r n1) rs + r)
+      a[u] = TRANS ? d.j * rs + d.ch : i + d.ch * (rs - n1);
       d.next(n1);
     }
 #pragma unroll
@@ -206,38 +208,62 @@ __device__ __forceinline__ void kin4_any(PLine<T> ln, const int2* m, int N, int 
 }
 
 // G^T on a real line of L points (periodic: fine index p, L = 2N; free: fine
-// index p-14): NO coarse outputs i0..i0+NO-1 (c and d) from fine
-// I = 2i0-14+qq, qq < 2 NO + 29.
-template <typename T, bool INTERIOR, int NO = 4>
-__device__ __forceinline__ void shrink4p(PLine<T> ln, int N, int L, int i0, int per,
-                                         T oc[NO], T od[NO]) {
+// index p-14): the taps of fine input I = 2i0-14+qq (qq < 2 NO + 29) on the
+// NO coarse outputs i0..i0+NO-1 (c and d)
+template <typename T, int NO>
+__device__ __forceinline__ void shrink_tap(int qq, T x, T oc[NO], T od[NO]) {
+#pragma unroll
+  for (int r = 0; r < NO; ++r) {
+    const int j = qq - 2 * r - 14;
+    if (j >= -14 && j <= 16) {
+      oc[r] = fma((T)Fil::F2S[j + 14], x, oc[r]);
+      od[r] = fma((T)Fil::F2W[j + 14], x, od[r]);
+    }
+  }
+}
+
+// Fine-line map of a periodic line for the backward pass: entry e = 2 i0 + qq
+// of a shrink chunk holds the byte offset of its input, fine index e - 14
+// modulo L.  Every chunk's entries lie below 2 N + 2 NO + 27 <= fmap_len(N).
+__host__ __device__ constexpr int fmap_len(int N) { return 2 * N + 40; }
+__device__ __forceinline__ void build_fmap(int* m, int N, int L, int es, int tid, int nt) {
+  for (int e = tid; e < fmap_len(N); e += nt) {
+    int p = (e - 14) % L;
+    p += p < 0 ? L : 0;
+    m[e] = p * es;
+  }
+}
+
+// G^T on a fine line stored as an LDS row: NO coarse outputs, read directly
+// where the chunk's inputs lie inside the line; else, periodic, through the
+// map m, free (only the line's last chunk), with fine points past L as zero
+template <typename T, int NO = 4>
+__device__ __forceinline__ void shrink4_any(const T* ln, const int* m, int L, int i0, int per,
+                                            T oc[NO], T od[NO]) {
 #pragma unroll
   for (int r = 0; r < NO; ++r) {
     oc[r] = (T)0;
     od[r] = (T)0;
   }
+  const bool in = per ? (2 * i0 - 14 >= 0 && 2 * i0 + 2 * NO + 14 < L)
+                      : (2 * i0 + 2 * NO + 28 < L);
+  if (in) {
+    const T* q = ln + 2 * i0 - (per ? 14 : 0);
 #pragma unroll
-  for (int qq = 0; qq < 2 * NO + 29; ++qq) {
-    T x;
-    if (INTERIOR) {
-      x = ln.p[(2 * i0 + qq - (per ? 14 : 0)) * ln.es];
-    } else if (per) {
-      int p = 2 * i0 - 14 + qq;
-      p = p < 0 ? p + L : (p >= L ? p - L : p);
-      x = ln.p[p * ln.es];
-    } else {
+    for (int qq = 0; qq < 2 * NO + 29; ++qq) shrink_tap<T, NO>(qq, q[qq], oc, od);
+  } else if (per) {
+    const int* mm = m + 2 * i0;
+    const char* p = (const char*)ln;
+#pragma unroll
+    for (int qq = 0; qq < 2 * NO + 29; ++qq)
+      shrink_tap<T, NO>(qq, *(const T*)(p + mm[qq]), oc, od);
+  } else {
+#pragma unroll
+    for (int qq = 0; qq < 2 * NO + 29; ++qq) {
       const int p = 2 * i0 + qq;
       const bool v = p < L;
-      x = ln.p[(v ? p : 0) * ln.es];
-      x = v ? x : (T)0;
-    }
-#pragma unroll
-    for (int r = 0; r < NO; ++r) {
-      const int j = qq - 2 * r - 14;
-      if (j >= -14 && j <= 16) {
-        oc[r] = fma((T)Fil::F2S[j + 14], x, oc[r]);
-        od[r] = fma((T)Fil::F2W[j + 14], x, od[r]);
-      }
+      const T x = ln[v ? p : 0];
+      shrink_tap<T, NO>(qq, v ? x : (T)0, oc, od);
     }
   }
 }
@@ -259,15 +285,6 @@ __device__ __forceinline__ double block_sum_p(double v, double* red) {
 // first coarse index of the grow chunk starting at fine output p0
 __device__ __forceinline__ int grow_k0(int p0, int per) {
   return per ? (p0 >> 1) - 8 : (p0 >> 1) - 15;
-}
-
-template <typename T, int NO = 4>
-__device__ __forceinline__ void shrink4_any(PLine<T> ln, int N, int L, int i0, int per,
-                                            T oc[NO], T od[NO]) {
-  const bool in = per ? (2 * i0 - 14 >= 0 && 2 * i0 + 2 * NO + 14 < L)
-                      : (2 * i0 + 2 * NO + 28 < L);
-  if (in) shrink4p<T, true, NO>(ln, N, L, i0, per, oc, od);
-  else shrink4p<T, false, NO>(ln, N, L, i0, per, oc, od);
 }
 
 // LDS layout of the forward and preconditioner plane kernels: the W plane A
@@ -482,11 +499,16 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   const int N1 = bd.N[0], N2 = bd.N[1], N3 = bd.N[2], L1 = bd.L[0], L2 = bd.L[1],
             per = bd.per;
   const int W1 = 2 * N1, W2 = 2 * N2;
-  const int RU = L1 + 1;            // T2 plane rows (read by columns)
-  const int RQ = odd_stride(L1);    // G_y^T output rows (read by rows)
+  // odd row strides: a free box's fine extents are odd already, so its rows
+  // take no padding (with the periodic maps, the plane takes no more LDS
+  // than with untransposed rows of L1 + 1)
+  const int RU = L2 | 1;            // T2 plane, transposed: y lines as rows
+  const int RQ = L1 | 1;            // G_y^T output rows (read by rows)
   const int RO = W1 + 1;
-  T* U = (T*)smem;                // T2 plane: U[p2*RU + p1], later output O
-  T* Bq = U + max(L2 * RU, W2 * RO);  // G_y^T output: Bq[j2*RQ + p1]
+  T* U = (T*)smem;                // T2 plane: U[p1*RU + p2], later output O
+  T* Bq = U + max(L1 * RU, W2 * RO);  // G_y^T output: Bq[j2*RQ + p1]
+  int* fmx = (int*)(Bq + W2 * RQ);     // periodic fine-line maps (x: rows of Bq,
+  int* fmy = fmx + fmap_len(N1);       // y: rows of U)
   const int tid = threadIdx.x;
   const T* src = (const T*)B.T2 + bd.t2_off + (long long)L1 * L2 * j3;
   // the compress's first point per thread (indices, K, old hpsi) is loaded
@@ -497,9 +519,13 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
   const T* Kp = (const T*)B.K + bd.w_off + (long long)W1 * W2 * j3;
   CompressStep<T, UP> st0;
   compress_load<UP, NTP>(bd, j3, Kp, B.hpsi, tid, st0);
-  plane_load<T, NTP>(U, RU, src, L1, L2);
+  plane_load<T, NTP, true>(U, RU, src, L1, L2);
+  if (per) {
+    build_fmap(fmx, N1, L1, sizeof(T), tid, NTP);
+    build_fmap(fmy, N2, L2, sizeof(T), tid, NTP);
+  }
   __syncthreads();
-  // G_y^T: columns p1 -> Bq rows (c: i2, d: N2+i2), NOY coarse outputs
+  // G_y^T: y lines p1 (rows of U) -> Bq rows (c: i2, d: N2+i2), NOY coarse outputs
   // per work item (NOX for G_x^T): two in FP32, so that the last round of
   // items leaves fewer lanes idle between the barriers (C4 FP32 backward
   // pass 3.41 -> 3.22 ms); four in FP64 (two was slower on the periodic
@@ -510,7 +536,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
     const int ch = d.ch, p1 = d.j;
     const int i0 = NOY * ch;
     T oc[NOY], od[NOY];
-    shrink4_any<T, NOY>(PLine<T>{U + p1, RU}, N2, L2, i0, per, oc, od);
+    shrink4_any<T, NOY>(U + p1 * RU, fmy, L2, i0, per, oc, od);
 #pragma unroll
     for (int r = 0; r < NOY; ++r)
       if (i0 + r < N2) {
@@ -526,7 +552,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable 
     const int ch = d.ch, j2 = d.j;
     const int i0 = NOX * ch;
     T oc[NOX], od[NOX];
-    shrink4_any<T, NOX>(PLine<T>{Bq + j2 * RQ, 1}, N1, L1, i0, per, oc, od);
+    shrink4_any<T, NOX>(Bq + j2 * RQ, fmx, L1, i0, per, oc, od);
 #pragma unroll
     for (int r = 0; r < NOX; ++r)
       if (i0 + r < N1) {
@@ -553,9 +579,10 @@ size_t plane_lds_fwd(int N1, int N2, int L1, int elem) {
          8 * (size_t)(map_len(N1) + 2 * map_len(N2));
 }
 size_t plane_lds_bwd(int N1, int N2, int L1, int L2, int elem) {
-  const size_t RU = L1 + 1, RQ = odd_stride(L1), RO = 2 * N1 + 1;
-  const size_t u = (size_t)L2 * RU > 2 * N2 * RO ? (size_t)L2 * RU : 2 * N2 * RO;
-  return (u + 2 * N2 * RQ) * elem;
+  const size_t RU = L2 | 1, RQ = L1 | 1, RO = 2 * N1 + 1;
+  const size_t u = (size_t)L1 * RU > 2 * N2 * RO ? (size_t)L1 * RU : 2 * N2 * RO;
+  const bool per = L1 == 2 * N1;
+  return (u + 2 * N2 * RQ) * elem + (per ? 4 * (size_t)(fmap_len(N1) + fmap_len(N2)) : 0);
 }
 
 template <typename T>
