@@ -418,6 +418,13 @@ struct PassTables {
   TileTable vbg[4];
   int nt_vbg[4];
   int n_vbg;
+  // forward plane pass of the FP64 / FP32 plane-fused sequence: the boxes
+  // in groups by how many of their work-groups fit a CU (LDS and threads),
+  // one launch per group; the kinetic partials keep t[P_PF]'s tile numbers
+  TileTable pfg[4];
+  int nt_pfg[4];
+  size_t lds_pfg[4];
+  int n_pfg;
   // preconditioner: x and y operator in one plane pass (FP64 planes that fit)
   int pc_plane;
   size_t lds_pc;
@@ -572,8 +579,8 @@ hipError_t launch_plane_kin_pc(const Batch& B, const TileTable& t, int ntiles, s
                                const double* pc, hipStream_t s);
 size_t plane_attributes();
 template <typename T>
-hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
-                            double* part, hipStream_t s);
+hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, const TileTable& ft,
+                            int ntiles, size_t lds, double* part, hipStream_t s);
 template <typename T>
 hipError_t launch_plane_grow(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                              hipStream_t s);

@@ -1296,8 +1296,15 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
   EnergyTables et;
   if (PT.plane) {
     mark(S_PF, 0);
-    if (PT.ntiles[P_PF] > 0)
-      launch_plane_fwd<T>(B, PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pf, PT.part[0], s);
+    if (PT.n_pfg > 0) {
+      for (int k = 0; k < PT.n_pfg; ++k)
+        if (PT.nt_pfg[k] > 0)
+          launch_plane_fwd<T>(B, PT.pfg[k], PT.t[P_PF], PT.nt_pfg[k], PT.lds_pfg[k],
+                              PT.part[0], s);
+    } else if (PT.ntiles[P_PF] > 0) {
+      launch_plane_fwd<T>(B, PT.t[P_PF], PT.t[P_PF], PT.ntiles[P_PF], PT.lds_pf, PT.part[0],
+                          s);
+    }
     mark(S_PF, 1);
     BDFT_STAGE(S_KZ, P_KZ, lds, (k_kin<T, 2>), B, PT.t[P_KZ], PT.part[2])
     mark(S_ZP, 0);

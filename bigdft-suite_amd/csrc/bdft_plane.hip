@@ -307,17 +307,20 @@ __device__ __forceinline__ void plane_zero_slots(T* A, int RA, int W1, int W2, T
 
 // ---------------------------------------------------------------------------
 // forward plane kernel; KIN = false: T2 only (the density's x and y grow,
-// daub_to_isf in sumrho.f90:377-469), no kinetic, no K, no ekin
+// daub_to_isf in sumrho.f90:377-469), no kinetic, no K, no ekin.  tt: the
+// launch's tiles (all boxes, or one group of them); ft: the table over all
+// boxes whose tile numbers index the kinetic partials
 // ---------------------------------------------------------------------------
 template <typename T, bool KIN = true>
 __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable tt,
-                                                             double* part) {
+                                                             TileTable ft, double* part) {
   constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NTP / 64];
   const int tile = tile_of(tt);
   const int lb = find_box(tt, tile);
-  const BoxDesc bd = B.box[table_box(tt, lb)];
+  const int b = table_box(tt, lb);
+  const BoxDesc bd = B.box[b];
   const int j3 = tile - box_first_tile(tt, lb);
   const int N1 = bd.N[0], N2 = bd.N[1], L1 = bd.L[0], L2 = bd.L[1], per = bd.per;
   const int W1 = 2 * N1, W2 = 2 * N2;
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_fwd(Batch B, TileTable 
       }
   }
   const double eks = block_sum_p<NTP / 64>(ek, red);
-  if (tid == 0) part[tile] = eks;
+  if (tid == 0) part[box_first_tile(ft, b) + j3] = eks;
   __syncthreads();
   // K plane -> memory (contiguous)
   {
@@ -617,15 +620,15 @@ size_t plane_attributes() {
 }
 
 template <typename T>
-hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
-                            double* part, hipStream_t s) {
-  k_plane_fwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t, part);
+hipError_t launch_plane_fwd(const Batch& B, const TileTable& t, const TileTable& ft,
+                            int ntiles, size_t lds, double* part, hipStream_t s) {
+  k_plane_fwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t, ft, part);
   return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_plane_grow(const Batch& B, const TileTable& t, int ntiles, size_t lds,
                              hipStream_t s) {
-  k_plane_fwd<T, false><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t, nullptr);
+  k_plane_fwd<T, false><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t, t, nullptr);
   return hipGetLastError();
 }
 template <typename T>
@@ -634,10 +637,10 @@ hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size
   k_plane_bwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t);
   return hipGetLastError();
 }
-template hipError_t launch_plane_fwd<double>(const Batch&, const TileTable&, int, size_t,
-                                             double*, hipStream_t);
-template hipError_t launch_plane_fwd<float>(const Batch&, const TileTable&, int, size_t,
-                                            double*, hipStream_t);
+template hipError_t launch_plane_fwd<double>(const Batch&, const TileTable&, const TileTable&,
+                                             int, size_t, double*, hipStream_t);
+template hipError_t launch_plane_fwd<float>(const Batch&, const TileTable&, const TileTable&,
+                                            int, size_t, double*, hipStream_t);
 template hipError_t launch_plane_grow<double>(const Batch&, const TileTable&, int, size_t,
                                               hipStream_t);
 template hipError_t launch_plane_grow<float>(const Batch&, const TileTable&, int, size_t,
