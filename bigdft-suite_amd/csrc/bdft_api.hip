@@ -965,6 +965,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   // second pair of launches
   int g_off[4] = {-1, -1, -1, -1};  // small starts, small map, large starts, large map
   int pf_off[4][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // forward plane groups
+  int pb_off[4][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // backward plane groups
   int vb_off[4][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // FP32 backward groups
   if (mfma && ml.fused) {
     const size_t half = ctx->plane_lds / 2;
@@ -1070,14 +1071,13 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     // at most two; FP32: 512, at most four), one launch per group with the
     // group's largest LDS, so that the largest box does not set every
     // work-group's LDS; one group: the single launch over t[P_PF]
-    T.n_pfg = 0;
-    if (T.plane && !mfma && !density && ctx->plane_lds > 0) {
-      const int cap = esz == 8 ? 2 : 4;
+    auto group = [&](int pass, int cap, auto lds_of, TileTable* tg, int* nt, size_t* lg,
+                     int& n, int off[4][2]) {
+      n = 0;
       std::vector<int> grp[4];
       size_t gl[4] = {0, 0, 0, 0};
       for (int i = 0; i < norb; ++i) {
-        const BoxDesc& b = box[i];
-        const size_t v = plane_lds_fwd(b.N[0], b.N[1], b.L[0], (int)esz);
+        const size_t v = lds_of(box[i]);
         const size_t fit = ctx->plane_lds / std::max(v, (size_t)1);
         const int c = (int)std::min((size_t)cap, std::max((size_t)1, fit));
         grp[c - 1].push_back(i);
@@ -1085,24 +1085,36 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       }
       int ng = 0;
       for (int c = 0; c < 4; ++c) ng += grp[c].empty() ? 0 : 1;
-      if (ng > 1)
-        for (int c = 4; c >= 1; --c)
-          if (!grp[c - 1].empty()) {
-            const int k = T.n_pfg++;
-            pf_off[k][0] = (int)starts.size();
-            int acc = 0;
-            for (int i : grp[c - 1]) {
-              starts.push_back(acc);
-              acc += box_tiles(box[i], P_PF, zg);
-            }
+      if (ng < 2) return;
+      for (int c = 4; c >= 1; --c)
+        if (!grp[c - 1].empty()) {
+          const int k = n++;
+          off[k][0] = (int)starts.size();
+          int acc = 0;
+          for (int i : grp[c - 1]) {
             starts.push_back(acc);
-            pf_off[k][1] = (int)starts.size();
-            for (int i : grp[c - 1]) starts.push_back(i);
-            T.nt_pfg[k] = acc;
-            T.pfg[k].nbox = (int)grp[c - 1].size();
-            T.pfg[k].uniform = 0;
-            T.lds_pfg[k] = gl[c - 1];
+            acc += box_tiles(box[i], pass, zg);
           }
+          starts.push_back(acc);
+          off[k][1] = (int)starts.size();
+          for (int i : grp[c - 1]) starts.push_back(i);
+          nt[k] = acc;
+          tg[k].nbox = (int)grp[c - 1].size();
+          tg[k].uniform = 0;
+          lg[k] = gl[c - 1];
+        }
+    };
+    T.n_pfg = T.n_pbg = 0;
+    if (T.plane && !mfma && !density && ctx->plane_lds > 0) {
+      group(P_PF, esz == 8 ? 2 : 4,
+            [&](const BoxDesc& b) { return plane_lds_fwd(b.N[0], b.N[1], b.L[0], (int)esz); },
+            T.pfg, T.nt_pfg, T.lds_pfg, T.n_pfg, pf_off);
+      if (esz == 4)
+        group(P_PB, 4,
+              [&](const BoxDesc& b) {
+                return plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)esz);
+              },
+              T.pbg, T.nt_pbg, T.lds_pbg, T.n_pbg, pb_off);
     }
     // preconditioner plans (FP64): the x and y operator as one plane pass
     // where the W and K planes of the largest box fit in LDS
@@ -1121,7 +1133,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     const int po = ctx->opt_order == 1 ? 1 : 0;
     T.t[P_PF].xcd = T.t[P_PB].xcd = T.t[P_UNC].xcd = po;
     T.pf2.xcd = T.pb2.xcd = po;
-    for (int k = 0; k < 4; ++k) T.vbg[k].xcd = T.pfg[k].xcd = po;
+    for (int k = 0; k < 4; ++k) T.vbg[k].xcd = T.pfg[k].xcd = T.pbg[k].xcd = po;
   }
   if (force) {
     xbig = force->xbig;
@@ -1154,6 +1166,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
   int tb_g[2] = {-1, -1};  // the two plane-kernel box groups
   int tb_v[4] = {-1, -1, -1, -1};  // the FP32 backward groups
   int tb_f[4] = {-1, -1, -1, -1};  // the forward plane groups
+  int tb_b[4] = {-1, -1, -1, -1};  // the backward plane groups
   {
     constexpr int TB_MAX = 1 << 22;
     auto tiles_of = [&](int so, int nb) {
@@ -1173,6 +1186,8 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       tb_v[k] = T.nt_vbg[k] <= TB_MAX ? tiles_of(vb_off[k][0], T.vbg[k].nbox) : -1;
     for (int k = 0; k < T.n_pfg; ++k)
       tb_f[k] = T.nt_pfg[k] <= TB_MAX ? tiles_of(pf_off[k][0], T.pfg[k].nbox) : -1;
+    for (int k = 0; k < T.n_pbg; ++k)
+      tb_b[k] = T.nt_pbg[k] <= TB_MAX ? tiles_of(pb_off[k][0], T.pbg[k].nbox) : -1;
   }
   // work queue of the persistent z potential pass: the boxes in order of
   // decreasing work (fine points), each box's tiles together
@@ -1262,6 +1277,11 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     T.pfg[k].start = dst + pf_off[k][0];
     T.pfg[k].bmap = dst + pf_off[k][1];
     T.pfg[k].tbox = tb_f[k] >= 0 ? dst + tb_f[k] : nullptr;
+  }
+  for (int k = 0; k < T.n_pbg; ++k) {
+    T.pbg[k].start = dst + pb_off[k][0];
+    T.pbg[k].bmap = dst + pb_off[k][1];
+    T.pbg[k].tbox = tb_b[k] >= 0 ? dst + tb_b[k] : nullptr;
   }
   T.zq = zq_off >= 0 ? dst + zq_off : nullptr;
   T.zq_grid = 0;

@@ -425,6 +425,12 @@ struct PassTables {
   int nt_pfg[4];
   size_t lds_pfg[4];
   int n_pfg;
+  // backward plane pass of the FP32 plane-fused sequence, grouped the same
+  // way (the FP64 kernel's registers allow one work-group per CU)
+  TileTable pbg[4];
+  int nt_pbg[4];
+  size_t lds_pbg[4];
+  int n_pbg;
   // preconditioner: x and y operator in one plane pass (FP64 planes that fit)
   int pc_plane;
   size_t lds_pc;

@@ -1311,8 +1311,13 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
     zpot_pass<T>(B, PT, lds, s);
     mark(S_ZP, 1);
     mark(S_PB, 0);
-    if (PT.ntiles[P_PB] > 0)
+    if (PT.n_pbg > 0) {
+      for (int k = 0; k < PT.n_pbg; ++k)
+        if (PT.nt_pbg[k] > 0)
+          launch_plane_bwd<T>(B, PT.pbg[k], PT.nt_pbg[k], PT.lds_pbg[k], s);
+    } else if (PT.ntiles[P_PB] > 0) {
       launch_plane_bwd<T>(B, PT.t[P_PB], PT.ntiles[P_PB], PT.lds_pb, s);
+    }
     mark(S_PB, 1);
     et.t[0] = PT.t[P_PF];
     et.p[0] = PT.part[0];
