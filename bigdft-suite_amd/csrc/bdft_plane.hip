@@ -19,9 +19,10 @@
 
 namespace bdft {
 
-// threads per plane work-group: an FP64 plane fills most of a CU's LDS, so
-// one work-group per CU runs and it takes 12 waves; FP32 planes run two
-// work-groups of 8 waves per CU
+// threads per plane work-group: a large FP64 plane fills most of a CU's
+// LDS, so one work-group runs per CU and it takes 12 waves (smaller boxes'
+// forward planes, launched in their own group, two per CU); FP32 planes run
+// two to four work-groups of 8 waves per CU
 template <typename T>
 struct PlaneNT {
   static constexpr int v = sizeof(T) == 8 ? 768 : 512;
