@@ -38,8 +38,9 @@ import os
 
 from . import PRECISIONS, BigDFTError
 
-# version of the option table: bumped when an option's meaning changes
-TUNE_VERSION = 6
+# version of the option table: bumped when an option's meaning changes, or
+# when kernel changes move which options win (a cached choice is then re-timed)
+TUNE_VERSION = 7
 
 ERR_UNSUPPORTED = 5   # BDFT_ERR_UNSUPPORTED: the option set does not fit this batch
 
