@@ -909,12 +909,11 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
       ml.zp = std::max(ml.zp, mfma_lds_zp(tb[b.opG[2]].Kp, tb[b.opG[2]].Dp, zg));
     }
     // the FP32 plane backward kernel can stand in for the matrix-core one
-    // where every periodic in-plane axis has >= 18 points and its planes fit
+    // where its planes fit
     valu_bwd_ok = ctx->opt_bwd != 1;
     size_t vb_max = 0;
     for (int i = 0; i < norb; ++i) {
       const BoxDesc& b = box[i];
-      if (b.per && (b.N[0] < 18 || b.N[1] < 18)) valu_bwd_ok = false;
       vb_max = std::max(vb_max, plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1],
                                               (int)sizeof(float)));
     }
@@ -1057,7 +1056,6 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     size_t lpf = 0, lpb = 0;
     for (int i = 0; i < norb && plane; ++i) {
       const BoxDesc& b = box[i];
-      if (b.per && (b.N[0] < 18 || b.N[1] < 18)) plane = false;
       lpf = std::max(lpf, plane_lds_fwd(b.N[0], b.N[1], b.L[0], (int)esz));
       lpb = std::max(lpb, plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)esz));
     }

@@ -339,7 +339,7 @@ int bdft_ctx_info(const bdft_ctx *ctx, int64_t *lds_per_block,
  *                         FP32 vector-unit plane kernel for free-BC
  *                         batches, the matrix cores otherwise), 1 matrix
  *                         cores, 2 FP32 vector units where the plane kernel
- *                         applies (periodic in-plane axes >= 18 points)
+ *                         applies (its planes fit in LDS)
  *   BDFT_OPT_CHUNK_MB     Hamiltonian batches and plans: 0 one launch per
  *                         stage over the whole batch; n > 0 the orbitals in
  *                         consecutive chunks whose working grids take at

@@ -535,8 +535,9 @@ def test_mixed_sphere_batch(ctx):
 
 
 # ---------------------------------------------------------------------------
-# both kernel sequences: plane-fused (default where it applies: periodic axes
-# of at least 18 points, planes fitting in LDS) and the line passes
+# both kernel sequences: plane-fused (default where its planes fit in LDS)
+# and the line passes; the small periodic boxes wrap each stencil several
+# times (the plane kernels' line maps take k modulo N)
 # ---------------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def ctx_lines():
@@ -553,7 +554,8 @@ def ctx_lines():
 
 
 @pytest.mark.parametrize("geo,n", [("P", (23, 19, 18)), ("F", (15, 13, 17)),
-                                   ("P", (31, 31, 31))])
+                                   ("P", (31, 31, 31)), ("P", (7, 11, 5)),
+                                   ("P", (2, 0, 4))])
 def test_line_and_plane_sequences_agree_with_oracle(ctx, ctx_lines, geo, n):
     reg = dense_region(geo, n, (0.41, 0.43, 0.39))
     for c in (ctx, ctx_lines):
