@@ -1070,7 +1070,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     // group's largest LDS, so that the largest box does not set every
     // work-group's LDS; one group: the single launch over t[P_PF]
     auto group = [&](int pass, int cap, auto lds_of, TileTable* tg, int* nt, size_t* lg,
-                     int& n, int off[4][2]) {
+                     int* fitv, int& n, int off[4][2]) {
       n = 0;
       std::vector<int> grp[4];
       size_t gl[4] = {0, 0, 0, 0};
@@ -1097,6 +1097,7 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
           off[k][1] = (int)starts.size();
           for (int i : grp[c - 1]) starts.push_back(i);
           nt[k] = acc;
+          if (fitv) fitv[k] = c;
           tg[k].nbox = (int)grp[c - 1].size();
           tg[k].uniform = 0;
           lg[k] = gl[c - 1];
@@ -1106,13 +1107,12 @@ static int make_plan(bdft_ctx* ctx, int norb, const bdft_lr* const* lrs,
     if (T.plane && !mfma && !density && ctx->plane_lds > 0) {
       group(P_PF, esz == 8 ? 2 : 4,
             [&](const BoxDesc& b) { return plane_lds_fwd(b.N[0], b.N[1], b.L[0], (int)esz); },
-            T.pfg, T.nt_pfg, T.lds_pfg, T.n_pfg, pf_off);
-      if (esz == 4)
-        group(P_PB, 4,
-              [&](const BoxDesc& b) {
-                return plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)esz);
-              },
-              T.pbg, T.nt_pbg, T.lds_pbg, T.n_pbg, pb_off);
+            T.pfg, T.nt_pfg, T.lds_pfg, nullptr, T.n_pfg, pf_off);
+      group(P_PB, esz == 8 ? 2 : 4,
+            [&](const BoxDesc& b) {
+              return plane_lds_bwd(b.N[0], b.N[1], b.L[0], b.L[1], (int)esz);
+            },
+            T.pbg, T.nt_pbg, T.lds_pbg, T.fit_pbg, T.n_pbg, pb_off);
     }
     // preconditioner plans (FP64): the x and y operator as one plane pass
     // where the W and K planes of the largest box fit in LDS

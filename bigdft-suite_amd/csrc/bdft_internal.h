@@ -425,11 +425,13 @@ struct PassTables {
   int nt_pfg[4];
   size_t lds_pfg[4];
   int n_pfg;
-  // backward plane pass of the FP32 plane-fused sequence, grouped the same
-  // way (the FP64 kernel's registers allow one work-group per CU)
+  // backward plane pass of the FP64 / FP32 plane-fused sequence, grouped
+  // the same way; fit_pbg: work-groups per CU of each group (an FP64 group
+  // of two runs the kernel bounded to six waves per SIMD)
   TileTable pbg[4];
   int nt_pbg[4];
   size_t lds_pbg[4];
+  int fit_pbg[4];
   int n_pbg;
   // preconditioner: x and y operator in one plane pass (FP64 planes that fit)
   int pc_plane;
@@ -592,7 +594,7 @@ hipError_t launch_plane_grow(const Batch& B, const TileTable& t, int ntiles, siz
                              hipStream_t s);
 template <typename T>
 hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
-                            hipStream_t s);
+                            hipStream_t s, bool occ2 = false);
 hipError_t launch_transfer(const Batch& B, const TileTable& t, int ntiles,
                            long long wtotal, bool compress, hipStream_t s);
 // ev: null, or 2*S_COUNT events recorded before/after each stage;

@@ -1314,7 +1314,8 @@ static hipError_t launch_seq(const Batch& B, const PassTables& PT, int maxN,
     if (PT.n_pbg > 0) {
       for (int k = 0; k < PT.n_pbg; ++k)
         if (PT.nt_pbg[k] > 0)
-          launch_plane_bwd<T>(B, PT.pbg[k], PT.nt_pbg[k], PT.lds_pbg[k], s);
+          launch_plane_bwd<T>(B, PT.pbg[k], PT.nt_pbg[k], PT.lds_pbg[k], s,
+                              sizeof(T) == 8 && PT.fit_pbg[k] == 2);
     } else if (PT.ntiles[P_PB] > 0) {
       launch_plane_bwd<T>(B, PT.t[P_PB], PT.ntiles[P_PB], PT.lds_pb, s);
     }

@@ -492,8 +492,11 @@ hipError_t launch_plane_kin_pc(const Batch& B, const TileTable& t, int ntiles, s
 // ---------------------------------------------------------------------------
 // backward plane kernel with compress-accumulate
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(PlaneNT<T>::v) void k_plane_bwd(Batch B, TileTable tt) {
+// OCC2: bounded to six waves per SIMD (two 768-thread work-groups per CU),
+// for the FP64 boxes whose planes fit half a CU's LDS
+template <typename T, bool OCC2 = false>
+__global__ __launch_bounds__(PlaneNT<T>::v, OCC2 ? 6 : 1) void k_plane_bwd(Batch B,
+                                                                         TileTable tt) {
   constexpr int NTP = PlaneNT<T>::v;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tile = tile_of(tt);
@@ -602,6 +605,9 @@ hipError_t plane_setup(int bytes) {
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_plane_bwd<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_plane_bwd<T, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   return e;
 }
 
@@ -634,8 +640,9 @@ hipError_t launch_plane_grow(const Batch& B, const TileTable& t, int ntiles, siz
 }
 template <typename T>
 hipError_t launch_plane_bwd(const Batch& B, const TileTable& t, int ntiles, size_t lds,
-                            hipStream_t s) {
-  k_plane_bwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t);
+                            hipStream_t s, bool occ2) {
+  if (occ2) k_plane_bwd<T, true><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t);
+  else k_plane_bwd<T><<<ntiles, PlaneNT<T>::v, lds, s>>>(B, t);
   return hipGetLastError();
 }
 template hipError_t launch_plane_fwd<double>(const Batch&, const TileTable&, const TileTable&,
@@ -647,8 +654,8 @@ template hipError_t launch_plane_grow<double>(const Batch&, const TileTable&, in
 template hipError_t launch_plane_grow<float>(const Batch&, const TileTable&, int, size_t,
                                              hipStream_t);
 template hipError_t launch_plane_bwd<double>(const Batch&, const TileTable&, int, size_t,
-                                             hipStream_t);
+                                             hipStream_t, bool);
 template hipError_t launch_plane_bwd<float>(const Batch&, const TileTable&, int, size_t,
-                                            hipStream_t);
+                                            hipStream_t, bool);
 
 }  // namespace bdft
